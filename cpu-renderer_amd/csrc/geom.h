@@ -1,0 +1,365 @@
+// geom.h — ray/triangle intersection and 4-wide BVH traversal (device).
+//
+// Closest hit is the lexicographic minimum of (t, triangle id) over accepted
+// hits with t in [tmin, tmax], t and the hit predicate being the reference's
+// binary64 watertight test (reference src/geometry.cpp:147-182, ordering at
+// :609, acceptance rules at :279-292).  The result does not depend on tree
+// shape, so the device uses its own 4-wide tree and visiting order.
+//
+// Each candidate is first tested in FP32 together with a rigorous forward
+// error bound that covers both the FP32 and the reference's binary64 rounding.
+// When the bound decides every edge sign and the t-interval checks, the FP32
+// decision equals the reference's; otherwise (rays grazing an edge, near-ties
+// in t) the triangle is re-evaluated with the exact binary64 emulation in the
+// reference's expression order.  Box tests are FP32 slabs widened so they can
+// only over-include.
+#pragma once
+#include "layout.h"
+
+namespace cr {
+
+constexpr float kU = 5.9604644775390625e-8f;  // 2^-24, unit roundoff of binary32
+
+struct RayQuery {
+    V3 o, d, olow;
+    float tmin, tmax;
+    uint32_t ignore;
+    int kx, ky, kz;
+    float sx, sy, sz;
+    float inv[3];
+    float slack;    // absolute widening of box intervals for split origins
+    float ovec;     // |olow|_1 * 2: absolute error of vertex offsets in FP32
+    bool precise;   // origin_low != 0
+    bool valid;
+    bool have_exact;
+    sf::f64 esx, esy, esz;  // binary64 shear constants, computed on first use
+};
+
+CR_HDI bool valid_ray(V3 o, V3 olow, V3 d, float tmin, float tmax) {
+    return finite(o) && finite(olow) && finite(d) && nonzero(d) && tmin == tmin && tmax == tmax && tmin <= tmax;
+}
+
+CR_HDI void setup_query(RayQuery& q, V3 o, V3 d, float tmin, float tmax, V3 olow, uint32_t ignore) {
+    q.o = o;
+    q.d = d;
+    q.olow = olow;
+    q.tmin = tmin;
+    q.tmax = tmax;
+    q.ignore = ignore;
+    q.valid = valid_ray(o, olow, d, tmin, tmax);
+    q.precise = nonzero(olow);
+    q.have_exact = false;
+    int kz = 0;
+    if (fabsf(d.y) > fabsf(at(d, kz))) kz = 1;
+    if (fabsf(d.z) > fabsf(at(d, kz))) kz = 2;
+    int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+    if (at(d, kz) < 0) {
+        const int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    q.kx = kx;
+    q.ky = ky;
+    q.kz = kz;
+    const float dz = at(d, kz);
+    q.sx = at(d, kx) / dz;
+    q.sy = at(d, ky) / dz;
+    q.sz = 1.0f / dz;
+    float slack = 0;
+    for (int a = 0; a < 3; ++a) {
+        const float c = at(d, a);
+        float iv = 1.0f / c;
+        // A reciprocal that overflows (subnormal component) would turn finite
+        // slab distances into infinities; NaN makes that axis unconstrained.
+        if (c != 0 && !finite1(iv)) iv = __builtin_nanf("");
+        q.inv[a] = iv;
+        if (q.precise && c != 0 && finite1(iv)) slack = fmaxf(slack, fabsf(at(olow, a) * iv) * 2.0f);
+    }
+    q.slack = slack;
+    q.ovec = (fabsf(olow.x) + fabsf(olow.y) + fabsf(olow.z)) * 2.0f;
+}
+
+CR_HDI void ensure_exact_shear(RayQuery& q) {
+    if (q.have_exact) return;
+    const sf::f64 dz = sf::from_float(at(q.d, q.kz));
+    q.esx = sf::div(sf::from_float(at(q.d, q.kx)), dz);
+    q.esy = sf::div(sf::from_float(at(q.d, q.ky)), dz);
+    q.esz = sf::div(sf::one(), dz);
+    q.have_exact = true;
+}
+
+struct ExactHit {
+    bool hit;
+    sf::f64 t, u, v;
+};
+
+// The reference's intersectTriangle in emulated binary64, same expression order.
+CR_HD inline ExactHit exact_triangle(RayQuery& q, V3 va, V3 vb, V3 vc, sf::f64 limit_in) {
+    ExactHit r;
+    r.hit = false;
+    if (!q.valid) return r;
+    const sf::f64 tmin = sf::from_float(q.tmin);
+    const sf::f64 limit = sf::min_(sf::from_float(q.tmax), limit_in);
+    if (sf::lt(limit, tmin)) return r;
+    ensure_exact_shear(q);
+    const D3 origin = dadd(widen(q.o), widen(q.olow));
+    const D3 a = dsub(widen(va), origin), b = dsub(widen(vb), origin), c = dsub(widen(vc), origin);
+    const int kx = q.kx, ky = q.ky, kz = q.kz;
+    const sf::f64 az = dat(a, kz), bz = dat(b, kz), cz = dat(c, kz);
+    const sf::f64 ax = sf::sub(dat(a, kx), sf::mul(q.esx, az)), ay = sf::sub(dat(a, ky), sf::mul(q.esy, az));
+    const sf::f64 bx = sf::sub(dat(b, kx), sf::mul(q.esx, bz)), by = sf::sub(dat(b, ky), sf::mul(q.esy, bz));
+    const sf::f64 cx = sf::sub(dat(c, kx), sf::mul(q.esx, cz)), cy = sf::sub(dat(c, ky), sf::mul(q.esy, cz));
+    const sf::f64 w0 = sf::sub(sf::mul(cx, by), sf::mul(cy, bx));
+    const sf::f64 w1 = sf::sub(sf::mul(ax, cy), sf::mul(ay, cx));
+    const sf::f64 w2 = sf::sub(sf::mul(bx, ay), sf::mul(by, ax));
+    const sf::f64 z = sf::zero(0);
+    const bool neg = sf::lt(w0, z) || sf::lt(w1, z) || sf::lt(w2, z);
+    const bool pos = sf::gt(w0, z) || sf::gt(w1, z) || sf::gt(w2, z);
+    if (neg && pos) return r;
+    const sf::f64 det = sf::add(sf::add(w0, w1), w2);
+    if (sf::is_zero(det) || !sf::is_finite(det)) return r;
+    const sf::f64 num = sf::add(sf::add(sf::mul(w0, sf::mul(q.esz, az)), sf::mul(w1, sf::mul(q.esz, bz))),
+                                sf::mul(w2, sf::mul(q.esz, cz)));
+    const sf::f64 t = sf::div(num, det);
+    if (!sf::is_finite(t) || sf::gt(sf::fabs_(t), sf::from_float(kFltMax)) || sf::lt(t, tmin) || sf::gt(t, limit))
+        return r;
+    r.hit = true;
+    r.t = t;
+    r.u = sf::div(w1, det);
+    r.v = sf::div(w2, det);
+    return r;
+}
+
+enum FastResult { kReject = 0, kAccept = 1, kUnsure = 2 };
+
+// FP32 watertight test with a forward error bound.  On kAccept, [tlo, thi]
+// encloses the reference's binary64 t and that t lies certainly in [tmin, tmax].
+CR_HDI int fast_triangle(const RayQuery& q, V3 va, V3 vb, V3 vc, float& tlo, float& thi) {
+    const V3 a = va - q.o, b = vb - q.o, c = vc - q.o;
+    const float az = at(a, q.kz), bz = at(b, q.kz), cz = at(c, q.kz);
+    const float ax = at(a, q.kx) - q.sx * az, ay = at(a, q.ky) - q.sy * az;
+    const float bx = at(b, q.kx) - q.sx * bz, by = at(b, q.ky) - q.sy * bz;
+    const float cx = at(c, q.kx) - q.sx * cz, cy = at(c, q.ky) - q.sy * cz;
+    const float w0 = cx * by - cy * bx;
+    const float w1 = ax * cy - ay * cx;
+    const float w2 = bx * ay - by * ax;
+    // Magnitude bound for every sheared coordinate (|s| <= 1 for sx, sy).
+    const float ma = fabsf(a.x) + fabsf(a.y) + fabsf(a.z);
+    const float mb = fabsf(b.x) + fabsf(b.y) + fabsf(b.z);
+    const float mc = fabsf(c.x) + fabsf(c.y) + fabsf(c.z);
+    const float m = fmaxf(ma, fmaxf(mb, mc)) + q.ovec;
+    const float ev = 8.0f * kU * m + q.ovec;            // error of each sheared coordinate
+    const float ew = (2.0f * m * ev + ev * ev + 4.0f * kU * m * m) * 1.25f;  // error of each w
+    if (!finite1(ew) || !finite1(w0) || !finite1(w1) || !finite1(w2)) return kUnsure;
+    const bool p0 = w0 > ew, p1 = w1 > ew, p2 = w2 > ew;
+    const bool n0 = w0 < -ew, n1 = w1 < -ew, n2 = w2 < -ew;
+    if ((p0 || p1 || p2) && (n0 || n1 || n2)) return kReject;  // certainly mixed signs
+    const bool allpos = p0 && p1 && p2, allneg = n0 && n1 && n2;
+    if (!allpos && !allneg) return kUnsure;
+    const float det = (w0 + w1) + w2;
+    const float edet = 3.0f * ew + 2.0f * kU * fabsf(det);
+    const float adet = fabsf(det);
+    if (!(adet > 2.0f * edet)) return kUnsure;
+    const float za = q.sz * az, zb = q.sz * bz, zc = q.sz * cz;
+    const float num = (w0 * za + w1 * zb) + w2 * zc;
+    const float ez = 3.0f * kU * m * fabsf(q.sz) + fabsf(q.sz) * q.ovec;  // error of each sz*A_kz
+    const float zmax = fmaxf(fabsf(za), fmaxf(fabsf(zb), fabsf(zc)));
+    const float wsum = fabsf(w0) + fabsf(w1) + fabsf(w2);
+    const float enum_ = (wsum * ez + 3.0f * ew * (zmax + ez)) + 4.0f * kU * wsum * zmax;
+    const float t = num / det;
+    if (!finite1(t) || !finite1(enum_)) return kUnsure;
+    const float et = ((enum_ + fabsf(t) * edet) / (adet - edet) + 2.0f * kU * fabsf(t)) * 1.5f + 1e-37f;
+    tlo = t - et;
+    thi = t + et;
+    if (!(tlo == tlo) || !(thi == thi) || !finite1(thi) || !finite1(tlo)) return kUnsure;
+    if (tlo > q.tmax || thi < q.tmin) return kReject;
+    if (tlo < q.tmin || thi > q.tmax) return kUnsure;
+    return kAccept;
+}
+
+// accepts(): single-sided opaque culling on the binary64 sign of dot(Ng, d)
+// (reference src/geometry.cpp:279-292).
+CR_HDI bool prim_accepts(const DevScene& s, uint32_t prim, uint32_t flags, V3 d) {
+    if (flags & kPrimNeverHit) return false;
+    if (flags & kPrimCull) {
+        const float4 n = s.prim_ng[prim];
+        if (dot_sign_exact(V3{n.x, n.y, n.z}, d) >= 0) return false;
+    }
+    return true;
+}
+
+// Slab test against the four children of a node.  Returns a 4-bit hit mask and
+// the (widened) entry distances.
+CR_HDI uint32_t node_hit(const RayQuery& q, const WNode& n, float limit, float near_out[4]) {
+    uint32_t mask = 0;
+    const bool px = !(q.inv[0] < 0), py = !(q.inv[1] < 0), pz = !(q.inv[2] < 0);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float nx = ((px ? n.lox[c] : n.hix[c]) - q.o.x) * q.inv[0];
+        const float fx = ((px ? n.hix[c] : n.lox[c]) - q.o.x) * q.inv[0];
+        const float ny = ((py ? n.loy[c] : n.hiy[c]) - q.o.y) * q.inv[1];
+        const float fy = ((py ? n.hiy[c] : n.loy[c]) - q.o.y) * q.inv[1];
+        const float nz = ((pz ? n.loz[c] : n.hiz[c]) - q.o.z) * q.inv[2];
+        const float fz = ((pz ? n.hiz[c] : n.loz[c]) - q.o.z) * q.inv[2];
+        float tn = fmaxf(fmaxf(q.tmin, nx), fmaxf(ny, nz));
+        float tf = fminf(fminf(limit, fx), fminf(fy, fz));
+        tn = tn - (fabsf(tn) * (4.0f * kU) + q.slack);
+        tf = tf + (fabsf(tf) * (4.0f * kU) + q.slack);
+        near_out[c] = tn;
+        if (tn <= tf && n.child[c] != kEmptyChild) mask |= 1u << c;
+    }
+    return mask;
+}
+
+constexpr int kStackSize = 160;
+
+struct StackEntry {
+    uint32_t ref;
+    float near;
+};
+
+struct ClosestResult {
+    uint32_t prim;  // leaf-order primitive, kNoTriangle on miss
+    uint32_t id;    // finalized triangle index
+    bool exact;     // t,u,v hold the binary64 values
+    bool overflow;  // traversal stack exhausted (reported as an error)
+    sf::f64 t, u, v;
+    float tlo, thi;  // FP32 enclosure of the binary64 t (or the query limit)
+};
+
+// Replace the current best by an exactly evaluated candidate when it wins the
+// reference's (t, id) ordering.
+CR_HDI void take_exact(ClosestResult& best, uint32_t prim, uint32_t id, const ExactHit& e) {
+    best.prim = prim;
+    best.id = id;
+    best.exact = true;
+    best.t = e.t;
+    best.u = e.u;
+    best.v = e.v;
+    const float tf = sf::to_float(e.t);
+    best.tlo = next_down(tf);
+    best.thi = next_up(tf);
+}
+
+template <bool ANY>
+CR_HD inline bool leaf_prims(const DevScene& s, RayQuery& q, uint32_t first, uint32_t count, ClosestResult& best,
+                             uint32_t& fb) {
+    for (uint32_t i = first; i < first + count; ++i) {
+        const PrimRec p = s.prims[i];
+        if (p.id == q.ignore) continue;
+        const V3 va{p.ax, p.ay, p.az}, vb{p.bx, p.by, p.bz}, vc{p.cx, p.cy, p.cz};
+        float clo = 0, chi = 0;
+        const int fr = fast_triangle(q, va, vb, vc, clo, chi);
+        if (fr == kReject) continue;
+        if (fr == kAccept) {
+            if (!ANY && clo > best.thi) continue;  // certainly beyond the current limit
+            if (!prim_accepts(s, i, p.flags, q.d)) continue;
+            if (ANY) {
+                best.prim = i;
+                best.id = p.id;
+                best.exact = false;
+                best.tlo = clo;
+                best.thi = chi;
+                return true;
+            }
+            if (chi < best.tlo) {  // certainly nearer than the current best
+                best.prim = i;
+                best.id = p.id;
+                best.exact = false;
+                best.tlo = clo;
+                best.thi = chi;
+                continue;
+            }
+        }
+        // Undecided in FP32: decide with the binary64 emulation.
+        ++fb;
+        if (!ANY && !best.exact) {
+            const PrimRec bp = s.prims[best.prim];
+            const ExactHit e = exact_triangle(q, V3{bp.ax, bp.ay, bp.az}, V3{bp.bx, bp.by, bp.bz},
+                                              V3{bp.cx, bp.cy, bp.cz}, sf::inf(0));
+            take_exact(best, best.prim, best.id, e);
+        }
+        const ExactHit e = exact_triangle(q, va, vb, vc, ANY ? sf::inf(0) : best.t);
+        if (!e.hit) continue;
+        if (!prim_accepts(s, i, p.flags, q.d)) continue;
+        if (ANY) {
+            take_exact(best, i, p.id, e);
+            return true;
+        }
+        if (best.prim == kNoTriangle || sf::lt(e.t, best.t) || (sf::eq(e.t, best.t) && p.id < best.id))
+            take_exact(best, i, p.id, e);
+    }
+    return false;
+}
+
+// Bvh::intersect on the 4-wide tree (reference src/geometry.cpp:554-639).
+template <bool ANY>
+CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, uint32_t& fallbacks) {
+    ClosestResult best;
+    best.prim = kNoTriangle;
+    best.id = 0;
+    best.exact = true;
+    best.overflow = false;
+    best.t = sf::from_float(q.tmax);
+    best.u = best.v = sf::zero(0);
+    best.tlo = best.thi = q.tmax;
+    uint32_t fb = 0;
+    if (!q.valid || s.node_count == 0 || q.tmax < q.tmin) return best;
+    StackEntry stack[kStackSize];
+    int sp = 0;
+    uint32_t ref = 0;
+    for (;;) {
+        if (!ref_is_leaf(ref)) {
+            const WNode& n = s.nodes[ref];
+            float nears[4];
+            uint32_t mask = node_hit(q, n, best.thi, nears);
+            int order[4];
+            int cnt = 0;
+            while (mask) {
+                const int c = __builtin_ctz(mask);
+                mask &= mask - 1;
+                int k = cnt++;
+                while (k > 0 && nears[order[k - 1]] < nears[c]) {
+                    order[k] = order[k - 1];
+                    --k;
+                }
+                order[k] = c;  // descending near: order[cnt-1] is the nearest
+            }
+            if (cnt > 0) {
+                if (sp + cnt - 1 > kStackSize) {
+                    best.overflow = true;
+                    break;
+                }
+                for (int k = 0; k < cnt - 1; ++k) {
+                    stack[sp].ref = n.child[order[k]];
+                    stack[sp].near = nears[order[k]];
+                    ++sp;
+                }
+                ref = n.child[order[cnt - 1]];
+                continue;
+            }
+        } else {
+            if (leaf_prims<ANY>(s, q, leaf_first(ref), leaf_count(ref), best, fb)) break;
+        }
+        // pop the next entry that may still hold a nearer hit
+        bool found = false;
+        while (sp > 0) {
+            const StackEntry e = stack[--sp];
+            if (!ANY && e.near > best.thi) continue;
+            ref = e.ref;
+            found = true;
+            break;
+        }
+        if (!found) break;
+    }
+    fallbacks += fb;
+    return best;
+}
+
+// Exact (t, u, v) of a known hit primitive.
+CR_HDI ExactHit exact_of(const DevScene& s, RayQuery& q, uint32_t prim) {
+    const PrimRec p = s.prims[prim];
+    return exact_triangle(q, V3{p.ax, p.ay, p.az}, V3{p.bx, p.by, p.bz}, V3{p.cx, p.cy, p.cz}, sf::inf(0));
+}
+
+}  // namespace cr

@@ -1,0 +1,600 @@
+// host_scene.cpp — scene finalization, 4-wide SAH build, light/environment
+// inversion tables and camera constants, all on the host.
+#include "host_scene.h"
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <future>
+#include <limits>
+#include <numeric>
+
+namespace cr {
+namespace {
+
+constexpr float kHostPi = 3.14159265358979323846f;
+
+struct DV {
+    double x, y, z;
+};
+DV dv(HVec3 v) { return {v.x, v.y, v.z}; }
+DV operator+(DV a, DV b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+DV operator-(DV a, DV b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+DV operator*(DV a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+double ddot_(DV a, DV b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+DV dcross(DV a, DV b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+// unit() of the reference (src/geometry.cpp:31-37).
+HVec3 unit_of(DV v, HVec3 fallback = {0, 0, 0}) {
+    const double sq = ddot_(v, v);
+    if (!(sq > 0) || !std::isfinite(sq)) return fallback;
+    const double inv = 1 / std::sqrt(sq);
+    return {float(v.x * inv), float(v.y * inv), float(v.z * inv)};
+}
+// orthogonalized() of the reference (src/geometry.cpp:40-47).
+HVec3 orthogonal_of(DV v, HVec3 n, HVec3 fallback = {0, 0, 0}) {
+    const double nn = ddot_(dv(n), dv(n));
+    if (!(nn > 0)) return fallback;
+    const DV proj = v - dv(n) * (ddot_(v, dv(n)) / nn);
+    constexpr double tol = 4 * std::numeric_limits<float>::epsilon();
+    if (ddot_(proj, proj) <= ddot_(v, v) * tol * tol) return fallback;
+    return unit_of(proj, fallback);
+}
+bool fin(float v) { return std::isfinite(v); }
+bool fin3(const float* v) { return fin(v[0]) && fin(v[1]) && fin(v[2]); }
+bool nonzero3(HVec3 v) { return v.x != 0 || v.y != 0 || v.z != 0; }
+float fdot(HVec3 a, HVec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+bool in_unit(float v) { return std::isfinite(v) && v >= 0 && v <= 1; }
+float luminance3(const float* c) { return c[0] * 0.2126f + c[1] * 0.7152f + c[2] * 0.0722f; }
+
+}  // namespace
+
+HostScene finalize_scene(const crg_scene_desc& d) {
+    HostScene s;
+    s.materials.assign(d.materials, d.materials + d.material_count);
+    s.lights.assign(d.lights, d.lights + d.light_count);
+    for (uint32_t i = 0; i < d.image_count; ++i) {
+        const crg_image& im = d.images[i];
+        if (im.width <= 0 || im.height <= 0 || !im.rgba)
+            throw std::runtime_error("Image " + std::to_string(i) + " has invalid dimensions or pixel count");
+        const size_t n = size_t(im.width) * size_t(im.height) * 4;
+        for (size_t k = 0; k < n; ++k)
+            if (!fin(im.rgba[k])) throw std::runtime_error("Image " + std::to_string(i) + " contains non-finite pixels");
+    }
+    for (uint32_t i = 0; i < d.texture_count; ++i) {
+        const crg_texture& t = d.textures[i];
+        if (t.image < 0 || uint32_t(t.image) >= d.image_count)
+            throw std::runtime_error("Texture " + std::to_string(i) + " has an invalid image index");
+        auto ok = [](int w) { return w == 0 || w == 1 || w == 2; };
+        if (!ok(t.wrap_s) || !ok(t.wrap_t))
+            throw std::runtime_error("Texture " + std::to_string(i) + " has an invalid wrap mode");
+    }
+    for (size_t i = 0; i < s.materials.size(); ++i) {
+        const crg_material& m = s.materials[i];
+        const std::string label = "Material " + std::to_string(i);
+        if (!fin(m.base_color[0]) || !fin(m.base_color[1]) || !fin(m.base_color[2]) || !fin(m.base_color[3]) ||
+            !in_unit(m.base_color[0]) || !in_unit(m.base_color[1]) || !in_unit(m.base_color[2]) ||
+            !in_unit(m.base_color[3]) || !in_unit(m.metallic) || !in_unit(m.roughness) || !in_unit(m.transmission) ||
+            !in_unit(m.alpha_cutoff) || !fin3(m.emission) || m.emission[0] < 0 || m.emission[1] < 0 ||
+            m.emission[2] < 0 || !fin(m.ior) || m.ior <= 0 || !fin(m.thickness) || m.thickness < 0 ||
+            std::isnan(m.attenuation_distance) || m.attenuation_distance <= 0 || !in_unit(m.attenuation_color[0]) ||
+            !in_unit(m.attenuation_color[1]) || !in_unit(m.attenuation_color[2]) || !fin(m.normal_scale))
+            throw std::runtime_error(label + " has non-finite or out-of-range parameters");
+        for (const crg_texture_slot* slot :
+             {&m.base_color_tex, &m.metallic_roughness_tex, &m.normal_tex, &m.emissive_tex, &m.transmission_tex}) {
+            if (slot->texture < -1 || (slot->texture >= 0 && uint32_t(slot->texture) >= d.texture_count))
+                throw std::runtime_error(label + " has an invalid texture index");
+            if (slot->texture >= 0 && (slot->texcoord < 0 || slot->texcoord > 1 || !fin(slot->offset[0]) ||
+                                       !fin(slot->offset[1]) || !fin(slot->scale[0]) || !fin(slot->scale[1]) ||
+                                       !fin(slot->rotation)))
+                throw std::runtime_error(label + " has invalid texture coordinates or transform");
+        }
+        if (m.normal_tex.texture >= 0 && (m.normal_tex.scale[0] == 0 || m.normal_tex.scale[1] == 0))
+            s.warnings.push_back(label + " has a singular normal-map UV transform; using the untransformed tangent frame");
+    }
+    const uint64_t n = d.triangle_count;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (d.material_ids[i] >= d.material_count)
+            throw std::runtime_error("Triangle " + std::to_string(i) + " has an invalid material index");
+        bool ok = true;
+        for (int k = 0; k < 9; ++k) ok = ok && fin(d.positions[9 * i + k]);
+        if (d.normals)
+            for (int k = 0; k < 9; ++k) ok = ok && fin(d.normals[9 * i + k]);
+        if (d.tangents)
+            for (int k = 0; k < 12; ++k) ok = ok && fin(d.tangents[12 * i + k]);
+        if (d.uvs)
+            for (int k = 0; k < 12; ++k) ok = ok && fin(d.uvs[12 * i + k]);
+        if (!ok) throw std::runtime_error("Triangle " + std::to_string(i) + " contains non-finite vertex data");
+    }
+    s.tris.reserve(n);
+    size_t degenerate = 0, first_degenerate = 0, generated = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        HTriangle t;
+        for (int k = 0; k < 3; ++k) {
+            t.p[k] = {d.positions[9 * i + 3 * k], d.positions[9 * i + 3 * k + 1], d.positions[9 * i + 3 * k + 2]};
+            t.n[k] = d.normals ? HVec3{d.normals[9 * i + 3 * k], d.normals[9 * i + 3 * k + 1], d.normals[9 * i + 3 * k + 2]}
+                               : HVec3{0, 0, 0};
+        }
+        t.material = d.material_ids[i];
+        const DV normal = dcross(dv(t.p[1]) - dv(t.p[0]), dv(t.p[2]) - dv(t.p[0]));
+        const double twice = std::sqrt(ddot_(normal, normal));
+        if (twice * 0.5 > std::numeric_limits<float>::max())
+            throw std::runtime_error("Triangle " + std::to_string(i) + " area exceeds the supported finite range");
+        t.area = float(twice * 0.5);
+        if (!(t.area > 0)) {
+            if (degenerate == 0) first_degenerate = i;
+            ++degenerate;
+            continue;
+        }
+        t.ng = unit_of(normal);
+        for (int k = 0; k < 3; ++k) {
+            HVec3 nn = unit_of(dv(t.n[k]));
+            if (!nonzero3(nn) || fdot(nn, t.ng) == 0) {
+                nn = t.ng;
+                ++generated;
+            } else if (fdot(nn, t.ng) < 0) {
+                nn = {-nn.x, -nn.y, -nn.z};
+            }
+            t.n[k] = nn;
+        }
+        t.ung = unit_of(dv(t.ng));
+        s.tris.push_back(t);
+    }
+    if (degenerate)
+        s.warnings.push_back("Skipped " + std::to_string(degenerate) + " degenerate triangles (first input triangle " +
+                             std::to_string(first_degenerate) + ")");
+    if (generated)
+        s.warnings.push_back("Generated flat normals for " + std::to_string(generated) +
+                             " missing or tangential vertex normals");
+    return s;
+}
+
+// ------------------------------------------------------------------ BVH
+namespace {
+
+struct BPrim {
+    float lo[3], hi[3], c[3];
+};
+
+struct Box {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const float* l, const float* h) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], l[a]);
+            hi[a] = std::max(hi[a], h[a]);
+        }
+    }
+    void grow(const Box& b) { grow(b.lo, b.hi); }
+    double area() const {
+        if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) return 0;
+        const double dx = double(hi[0]) - lo[0], dy = double(hi[1]) - lo[1], dz = double(hi[2]) - lo[2];
+        return 2 * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+constexpr int kBins = 16;
+constexpr uint32_t kLeafSize = 4;
+constexpr int kMaxWideDepth = 40;
+
+struct Builder {
+    const std::vector<BPrim>& prims;
+    std::vector<uint32_t>& idx;
+    std::vector<WNode>& nodes;
+    std::atomic<uint32_t> next{0};
+    std::atomic<uint64_t> leaves{0};
+
+    Box bounds(uint32_t b, uint32_t e, Box* centers) const {
+        Box box, cen;
+        for (uint32_t i = b; i < e; ++i) {
+            const BPrim& p = prims[idx[i]];
+            box.grow(p.lo, p.hi);
+            cen.grow(p.c, p.c);
+        }
+        if (centers) *centers = cen;
+        return box;
+    }
+
+    // Binned SAH split of [b,e); returns the split point or 0 when a leaf is cheaper.
+    uint32_t sah_split(uint32_t b, uint32_t e) {
+        Box cen;
+        const Box box = bounds(b, e, &cen);
+        const double parent = box.area();
+        if (!(parent > 0)) return 0;
+        struct Bin {
+            Box box;
+            uint32_t count = 0;
+        };
+        std::array<std::array<Bin, kBins>, 3> bins{};
+        auto bin_at = [&](float c, int a) {
+            const double ext = double(cen.hi[a]) - cen.lo[a];
+            if (!(ext > 0)) return 0;
+            return std::clamp(int((double(c) - cen.lo[a]) / ext * kBins), 0, kBins - 1);
+        };
+        for (uint32_t i = b; i < e; ++i) {
+            const BPrim& p = prims[idx[i]];
+            for (int a = 0; a < 3; ++a) {
+                Bin& bn = bins[a][bin_at(p.c[a], a)];
+                bn.box.grow(p.lo, p.hi);
+                ++bn.count;
+            }
+        }
+        int best_axis = -1, best_split = -1;
+        double best = double(e - b) * parent;
+        for (int a = 0; a < 3; ++a) {
+            if (cen.lo[a] == cen.hi[a]) continue;
+            std::array<Box, kBins> rb{};
+            std::array<uint32_t, kBins> rc{};
+            Box r;
+            uint32_t cnt = 0;
+            for (int k = kBins - 1; k >= 0; --k) {
+                if (bins[a][k].count) r.grow(bins[a][k].box);
+                cnt += bins[a][k].count;
+                rb[k] = r;
+                rc[k] = cnt;
+            }
+            Box l;
+            cnt = 0;
+            for (int k = 0; k < kBins - 1; ++k) {
+                if (bins[a][k].count) l.grow(bins[a][k].box);
+                cnt += bins[a][k].count;
+                if (!cnt || !rc[k + 1]) continue;
+                const double cost = parent + cnt * l.area() + rc[k + 1] * rb[k + 1].area();
+                if (cost < best) {
+                    best = cost;
+                    best_axis = a;
+                    best_split = k;
+                }
+            }
+        }
+        if (best_axis < 0) return 0;
+        const int a = best_axis;
+        uint32_t* mid = std::partition(idx.data() + b, idx.data() + e,
+                                       [&](uint32_t i) { return bin_at(prims[i].c[a], a) <= best_split; });
+        const uint32_t m = uint32_t(mid - idx.data());
+        if (m == b || m == e) return 0;
+        return m;
+    }
+
+    // Median split on the widest centroid axis (or by index when centroids coincide).
+    uint32_t median_split(uint32_t b, uint32_t e) {
+        Box cen;
+        bounds(b, e, &cen);
+        int a = 0;
+        float ext = -1;
+        for (int k = 0; k < 3; ++k)
+            if (cen.hi[k] - cen.lo[k] > ext) {
+                ext = cen.hi[k] - cen.lo[k];
+                a = k;
+            }
+        const uint32_t m = b + (e - b) / 2;
+        if (ext > 0)
+            std::nth_element(idx.data() + b, idx.data() + m, idx.data() + e,
+                             [&](uint32_t x, uint32_t y) { return prims[x].c[a] < prims[y].c[a]; });
+        return m;
+    }
+
+    uint32_t build(uint32_t b, uint32_t e, int depth) {
+        const uint32_t me = next.fetch_add(1);
+        if (me >= nodes.size()) throw std::logic_error("wide BVH node bound exceeded");
+        struct Range {
+            uint32_t b, e;
+            bool leaf;
+        };
+        std::array<Range, 4> r{};
+        int nr = 1;
+        r[0] = {b, e, false};
+        bool forced = depth >= kMaxWideDepth;
+        while (nr < 4) {
+            int pick = -1;
+            uint32_t most = 0;
+            for (int k = 0; k < nr; ++k)
+                if (!r[k].leaf && r[k].e - r[k].b > most) {
+                    most = r[k].e - r[k].b;
+                    pick = k;
+                }
+            if (pick < 0) break;
+            Range& g = r[pick];
+            const uint32_t n = g.e - g.b;
+            uint32_t m = 0;
+            if (n > kLeafSize && !forced) m = sah_split(g.b, g.e);
+            if (m == 0 && (n > kLeafMaxCount || (forced && n > kLeafSize))) m = median_split(g.b, g.e);
+            if (m == 0) {
+                g.leaf = true;
+                continue;
+            }
+            const Range right{m, g.e, false};
+            g.e = m;
+            r[nr++] = right;
+        }
+        std::array<uint32_t, 4> child{kEmptyChild, kEmptyChild, kEmptyChild, kEmptyChild};
+        std::array<Box, 4> boxes{};
+        std::array<std::future<uint32_t>, 4> pending;
+        for (int k = 0; k < nr; ++k) {
+            boxes[k] = bounds(r[k].b, r[k].e, nullptr);
+            const uint32_t n = r[k].e - r[k].b;
+            const bool leaf = n <= kLeafSize || (r[k].leaf && n <= kLeafMaxCount);
+            if (leaf) {
+                if (r[k].b > kLeafMaxFirst) throw std::length_error("Too many triangles for the device BVH");
+                child[k] = make_leaf(r[k].b, n);
+                leaves.fetch_add(1);
+            } else if (n > 65536) {
+                const uint32_t cb = r[k].b, ce = r[k].e;
+                pending[k] = std::async(std::launch::async, [this, cb, ce, depth] { return build(cb, ce, depth + 1); });
+            } else {
+                child[k] = build(r[k].b, r[k].e, depth + 1);
+            }
+        }
+        for (int k = 0; k < nr; ++k)
+            if (pending[k].valid()) child[k] = pending[k].get();
+        WNode& node = nodes[me];
+        for (int k = 0; k < 4; ++k) {
+            const bool used = k < nr;
+            node.lox[k] = used ? boxes[k].lo[0] : INFINITY;
+            node.loy[k] = used ? boxes[k].lo[1] : INFINITY;
+            node.loz[k] = used ? boxes[k].lo[2] : INFINITY;
+            node.hix[k] = used ? boxes[k].hi[0] : -INFINITY;
+            node.hiy[k] = used ? boxes[k].hi[1] : -INFINITY;
+            node.hiz[k] = used ? boxes[k].hi[2] : -INFINITY;
+            node.child[k] = child[k];
+            node.pad[k] = 0;
+        }
+        return me;
+    }
+};
+
+}  // namespace
+
+WideBvh build_wide_bvh(const HostScene& s) {
+    WideBvh out;
+    const size_t n = s.tris.size();
+    if (n == 0) return out;
+    if (n > size_t(std::numeric_limits<uint32_t>::max() / 2))
+        throw std::length_error("Too many triangles for 32-bit BVH node indices");
+    std::vector<BPrim> prims(n);
+    for (size_t i = 0; i < n; ++i) {
+        const HTriangle& t = s.tris[i];
+        BPrim& p = prims[i];
+        for (int a = 0; a < 3; ++a) {
+            const float v0 = a == 0 ? t.p[0].x : (a == 1 ? t.p[0].y : t.p[0].z);
+            const float v1 = a == 0 ? t.p[1].x : (a == 1 ? t.p[1].y : t.p[1].z);
+            const float v2 = a == 0 ? t.p[2].x : (a == 1 ? t.p[2].y : t.p[2].z);
+            p.lo[a] = std::min({v0, v1, v2});
+            p.hi[a] = std::max({v0, v1, v2});
+            p.c[a] = float((double(v0) + double(v1) + double(v2)) * (1.0 / 3.0));
+        }
+    }
+    out.order.resize(n);
+    std::iota(out.order.begin(), out.order.end(), 0u);
+    out.nodes.resize(2 * n + 2);  // every wide node owns >= 1 leaf or >= 2 children
+    Builder b{prims, out.order, out.nodes};
+    b.build(0, uint32_t(n), 0);
+    out.nodes.resize(b.next.load());
+    out.binary_nodes = 2 * b.leaves.load() - 1;
+    return out;
+}
+
+// ------------------------------------------------------------------ environment
+HostEnv build_env(const crg_env_desc& d) {
+    HostEnv e;
+    if (!d.rgba) {
+        if (!fin3(d.constant) || std::min({d.constant[0], d.constant[1], d.constant[2]}) < 0)
+            throw std::runtime_error("Invalid environment radiance");
+        e.is_image = false;
+        std::memcpy(e.constant, d.constant, sizeof(e.constant));
+        e.integral = 4 * kHostPi * luminance3(d.constant);
+        return e;
+    }
+    if (d.width <= 0 || d.height <= 0) throw std::runtime_error("Invalid environment image dimensions");
+    if (!std::isfinite(d.rotation_degrees) || !std::isfinite(d.intensity) || d.intensity < 0)
+        throw std::runtime_error("Invalid environment rotation/intensity");
+    e.is_image = true;
+    e.width = d.width;
+    e.height = d.height;
+    e.rotation = std::fmod(d.rotation_degrees, 360.0f) * kHostPi / 180;
+    e.intensity = d.intensity;
+    const size_t n = size_t(d.width) * size_t(d.height);
+    e.texels.assign(d.rgba, d.rgba + 4 * n);
+    std::vector<double> cdf;
+    cdf.reserve(n);
+    auto solid_angle = [&](int row) {
+        const double a = double(kHostPi) * row / d.height, b = double(kHostPi) * (row + 1) / d.height;
+        return (2 * double(kHostPi) / d.width) * (std::cos(a) - std::cos(b));
+    };
+    for (int y = 0; y < d.height; ++y) {
+        const double omega = solid_angle(y);
+        for (int x = 0; x < d.width; ++x) {
+            const float* p = d.rgba + 4 * (size_t(y) * d.width + x);
+            if (!fin3(p) || std::min({p[0], p[1], p[2]}) < 0)
+                throw std::runtime_error("Non-finite or negative HDR environment pixel");
+            const double light = luminance3(p) * e.intensity;
+            e.integral += light * omega;
+            e.weight_sum += (light + 1e-9) * omega;
+            cdf.push_back(e.weight_sum);
+        }
+    }
+    e.pdf.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        const int y = int(i / size_t(d.width));
+        e.pdf[i] = float((cdf[i] - (i ? cdf[i - 1] : 0)) / (e.weight_sum * solid_angle(y)));
+    }
+    // T_i = first 24-bit draw k with pick(k) = double(k*2^-24)*W >= cdf[i]; the
+    // texel picked for k is then #{i : T_i <= k} (upper_bound semantics).
+    const double W = e.weight_sum;
+    auto pick = [&](uint32_t k) { return double(float(k) * (1.0f / 16777216.0f)) * W; };
+    e.threshold.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t lo = 0, hi = 1u << 24;  // search in [0, 2^24]; 2^24 means "never"
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (pick(mid) >= cdf[i]) hi = mid;
+            else lo = mid + 1;
+        }
+        e.threshold[i] = lo;
+    }
+    e.guide.resize(65537);
+    for (uint32_t j = 0; j <= 65536; ++j) {
+        const uint32_t k = j << 8;
+        e.guide[j] = uint32_t(std::upper_bound(e.threshold.begin(), e.threshold.end(), k) - e.threshold.begin());
+        if (j == 65536) e.guide[j] = uint32_t(n);
+    }
+    // The guide bounds the search window [G[k>>8], G[(k>>8)+1]] for any k.
+    e.rowcos.resize(size_t(d.height) + 1);
+    for (int y = 0; y <= d.height; ++y) e.rowcos[y] = std::cos(kHostPi * float(y) / float(d.height));
+    return e;
+}
+
+// ------------------------------------------------------------------ lights
+SceneLights scene_lights(const HostScene& s) {
+    SceneLights sl;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (const HTriangle& t : s.tris)
+        for (int k = 0; k < 3; ++k) {
+            const float v[3] = {t.p[k].x, t.p[k].y, t.p[k].z};
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], v[a]);
+                hi[a] = std::max(hi[a], v[a]);
+            }
+        }
+    const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+    sl.radius2 = s.tris.empty() ? 1 : std::max(1.0, double(ex * ex + ey * ey + ez * ez) * 0.25);
+    for (const crg_light& l : s.lights) {
+        sl.punctual_weight.push_back(double(luminance3(l.color)) * l.intensity *
+                                     (l.type == 1 ? kHostPi * sl.radius2 : 4 * kHostPi));
+        DevPunctual p{};
+        p.type = uint32_t(l.type);
+        std::memcpy(p.position, l.position, sizeof(p.position));
+        std::memcpy(p.direction, l.direction, sizeof(p.direction));
+        std::memcpy(p.color, l.color, sizeof(p.color));
+        p.intensity = l.intensity;
+        p.range = l.range;
+        p.cos_outer = std::cos(l.outer_cone);
+        p.cos_inner = std::cos(l.inner_cone);
+        sl.punctual.push_back(p);
+    }
+    for (uint32_t i = 0; i < s.tris.size(); ++i) {
+        const HTriangle& t = s.tris[i];
+        const crg_material& m = s.materials[t.material];
+        const double w = double(t.area) * kHostPi * luminance3(m.emission) * (m.double_sided ? 2 : 1);
+        if (w > 0) {
+            sl.emissive.push_back(i);
+            sl.emissive_weight.push_back(w);
+        }
+    }
+    return sl;
+}
+
+std::vector<uint32_t> light_ranks(const SceneLights& sl, const std::vector<uint32_t>& tri_to_prim, size_t prims) {
+    std::vector<uint32_t> rank(prims, kNoTriangle);
+    uint32_t base = 0;
+    for (double w : sl.punctual_weight)
+        if (w > 0) ++base;
+    for (size_t k = 0; k < sl.emissive.size(); ++k) rank[tri_to_prim[sl.emissive[k]]] = base + uint32_t(k);
+    return rank;
+}
+
+LightTable build_light_table(const SceneLights& sl, const std::vector<uint32_t>& tri_to_prim, double env_integral) {
+    struct Entry {
+        uint32_t type, index;
+        double cdf;
+        float probability;
+    };
+    std::vector<Entry> entries;
+    double total = 0;
+    auto add = [&](uint32_t type, uint32_t index, double w) {
+        if (w > 0) {
+            total += w;
+            entries.push_back({type, index, total, 0});
+        }
+    };
+    for (uint32_t i = 0; i < sl.punctual_weight.size(); ++i) add(0, i, sl.punctual_weight[i]);
+    for (size_t k = 0; k < sl.emissive.size(); ++k) add(1, tri_to_prim[sl.emissive[k]], sl.emissive_weight[k]);
+    add(2, 0, env_integral * kHostPi * sl.radius2);
+    LightTable lt;
+    double last = 0, cumulative = 0;
+    for (Entry& e : entries) {
+        const double p = 0.95 * (e.cdf - last) / total + 0.05 / double(entries.size());
+        last = e.cdf;
+        cumulative += p;
+        e.probability = float(p);
+        e.cdf = cumulative;
+        if (e.type == 2) lt.env_probability = e.probability;
+    }
+    if (!entries.empty()) entries.back().cdf = 1;
+    for (const Entry& e : entries) {
+        // lower_bound(cdf >= u) with u = k*2^-24 exactly: chosen iff k <= floor(cdf*2^24).
+        const double scaled = std::floor(e.cdf * 16777216.0);
+        const uint32_t th = scaled >= 16777216.0 ? 16777216u : (scaled < 0 ? 0u : uint32_t(scaled));
+        lt.entries.push_back({e.type, e.index, th, e.probability});
+    }
+    return lt;
+}
+
+// ------------------------------------------------------------------ camera / options / materials
+DevCamera prepare_camera(const crg_camera& c, int width, int height) {
+    const HVec3 pos{c.position[0], c.position[1], c.position[2]}, fw{c.forward[0], c.forward[1], c.forward[2]},
+        rt{c.right[0], c.right[1], c.right[2]}, up{c.up[0], c.up[1], c.up[2]};
+    auto fin_v = [](HVec3 v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z); };
+    if (width <= 0 || height <= 0 || !fin_v(pos) || !fin_v(fw) || !fin_v(rt) || !fin_v(up) || !std::isfinite(c.yfov) ||
+        c.yfov <= 0 || c.yfov >= kHostPi || !std::isfinite(c.aspect) || c.aspect < 0 || !std::isfinite(c.near_clip) ||
+        c.near_clip < 0 || std::isnan(c.far_clip) || c.far_clip <= c.near_clip)
+        throw std::invalid_argument("Invalid perspective camera or image dimensions");
+    const HVec3 f = unit_of(dv(fw));
+    const HVec3 r = orthogonal_of(dv(rt), f);
+    const HVec3 u = orthogonal_of(dv(orthogonal_of(dv(up), f)), r);
+    if (!nonzero3(f) || !nonzero3(r) || !nonzero3(u)) throw std::invalid_argument("Degenerate camera basis");
+    DevCamera cam{};
+    std::memcpy(cam.position, c.position, sizeof(cam.position));
+    cam.f[0] = f.x; cam.f[1] = f.y; cam.f[2] = f.z;
+    cam.r[0] = r.x; cam.r[1] = r.y; cam.r[2] = r.z;
+    cam.u[0] = u.x; cam.u[1] = u.y; cam.u[2] = u.z;
+    const double ratio = c.aspect > 0 ? double(c.aspect) : double(width) / height;
+    const double half = std::tan(double(c.yfov) * 0.5);
+    std::memcpy(&cam.ratio.b, &ratio, 8);
+    std::memcpy(&cam.half_height.b, &half, 8);
+    cam.near_clip = c.near_clip;
+    cam.far_clip = c.far_clip;
+    cam.width = width;
+    cam.height = height;
+    return cam;
+}
+
+void validate_options(const crg_options& o) {
+    if (o.width < 1 || o.height < 1 || o.width > 16384 || o.height > 16384)
+        throw std::runtime_error("Width and height must be in [1, 16384]");
+    const uint64_t pixels = uint64_t(o.width) * uint64_t(o.height);
+    if (pixels > 67108864) throw std::runtime_error("Image exceeds the 64-megapixel allocation limit");
+    if (o.samples < 1 || o.samples > 1048576) throw std::runtime_error("Samples must be in [1, 1048576]");
+    if (o.bounces < 1 || o.bounces > 64) throw std::runtime_error("Bounces must be in [1, 64]");
+    if (o.threads < 0 || o.threads > 256) throw std::runtime_error("Threads must be 0 (automatic) or in [1, 256]");
+}
+
+std::vector<DevMaterial> device_materials(const HostScene& s) {
+    std::vector<DevMaterial> out;
+    for (const crg_material& m : s.materials) {
+        DevMaterial d{};
+        std::memcpy(d.base, m.base_color, sizeof(d.base));
+        d.metallic = m.metallic;
+        d.roughness = m.roughness;
+        std::memcpy(d.emission, m.emission, sizeof(d.emission));
+        d.transmission = m.transmission;
+        d.ior = m.ior;
+        d.thickness = m.thickness;
+        d.att_distance = m.attenuation_distance;
+        std::memcpy(d.att_color, m.attenuation_color, sizeof(d.att_color));
+        if (!std::isfinite(m.attenuation_distance)) {
+            d.sigma[0] = d.sigma[1] = d.sigma[2] = 0;
+        } else {
+            for (int k = 0; k < 3; ++k) {
+                const float c = std::clamp(m.attenuation_color[k], 1e-8f, 1.0f);
+                d.sigma[k] = -std::log(c) / m.attenuation_distance;
+            }
+        }
+        d.double_sided = m.double_sided ? 1u : 0u;
+        out.push_back(d);
+    }
+    return out;
+}
+
+}  // namespace cr

@@ -1,0 +1,96 @@
+// host_scene.h — host-side scene preparation for the device (C++20).
+//
+// Everything here runs once per scene/environment/camera on the CPU, in
+// native binary64 where the reference uses double, so the tables the device
+// consumes are bit-identical to the values the reference computes per call.
+#pragma once
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/crgpu.h"
+#include "layout.h"
+
+namespace cr {
+
+struct HVec3 {
+    float x, y, z;
+};
+
+struct HTriangle {
+    HVec3 p[3];
+    HVec3 n[3];
+    uint32_t material;
+    HVec3 ng;   // geometric normal (finalize)
+    HVec3 ung;  // unit(ng) in binary64 (offsetRay)
+    float area;
+};
+
+struct HostScene {
+    std::vector<HTriangle> tris;
+    std::vector<crg_material> materials;
+    std::vector<crg_light> lights;
+    std::vector<std::string> warnings;
+};
+
+// Validate and repair imported triangles (reference src/geometry.cpp:296-397).
+HostScene finalize_scene(const crg_scene_desc& desc);
+
+struct WideBvh {
+    std::vector<WNode> nodes;
+    std::vector<uint32_t> order;  // prim i -> finalized triangle index
+    uint64_t binary_nodes = 0;    // equivalent binary node count (2*leaves-1)
+};
+
+// Binned SAH (16 bins, leaves <= 4, depth <= 48) emitted as 4-wide nodes.
+WideBvh build_wide_bvh(const HostScene& s);
+
+// Environment-independent part of the light mixture of one scene.
+struct SceneLights {
+    double radius2 = 1;
+    std::vector<double> punctual_weight;  // per punctual light
+    std::vector<uint32_t> emissive;       // finalized triangle indices with weight > 0
+    std::vector<double> emissive_weight;
+    std::vector<DevPunctual> punctual;
+};
+
+struct LightTable {
+    std::vector<DevLightEntry> entries;
+    float env_probability = 0;
+};
+
+struct HostEnv {
+    bool is_image = false;
+    float constant[3] = {0, 0, 0};
+    int width = 0, height = 0;
+    float rotation = 0, intensity = 1;
+    double integral = 0, weight_sum = 0;
+    std::vector<float> texels;  // rgba
+    std::vector<float> pdf;
+    std::vector<uint32_t> threshold;
+    std::vector<uint32_t> guide;
+    std::vector<float> rowcos;
+};
+
+// Environment construction (reference src/render.cpp:12-42, :67-94).
+HostEnv build_env(const crg_env_desc& d);
+
+// Lights mixture (reference src/render.cpp:131-160): weights per scene, then the
+// normalized table for a given environment integral.  tri_to_prim maps a
+// finalized triangle index to its device primitive.
+SceneLights scene_lights(const HostScene& s);
+LightTable build_light_table(const SceneLights& sl, const std::vector<uint32_t>& tri_to_prim, double env_integral);
+// Per-prim index of its entry in any table built from `sl` (kNoTriangle if not emissive).
+std::vector<uint32_t> light_ranks(const SceneLights& sl, const std::vector<uint32_t>& tri_to_prim, size_t prims);
+
+// Camera basis and raster constants (reference src/geometry.cpp:213-227).
+DevCamera prepare_camera(const crg_camera& c, int width, int height);
+
+// RenderOptions::validate (reference src/render.cpp:97-104).
+void validate_options(const crg_options& o);
+
+// Material table with precomputed absorption (reference src/bsdf.cpp:63-68).
+std::vector<DevMaterial> device_materials(const HostScene& s);
+
+}  // namespace cr

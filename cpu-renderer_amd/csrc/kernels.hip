@@ -1,0 +1,534 @@
+// kernels.hip — gfx950 wavefront path-tracing kernels.
+//
+// One render (or a batch of renders) is a pool of path slots processed in
+// iterations; each iteration is
+//   extend  : closest-hit traversal of every queued ray          (Bvh::intersect)
+//   shade   : hit -> surface, emission/MIS, NEE light sample, BSDF sample,
+//             media, Russian roulette; emits shadow rays and next rays
+//   shadow  : any-hit visibility of the NEE rays, adds their contribution
+//   retire  : finished paths write their radiance into the job's sample
+//             buffer and the slot immediately takes the next work item
+// All per-path state lives in HBM as structure-of-arrays, so the result of a
+// path never depends on which slot, wave or iteration ran it; the film is
+// summed per pixel in sample order afterwards (accumulate), exactly like the
+// reference's per-pixel double accumulation (reference src/render.cpp:384-415).
+#include <hip/hip_runtime.h>
+
+#include "geom.h"
+#include "kernels.h"
+#include "shade.h"
+
+namespace cr {
+
+// ------------------------------------------------------------------ helpers
+__device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    if (!mask) return 0xffffffffu;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+    base = __shfl(base, leader);
+    return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+}
+
+__device__ inline unsigned long long wave_claim(unsigned long long* counter, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    if (!mask) return ~0ull;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+__device__ inline void report_error(Control* ctl, uint32_t job, uint64_t key, uint32_t code, uint32_t depth,
+                                    uint32_t tri, uint32_t material, sf::f64 t, V3 point, V3 origin, V3 dir) {
+    const uint32_t slot = atomicAdd(&ctl->err_count, 1u);
+    if (slot >= uint32_t(kMaxErrors)) return;
+    ErrorRec& e = ctl->errors[slot];
+    e.key = key;
+    e.job = job;
+    e.code = code;
+    e.depth = depth;
+    e.triangle = tri;
+    e.material = material;
+    e.tbits = t.b;
+    e.point[0] = point.x; e.point[1] = point.y; e.point[2] = point.z;
+    e.origin[0] = origin.x; e.origin[1] = origin.y; e.origin[2] = origin.z;
+    e.direction[0] = dir.x; e.direction[1] = dir.y; e.direction[2] = dir.z;
+}
+
+__device__ inline V3 xyz(float4 v) { return V3{v.x, v.y, v.z}; }
+
+// ------------------------------------------------------------------ ray queries
+// Bvh::intersect over an array of rays (parity harness).  One lane per ray.
+__global__ void __launch_bounds__(256) k_trace_rays(const DevScene* scene, const RayIn* rays, uint64_t n,
+                                                    HitOut* hits, int any, Control* ctl) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DevScene& s = *scene;
+    const RayIn r = rays[i];
+    RayQuery q;
+    setup_query(q, V3{r.o[0], r.o[1], r.o[2]}, V3{r.d[0], r.d[1], r.d[2]}, r.tmin, r.tmax,
+                V3{r.lo[0], r.lo[1], r.lo[2]}, r.ignore);
+    uint32_t fb = 0;
+    ClosestResult best = any ? traverse<true>(s, q, fb) : traverse<false>(s, q, fb);
+    HitOut h;
+    h.hit = best.prim != kNoTriangle ? 1u : 0u;
+    h.triangle = h.hit ? best.id : kNoTriangle;
+    h.t = h.u = h.v = 0;
+    if (best.overflow) h.hit = 2u;
+    if (h.hit == 1u) {
+        if (!best.exact) {
+            const ExactHit e = exact_of(s, q, best.prim);
+            best.t = e.t;
+            best.u = e.u;
+            best.v = e.v;
+        }
+        h.t = best.t.b;
+        h.u = best.u.b;
+        h.v = best.v.b;
+    }
+    hits[i] = h;
+    if (fb) atomicAdd(&ctl->fallbacks, (unsigned long long)fb);
+}
+
+// ------------------------------------------------------------------ path state
+__device__ inline uint32_t find_job(const DevJob* jobs, uint32_t njobs, uint64_t item) {
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].item_base <= item) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Camera sample -> path slot (reference src/render.cpp:396-401).  Returns false
+// when the camera cannot form the ray (reported as an error).
+__device__ inline bool init_path(const PathSoA& P, uint32_t slot, const DevJob& J, uint32_t job_index,
+                                 uint64_t local, uint32_t precise, Control* ctl) {
+    const uint64_t npix = uint64_t(J.width) * uint64_t(J.height);
+    const uint32_t sample = uint32_t(local / npix);
+    const uint32_t pixel = uint32_t(local - uint64_t(sample) * npix);
+    const int x = int(pixel % uint32_t(J.width)), y = int(pixel / uint32_t(J.width));
+    Rng rng{sample_seed(J.seed, pixel, sample)};
+    const float dx = rng.uniform(), dy = rng.uniform();
+    V3 o, d;
+    float tmin, tmax;
+    const bool ok = camera_ray(J.cam, float(x) + dx, float(y) + dy, o, d, tmin, tmax);
+    P.job[slot] = job_index;
+    P.pixel[slot] = pixel;
+    P.sample[slot] = sample;
+    P.rng[slot] = rng.state;
+    P.ray_o[slot] = make_float4(o.x, o.y, o.z, tmin);
+    P.ray_d[slot] = make_float4(d.x, d.y, d.z, tmax);
+    P.ray_lo[slot] = make_float4(0, 0, 0, 0);
+    P.thr[slot] = make_float4(1, 1, 1, 1);
+    P.rad[slot] = make_float4(0, 0, 0, 0);
+    P.lastpos[slot] = make_float4(0, 0, 0, 0);
+    P.absorb[slot] = make_float4(0, 0, 0, 0);
+    P.medium[slot] = -1;
+    P.flags[slot] = kLastDelta | (precise ? kPrecise : 0u);
+    P.prev[slot] = kNoTriangle;
+    P.hit[slot] = kNoTriangle;
+    if (!ok) {
+        report_error(ctl, job_index, uint64_t(pixel) * uint64_t(J.spp) + sample, kErrCamera, 0, 0, 0, sf::zero(0), V3{0, 0, 0},
+                     V3{0, 0, 0}, V3{0, 0, 0});
+    }
+    return ok;
+}
+
+__global__ void __launch_bounds__(256) k_seed(PathSoA P, const DevJob* jobs, uint32_t njobs, uint32_t pool,
+                                              uint32_t* q0, Control* ctl) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    bool queued = false;
+    if (slot < pool && uint64_t(slot) < ctl->total) {
+        const uint32_t j = find_job(jobs, njobs, slot);
+        queued = init_path(P, slot, jobs[j], j, uint64_t(slot) - jobs[j].item_base, 0, ctl);
+        if (!queued) {
+            // Nothing to trace: park it with zero radiance for retirement.
+            queued = false;
+        }
+    }
+    const uint32_t pos = wave_append(&ctl->qcount[0], queued);
+    if (queued) q0[pos] = slot;
+}
+
+// ------------------------------------------------------------------ extend
+__global__ void __launch_bounds__(256) k_extend(PathSoA P, const DevJob* jobs, const uint32_t* queue, int parity,
+                                                Control* ctl) {
+    const uint32_t n = ctl->qcount[parity];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t fb = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t path = queue[i];
+        const DevScene& s = *jobs[P.job[path]].scene;
+        const float4 ro = P.ray_o[path], rd = P.ray_d[path], rl = P.ray_lo[path];
+        RayQuery q;
+        setup_query(q, xyz(ro), xyz(rd), ro.w, rd.w, xyz(rl), P.prev[path]);
+        const ClosestResult best = traverse<false>(s, q, fb);
+        P.hit[path] = best.overflow ? kNoTriangle - 1 : best.prim;
+    }
+    if (fb) atomicAdd(&ctl->fallbacks, (unsigned long long)fb);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->extend_rays, (unsigned long long)n);
+}
+
+// ------------------------------------------------------------------ shade
+enum Action : uint32_t { kContinue = 0, kRetire = 1, kReplay = 2 };
+
+struct ShadowOut {
+    bool emit;
+    V3 o, d, lo, c;
+    float tmax;
+    uint32_t ignore;
+};
+
+// One bounce of the reference's trace() loop (reference src/render.cpp:240-344)
+// for the path in `slot`, whose closest hit is already in P.hit.
+__device__ inline uint32_t shade_path(const PathSoA& P, const DevJob* jobs, uint32_t slot, ShadowOut& sh,
+                                      Control* ctl) {
+    sh.emit = false;
+    const uint32_t ji = P.job[slot];
+    const DevJob& J = jobs[ji];
+    const DevScene& sc = *J.scene;
+    const DevEnv& env = *J.env;
+    const float4 ro = P.ray_o[slot], rd = P.ray_d[slot], rl = P.ray_lo[slot];
+    const V3 ray_o = xyz(ro), ray_d = xyz(rd), ray_lo = xyz(rl);
+    float4 thr4 = P.thr[slot], rad4 = P.rad[slot];
+    V3 T = xyz(thr4), L = xyz(rad4);
+    float eta_scale = thr4.w, last_pdf = rad4.w;
+    uint32_t flags = P.flags[slot];
+    const uint32_t depth = flags & kDepthMask;
+    const bool last_delta = (flags & kLastDelta) != 0, precise = (flags & kPrecise) != 0;
+    int32_t medium = P.medium[slot];
+    V3 sigma = xyz(P.absorb[slot]);
+    const V3 last_pos = xyz(P.lastpos[slot]);
+    const uint32_t prim = P.hit[slot];
+    const bool nee = J.nee != 0;
+    const uint64_t key = uint64_t(P.pixel[slot]) * uint64_t(J.spp) + P.sample[slot];
+
+    if (prim == kNoTriangle - 1) {
+        report_error(ctl, ji, key, kErrStack, depth, 0, 0, sf::zero(0), ray_o, ray_o, ray_d);
+        P.rad[slot] = make_float4(0, 0, 0, last_pdf);
+        return kRetire;
+    }
+    if (prim == kNoTriangle) {
+        if (medium >= 0) T = T * beer_lambert(sigma, kInf);
+        const float w = last_delta || !nee ? 1 : power_heuristic(last_pdf, environment_pdf(J.lights, env, ray_d));
+        L = L + T * env_radiance(env, ray_d) * w;
+        P.rad[slot] = make_float4(L.x, L.y, L.z, last_pdf);
+        return kRetire;
+    }
+    RayQuery q;
+    setup_query(q, ray_o, ray_d, ro.w, rd.w, ray_lo, P.prev[slot]);
+    const ExactHit hit = exact_of(sc, q, prim);
+    const uint32_t tri_id = sc.prims[prim].id;
+    if (medium >= 0) T = T * beer_lambert(sigma, sf::to_float(hit.t));
+    const Surf s = surface_at(sc, prim, ray_d, hit.u, hit.v);
+    if ((s.front || s.double_sided) && max_component(s.emission) > 0) {
+        const float w = last_delta || !nee
+                            ? 1
+                            : power_heuristic(last_pdf, triangle_pdf(sc, J.lights, prim, last_pos, s.position));
+        L = L + T * s.emission * w;
+    }
+    if (int(depth) == J.bounces) {
+        P.rad[slot] = make_float4(L.x, L.y, L.z, last_pdf);
+        return kRetire;
+    }
+    const V3 wo = -ray_d;
+    Rng rng{P.rng[slot]};
+    if (nee) {
+        const LightSample light = sample_light(sc, J.lights, env, s.position, rng);
+        const bool enclosed_infinite = medium >= 0 && s.thickness <= 0 && !finite1(light.distance);
+        if (!enclosed_infinite && light.pdf > 0 && max_component(light.radiance) > 0) {
+            const BsdfEval bsdf = evaluate_bsdf(s, wo, light.direction);
+            if (max_component(bsdf.value) > 0) {
+                V3 so, sd, slo{0, 0, 0};
+                float stmax = kInf;
+                bool ok = true;
+                if (precise) {
+                    so = s.position;
+                    sd = light.direction;
+                    slo = s.position_low;
+                } else {
+                    const PrimAux& ax = sc.aux[prim];
+                    const uint32_t err = offset_ray(s.position, s.ng, V3{ax.ung[0], ax.ung[1], ax.ung[2]},
+                                                    light.direction, kInf, so, sd, stmax);
+                    if (err) {
+                        report_error(ctl, ji, key, err, depth, tri_id, s.material, sf::zero(0), s.position, ray_o, ray_d);
+                        ok = false;
+                    }
+                }
+                if (ok && finite1(light.distance)) {
+                    V3 end = light.target;
+                    if (length_sq(light.target_normal) > 0) {
+                        V3 eo, ed;
+                        float et;
+                        const uint32_t err =
+                            offset_ray(end, light.target_normal, light.target_unormal, -light.direction, kInf, eo, ed, et);
+                        if (err) {
+                            report_error(ctl, ji, key, err, depth, tri_id, s.material, sf::zero(0), end, ray_o, ray_d);
+                            ok = false;
+                        }
+                        end = eo;
+                    }
+                    const V3 delta = end - so;
+                    stmax = length(delta) * (1 - 2e-6f);
+                    sd = normalize(delta);
+                }
+                if (ok) {
+                    const float w = light.delta ? 1 : power_heuristic(light.pdf, bsdf.pdf);
+                    V3 shadow_sigma = sigma;
+                    bool shadow_in_medium = medium >= 0;
+                    if (s.thickness > 0 && dot(wo, s.ng) * dot(light.direction, s.ng) < 0) {
+                        shadow_in_medium = s.front;
+                        const DevMaterial& m = sc.materials[s.material];
+                        shadow_sigma = s.front ? V3{m.sigma[0], m.sigma[1], m.sigma[2]} : V3{0, 0, 0};
+                    }
+                    const V3 tr = shadow_in_medium ? beer_lambert(shadow_sigma, light.distance) : v3(1);
+                    sh.emit = true;
+                    sh.o = so;
+                    sh.d = sd;
+                    sh.lo = slo;
+                    sh.tmax = stmax;
+                    sh.ignore = precise ? tri_id : kNoTriangle;
+                    sh.c = T * bsdf.value * light.radiance * tr * (fabsf(dot(s.normal, light.direction)) * w / light.pdf);
+                }
+            }
+        }
+    }
+    const BsdfSample smp = sample_bsdf(s, wo, rng);
+    if (!sample_ok(smp)) {
+        P.rad[slot] = make_float4(L.x, L.y, L.z, last_pdf);
+        return kRetire;
+    }
+    T = T * smp.weight;
+    if (!finite(T)) {
+        report_error(ctl, ji, key, kErrNonFiniteThroughput, depth, tri_id, s.material, sf::zero(0), s.position, ray_o, ray_d);
+        sh.emit = false;
+        P.rad[slot] = make_float4(0, 0, 0, last_pdf);
+        return kRetire;
+    }
+    if (smp.transmitted && s.thickness > 0) {
+        uint32_t boundary = 0;
+        if (s.front) {
+            if (medium >= 0) boundary = kErrNested;
+            else {
+                medium = int32_t(s.material);
+                const DevMaterial& m = sc.materials[s.material];
+                sigma = V3{m.sigma[0], m.sigma[1], m.sigma[2]};
+            }
+        } else {
+            if (medium < 0) boundary = kErrExitWithoutEntry;
+            else {
+                medium = -1;
+                sigma = V3{0, 0, 0};
+            }
+        }
+        if (boundary) {
+            sh.emit = false;
+            if (!precise) return kReplay;  // precision replay of the whole sample
+            report_error(ctl, ji, key, boundary, depth, tri_id, s.material, hit.t, s.position, ray_o,
+                         ray_d);
+            P.rad[slot] = make_float4(0, 0, 0, last_pdf);
+            return kRetire;
+        }
+        eta_scale *= smp.eta * smp.eta;
+    }
+    V3 no, nd, nlo{0, 0, 0};
+    float ntmax = kInf;
+    if (precise) {
+        no = s.position;
+        nd = smp.direction;
+        nlo = s.position_low;
+    } else {
+        const PrimAux& ax = sc.aux[prim];
+        const uint32_t err =
+            offset_ray(s.position, s.ng, V3{ax.ung[0], ax.ung[1], ax.ung[2]}, smp.direction, kInf, no, nd, ntmax);
+        if (err) {
+            report_error(ctl, ji, key, err, depth, tri_id, s.material, sf::zero(0), s.position, ray_o, ray_d);
+            sh.emit = false;
+            P.rad[slot] = make_float4(0, 0, 0, last_pdf);
+            return kRetire;
+        }
+    }
+    uint32_t action = kContinue;
+    if (J.rr && depth >= 3) {
+        const float survive = clampf(max_component(T) * eta_scale, 0.05f, 0.95f);
+        if (rng.uniform() >= survive) action = kRetire;
+        else T = T * (1 / survive);
+    }
+    P.rng[slot] = rng.state;
+    P.thr[slot] = make_float4(T.x, T.y, T.z, eta_scale);
+    P.rad[slot] = make_float4(L.x, L.y, L.z, smp.pdf);
+    P.lastpos[slot] = make_float4(s.position.x, s.position.y, s.position.z, 0);
+    P.absorb[slot] = make_float4(sigma.x, sigma.y, sigma.z, 0);
+    P.medium[slot] = medium;
+    P.ray_o[slot] = make_float4(no.x, no.y, no.z, 0.0f);
+    P.ray_d[slot] = make_float4(nd.x, nd.y, nd.z, ntmax);
+    P.ray_lo[slot] = make_float4(nlo.x, nlo.y, nlo.z, 0);
+    P.prev[slot] = precise ? tri_id : kNoTriangle;
+    P.flags[slot] = (depth + 1) | (smp.delta ? kLastDelta : 0u) | (precise ? kPrecise : 0u);
+    return action;
+}
+
+__global__ void __launch_bounds__(256) k_shade(PathSoA P, ShadowSoA S, const DevJob* jobs, const uint32_t* queue,
+                                               uint32_t* next_queue, uint32_t* retire, int parity, Control* ctl) {
+    const uint32_t n = ctl->qcount[parity];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t rounds = (n + stride - 1) / stride;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t r = 0; r < rounds; ++r, i += stride) {
+        const bool live = i < n;
+        uint32_t path = 0, action = kRetire;
+        ShadowOut sh;
+        sh.emit = false;
+        if (live) {
+            path = queue[i];
+            action = shade_path(P, jobs, path, sh, ctl);
+            if (action == kReplay) {
+                const uint32_t ji = P.job[path];
+                const DevJob& J = jobs[ji];
+                const uint64_t npix = uint64_t(J.width) * uint64_t(J.height);
+                const uint64_t local = uint64_t(P.sample[path]) * npix + P.pixel[path];
+                atomicAdd(&ctl->replays, 1ull);
+                init_path(P, path, J, ji, local, 1u, ctl);
+                action = kContinue;
+            }
+        }
+        const uint32_t spos = wave_append(&ctl->scount, live && sh.emit);
+        if (live && sh.emit) {
+            S.path[spos] = path;
+            S.o[spos] = make_float4(sh.o.x, sh.o.y, sh.o.z, 0.0f);
+            S.d[spos] = make_float4(sh.d.x, sh.d.y, sh.d.z, sh.tmax);
+            S.lo[spos] = make_float4(sh.lo.x, sh.lo.y, sh.lo.z, __uint_as_float(sh.ignore));
+            S.c[spos] = make_float4(sh.c.x, sh.c.y, sh.c.z, 0);
+        }
+        const uint32_t qpos = wave_append(&ctl->qcount[parity ^ 1], live && action == kContinue);
+        if (live && action == kContinue) next_queue[qpos] = path;
+        const uint32_t rpos = wave_append(&ctl->rcount, live && action == kRetire);
+        if (live && action == kRetire) retire[rpos] = path;
+    }
+}
+
+// ------------------------------------------------------------------ shadow
+__global__ void __launch_bounds__(256) k_shadow(PathSoA P, ShadowSoA S, const DevJob* jobs, Control* ctl) {
+    const uint32_t n = ctl->scount;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t fb = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t path = S.path[i];
+        const DevScene& s = *jobs[P.job[path]].scene;
+        const float4 so = S.o[i], sd = S.d[i], sl = S.lo[i];
+        RayQuery q;
+        setup_query(q, xyz(so), xyz(sd), so.w, sd.w, xyz(sl), __float_as_uint(sl.w));
+        const ClosestResult b = traverse<true>(s, q, fb);
+        if (b.prim == kNoTriangle && !b.overflow) {
+            const float4 c = S.c[i];
+            float4 r = P.rad[path];
+            r.x = r.x + c.x;
+            r.y = r.y + c.y;
+            r.z = r.z + c.z;
+            P.rad[path] = r;
+        }
+    }
+    if (fb) atomicAdd(&ctl->fallbacks, (unsigned long long)fb);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->shadow_rays, (unsigned long long)n);
+}
+
+// ------------------------------------------------------------------ retire + regenerate
+__global__ void __launch_bounds__(256) k_retire(PathSoA P, const DevJob* jobs, uint32_t njobs, const uint32_t* retire,
+                                                uint32_t* next_queue, int parity, Control* ctl) {
+    const uint32_t n = ctl->rcount;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t rounds = (n + stride - 1) / stride;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t r = 0; r < rounds; ++r, i += stride) {
+        const bool live = i < n;
+        uint32_t path = 0;
+        if (live) {
+            path = retire[i];
+            const uint32_t ji = P.job[path];
+            const DevJob& J = jobs[ji];
+            const float4 L = P.rad[path];
+            const V3 l = xyz(L);
+            const uint64_t npix = uint64_t(J.width) * uint64_t(J.height);
+            if (!finite(l) || min_component(l) < 0) {
+                report_error(ctl, ji, uint64_t(P.pixel[path]) * uint64_t(J.spp) + P.sample[path], kErrBadRadiance,
+                             0, 0, 0, sf::zero(0), V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0});
+            }
+            J.samples[uint64_t(P.sample[path]) * npix + P.pixel[path]] = make_float4(l.x, l.y, l.z, 0);
+        }
+        const unsigned long long item = wave_claim(&ctl->issued, live);
+        bool queued = false;
+        if (live && item < ctl->total) {
+            const uint32_t j = find_job(jobs, njobs, item);
+            queued = init_path(P, path, jobs[j], j, item - jobs[j].item_base, 0, ctl);
+        }
+        if (live) atomicAdd(&ctl->retired, 1ull);
+        const uint32_t qpos = wave_append(&ctl->qcount[parity ^ 1], queued);
+        if (queued) next_queue[qpos] = path;
+    }
+}
+
+__global__ void k_reset(int parity, Control* ctl) {
+    ctl->qcount[parity] = 0;
+    ctl->scount = 0;
+    ctl->rcount = 0;
+}
+
+// ------------------------------------------------------------------ accumulate
+// film = float(sum_s L_s / spp) with the sum in binary64, samples in order.
+__global__ void __launch_bounds__(256) k_accumulate(const DevJob* jobs, uint32_t job, float* film) {
+    const DevJob& J = jobs[job];
+    const uint64_t npix = uint64_t(J.width) * uint64_t(J.height);
+    const uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    sf::f64 r = sf::zero(0), g = sf::zero(0), b = sf::zero(0);
+    for (int s = 0; s < J.spp; ++s) {
+        const float4 v = J.samples[uint64_t(s) * npix + p];
+        r = sf::add(r, sf::from_float(v.x));
+        g = sf::add(g, sf::from_float(v.y));
+        b = sf::add(b, sf::from_float(v.z));
+    }
+    const sf::f64 n = sf::from_i32(J.spp);
+    film[3 * p + 0] = sf::to_float(sf::div(r, n));
+    film[3 * p + 1] = sf::to_float(sf::div(g, n));
+    film[3 * p + 2] = sf::to_float(sf::div(b, n));
+}
+
+// ------------------------------------------------------------------ launchers
+static inline unsigned blocks_for(uint64_t n, unsigned bs) { return unsigned((n + bs - 1) / bs); }
+
+hipError_t launch_trace_rays(const DevScene* scene, const RayIn* rays, uint64_t n, HitOut* hits, int any, Control* ctl,
+                             hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_trace_rays, dim3(blocks_for(n, 256)), dim3(256), 0, st, scene, rays, n, hits, any, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_seed(const PathSoA& P, const DevJob* jobs, uint32_t njobs, uint32_t pool, uint32_t* q0, Control* ctl,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(k_seed, dim3(blocks_for(pool, 256)), dim3(256), 0, st, P, jobs, njobs, pool, q0, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_iteration(const PathSoA& P, const ShadowSoA& S, const DevJob* jobs, uint32_t njobs, uint32_t* const q[2],
+                            uint32_t* retire, int parity, Control* ctl, unsigned grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_extend, dim3(grid), dim3(256), 0, st, P, jobs, q[parity], parity, ctl);
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, st, P, S, jobs, q[parity], q[parity ^ 1], retire, parity, ctl);
+    hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(256), 0, st, P, S, jobs, ctl);
+    hipLaunchKernelGGL(k_retire, dim3(grid), dim3(256), 0, st, P, jobs, njobs, retire, q[parity ^ 1], parity, ctl);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, parity, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_accumulate(const DevJob* jobs, uint32_t job, uint64_t npix, float* film, hipStream_t st) {
+    hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(npix, 256)), dim3(256), 0, st, jobs, job, film);
+    return hipGetLastError();
+}
+
+}  // namespace cr

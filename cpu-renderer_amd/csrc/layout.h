@@ -1,0 +1,206 @@
+// layout.h — device-resident data layout of the wavefront path tracer.
+//
+// Scene (per resident scene, all in HBM):
+//   WNode   128-byte 4-wide BVH node: child boxes as SoA lanes + child refs.
+//   PrimRec 48-byte traversal record per triangle in leaf order (v0,id | v1,flags | v2,material).
+//   PrimAux 64-byte shading record per triangle in leaf order (vertex normals,
+//           geometric normal, unit geometric normal, area); fetched only by shade.
+// Environment: texels + per-texel float pdf + integer inversion thresholds.
+// Path state: structure-of-arrays, one slot per in-flight camera sample.
+#pragma once
+#include <stdint.h>
+
+#include "vmath.h"
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#include <hip/hip_runtime.h>
+#endif
+
+namespace cr {
+
+// Child reference: bit 31 clear -> inner node index; bit 31 set -> leaf with
+// count-1 in bits 27..30 (1..16 primitives) and the first primitive in bits 0..26.
+constexpr uint32_t kEmptyChild = 0x7fffffffu;
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kLeafMaxCount = 16;
+constexpr uint32_t kLeafMaxFirst = (1u << 27) - 1;
+CR_HDI bool ref_is_leaf(uint32_t r) { return (r & kLeafBit) != 0; }
+CR_HDI uint32_t leaf_first(uint32_t r) { return r & kLeafMaxFirst; }
+CR_HDI uint32_t leaf_count(uint32_t r) { return ((r >> 27) & 15u) + 1; }
+CR_HDI uint32_t make_leaf(uint32_t first, uint32_t count) { return kLeafBit | ((count - 1) << 27) | first; }
+
+struct alignas(128) WNode {
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    uint32_t child[4];
+    uint32_t pad[4];
+};
+static_assert(sizeof(WNode) == 128, "WNode must be one 128-byte line");
+
+enum PrimFlags : uint32_t {
+    kPrimCull = 1u,        // single-sided opaque: reject when dot(Ng, d) >= 0
+    kPrimNeverHit = 2u,    // MASK material whose constant alpha is below its cutoff
+    kPrimMaskTexture = 4u  // MASK material with a base-colour texture (unsupported on device)
+};
+
+struct alignas(16) PrimRec {
+    float ax, ay, az;
+    uint32_t id;  // triangle index in the finalized scene (tie-break + output)
+    float bx, by, bz;
+    uint32_t flags;
+    float cx, cy, cz;
+    uint32_t material;
+};
+static_assert(sizeof(PrimRec) == 48, "PrimRec is three 16-byte vectors");
+
+struct alignas(16) PrimAux {
+    float n0[3], n1[3], n2[3];
+    float ng[3];   // geometric normal as stored by finalize
+    float ung[3];  // unit(ng) evaluated in binary64 (offsetRay's normalization)
+    float area;
+};
+static_assert(sizeof(PrimAux) == 64, "PrimAux is one 64-byte record");
+
+struct DevMaterial {
+    float base[4];
+    float metallic, roughness;
+    float emission[3];
+    float transmission, ior, thickness, att_distance;
+    float att_color[3];
+    float sigma[3];  // absorptionCoefficient(att_color, att_distance), evaluated on the host
+    uint32_t double_sided;
+    uint32_t pad;
+};
+
+struct DevLightEntry {
+    uint32_t type;       // 0 punctual, 1 triangle, 2 environment
+    uint32_t index;      // punctual index or prim index
+    uint32_t threshold;  // entry chosen when 24-bit draw k <= threshold
+    float probability;
+};
+
+struct DevPunctual {
+    uint32_t type;
+    float position[3], direction[3], color[3];
+    float intensity, range, cos_outer, cos_inner;
+};
+
+struct DevScene {
+    const WNode* nodes;
+    const PrimRec* prims;
+    const float4* prim_ng;        // geometric normal per prim (cull test)
+    const PrimAux* aux;
+    const DevMaterial* materials;
+    const uint32_t* light_rank;   // per prim: entry index in the light table, or kNoTriangle
+    const DevPunctual* punctual;
+    uint32_t node_count, prim_count, punctual_count, pad;
+};
+
+// Light-selection mixture of one (scene, environment) pair.
+struct DevLights {
+    const DevLightEntry* entries;
+    uint32_t count;
+    float env_probability;
+};
+
+struct DevEnv {
+    const float4* texels;       // rgb, 1
+    const float* pdf;           // per-texel solid-angle pdf (binary64 on host, rounded)
+    const uint32_t* threshold;  // T_i: first 24-bit draw whose CDF pick reaches texel i
+    const uint32_t* guide;      // 65537 entries: texel index at k = j << 8
+    const float* rowcos;        // cosf(pi*y/H), y = 0..H
+    int32_t width, height;
+    float rotation, intensity;
+    float constant[3];
+    int32_t is_image;
+};
+
+struct DevCamera {
+    float position[3], f[3], r[3], u[3];
+    sf::f64 ratio, half_height;
+    float near_clip, far_clip;
+    int32_t width, height;
+};
+
+// One render request: scene/environment/camera/options plus its sample buffer.
+struct DevJob {
+    const DevScene* scene;
+    const DevEnv* env;
+    DevLights lights;
+    DevCamera cam;
+    int32_t width, height, spp, bounces;
+    uint64_t seed;
+    int32_t nee, rr;
+    float4* samples;     // [sample][pixel] radiance (xyz)
+    uint64_t item_base;  // first global work item of this job
+    uint64_t items;      // width*height*spp
+};
+
+enum PathFlag : uint32_t {
+    kDepthMask = 0xffu,
+    kLastDelta = 1u << 8,
+    kPrecise = 1u << 9,
+};
+
+struct PathSoA {
+    float4* ray_o;    // origin, tmin
+    float4* ray_d;    // direction, tmax
+    float4* ray_lo;   // originLow, (unused)
+    float4* thr;      // throughput, etaScale
+    float4* rad;      // radiance, lastPdf
+    float4* lastpos;  // lastPosition
+    float4* absorb;   // active medium absorption
+    uint64_t* rng;
+    uint32_t* job;
+    uint32_t* pixel;
+    uint32_t* sample;
+    int32_t* medium;
+    uint32_t* flags;
+    uint32_t* prev;  // departed triangle id (precise replay) or kNoTriangle
+    uint32_t* hit;   // hit prim or kNoTriangle
+};
+
+struct ShadowSoA {
+    uint32_t* path;
+    float4* o;   // origin, tmin
+    float4* d;   // direction, tmax
+    float4* lo;  // originLow, ignore (as bits)
+    float4* c;   // contribution
+};
+
+struct ErrorRec {
+    uint64_t key;    // pixel*spp + sample (smallest reported)
+    uint64_t tbits;  // binary64 hit distance
+    uint32_t job, code, depth, triangle, material, pad;
+    float point[3], origin[3], direction[3], pad2;
+};
+
+enum ErrorCode : uint32_t {
+    kErrNonFiniteThroughput = 1,
+    kErrBadRadiance = 2,
+    kErrNested = 3,
+    kErrExitWithoutEntry = 4,
+    kErrInvalidSecondary = 5,
+    kErrOffsetOverflow = 6,
+    kErrCamera = 7,
+    kErrStack = 8,
+};
+
+constexpr int kMaxErrors = 64;
+
+struct Control {
+    uint32_t qcount[2];
+    uint32_t scount, rcount;
+    uint32_t fetch[2];
+    uint32_t err_count;
+    uint32_t pad0;
+    unsigned long long issued;
+    unsigned long long total;
+    unsigned long long retired;
+    unsigned long long replays;
+    unsigned long long extend_rays;
+    unsigned long long shadow_rays;
+    unsigned long long fallbacks;
+    ErrorRec errors[kMaxErrors];
+};
+
+}  // namespace cr

@@ -1,0 +1,343 @@
+// sf64.h — exact IEEE-754 binary64 arithmetic from 64-bit integer operations.
+//
+// The reference renderer evaluates its geometric predicates in double
+// (reference src/geometry.cpp:147-182 intersectTriangle, :186-209 offsetRay,
+// :213-234 Camera::generate, :401-459 surfaceAt; src/math.h:124-128
+// powerHeuristic; src/render.cpp:390-415 film accumulation).  The device path
+// executes no FP64 instructions; wherever a result must equal the reference's
+// double result bit for bit it uses this emulation instead.  Every operation
+// is correctly rounded (round-to-nearest-even) including subnormal results,
+// signed zeros, infinities and NaN, so `sf64 a*b` equals `double a*b` for all
+// inputs.  The same source compiles for the host, where tests/test_sf64*.py
+// checks it against native double on random and edge-case operands.
+#pragma once
+#include <stdint.h>
+
+#ifndef CR_HD
+#if defined(__HIPCC__) || defined(__HIP__)
+#define CR_HD __host__ __device__
+#define CR_HDI __host__ __device__ inline
+#define CR_NOINLINE __noinline__
+#else
+#define CR_HD
+#define CR_HDI inline
+#define CR_NOINLINE __attribute__((noinline))
+#endif
+#endif
+
+namespace sf {
+
+struct f64 {
+    uint64_t b;
+};
+
+CR_HDI f64 bits(uint64_t b) { return f64{b}; }
+CR_HDI uint32_t sign_of(f64 a) { return uint32_t(a.b >> 63); }
+CR_HDI int32_t exp_of(f64 a) { return int32_t((a.b >> 52) & 0x7ff); }
+CR_HDI uint64_t frac_of(f64 a) { return a.b & 0x000fffffffffffffull; }
+CR_HDI bool is_nan(f64 a) { return exp_of(a) == 0x7ff && frac_of(a) != 0; }
+CR_HDI bool is_inf(f64 a) { return (a.b & 0x7fffffffffffffffull) == 0x7ff0000000000000ull; }
+CR_HDI bool is_finite(f64 a) { return exp_of(a) != 0x7ff; }
+CR_HDI bool is_zero(f64 a) { return (a.b & 0x7fffffffffffffffull) == 0; }
+CR_HDI f64 neg(f64 a) { return f64{a.b ^ 0x8000000000000000ull}; }
+CR_HDI f64 fabs_(f64 a) { return f64{a.b & 0x7fffffffffffffffull}; }
+CR_HDI f64 qnan() { return f64{0x7ff8000000000000ull}; }
+CR_HDI f64 inf(uint32_t s) { return f64{(uint64_t(s) << 63) | 0x7ff0000000000000ull}; }
+CR_HDI f64 zero(uint32_t s) { return f64{uint64_t(s) << 63}; }
+
+CR_HDI int clz64(uint64_t x) { return x ? __builtin_clzll(x) : 64; }
+
+CR_HDI uint64_t shift_right_jam(uint64_t x, int n) {
+    if (n <= 0) return x;
+    if (n >= 64) return x != 0;
+    return (x >> n) | uint64_t((x << (64 - n)) != 0);
+}
+
+CR_HDI uint64_t mul_hi(uint64_t a, uint64_t b) { return uint64_t((unsigned __int128)a * b >> 64); }
+
+// Round and pack. `sig` holds the significand with its leading one at bit 62
+// (bits 9..0 are guard/sticky bits); the value is sig * 2^(e - 62).
+CR_HDI f64 round_pack(uint32_t s, int32_t e, uint64_t sig) {
+    int32_t be = e + 1023;
+    if (be >= 0x7ff) return inf(s);
+    bool sub = false;
+    if (be <= 0) {
+        sig = shift_right_jam(sig, 1 - be);
+        be = 0;
+        sub = true;
+    }
+    const uint32_t rb = uint32_t(sig & 0x3ff);
+    sig = (sig + 0x200) >> 10;
+    if (rb == 0x200) sig &= ~uint64_t(1);
+    if (sub) return f64{(uint64_t(s) << 63) + sig};
+    return f64{(uint64_t(s) << 63) + (uint64_t(be - 1) << 52) + sig};
+}
+
+// Normalize a nonzero finite operand: sig gets its leading one at bit 52.
+CR_HDI void unpack(f64 a, int32_t& e, uint64_t& sig) {
+    const int32_t ex = exp_of(a);
+    const uint64_t fr = frac_of(a);
+    if (ex == 0) {
+        const int sh = clz64(fr) - 11;
+        sig = fr << sh;
+        e = -1022 - sh;
+    } else {
+        sig = fr | 0x0010000000000000ull;
+        e = ex - 1023;
+    }
+}
+
+CR_HDI f64 add_mags(f64 a, f64 b, uint32_t s) {
+    // |a| + |b| with both finite and nonzero.
+    int32_t ea, eb;
+    uint64_t sa, sb;
+    unpack(a, ea, sa);
+    unpack(b, eb, sb);
+    if (ea < eb) {
+        int32_t t = ea; ea = eb; eb = t;
+        uint64_t u = sa; sa = sb; sb = u;
+    }
+    sa <<= 10;
+    sb = shift_right_jam(sb << 10, ea - eb);
+    uint64_t sum = sa + sb;
+    if (sum >> 63) {
+        sum = shift_right_jam(sum, 1);
+        ++ea;
+    }
+    return round_pack(s, ea, sum);
+}
+
+CR_HDI f64 sub_mags(f64 a, f64 b, uint32_t s) {
+    // |a| - |b| carrying sign s for a positive result; both finite nonzero.
+    int32_t ea, eb;
+    uint64_t sa, sb;
+    unpack(a, ea, sa);
+    unpack(b, eb, sb);
+    if (ea < eb || (ea == eb && sa < sb)) {
+        int32_t t = ea; ea = eb; eb = t;
+        uint64_t u = sa; sa = sb; sb = u;
+        s ^= 1;
+    }
+    if (ea == eb && sa == sb) return zero(0);
+    sa <<= 10;
+    sb = shift_right_jam(sb << 10, ea - eb);
+    uint64_t d = sa - sb;
+    const int sh = clz64(d) - 1;
+    d <<= sh;
+    return round_pack(s, ea - sh, d);
+}
+
+CR_HD CR_NOINLINE inline f64 add(f64 a, f64 b) {
+    if (is_nan(a) || is_nan(b)) return qnan();
+    const uint32_t sa = sign_of(a), sb = sign_of(b);
+    if (is_inf(a)) {
+        if (is_inf(b) && sa != sb) return qnan();
+        return a;
+    }
+    if (is_inf(b)) return b;
+    if (is_zero(a)) {
+        if (is_zero(b)) return zero(sa & sb);
+        return b;
+    }
+    if (is_zero(b)) return a;
+    return sa == sb ? add_mags(a, b, sa) : sub_mags(a, b, sa);
+}
+
+CR_HDI f64 sub(f64 a, f64 b) { return add(a, is_nan(b) ? b : neg(b)); }
+
+CR_HD CR_NOINLINE inline f64 mul(f64 a, f64 b) {
+    if (is_nan(a) || is_nan(b)) return qnan();
+    const uint32_t s = sign_of(a) ^ sign_of(b);
+    if (is_inf(a) || is_inf(b)) {
+        if (is_zero(a) || is_zero(b)) return qnan();
+        return inf(s);
+    }
+    if (is_zero(a) || is_zero(b)) return zero(s);
+    int32_t ea, eb;
+    uint64_t ma, mb;
+    unpack(a, ea, ma);
+    unpack(b, eb, mb);
+    ma <<= 10;
+    mb <<= 11;
+    const uint64_t hi = mul_hi(ma, mb), lo = ma * mb;
+    uint64_t sig = hi | uint64_t(lo != 0);
+    int32_t e = ea + eb + 1;
+    if (sig < 0x4000000000000000ull) {
+        sig <<= 1;
+        --e;
+    }
+    return round_pack(s, e, sig);
+}
+
+CR_HD CR_NOINLINE inline f64 div(f64 a, f64 b) {
+    if (is_nan(a) || is_nan(b)) return qnan();
+    const uint32_t s = sign_of(a) ^ sign_of(b);
+    if (is_inf(a)) return is_inf(b) ? qnan() : inf(s);
+    if (is_inf(b)) return zero(s);
+    if (is_zero(b)) return is_zero(a) ? qnan() : inf(s);
+    if (is_zero(a)) return zero(s);
+    int32_t ea, eb;
+    uint64_t ma, mb;
+    unpack(a, ea, ma);
+    unpack(b, eb, mb);
+    int32_t e = ea - eb;
+    if (ma < mb) {
+        ma <<= 1;
+        --e;
+    }
+    // Restoring division: 63 quotient bits with the leading one at bit 62.
+    uint64_t q = 0, r = ma;
+    for (int i = 0; i < 63; ++i) {
+        q <<= 1;
+        if (r >= mb) {
+            r -= mb;
+            q |= 1;
+        }
+        r <<= 1;
+    }
+    q |= uint64_t(r != 0);
+    return round_pack(s, e, q);
+}
+
+CR_HD CR_NOINLINE inline f64 sqrt(f64 a) {
+    if (is_nan(a)) return qnan();
+    if (is_zero(a)) return a;
+    if (sign_of(a)) return qnan();
+    if (is_inf(a)) return a;
+    int32_t e;
+    uint64_t m;
+    unpack(a, e, m);
+    if (e & 1) {
+        m <<= 1;
+        --e;
+    }
+    // m in [2^52, 2^54).  Radicand N = m * 2^56 in [2^108, 2^110); its integer
+    // square root has 55 bits.  Digit-by-digit over the two 64-bit words.
+    const uint64_t nhi = m >> 8, nlo = m << 56;
+    uint64_t root = 0, rem = 0;
+    for (int i = 54; i >= 0; --i) {
+        const int bit = 2 * i;
+        uint64_t two;
+        if (bit >= 64) two = (nhi >> (bit - 64)) & 3;
+        else two = (nlo >> bit) & 3;
+        rem = (rem << 2) | two;
+        const uint64_t trial = (root << 2) | 1;
+        root <<= 1;
+        if (rem >= trial) {
+            rem -= trial;
+            root |= 1;
+        }
+    }
+    // root in [2^54, 2^55): move the leading one to bit 62, jam the remainder.
+    const uint64_t sig = (root << 8) | uint64_t(rem != 0);
+    return round_pack(0, e / 2, sig);
+}
+
+// ---- comparisons (IEEE semantics: NaN compares false, -0 == +0) ----------
+CR_HDI int64_t order_key(f64 a) {
+    // Monotone integer key for non-NaN values.
+    return sign_of(a) ? -int64_t(a.b & 0x7fffffffffffffffull) : int64_t(a.b);
+}
+CR_HDI bool lt(f64 a, f64 b) {
+    if (is_nan(a) || is_nan(b)) return false;
+    if (is_zero(a) && is_zero(b)) return false;
+    return order_key(a) < order_key(b);
+}
+CR_HDI bool le(f64 a, f64 b) {
+    if (is_nan(a) || is_nan(b)) return false;
+    if (is_zero(a) && is_zero(b)) return true;
+    return order_key(a) <= order_key(b);
+}
+CR_HDI bool eq(f64 a, f64 b) {
+    if (is_nan(a) || is_nan(b)) return false;
+    if (is_zero(a) && is_zero(b)) return true;
+    return a.b == b.b;
+}
+CR_HDI bool gt(f64 a, f64 b) { return lt(b, a); }
+CR_HDI bool ge(f64 a, f64 b) { return le(b, a); }
+// std::max / std::min semantics: max(a,b) = (a < b) ? b : a.
+CR_HDI f64 max_(f64 a, f64 b) { return lt(a, b) ? b : a; }
+CR_HDI f64 min_(f64 a, f64 b) { return lt(b, a) ? b : a; }
+
+// ---- conversions -----------------------------------------------------------
+CR_HDI uint32_t fbits(float x) {
+    union { float f; uint32_t u; } c;
+    c.f = x;
+    return c.u;
+}
+CR_HDI float bitsf(uint32_t u) {
+    union { float f; uint32_t u; } c;
+    c.u = u;
+    return c.f;
+}
+
+// Exact widening float -> double.
+CR_HDI f64 from_float(float x) {
+    const uint32_t u = fbits(x);
+    const uint32_t s = u >> 31;
+    const int32_t ex = int32_t((u >> 23) & 0xff);
+    uint32_t fr = u & 0x7fffff;
+    if (ex == 0xff) {
+        if (fr) return qnan();
+        return inf(s);
+    }
+    if (ex == 0) {
+        if (fr == 0) return zero(s);
+        // float subnormal: normalize
+        int32_t e = -126;
+        while (!(fr & 0x800000)) {
+            fr <<= 1;
+            --e;
+        }
+        fr &= 0x7fffff;
+        return f64{(uint64_t(s) << 63) | (uint64_t(e + 1023) << 52) | (uint64_t(fr) << 29)};
+    }
+    return f64{(uint64_t(s) << 63) | (uint64_t(ex - 127 + 1023) << 52) | (uint64_t(fr) << 29)};
+}
+
+// Correctly rounded narrowing double -> float (round-to-nearest-even).
+CR_HD CR_NOINLINE inline float to_float(f64 a) {
+    const uint32_t s = sign_of(a);
+    if (is_nan(a)) return bitsf(0x7fc00000u);
+    if (is_inf(a)) return bitsf((s << 31) | 0x7f800000u);
+    if (is_zero(a)) return bitsf(s << 31);
+    int32_t e;
+    uint64_t m;
+    unpack(a, e, m);
+    uint64_t sig = m << 10;  // leading one at bit 62
+    int32_t be = e + 127;
+    if (be >= 0xff) return bitsf((s << 31) | 0x7f800000u);
+    bool sub = false;
+    if (be <= 0) {
+        sig = shift_right_jam(sig, 1 - be);
+        be = 0;
+        sub = true;
+    }
+    const uint64_t half = uint64_t(1) << 38;
+    const uint64_t rb = sig & ((uint64_t(1) << 39) - 1);
+    uint64_t r = (sig + half) >> 39;
+    if (rb == half) r &= ~uint64_t(1);
+    if (sub) return bitsf((s << 31) + uint32_t(r));
+    const uint64_t packed = (uint64_t(s) << 31) + (uint64_t(be - 1) << 23) + r;
+    if (((packed >> 23) & 0xff) == 0xff) return bitsf((s << 31) | 0x7f800000u);
+    return bitsf(uint32_t(packed));
+}
+
+CR_HDI f64 from_u64(uint64_t v) {
+    if (v == 0) return zero(0);
+    const int lz = clz64(v);
+    uint64_t sig;
+    if (lz >= 1) sig = v << (lz - 1);  // leading one at bit 62
+    else sig = shift_right_jam(v, 1);
+    return round_pack(0, 63 - lz, sig);
+}
+CR_HDI f64 from_i32(int32_t v) {
+    if (v < 0) return neg(from_u64(uint64_t(-int64_t(v))));
+    return from_u64(uint64_t(v));
+}
+
+// Convenience constants.
+CR_HDI f64 one() { return f64{0x3ff0000000000000ull}; }
+CR_HDI f64 two() { return f64{0x4000000000000000ull}; }
+
+}  // namespace sf

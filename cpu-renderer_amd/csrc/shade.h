@@ -1,0 +1,510 @@
+// shade.h — per-hit transport pieces of the wavefront integrator (device).
+//
+// Each function is a restatement of one reference routine, evaluated in the
+// same float expression order so that per-sample results agree bit for bit
+// wherever the math library agrees; binary64 parts use the sf64 emulation:
+//   camera_ray      Camera::generate        reference src/geometry.cpp:213-234
+//   offset_ray      offsetRay               reference src/geometry.cpp:186-209
+//   surface_at      surfaceAt (no textures) reference src/geometry.cpp:401-459
+//   Bsdf*           Context/evalLocal/sampleBsdf/evaluateBsdf  reference src/bsdf.cpp:6-237
+//   env_*           Environment             reference src/render.cpp:12-94
+//   sample_light    Lights::sample          reference src/render.cpp:172-228
+//   triangle_pdf    Lights::trianglePdf     reference src/render.cpp:165-170
+#pragma once
+#include "layout.h"
+
+namespace cr {
+
+// ---------------------------------------------------------------- camera
+CR_HDI bool camera_ray(const DevCamera& c, float px, float py, V3& o, V3& d, float& tmin, float& tmax) {
+    const sf::f64 one = sf::one(), two = sf::two();
+    const sf::f64 x = sf::mul(
+        sf::mul(sf::sub(sf::div(sf::mul(two, sf::from_float(px)), sf::from_i32(c.width)), one), c.ratio),
+        c.half_height);
+    const sf::f64 y =
+        sf::mul(sf::sub(one, sf::div(sf::mul(two, sf::from_float(py)), sf::from_i32(c.height))), c.half_height);
+    const V3 f{c.f[0], c.f[1], c.f[2]}, r{c.r[0], c.r[1], c.r[2]}, u{c.u[0], c.u[1], c.u[2]};
+    const D3 dir = dadd(dadd(widen(f), dscale(widen(r), x)), dscale(widen(u), y));
+    d = dunit(dir, V3{0, 0, 0});
+    const sf::f64 proj = ddot(widen(d), widen(f));
+    const sf::f64 near_t = sf::div(sf::from_float(c.near_clip), proj);
+    if (!sf::gt(proj, sf::zero(0)) || !sf::is_finite(near_t) || sf::gt(near_t, sf::from_float(kFltMax))) return false;
+    o = V3{c.position[0], c.position[1], c.position[2]};
+    tmin = sf::to_float(near_t);
+    tmax = sf::to_float(sf::div(sf::from_float(c.far_clip), proj));
+    return true;
+}
+
+// ---------------------------------------------------------------- offsetRay
+// `ung` is unit(geometricNormal) (binary64 normalization done once per triangle
+// on the host); `ng` is the stored geometric normal used for the validity test.
+CR_HDI uint32_t offset_ray(V3 p, V3 ng, V3 ung, V3 direction, float max_distance, V3& origin, V3& dir,
+                           float& tmax) {
+    if (!finite(p) || !finite(ng) || !finite(direction) || !nonzero(ng) || !nonzero(direction) ||
+        max_distance != max_distance || max_distance < 0)
+        return kErrInvalidSecondary;
+    dir = dunit(widen(direction), V3{0, 0, 0});
+    V3 n = ung;
+    if (dot_sign_exact(n, dir) < 0) n = -n;
+    float scale = fabsf(p.x);
+    if (scale < fabsf(p.y)) scale = fabsf(p.y);
+    if (scale < fabsf(p.z)) scale = fabsf(p.z);
+    const sf::f64 distance = sf::mul(sf::from_float(4 * kFltEps), sf::from_float(scale));
+    for (int a = 0; a < 3; ++a) {
+        const float na = at(n, a);
+        float v = sf::to_float(sf::add(sf::from_float(at(p, a)), sf::mul(distance, sf::from_float(na))));
+        if (na != 0) v = na > 0 ? next_up(v) : next_down(v);
+        set(origin, a, v);
+    }
+    if (!finite(origin)) return kErrOffsetOverflow;
+    float limit = max_distance;
+    if (finite1(limit)) {
+        const sf::f64 advance = ddot(dsub(widen(origin), widen(p)), widen(dir));
+        limit = sf::to_float(sf::max_(sf::zero(0), sf::sub(sf::from_float(limit), advance)));
+        limit = next_toward_zero(limit);
+    }
+    tmax = limit;
+    return 0;
+}
+
+// ---------------------------------------------------------------- surfaceAt
+struct Surf {
+    V3 position, position_low, ng, normal;
+    V3 base, emission, att_color;
+    float metallic, roughness, transmission, ior, thickness, att_distance;
+    uint32_t material;
+    bool front, double_sided;
+};
+
+CR_HDI void material_into(const DevMaterial& m, Surf& s) {
+    s.base = V3{m.base[0], m.base[1], m.base[2]};
+    s.metallic = clampf(m.metallic, 0.0f, 1.0f);
+    s.roughness = clampf(m.roughness, 0.0f, 1.0f);
+    s.emission = V3{m.emission[0], m.emission[1], m.emission[2]};
+    s.transmission = clampf(m.transmission, 0.0f, 1.0f);
+    s.ior = m.ior;
+    s.thickness = m.thickness;
+    s.att_distance = m.att_distance;
+    s.att_color = V3{m.att_color[0], m.att_color[1], m.att_color[2]};
+    s.double_sided = m.double_sided != 0;
+}
+
+CR_HDI Surf surface_at(const DevScene& sc, uint32_t prim, V3 ray_dir, sf::f64 hu, sf::f64 hv) {
+    const PrimRec p = sc.prims[prim];
+    const PrimAux& ax = sc.aux[prim];
+    Surf s;
+    const sf::f64 w = sf::sub(sf::sub(sf::one(), hu), hv);
+    const D3 P = dadd(dadd(dscale(widen(V3{p.ax, p.ay, p.az}), w), dscale(widen(V3{p.bx, p.by, p.bz}), hu)),
+                      dscale(widen(V3{p.cx, p.cy, p.cz}), hv));
+    s.position = narrow(P);
+    s.position_low = V3{sf::to_float(sf::sub(P.x, sf::from_float(s.position.x))),
+                        sf::to_float(sf::sub(P.y, sf::from_float(s.position.y))),
+                        sf::to_float(sf::sub(P.z, sf::from_float(s.position.z)))};
+    s.ng = V3{ax.ng[0], ax.ng[1], ax.ng[2]};
+    s.front = dot_sign_exact(ray_dir, s.ng) < 0;
+    const D3 N = dadd(dadd(dscale(widen(V3{ax.n0[0], ax.n0[1], ax.n0[2]}), w),
+                           dscale(widen(V3{ax.n1[0], ax.n1[1], ax.n1[2]}), hu)),
+                      dscale(widen(V3{ax.n2[0], ax.n2[1], ax.n2[2]}), hv));
+    s.normal = dunit(N, s.ng);
+    if (dot(s.normal, s.ng) < 0) s.normal = -s.normal;
+    s.material = p.material;
+    material_into(sc.materials[p.material], s);
+    return s;
+}
+
+// ---------------------------------------------------------------- BSDF
+CR_HDI float fresnel_dielectric(float cosine, float eta_i, float eta_t) {
+    if (eta_i == eta_t) return 0;
+    cosine = clampf(fabsf(cosine), 0.0f, 1.0f);
+    const float eta = eta_i / eta_t;
+    const float sin_t2 = eta * eta * fmax_(0.0f, 1 - cosine * cosine);
+    if (sin_t2 >= 1) return 1;
+    const float cos_t = sqrtf(1 - sin_t2);
+    const float ds = eta_i * cosine + eta_t * cos_t, dp = eta_t * cosine + eta_i * cos_t;
+    if (ds == 0 || dp == 0) return 1;
+    const float rs = (eta_i * cosine - eta_t * cos_t) / ds, rp = (eta_t * cosine - eta_i * cos_t) / dp;
+    return 0.5f * (rs * rs + rp * rp);
+}
+
+CR_HDI bool refract_direction(V3 wo, V3 n, float ratio, V3& wi) {
+    const float cos_i = dot(wo, n);
+    const float sin_t2 = ratio * ratio * fmax_(0.0f, 1 - cos_i * cos_i);
+    if (sin_t2 >= 1) return false;
+    wi = normalize(-wo * ratio + n * (ratio * cos_i - sqrtf(1 - sin_t2)));
+    return true;
+}
+
+// x^5 for the Schlick term: powf(x, 5) correctly rounded, from an exact
+// binary64 product chain (x^5 of a float needs at most 120 bits; the chain
+// below rounds once per step at 53 bits, far inside the float rounding margin).
+CR_HDI float pow5(float x) {
+    const sf::f64 d = sf::from_float(x);
+    const sf::f64 d2 = sf::mul(d, d);
+    return sf::to_float(sf::mul(sf::mul(d2, d2), d));
+}
+
+CR_HDI float ggx_d(float alpha, V3 h) {
+    if (h.z <= 0) return 0;
+    const float a2 = alpha * alpha;
+    const float d = (h.x * h.x + h.y * h.y) + a2 * h.z * h.z;
+    return a2 / (kPi * d * d);
+}
+
+CR_HDI float smith_g1(float alpha, V3 v) {
+    const float cos2 = v.z * v.z;
+    if (cos2 == 0) return 0;
+    return 2 / (1 + sqrtf(1 + alpha * alpha * (v.x * v.x + v.y * v.y) / cos2));
+}
+
+CR_HDI V3 visible_normal(V3 wo, float alpha, float u, float v) {
+    const V3 vh = normalize(V3{alpha * wo.x, alpha * wo.y, wo.z});
+    const float lensq = vh.x * vh.x + vh.y * vh.y;
+    const V3 t1 = lensq > 0 ? V3{-vh.y, vh.x, 0} / sqrtf(lensq) : V3{1, 0, 0};
+    const V3 t2 = cross(vh, t1);
+    const float r = sqrtf(u), phi = 2 * kPi * v;
+    const float x = r * cosf(phi);
+    float y = r * sinf(phi);
+    const float s = 0.5f * (1 + vh.z);
+    y = (1 - s) * sqrtf(fmax_(0.0f, 1 - x * x)) + s * y;
+    const V3 nh = t1 * x + t2 * y + vh * sqrtf(fmax_(0.0f, 1 - x * x - y * y));
+    return normalize(V3{alpha * nh.x, alpha * nh.y, fmax_(0.0f, nh.z)});
+}
+
+struct Bsdf {
+    const Surf* s;
+    Frame frame;
+    V3 wo;
+    float eta_i, eta_t, eta, alpha, diffuse_p, delta_t_p;
+    bool thin, smooth, matched;
+
+    CR_HDI float fresnel(float cosine) const {
+        const float f = fresnel_dielectric(cosine, eta_i, eta_t);
+        return thin ? 2 * f / (1 + f) : f;
+    }
+    CR_HDI V3 reflection(float f, float cosine) const {
+        const float schlick = pow5(1 - clampf(fabsf(cosine), 0.0f, 1.0f));
+        return v3((1 - s->metallic) * f) + (s->base + (v3(1) - s->base) * schlick) * s->metallic;
+    }
+    CR_HDI V3 transmission(float f) const { return s->base * ((1 - s->metallic) * s->transmission * (1 - f)); }
+    CR_HDI float reflection_probability(float f, float cos_h) const {
+        if (matched) return 1;
+        const float r = luminance(reflection(f, cos_h)), t = luminance(transmission(f));
+        return r + t > 0 ? r / (r + t) : 1;
+    }
+    CR_HDI float micro_probability() const { return fmax_(0.0f, 1 - diffuse_p - delta_t_p); }
+};
+
+CR_HDI void bsdf_init(Bsdf& c, const Surf& s, V3 outgoing) {
+    c.s = &s;
+    c.frame = make_frame(dot(outgoing, s.normal) > 0 ? s.normal : -s.normal);
+    c.wo = to_local(c.frame, outgoing);
+    const bool exit = s.thickness > 0 && !s.front;
+    c.eta_i = exit ? s.ior : 1;
+    c.eta_t = exit ? 1 : s.ior;
+    c.eta = c.eta_t / c.eta_i;
+    c.alpha = fmax_(0.001f, s.roughness * s.roughness);
+    c.diffuse_p = 0;
+    c.delta_t_p = 0;
+    c.thin = s.thickness <= 0 && s.transmission > 0;
+    c.smooth = s.roughness <= 0.001f;
+    c.matched = false;
+    if (c.thin) {
+        c.eta_i = 1;
+        c.eta_t = s.ior;
+        c.eta = s.ior;
+    }
+    c.matched = fabsf(c.eta_i - c.eta_t) <= sqrtf(kFltEps) * fmax_(c.eta_i, c.eta_t);
+    if (c.matched) c.eta_i = c.eta_t = c.eta = 1;
+    const float f = c.fresnel(c.wo.z);
+    const float diffuse = (1 - s.metallic) * (1 - s.transmission) * luminance(s.base) * (1 - f);
+    const float transmit = (1 - s.metallic) * s.transmission * (1 - f) * luminance(s.base);
+    const float total = diffuse + luminance(c.reflection(f, c.wo.z)) + transmit;
+    if (total > 0) {
+        c.diffuse_p = diffuse / total;
+        if (c.matched) c.delta_t_p = transmit / total;
+    }
+}
+
+struct BsdfEval {
+    V3 value;
+    float pdf;
+};
+
+CR_HDI BsdfEval eval_local(const Bsdf& c, V3 wi) {
+    BsdfEval r{V3{0, 0, 0}, 0};
+    const float no = c.wo.z, ni = fabsf(wi.z);
+    if (no <= 1e-7f || ni <= 1e-7f) return r;
+    const bool refl = wi.z > 0;
+    if (!refl && (c.matched || c.thin)) return r;
+    const Surf& s = *c.s;
+    if (refl) {
+        const float diffuse = (1 - s.metallic) * (1 - s.transmission) * (1 - c.fresnel(no)) * (1 - c.fresnel(ni));
+        r.value = s.base * (diffuse / kPi);
+        r.pdf = c.diffuse_p * ni / kPi;
+    }
+    if (c.smooth) return r;
+    V3 h = normalize(refl ? c.wo + wi : c.wo + wi * c.eta);
+    if (h.z < 0) h = -h;
+    const float oh = dot(c.wo, h), ih = dot(wi, h);
+    if (oh <= 0 || (refl ? ih <= 0 : ih >= 0)) return r;
+    const float f = c.fresnel(oh), pr = c.reflection_probability(f, oh);
+    const float d = ggx_d(c.alpha, h), g1 = smith_g1(c.alpha, c.wo), g = g1 * smith_g1(c.alpha, wi);
+    const float pdf_h = d * g1 * oh / no;
+    if (refl) {
+        r.value = r.value + c.reflection(f, oh) * (d * g / (4 * no * ni));
+        r.pdf += c.micro_probability() * pr * pdf_h / (4 * oh);
+    } else {
+        const float denom = oh + c.eta * ih, denom2 = denom * denom;
+        if (denom2 < 1e-20f) return BsdfEval{V3{0, 0, 0}, 0};
+        r.value = c.transmission(f) * (d * g * fabsf(oh * ih) / (no * ni * denom2));
+        r.pdf = c.micro_probability() * (1 - pr) * pdf_h * c.eta * c.eta * fabsf(ih) / denom2;
+    }
+    return r;
+}
+
+CR_HDI bool correct_side(const Surf& s, V3 wo, V3 wi, bool transmitted) {
+    const float d = dot(wo, s.ng) * dot(wi, s.ng);
+    return transmitted ? d < 0 : d > 0;
+}
+
+CR_HDI BsdfEval evaluate_bsdf(const Surf& s, V3 wo, V3 wi) {
+    Bsdf c;
+    bsdf_init(c, s, wo);
+    const V3 local = to_local(c.frame, wi);
+    if (!correct_side(s, wo, wi, local.z < 0)) return BsdfEval{V3{0, 0, 0}, 0};
+    return eval_local(c, local);
+}
+
+struct BsdfSample {
+    V3 direction, weight;
+    float pdf, eta;
+    bool delta, transmitted;
+};
+
+CR_HDI BsdfSample empty_sample() { return BsdfSample{V3{0, 0, 0}, V3{0, 0, 0}, 0, 1, false, false}; }
+CR_HDI bool sample_ok(const BsdfSample& b) {
+    return max_component(b.weight) > 0 && finite(b.weight) && finite(b.direction);
+}
+
+CR_HDI BsdfSample sample_bsdf(const Surf& s, V3 wo, Rng& rng) {
+    Bsdf c;
+    bsdf_init(c, s, wo);
+    BsdfSample out = empty_sample();
+    if (c.wo.z <= 1e-7f) return out;
+    V3 wi;
+    const float lobe = rng.uniform();
+    if (lobe < c.diffuse_p) {
+        const float u = rng.uniform(), phi = 2 * kPi * rng.uniform();
+        wi = V3{sqrtf(u) * cosf(phi), sqrtf(u) * sinf(phi), sqrtf(1 - u)};
+    } else if (lobe < c.diffuse_p + c.delta_t_p) {
+        out.direction = -wo;
+        out.delta = true;
+        out.transmitted = true;
+        out.pdf = c.delta_t_p;
+        out.weight = s.base * ((1 - s.metallic) * s.transmission / out.pdf);
+        return correct_side(s, wo, out.direction, true) ? out : empty_sample();
+    } else {
+        const float u = rng.uniform(), v = rng.uniform();
+        const V3 h = c.smooth ? V3{0, 0, 1} : visible_normal(c.wo, c.alpha, u, v);
+        const float oh = dot(c.wo, h), f = c.fresnel(oh), pr = c.reflection_probability(f, oh);
+        const bool refl = rng.uniform() < pr;
+        if (refl) wi = reflect(-c.wo, h);
+        else if (c.thin) wi = -c.wo;
+        else if (!refract_direction(c.wo, h, 1 / c.eta, wi)) return out;
+        if ((refl && wi.z <= 0) || (!refl && wi.z >= 0)) return out;
+        out.transmitted = !refl;
+        out.eta = out.transmitted && !c.thin ? c.eta : 1;
+        if (c.smooth || (!refl && c.thin)) {
+            out.delta = true;
+            out.direction = normalize(to_world(c.frame, wi));
+            if (!correct_side(s, wo, out.direction, out.transmitted)) return empty_sample();
+            const float prob = c.micro_probability() * (refl ? pr : 1 - pr);
+            if (prob <= 0) return empty_sample();
+            out.pdf = prob;
+            out.weight = (refl ? c.reflection(f, oh) : c.transmission(f) / (out.eta * out.eta)) / prob;
+            return out;
+        }
+    }
+    out.direction = normalize(to_world(c.frame, wi));
+    wi = to_local(c.frame, out.direction);
+    if ((wi.z < 0) != out.transmitted || !correct_side(s, wo, out.direction, out.transmitted)) return empty_sample();
+    const BsdfEval e = eval_local(c, wi);
+    if (e.pdf <= 0 || !finite1(e.pdf)) return empty_sample();
+    out.pdf = e.pdf;
+    out.weight = e.value * (fabsf(wi.z) / e.pdf);
+    return out;
+}
+
+// Beer-Lambert transmittance over `distance` (reference src/bsdf.cpp:71-74).
+CR_HDI V3 beer_lambert(V3 sigma, float distance) {
+    if (distance == kInf) return V3{sigma.x > 0 ? 0.0f : 1.0f, sigma.y > 0 ? 0.0f : 1.0f, sigma.z > 0 ? 0.0f : 1.0f};
+    const V3 a = sigma * (-fmax_(0.0f, distance));
+    return V3{expf(a.x), expf(a.y), expf(a.z)};
+}
+
+// ---------------------------------------------------------------- environment
+CR_HDI void env_uv(const DevEnv& e, V3 d, float& u, float& v) {
+    u = (atan2f(d.z, d.x) - e.rotation) / (2 * kPi) + 0.5f;
+    u -= floorf(u);
+    v = acosf(clampf(d.y, -1.0f, 1.0f)) / kPi;
+}
+
+CR_HDI V3 env_texel(const DevEnv& e, int ix, int iy) {
+    ix = (ix % e.width + e.width) % e.width;
+    iy = iy < 0 ? 0 : (iy > e.height - 1 ? e.height - 1 : iy);
+    const float4 t = e.texels[size_t(iy) * size_t(e.width) + size_t(ix)];
+    return V3{t.x, t.y, t.z};
+}
+
+CR_HDI V3 env_radiance(const DevEnv& e, V3 d) {
+    if (!e.is_image) return V3{e.constant[0], e.constant[1], e.constant[2]};
+    float u, v;
+    env_uv(e, d, u, v);
+    const float px = u * float(e.width) - 0.5f, py = v * float(e.height) - 0.5f;
+    const int x = int(floorf(px)), y = int(floorf(py));
+    const float tx = px - float(x), ty = py - float(y);
+    return mix(mix(env_texel(e, x, y), env_texel(e, x + 1, y), tx),
+               mix(env_texel(e, x, y + 1), env_texel(e, x + 1, y + 1), tx), ty) *
+           e.intensity;
+}
+
+CR_HDI float env_pdf(const DevEnv& e, V3 d) {
+    if (!e.is_image) return 1 / (4 * kPi);
+    float u, v;
+    env_uv(e, d, u, v);
+    int x = int(u * float(e.width));
+    if (x > e.width - 1) x = e.width - 1;
+    int y = int(v * float(e.height));
+    if (y > e.height - 1) y = e.height - 1;
+    return e.pdf[size_t(y) * size_t(e.width) + size_t(x)];
+}
+
+// Texel index for a 24-bit draw k: the count of thresholds T_i <= k, i.e. the
+// reference's upper_bound over its binary64 CDF (clamped to the last texel).
+CR_HDI uint32_t env_invert(const DevEnv& e, uint32_t k) {
+    const uint32_t n = uint32_t(e.width) * uint32_t(e.height);
+    uint32_t lo = e.guide[k >> 8], hi = e.guide[(k >> 8) + 1];
+    while (lo < hi) {  // first index in [lo, hi) with T > k
+        const uint32_t mid = (lo + hi) >> 1;
+        if (e.threshold[mid] <= k) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n ? lo : n - 1;
+}
+
+CR_HDI V3 env_sample(const DevEnv& e, Rng& rng, float& density) {
+    if (!e.is_image) {
+        const float y = 1 - 2 * rng.uniform(), phi = 2 * kPi * rng.uniform(), r = sqrtf(fmax_(0.0f, 1 - y * y));
+        density = 1 / (4 * kPi);
+        return V3{r * cosf(phi), y, r * sinf(phi)};
+    }
+    const uint32_t index = env_invert(e, rng.bits24());
+    const int x = int(index % uint32_t(e.width)), y = int(index / uint32_t(e.width));
+    const float u = (float(x) + rng.uniform()) / float(e.width);
+    const float c0 = e.rowcos[y], c1 = e.rowcos[y + 1];
+    const float cos_t = c0 + (c1 - c0) * rng.uniform(), sin_t = sqrtf(fmax_(0.0f, 1 - cos_t * cos_t));
+    const float phi = (u - 0.5f) * 2 * kPi + e.rotation;
+    density = e.pdf[index];
+    return V3{sin_t * cosf(phi), cos_t, sin_t * sinf(phi)};
+}
+
+// ---------------------------------------------------------------- lights
+struct LightSample {
+    V3 direction, radiance, target, target_normal, target_unormal;
+    float distance, pdf;
+    bool delta;
+};
+
+CR_HDI LightSample no_light() {
+    LightSample l;
+    l.direction = l.radiance = l.target = l.target_normal = l.target_unormal = V3{0, 0, 0};
+    l.distance = kInf;
+    l.pdf = 0;
+    l.delta = false;
+    return l;
+}
+
+CR_HDI float triangle_pdf(const DevScene& sc, const DevLights& lt, uint32_t prim, V3 from, V3 point) {
+    const PrimAux& ax = sc.aux[prim];
+    const V3 ng{ax.ng[0], ax.ng[1], ax.ng[2]};
+    const V3 dir = point - from;
+    const uint32_t rank = sc.light_rank[prim];
+    const float prob = rank == kNoTriangle ? 0.0f : lt.entries[rank].probability;
+    const float len2 = length_sq(dir), cosine = fabsf(dot(ng, normalize(dir)));
+    return cosine > 1e-8f ? prob * len2 / (ax.area * cosine) : 0;
+}
+
+CR_HDI float environment_pdf(const DevLights& lt, const DevEnv& e, V3 d) { return lt.env_probability * env_pdf(e, d); }
+
+CR_HDI LightSample sample_light(const DevScene& sc, const DevLights& lt, const DevEnv& env, V3 from, Rng& rng) {
+    LightSample out = no_light();
+    if (lt.count == 0) return out;
+    const uint32_t k = rng.bits24();
+    uint32_t lo = 0, hi = lt.count;  // first entry whose threshold >= k
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (lt.entries[mid].threshold < k) lo = mid + 1;
+        else hi = mid;
+    }
+    const DevLightEntry le = lt.entries[lo < lt.count ? lo : lt.count - 1];
+    if (le.type == 2) {
+        out.direction = env_sample(env, rng, out.pdf);
+        out.pdf *= le.probability;
+        out.radiance = env_radiance(env, out.direction);
+        return out;
+    }
+    if (le.type == 0) {
+        const DevPunctual& l = sc.punctual[le.index];
+        out.delta = true;
+        out.pdf = le.probability;
+        float attenuation = 1;
+        if (l.type == 1) {
+            out.direction = -V3{l.direction[0], l.direction[1], l.direction[2]};
+        } else {
+            const V3 lp{l.position[0], l.position[1], l.position[2]};
+            const V3 delta = lp - from;
+            const float r2 = length_sq(delta);
+            if (r2 < 1e-12f) return no_light();
+            out.distance = sqrtf(r2);
+            out.direction = delta / out.distance;
+            out.target = lp;
+            attenuation = 1 / r2;
+            if (finite1(l.range)) {
+                const float q = out.distance / l.range;
+                attenuation *= fmax_(0.0f, 1 - q * q * q * q);
+            }
+            if (l.type == 2) {
+                const V3 ld{l.direction[0], l.direction[1], l.direction[2]};
+                const float cos_a = dot(ld, -out.direction);
+                const float falloff =
+                    clampf((cos_a - l.cos_outer) / fmax_(1e-7f, l.cos_inner - l.cos_outer), 0.0f, 1.0f);
+                attenuation *= falloff * falloff;
+            }
+        }
+        out.radiance = V3{l.color[0], l.color[1], l.color[2]} * (l.intensity * attenuation);
+        return out;
+    }
+    // Emissive triangle: uniform area sample by the square-root warp.
+    const uint32_t prim = le.index;
+    const PrimRec p = sc.prims[prim];
+    const PrimAux& ax = sc.aux[prim];
+    const float r = sqrtf(rng.uniform()), u1 = 1 - r, u2 = r * rng.uniform(), u0 = 1 - u1 - u2;
+    out.target = V3{p.ax, p.ay, p.az} * u0 + V3{p.bx, p.by, p.bz} * u1 + V3{p.cx, p.cy, p.cz} * u2;
+    out.target_normal = V3{ax.ng[0], ax.ng[1], ax.ng[2]};
+    out.target_unormal = V3{ax.ung[0], ax.ung[1], ax.ung[2]};
+    const V3 delta = out.target - from;
+    out.distance = length(delta);
+    if (out.distance < 1e-7f) return no_light();
+    out.direction = delta / out.distance;
+    if (p.flags & kPrimNeverHit) return no_light();
+    const int side = dot_sign_exact(out.target_normal, out.direction);
+    if ((p.flags & kPrimCull) && side >= 0) return no_light();
+    const DevMaterial& m = sc.materials[p.material];
+    const bool front = side < 0;
+    if (!front && !m.double_sided) return no_light();
+    out.radiance = V3{m.emission[0], m.emission[1], m.emission[2]};
+    out.pdf = triangle_pdf(sc, lt, prim, from, out.target);
+    return out;
+}
+
+}  // namespace cr

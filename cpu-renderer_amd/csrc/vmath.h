@@ -1,0 +1,169 @@
+// vmath.h — float vector algebra and the per-sample random stream, host+device.
+//
+// Operation order follows the reference value types exactly (reference
+// src/math.h:15-128) because per-sample parity requires identical float
+// rounding: scalar division is multiplication by the reciprocal, dot products
+// sum x, y, z left to right, luminance uses the Rec.709 weights, and every
+// product/sum is a separate rounded operation (device code is compiled with
+// -ffp-contract=off so no multiply-add is fused behind our back).
+#pragma once
+#include "sf64.h"
+
+#include <math.h>
+#include <stdint.h>
+
+namespace cr {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInf = __builtin_huge_valf();
+constexpr float kFltMax = 3.40282346638528859812e+38f;
+constexpr float kFltEps = 1.1920928955078125e-7f;
+constexpr uint32_t kNoTriangle = 0xffffffffu;
+
+struct V3 {
+    float x, y, z;
+};
+
+CR_HDI V3 v3(float s) { return {s, s, s}; }
+CR_HDI V3 v3(float a, float b, float c) { return {a, b, c}; }
+CR_HDI float at(V3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+CR_HDI void set(V3& v, int i, float s) {
+    if (i == 0) v.x = s;
+    else if (i == 1) v.y = s;
+    else v.z = s;
+}
+CR_HDI V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+CR_HDI V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+CR_HDI V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+CR_HDI V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+CR_HDI V3 operator*(V3 a, float b) { return {a.x * b, a.y * b, a.z * b}; }
+CR_HDI V3 operator*(float a, V3 b) { return b * a; }
+CR_HDI V3 operator/(V3 a, float b) { return a * (1.0f / b); }
+CR_HDI float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+CR_HDI V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+CR_HDI float length_sq(V3 a) { return dot(a, a); }
+CR_HDI float length(V3 a) { return sqrtf(length_sq(a)); }
+CR_HDI V3 normalize(V3 a) {
+    const float l = length(a);
+    return l > 0 ? a / l : V3{0, 0, 0};
+}
+CR_HDI bool finite1(float x) { return (sf::fbits(x) & 0x7f800000u) != 0x7f800000u; }
+CR_HDI bool finite(V3 a) { return finite1(a.x) && finite1(a.y) && finite1(a.z); }
+CR_HDI bool nonzero(V3 a) { return a.x != 0 || a.y != 0 || a.z != 0; }
+// std::min / std::max / std::clamp semantics (first argument wins ties and NaN).
+CR_HDI float fmin_(float a, float b) { return b < a ? b : a; }
+CR_HDI float fmax_(float a, float b) { return a < b ? b : a; }
+CR_HDI float clampf(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
+CR_HDI V3 clamp(V3 a, float lo, float hi) { return {clampf(a.x, lo, hi), clampf(a.y, lo, hi), clampf(a.z, lo, hi)}; }
+CR_HDI float max_component(V3 a) {
+    float m = a.x;
+    if (m < a.y) m = a.y;
+    if (m < a.z) m = a.z;
+    return m;
+}
+CR_HDI float min_component(V3 a) {
+    float m = a.x;
+    if (a.y < m) m = a.y;
+    if (a.z < m) m = a.z;
+    return m;
+}
+CR_HDI float luminance(V3 a) { return dot(a, V3{0.2126f, 0.7152f, 0.0722f}); }
+CR_HDI V3 mix(V3 a, V3 b, float t) { return a * (1 - t) + b * t; }
+CR_HDI V3 reflect(V3 i, V3 n) { return i - 2 * dot(i, n) * n; }
+
+// Orthonormal basis whose local +Z is the given unit normal.
+struct Frame {
+    V3 x, y, z;
+};
+CR_HDI Frame make_frame(V3 n) {
+    Frame f;
+    f.z = n;
+    f.x = normalize(cross(fabsf(n.z) < 0.999f ? V3{0, 0, 1} : V3{0, 1, 0}, n));
+    f.y = cross(n, f.x);
+    return f;
+}
+CR_HDI V3 to_local(const Frame& f, V3 v) { return {dot(v, f.x), dot(v, f.y), dot(v, f.z)}; }
+CR_HDI V3 to_world(const Frame& f, V3 v) { return f.x * v.x + f.y * v.y + f.z * v.z; }
+
+// nextafterf toward +inf / -inf / zero for finite or infinite inputs.
+CR_HDI float next_up(float x) {
+    if (x != x || x == kInf) return x;
+    if (x == 0) return sf::bitsf(1);
+    const uint32_t u = sf::fbits(x);
+    return sf::bitsf(x > 0 ? u + 1 : u - 1);
+}
+CR_HDI float next_down(float x) { return -next_up(-x); }
+CR_HDI float next_toward_zero(float x) {
+    if (x != x || x == 0) return x;
+    return sf::bitsf(sf::fbits(x) - 1);
+}
+
+// SplitMix64 stream, one per (seed, pixel, sample) (reference src/math.h:106-121).
+struct Rng {
+    uint64_t state;
+    CR_HDI uint32_t next() {
+        state += 0x9e3779b97f4a7c15ull;
+        uint64_t z = state;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return uint32_t((z ^ (z >> 31)) >> 32);
+    }
+    // 24 random bits -> float in [0,1); bits24() exposes the integer for
+    // exact CDF inversion by threshold tables.
+    CR_HDI uint32_t bits24() { return next() >> 8; }
+    CR_HDI float uniform() { return float(bits24()) * (1.0f / 16777216.0f); }
+};
+
+CR_HDI uint64_t sample_seed(uint64_t seed, uint64_t pixel, uint64_t sample) {
+    Rng h{seed ^ (pixel * 0xd2b74407b1ce6e93ull) ^ (sample * 0xca5a826395121157ull)};
+    const uint64_t hi = h.next();
+    return (hi << 32) | h.next();
+}
+
+// ---- binary64 helpers on float vectors ------------------------------------
+struct D3 {
+    sf::f64 x, y, z;
+};
+CR_HDI D3 widen(V3 v) { return {sf::from_float(v.x), sf::from_float(v.y), sf::from_float(v.z)}; }
+CR_HDI sf::f64 dat(const D3& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+CR_HDI D3 dadd(const D3& a, const D3& b) { return {sf::add(a.x, b.x), sf::add(a.y, b.y), sf::add(a.z, b.z)}; }
+CR_HDI D3 dsub(const D3& a, const D3& b) { return {sf::sub(a.x, b.x), sf::sub(a.y, b.y), sf::sub(a.z, b.z)}; }
+CR_HDI D3 dscale(const D3& a, sf::f64 s) { return {sf::mul(a.x, s), sf::mul(a.y, s), sf::mul(a.z, s)}; }
+CR_HDI sf::f64 ddot(const D3& a, const D3& b) {
+    return sf::add(sf::add(sf::mul(a.x, b.x), sf::mul(a.y, b.y)), sf::mul(a.z, b.z));
+}
+CR_HDI V3 narrow(const D3& v) { return {sf::to_float(v.x), sf::to_float(v.y), sf::to_float(v.z)}; }
+
+// unit(): double-precision normalization with a fallback for zero/invalid input
+// (reference src/geometry.cpp:31-37).
+CR_HDI V3 dunit(const D3& v, V3 fallback) {
+    const sf::f64 sq = ddot(v, v);
+    if (!sf::gt(sq, sf::zero(0)) || !sf::is_finite(sq)) return fallback;
+    const sf::f64 inv = sf::div(sf::one(), sf::sqrt(sq));
+    return {sf::to_float(sf::mul(v.x, inv)), sf::to_float(sf::mul(v.y, inv)), sf::to_float(sf::mul(v.z, inv))};
+}
+
+// Sign of the binary64 dot product of two float vectors, exactly as the
+// reference evaluates preciseDot(DVec3(a), DVec3(b)).  Float products are
+// exact in binary64, so a float-only filter decides almost every call; ties
+// near zero fall back to the emulated sum.  Returns -1, 0 or +1.
+CR_HDI int dot_sign_exact(V3 a, V3 b) {
+    const float p0 = a.x * b.x, p1 = a.y * b.y, p2 = a.z * b.z;
+    const float s = (p0 + p1) + p2;
+    const float mag = fabsf(p0) + fabsf(p1) + fabsf(p2);
+    if (finite1(s) && finite1(mag) && fabsf(s) > mag * 4.0e-7f && mag > 1.0e-30f) return s > 0 ? 1 : -1;
+    const sf::f64 d = ddot(widen(a), widen(b));
+    if (sf::is_nan(d)) return 0;
+    if (sf::is_zero(d)) return 0;
+    return sf::sign_of(d) ? -1 : 1;
+}
+
+// powerHeuristic in binary64 (reference src/math.h:124-128).
+CR_HDI float power_heuristic(float a, float b) {
+    if (a <= 0) return 0;
+    const sf::f64 da = sf::from_float(a), db = sf::from_float(b);
+    const sf::f64 aa = sf::mul(da, da), bb = sf::mul(db, db);
+    return sf::to_float(sf::div(aa, sf::add(aa, bb)));
+}
+
+}  // namespace cr

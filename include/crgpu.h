@@ -1,0 +1,203 @@
+/*
+ * crgpu.h — C ABI of the MI355X (gfx950) wavefront path tracer.
+ *
+ * This is the drop-in boundary for the reference renderer's hot path:
+ *   RenderResult cr::render(const Scene&, const Bvh&, const Camera&,
+ *                           const Environment&, const RenderOptions&, Progress)
+ *                                                   (reference src/render.h:53)
+ *   bool Bvh::intersect(const Scene&, const Ray&, Hit&, bool any, uint32_t ignore)
+ *                                                   (reference src/scene.h:255-256)
+ *   void Bvh::build(const Scene&)                   (reference src/scene.h:252)
+ *
+ * Every entry point takes plain pointers and sizes, never throws, and returns
+ * an int status (CRG_OK or one of the CRG_E* codes).  The text of the most
+ * recent failure on a context is returned by crg_last_error(), in the same
+ * wording the reference throws (e.g. "Pixel (x,y), sample s: <cause>",
+ * reference src/render.cpp:437-438).
+ *
+ * Descriptor structs below mirror the reference scene model after import
+ * (reference src/scene.h:139-233, src/render.h:8-39) as flat arrays.  The same
+ * descriptors are consumed by the CPU oracle (oracle/) so that every parity
+ * test feeds identical inputs to both sides.
+ */
+#ifndef CRGPU_H
+#define CRGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define CRG_OK        0
+#define CRG_EINVAL    1  /* invalid argument / option validation failure    */
+#define CRG_ENOMEM    2  /* host or device allocation failed (bad_alloc)     */
+#define CRG_EPATH     3  /* non-finite/negative transport or replay failure  */
+#define CRG_EDEVICE   4  /* HIP runtime failure or no usable gfx950 device   */
+#define CRG_ENOTFOUND 5  /* unknown scene / environment / request handle     */
+#define CRG_EBUSY     6  /* admission refused: footprint over budget         */
+
+/* ---- scene model ------------------------------------------------------- */
+/* Texture binding: texture = -1 means "no map" (reference src/scene.h:179-182). */
+typedef struct crg_texture_slot {
+    int32_t texture, texcoord;
+    float offset[2], scale[2], rotation;
+} crg_texture_slot;
+
+/* Material factors (reference src/scene.h:187-199).  alpha_mode: 0 OPAQUE, 1 MASK. */
+typedef struct crg_material {
+    float base_color[4];
+    float metallic, roughness;
+    float emission[3];
+    float transmission, ior;
+    float thickness, attenuation_distance;
+    float attenuation_color[3];
+    int32_t double_sided;
+    int32_t alpha_mode;
+    float alpha_cutoff, normal_scale;
+    crg_texture_slot base_color_tex, metallic_roughness_tex, normal_tex, emissive_tex, transmission_tex;
+} crg_material;
+
+/* Punctual light (reference src/scene.h:208-215). type: 0 point, 1 directional, 2 spot. */
+typedef struct crg_light {
+    int32_t type;
+    float position[3], direction[3], color[3];
+    float intensity, range, inner_cone, outer_cone;
+} crg_light;
+
+/* Decoded image: rgba is width*height*4 floats (reference src/scene.h:176). */
+typedef struct crg_image {
+    int32_t width, height;
+    const float* rgba;
+} crg_image;
+
+/* Sampler binding of an image. wrap: 0 repeat, 1 clamp, 2 mirror (reference src/scene.h:173-177). */
+typedef struct crg_texture {
+    int32_t image, wrap_s, wrap_t, linear;
+} crg_texture;
+
+/*
+ * Triangle soup in world space, before Scene::finalize (reference
+ * src/scene.h:155-163).  positions: 9 floats per triangle (v0,v1,v2).
+ * normals: 9 floats per triangle or NULL (zero normals are repaired by
+ * finalize).  tangents: 12 floats per triangle (xyzw per vertex) or NULL for
+ * (0,0,0,1).  uvs: 12 floats per triangle laid out [vertex][set][uv] or NULL.
+ */
+typedef struct crg_scene_desc {
+    uint64_t triangle_count;
+    const float* positions;
+    const float* normals;
+    const float* tangents;
+    const float* uvs;
+    const uint32_t* material_ids;
+    uint32_t material_count;
+    const crg_material* materials;
+    uint32_t light_count;
+    const crg_light* lights;
+    uint32_t image_count;
+    const crg_image* images;
+    uint32_t texture_count;
+    const crg_texture* textures;
+} crg_scene_desc;
+
+/* Perspective camera (reference src/scene.h:202-207). */
+typedef struct crg_camera {
+    float position[3], forward[3], right[3], up[3];
+    float yfov, aspect, near_clip, far_clip;
+} crg_camera;
+
+/*
+ * Environment (reference src/render.h:8-28).  rgba == NULL selects the
+ * constant environment `constant`; otherwise a width*height lat-long image
+ * with rotation (degrees) and intensity.
+ */
+typedef struct crg_env_desc {
+    float constant[3];
+    int32_t width, height;
+    const float* rgba;
+    float rotation_degrees, intensity;
+} crg_env_desc;
+
+/* Render options (reference src/render.h:31-39). threads is ignored on the GPU. */
+typedef struct crg_options {
+    int32_t width, height, samples, bounces, threads;
+    int32_t next_event_estimation, russian_roulette;
+    uint64_t seed;
+} crg_options;
+
+/* Result statistics (reference src/render.h:41-48 plus device counters). */
+typedef struct crg_stats {
+    double seconds;            /* render time, excluding upload and BVH build  */
+    uint64_t camera_samples;
+    uint64_t boundary_retries; /* precision replays (reference render.cpp:409) */
+    uint64_t extend_rays;      /* closest-hit queries, replays included        */
+    uint64_t shadow_rays;      /* any-hit queries                              */
+    int32_t workers;           /* 0 = device                                   */
+    int32_t iterations;        /* wavefront iterations executed                */
+} crg_stats;
+
+/*
+ * Ray query for the traversal parity harness (reference src/scene.h:139-143):
+ * the ray is origin + origin_low + t*direction on [tmin, tmax]; `ignore` is
+ * the excluded source triangle (0xffffffff = none, reference src/scene.h:255).
+ */
+typedef struct crg_ray {
+    float origin[3], direction[3];
+    float tmin, tmax;
+    float origin_low[3];
+    uint32_t ignore;
+} crg_ray;
+
+/* Closest/any-hit result.  t,u,v are the binary64 values the reference's
+ * Hit holds (reference src/scene.h:166); the device computes them with an
+ * exact integer emulation of binary64 and never executes FP64 instructions. */
+typedef struct crg_hit {
+    uint32_t hit, triangle;
+    double t, u, v;
+} crg_hit;
+
+/* Scene metadata returned by upload. */
+typedef struct crg_scene_info {
+    uint64_t triangles;        /* after finalize (degenerates removed) */
+    uint64_t bvh_nodes;        /* binary SAH nodes built on the host   */
+    uint64_t wide_nodes;       /* 4-wide 128-byte device nodes         */
+    uint64_t device_bytes;     /* resident HBM footprint               */
+    double build_seconds;      /* finalize + SAH build + flatten       */
+    double upload_seconds;
+} crg_scene_info;
+
+typedef struct crg_ctx crg_ctx;
+
+/* ---- context ----------------------------------------------------------- */
+int crg_context_create(int device, crg_ctx** out);
+void crg_context_destroy(crg_ctx* ctx);
+const char* crg_last_error(const crg_ctx* ctx);
+/* Library build identity ("gfx950 ..."), usable without a device. */
+const char* crg_version(void);
+
+/* ---- resident scenes / environments ------------------------------------ */
+/* Scene::finalize + Bvh::build on the host, then upload (reference
+ * src/geometry.cpp:296-397, :463-550).  Returns a resident handle. */
+int crg_scene_upload(crg_ctx* ctx, const crg_scene_desc* desc, uint64_t* scene_id, crg_scene_info* info);
+int crg_scene_release(crg_ctx* ctx, uint64_t scene_id);
+/* Environment construction (reference src/render.cpp:12-42). */
+int crg_env_upload(crg_ctx* ctx, const crg_env_desc* desc, uint64_t* env_id, uint64_t* device_bytes);
+int crg_env_release(crg_ctx* ctx, uint64_t env_id);
+
+/* ---- hot path ---------------------------------------------------------- */
+/* render(): film_rgb receives width*height*3 linear floats
+ * (reference src/render.cpp:352-456). */
+int crg_render(crg_ctx* ctx, uint64_t scene_id, uint64_t env_id, const crg_camera* camera,
+               const crg_options* options, float* film_rgb, crg_stats* stats);
+
+/* Bvh::intersect over a host ray array (reference src/geometry.cpp:554-639).
+ * any != 0 selects visibility queries (reference src/geometry.cpp:608). */
+int crg_trace_rays(crg_ctx* ctx, uint64_t scene_id, const crg_ray* rays, uint64_t count,
+                   crg_hit* hits, int any);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CRGPU_H */

@@ -1,0 +1,211 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the probe interface (probe.h).
+
+`Probe("orc")` loads oracle/liboracle.so, the CPU restatement of the
+reference hot path; `Probe("ref")` loads oracle/_ref/libcrref.so, the
+reference's own sources compiled from where they lie (present only where the
+reference tree was available at build time).  Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg use this module, and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import importlib.util
+import pathlib
+import sys
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+ROOT = HERE.parent
+
+
+def _abi():
+    if "cr_amd" not in sys.modules:
+        pkg = ROOT / "cpu-renderer_amd"
+        spec = importlib.util.spec_from_file_location("cr_amd", pkg / "__init__.py",
+                                                      submodule_search_locations=[str(pkg)])
+        mod = importlib.util.module_from_spec(spec)
+        sys.modules["cr_amd"] = mod
+        spec.loader.exec_module(mod)
+    from cr_amd import abi
+    return abi
+
+
+abi = _abi()
+
+PATHS = {"orc": HERE / "liboracle.so", "ref": HERE / "_ref" / "libcrref.so"}
+
+
+class Surface(C.Structure):
+    _fields_ = [(n, C.c_float * 3) for n in ("position", "position_low", "geometric_normal", "normal", "tangent",
+                                             "bitangent", "base_color", "emission", "attenuation_color")] + \
+               [(n, C.c_float) for n in ("metallic", "roughness", "transmission", "ior", "thickness",
+                                         "attenuation_distance")] + \
+               [("material", C.c_uint32), ("front_face", C.c_int32), ("double_sided", C.c_int32)]
+
+
+class BsdfSample(C.Structure):
+    _fields_ = [("direction", C.c_float * 3), ("weight", C.c_float * 3), ("pdf", C.c_float), ("eta", C.c_float),
+                ("delta", C.c_int32), ("transmitted", C.c_int32), ("valid", C.c_int32)]
+
+
+def available(kind: str) -> bool:
+    return PATHS[kind].exists()
+
+
+class Probe:
+    def __init__(self, kind: str = "orc"):
+        if kind not in PATHS:
+            raise ValueError(kind)
+        if not PATHS[kind].exists():
+            raise FileNotFoundError(f"{PATHS[kind]} is not built")
+        self.kind = kind
+        self.lib = C.CDLL(str(PATHS[kind]))
+        p = kind + "_"
+        vp, u64, f = C.c_void_p, C.c_uint64, C.c_float
+        fp = C.POINTER(C.c_float)
+        sig = {
+            "last_error": (C.c_char_p, []),
+            "sample_seed": (u64, [u64, u64, u64]),
+            "rng_draw": (None, [C.POINTER(u64), C.c_int, fp]),
+            "camera_generate": (C.c_int, [C.POINTER(abi.Camera), f, f, C.c_int, C.c_int, C.POINTER(abi.Ray)]),
+            "scene_create": (vp, [C.POINTER(abi.SceneDesc)]),
+            "scene_destroy": (None, [vp]),
+            "scene_triangles": (u64, [vp]),
+            "scene_nodes": (u64, [vp]),
+            "trace": (C.c_int, [vp, vp, u64, vp, C.c_int]),
+            "offset_ray": (C.c_int, [fp, fp, fp, f, C.POINTER(abi.Ray)]),
+            "surface_at": (C.c_int, [vp, C.POINTER(abi.Ray), C.POINTER(abi.Hit), C.POINTER(Surface)]),
+            "bsdf_sample": (None, [C.POINTER(Surface), fp, C.POINTER(u64), C.POINTER(BsdfSample)]),
+            "bsdf_eval": (None, [C.POINTER(Surface), fp, fp, fp]),
+            "env_create": (vp, [C.POINTER(abi.EnvDesc)]),
+            "env_destroy": (None, [vp]),
+            "env_sample": (None, [vp, C.POINTER(u64), fp, fp]),
+            "env_pdf": (f, [vp, fp]),
+            "env_radiance": (None, [vp, fp, fp]),
+            "render": (C.c_int, [vp, vp, C.POINTER(abi.Camera), C.POINTER(abi.Options), fp, C.POINTER(abi.Stats)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(self.lib, p + name)
+            fn.restype = res
+            fn.argtypes = args
+            setattr(self, "_" + name, fn)
+
+    def error(self) -> str:
+        return self._last_error().decode()
+
+    # --- stateless pieces
+    def sample_seed(self, seed, pixel, sample) -> int:
+        return self._sample_seed(seed, pixel, sample)
+
+    def rng_draw(self, state: int, count: int):
+        st = C.c_uint64(state)
+        out = np.zeros(count, np.float32)
+        self._rng_draw(C.byref(st), count, out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out, st.value
+
+    def camera_generate(self, camera, px, py, width, height):
+        r = abi.Ray()
+        rc = self._camera_generate(C.byref(camera), px, py, width, height, C.byref(r))
+        return (r if rc == 0 else None), (self.error() if rc else "")
+
+    def offset_ray(self, position, normal, direction, max_distance=float("inf")):
+        r = abi.Ray()
+        arr = lambda v: (C.c_float * 3)(*v)
+        rc = self._offset_ray(arr(position), arr(normal), arr(direction), max_distance, C.byref(r))
+        return (r if rc == 0 else None), (self.error() if rc else "")
+
+    def bsdf_sample(self, surface: Surface, wo, state: int):
+        st = C.c_uint64(state)
+        out = BsdfSample()
+        self._bsdf_sample(C.byref(surface), (C.c_float * 3)(*wo), C.byref(st), C.byref(out))
+        return out, st.value
+
+    def bsdf_eval(self, surface: Surface, wo, wi):
+        out = (C.c_float * 4)()
+        self._bsdf_eval(C.byref(surface), (C.c_float * 3)(*wo), (C.c_float * 3)(*wi), out)
+        return np.array(out[:3], np.float32), np.float32(out[3])
+
+    # --- scene / environment objects
+    def scene(self, data) -> "ProbeScene":
+        return ProbeScene(self, data)
+
+    def env(self, data) -> "ProbeEnv":
+        return ProbeEnv(self, data)
+
+
+class ProbeScene:
+    def __init__(self, probe: Probe, data):
+        self.p = probe
+        self.data = data
+        desc = data.descriptor()
+        self.h = probe._scene_create(C.byref(desc))
+        if not self.h:
+            raise RuntimeError(probe.error())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.p._scene_destroy(self.h)
+            self.h = None
+
+    @property
+    def triangles(self) -> int:
+        return self.p._scene_triangles(self.h)
+
+    @property
+    def nodes(self) -> int:
+        return self.p._scene_nodes(self.h)
+
+    def trace(self, rays: np.ndarray, any_hit=False) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, abi.RAY_DTYPE)
+        hits = np.zeros(len(rays), abi.HIT_DTYPE)
+        rc = self.p._trace(self.h, rays.ctypes.data, len(rays), hits.ctypes.data, int(bool(any_hit)))
+        if rc:
+            raise RuntimeError(self.p.error())
+        return hits
+
+    def surface_at(self, ray, hit):
+        s = Surface()
+        rc = self.p._surface_at(self.h, C.byref(ray), C.byref(hit), C.byref(s))
+        if rc:
+            raise RuntimeError(self.p.error())
+        return s
+
+    def render(self, env: "ProbeEnv", camera, options):
+        film = np.zeros((options.height, options.width, 3), np.float32)
+        stats = abi.Stats()
+        rc = self.p._render(self.h, env.h, C.byref(camera), C.byref(options),
+                            film.ctypes.data_as(C.POINTER(C.c_float)), C.byref(stats))
+        if rc:
+            raise RuntimeError(self.p.error())
+        return film, stats
+
+
+class ProbeEnv:
+    def __init__(self, probe: Probe, data):
+        self.p = probe
+        self.data = data
+        desc = data.descriptor()
+        self.h = probe._env_create(C.byref(desc))
+        if not self.h:
+            raise RuntimeError(probe.error())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.p._env_destroy(self.h)
+            self.h = None
+
+    def sample(self, state: int):
+        st = C.c_uint64(state)
+        d = (C.c_float * 3)()
+        pdf = C.c_float()
+        self.p._env_sample(self.h, C.byref(st), d, C.byref(pdf))
+        return np.array(d[:], np.float32), np.float32(pdf.value), st.value
+
+    def pdf(self, direction) -> np.float32:
+        return np.float32(self.p._env_pdf(self.h, (C.c_float * 3)(*direction)))
+
+    def radiance(self, direction):
+        out = (C.c_float * 3)()
+        self.p._env_radiance(self.h, (C.c_float * 3)(*direction), out)
+        return np.array(out[:], np.float32)

@@ -1,0 +1,98 @@
+"""GPU parity against the committed golden vectors (written by the reference).
+
+Traversal: closest-hit triangle ids and hit flags bit-exact, t/u/v equal to
+the reference's binary64 values bit for bit, any-hit flags equal.
+Rendering: per-pixel radiance within 1e-4 relative on >= 99.9% of pixels (the
+rest are counted path divergences from ULP-level libm differences); precision
+replay counts equal.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import abi, load_golden, scenes
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-4          # per-pixel relative radiance tolerance
+PIXEL_FRACTION = 0.999  # fraction of pixels that must be within REL_TOL
+
+
+def _trace_sets():
+    import make_golden  # noqa: F401  (tests/golden is on sys.path via conftest)
+    return make_golden.trace_sets()
+
+
+@pytest.fixture(scope="module")
+def trace_golden():
+    return load_golden("trace.npz")
+
+
+def _sets():
+    import sys
+    from conftest import GOLDEN
+    sys.path.insert(0, str(GOLDEN))
+    import make_golden
+    return make_golden.trace_sets()
+
+
+@pytest.mark.parametrize("case", [s[0] for s in _sets()])
+def test_trace_matches_reference(gpu, trace_golden, case):
+    name, scene, rays, stored = next(s for s in _sets() if s[0] == case)
+    g = trace_golden
+    sid, info = gpu.upload_scene(scene)
+    try:
+        assert info.triangles == int(g[f"{name}/triangles"])
+        hits = gpu.trace(sid, rays, any_hit=False)
+        anyh = gpu.trace(sid, rays, any_hit=True)
+    finally:
+        gpu.release_scene(sid)
+    np.testing.assert_array_equal(hits["hit"], g[f"{name}/hit"], err_msg=f"{name}: hit flags")
+    np.testing.assert_array_equal(hits["triangle"], g[f"{name}/triangle"], err_msg=f"{name}: triangle ids")
+    for k in ("t", "u", "v"):
+        np.testing.assert_array_equal(hits[k].view(np.uint64), g[f"{name}/{k}"].view(np.uint64),
+                                      err_msg=f"{name}: binary64 {k}")
+    np.testing.assert_array_equal(anyh["hit"], g[f"{name}/any"], err_msg=f"{name}: any-hit flags")
+
+
+def _render_cases():
+    import sys
+    from conftest import GOLDEN
+    sys.path.insert(0, str(GOLDEN))
+    import make_golden
+    return make_golden.render_cases()
+
+
+def within(a, b, rel=REL_TOL):
+    """Per-pixel agreement: every channel within `rel` relative (or 1e-7 absolute near zero)."""
+    d = np.abs(a.astype(np.float64) - b.astype(np.float64))
+    ok = (d <= rel * np.abs(b.astype(np.float64))) | (d <= 1e-7)
+    return np.all(ok, axis=-1)
+
+
+@pytest.mark.parametrize("case", [c[0] for c in _render_cases()])
+def test_render_matches_reference(gpu, case):
+    name, scene, env, cam, opt, crop = next(c for c in _render_cases() if c[0] == case)
+    g = load_golden("render.npz")
+    sid, _ = gpu.upload_scene(scene)
+    eid, _ = gpu.upload_env(env)
+    try:
+        res = gpu.render(sid, eid, cam, opt)
+    finally:
+        gpu.release_scene(sid)
+        gpu.release_env(eid)
+    film = res.pixels
+    if crop is not None:
+        y0, y1, x0, x1 = crop
+        film = film[y0:y1, x0:x1]
+    ref = g[f"{name}/film"]
+    ok = within(film, ref)
+    frac = ok.mean()
+    exact = np.mean(np.all(film == ref, axis=-1))
+    print(f"{name}: within 1e-4 {frac * 100:.3f}%  bit-exact {exact * 100:.3f}%  retries {res.boundary_retries}")
+    if ok.size >= 1000:
+        assert frac >= PIXEL_FRACTION, f"{name}: only {frac * 100:.3f}% of pixels within {REL_TOL}"
+    else:
+        assert ok.all(), f"{name}: {film.reshape(-1, 3)[~ok.reshape(-1)]} vs {ref.reshape(-1, 3)[~ok.reshape(-1)]}"
+    assert res.boundary_retries == int(g[f"{name}/retries"])
