@@ -414,6 +414,7 @@ HostEnv build_env(const crg_env_desc& d) {
             cdf.push_back(e.weight_sum);
         }
     }
+    e.cdf = cdf;
     e.pdf.resize(n);
     for (size_t i = 0; i < n; ++i) {
         const int y = int(i / size_t(d.width));

@@ -71,6 +71,7 @@ struct HostEnv {
     std::vector<uint32_t> threshold;
     std::vector<uint32_t> guide;
     std::vector<float> rowcos;
+    std::vector<double> cdf;  // the reference's cumulative weights (host-side checks)
 };
 
 // Environment construction (reference src/render.cpp:12-42, :67-94).

@@ -14,9 +14,8 @@
 // reference's per-pixel double accumulation (reference src/render.cpp:384-415).
 #include <hip/hip_runtime.h>
 
-#include "geom.h"
 #include "kernels.h"
-#include "shade.h"
+#include "path.h"
 
 namespace cr {
 
@@ -106,38 +105,57 @@ __device__ inline uint32_t find_job(const DevJob* jobs, uint32_t njobs, uint64_t
     return lo;
 }
 
-// Camera sample -> path slot (reference src/render.cpp:396-401).  Returns false
-// when the camera cannot form the ray (reported as an error).
-__device__ inline bool init_path(const PathSoA& P, uint32_t slot, const DevJob& J, uint32_t job_index,
-                                 uint64_t local, uint32_t precise, Control* ctl) {
+__device__ inline PathState load_state(const PathSoA& P, uint32_t slot) {
+    PathState st;
+    const float4 ro = P.ray_o[slot], rd = P.ray_d[slot], rl = P.ray_lo[slot];
+    const float4 th = P.thr[slot], ra = P.rad[slot], lp = P.lastpos[slot], ab = P.absorb[slot];
+    st.o = xyz(ro);
+    st.tmin = ro.w;
+    st.d = xyz(rd);
+    st.tmax = rd.w;
+    st.lo = xyz(rl);
+    st.T = xyz(th);
+    st.eta_scale = th.w;
+    st.L = xyz(ra);
+    st.last_pdf = ra.w;
+    st.last_pos = xyz(lp);
+    st.sigma = xyz(ab);
+    st.medium = P.medium[slot];
+    st.flags = P.flags[slot];
+    st.prev = P.prev[slot];
+    st.rng = P.rng[slot];
+    return st;
+}
+
+__device__ inline void store_state(const PathSoA& P, uint32_t slot, const PathState& st) {
+    P.ray_o[slot] = make_float4(st.o.x, st.o.y, st.o.z, st.tmin);
+    P.ray_d[slot] = make_float4(st.d.x, st.d.y, st.d.z, st.tmax);
+    P.ray_lo[slot] = make_float4(st.lo.x, st.lo.y, st.lo.z, 0);
+    P.thr[slot] = make_float4(st.T.x, st.T.y, st.T.z, st.eta_scale);
+    P.rad[slot] = make_float4(st.L.x, st.L.y, st.L.z, st.last_pdf);
+    P.lastpos[slot] = make_float4(st.last_pos.x, st.last_pos.y, st.last_pos.z, 0);
+    P.absorb[slot] = make_float4(st.sigma.x, st.sigma.y, st.sigma.z, 0);
+    P.medium[slot] = st.medium;
+    P.flags[slot] = st.flags;
+    P.prev[slot] = st.prev;
+    P.rng[slot] = st.rng;
+}
+
+// Camera sample `local` of job `ji` into `slot`; false if the camera failed.
+__device__ inline bool init_path(const PathSoA& P, uint32_t slot, const DevJob& J, uint32_t ji, uint64_t local,
+                                 bool precise, Control* ctl) {
     const uint64_t npix = uint64_t(J.width) * uint64_t(J.height);
     const uint32_t sample = uint32_t(local / npix);
     const uint32_t pixel = uint32_t(local - uint64_t(sample) * npix);
-    const int x = int(pixel % uint32_t(J.width)), y = int(pixel / uint32_t(J.width));
-    Rng rng{sample_seed(J.seed, pixel, sample)};
-    const float dx = rng.uniform(), dy = rng.uniform();
-    V3 o, d;
-    float tmin, tmax;
-    const bool ok = camera_ray(J.cam, float(x) + dx, float(y) + dy, o, d, tmin, tmax);
-    P.job[slot] = job_index;
+    PathState st;
+    const bool ok = begin_sample(J, pixel, sample, precise, st);
+    P.job[slot] = ji;
     P.pixel[slot] = pixel;
     P.sample[slot] = sample;
-    P.rng[slot] = rng.state;
-    P.ray_o[slot] = make_float4(o.x, o.y, o.z, tmin);
-    P.ray_d[slot] = make_float4(d.x, d.y, d.z, tmax);
-    P.ray_lo[slot] = make_float4(0, 0, 0, 0);
-    P.thr[slot] = make_float4(1, 1, 1, 1);
-    P.rad[slot] = make_float4(0, 0, 0, 0);
-    P.lastpos[slot] = make_float4(0, 0, 0, 0);
-    P.absorb[slot] = make_float4(0, 0, 0, 0);
-    P.medium[slot] = -1;
-    P.flags[slot] = kLastDelta | (precise ? kPrecise : 0u);
-    P.prev[slot] = kNoTriangle;
-    P.hit[slot] = kNoTriangle;
-    if (!ok) {
-        report_error(ctl, job_index, uint64_t(pixel) * uint64_t(J.spp) + sample, kErrCamera, 0, 0, 0, sf::zero(0), V3{0, 0, 0},
-                     V3{0, 0, 0}, V3{0, 0, 0});
-    }
+    store_state(P, slot, st);
+    if (!ok)
+        report_error(ctl, ji, uint64_t(pixel) * uint64_t(J.spp) + sample, kErrCamera, 0, 0, 0, sf::zero(0),
+                     V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0});
     return ok;
 }
 
@@ -147,11 +165,7 @@ __global__ void __launch_bounds__(256) k_seed(PathSoA P, const DevJob* jobs, uin
     bool queued = false;
     if (slot < pool && uint64_t(slot) < ctl->total) {
         const uint32_t j = find_job(jobs, njobs, slot);
-        queued = init_path(P, slot, jobs[j], j, uint64_t(slot) - jobs[j].item_base, 0, ctl);
-        if (!queued) {
-            // Nothing to trace: park it with zero radiance for retirement.
-            queued = false;
-        }
+        queued = init_path(P, slot, jobs[j], j, uint64_t(slot) - jobs[j].item_base, false, ctl);
     }
     const uint32_t pos = wave_append(&ctl->qcount[0], queued);
     if (queued) q0[pos] = slot;
@@ -177,204 +191,6 @@ __global__ void __launch_bounds__(256) k_extend(PathSoA P, const DevJob* jobs, c
 }
 
 // ------------------------------------------------------------------ shade
-enum Action : uint32_t { kContinue = 0, kRetire = 1, kReplay = 2 };
-
-struct ShadowOut {
-    bool emit;
-    V3 o, d, lo, c;
-    float tmax;
-    uint32_t ignore;
-};
-
-// One bounce of the reference's trace() loop (reference src/render.cpp:240-344)
-// for the path in `slot`, whose closest hit is already in P.hit.
-__device__ inline uint32_t shade_path(const PathSoA& P, const DevJob* jobs, uint32_t slot, ShadowOut& sh,
-                                      Control* ctl) {
-    sh.emit = false;
-    const uint32_t ji = P.job[slot];
-    const DevJob& J = jobs[ji];
-    const DevScene& sc = *J.scene;
-    const DevEnv& env = *J.env;
-    const float4 ro = P.ray_o[slot], rd = P.ray_d[slot], rl = P.ray_lo[slot];
-    const V3 ray_o = xyz(ro), ray_d = xyz(rd), ray_lo = xyz(rl);
-    float4 thr4 = P.thr[slot], rad4 = P.rad[slot];
-    V3 T = xyz(thr4), L = xyz(rad4);
-    float eta_scale = thr4.w, last_pdf = rad4.w;
-    uint32_t flags = P.flags[slot];
-    const uint32_t depth = flags & kDepthMask;
-    const bool last_delta = (flags & kLastDelta) != 0, precise = (flags & kPrecise) != 0;
-    int32_t medium = P.medium[slot];
-    V3 sigma = xyz(P.absorb[slot]);
-    const V3 last_pos = xyz(P.lastpos[slot]);
-    const uint32_t prim = P.hit[slot];
-    const bool nee = J.nee != 0;
-    const uint64_t key = uint64_t(P.pixel[slot]) * uint64_t(J.spp) + P.sample[slot];
-
-    if (prim == kNoTriangle - 1) {
-        report_error(ctl, ji, key, kErrStack, depth, 0, 0, sf::zero(0), ray_o, ray_o, ray_d);
-        P.rad[slot] = make_float4(0, 0, 0, last_pdf);
-        return kRetire;
-    }
-    if (prim == kNoTriangle) {
-        if (medium >= 0) T = T * beer_lambert(sigma, kInf);
-        const float w = last_delta || !nee ? 1 : power_heuristic(last_pdf, environment_pdf(J.lights, env, ray_d));
-        L = L + T * env_radiance(env, ray_d) * w;
-        P.rad[slot] = make_float4(L.x, L.y, L.z, last_pdf);
-        return kRetire;
-    }
-    RayQuery q;
-    setup_query(q, ray_o, ray_d, ro.w, rd.w, ray_lo, P.prev[slot]);
-    const ExactHit hit = exact_of(sc, q, prim);
-    const uint32_t tri_id = sc.prims[prim].id;
-    if (medium >= 0) T = T * beer_lambert(sigma, sf::to_float(hit.t));
-    const Surf s = surface_at(sc, prim, ray_d, hit.u, hit.v);
-    if ((s.front || s.double_sided) && max_component(s.emission) > 0) {
-        const float w = last_delta || !nee
-                            ? 1
-                            : power_heuristic(last_pdf, triangle_pdf(sc, J.lights, prim, last_pos, s.position));
-        L = L + T * s.emission * w;
-    }
-    if (int(depth) == J.bounces) {
-        P.rad[slot] = make_float4(L.x, L.y, L.z, last_pdf);
-        return kRetire;
-    }
-    const V3 wo = -ray_d;
-    Rng rng{P.rng[slot]};
-    if (nee) {
-        const LightSample light = sample_light(sc, J.lights, env, s.position, rng);
-        const bool enclosed_infinite = medium >= 0 && s.thickness <= 0 && !finite1(light.distance);
-        if (!enclosed_infinite && light.pdf > 0 && max_component(light.radiance) > 0) {
-            const BsdfEval bsdf = evaluate_bsdf(s, wo, light.direction);
-            if (max_component(bsdf.value) > 0) {
-                V3 so, sd, slo{0, 0, 0};
-                float stmax = kInf;
-                bool ok = true;
-                if (precise) {
-                    so = s.position;
-                    sd = light.direction;
-                    slo = s.position_low;
-                } else {
-                    const PrimAux& ax = sc.aux[prim];
-                    const uint32_t err = offset_ray(s.position, s.ng, V3{ax.ung[0], ax.ung[1], ax.ung[2]},
-                                                    light.direction, kInf, so, sd, stmax);
-                    if (err) {
-                        report_error(ctl, ji, key, err, depth, tri_id, s.material, sf::zero(0), s.position, ray_o, ray_d);
-                        ok = false;
-                    }
-                }
-                if (ok && finite1(light.distance)) {
-                    V3 end = light.target;
-                    if (length_sq(light.target_normal) > 0) {
-                        V3 eo, ed;
-                        float et;
-                        const uint32_t err =
-                            offset_ray(end, light.target_normal, light.target_unormal, -light.direction, kInf, eo, ed, et);
-                        if (err) {
-                            report_error(ctl, ji, key, err, depth, tri_id, s.material, sf::zero(0), end, ray_o, ray_d);
-                            ok = false;
-                        }
-                        end = eo;
-                    }
-                    const V3 delta = end - so;
-                    stmax = length(delta) * (1 - 2e-6f);
-                    sd = normalize(delta);
-                }
-                if (ok) {
-                    const float w = light.delta ? 1 : power_heuristic(light.pdf, bsdf.pdf);
-                    V3 shadow_sigma = sigma;
-                    bool shadow_in_medium = medium >= 0;
-                    if (s.thickness > 0 && dot(wo, s.ng) * dot(light.direction, s.ng) < 0) {
-                        shadow_in_medium = s.front;
-                        const DevMaterial& m = sc.materials[s.material];
-                        shadow_sigma = s.front ? V3{m.sigma[0], m.sigma[1], m.sigma[2]} : V3{0, 0, 0};
-                    }
-                    const V3 tr = shadow_in_medium ? beer_lambert(shadow_sigma, light.distance) : v3(1);
-                    sh.emit = true;
-                    sh.o = so;
-                    sh.d = sd;
-                    sh.lo = slo;
-                    sh.tmax = stmax;
-                    sh.ignore = precise ? tri_id : kNoTriangle;
-                    sh.c = T * bsdf.value * light.radiance * tr * (fabsf(dot(s.normal, light.direction)) * w / light.pdf);
-                }
-            }
-        }
-    }
-    const BsdfSample smp = sample_bsdf(s, wo, rng);
-    if (!sample_ok(smp)) {
-        P.rad[slot] = make_float4(L.x, L.y, L.z, last_pdf);
-        return kRetire;
-    }
-    T = T * smp.weight;
-    if (!finite(T)) {
-        report_error(ctl, ji, key, kErrNonFiniteThroughput, depth, tri_id, s.material, sf::zero(0), s.position, ray_o, ray_d);
-        sh.emit = false;
-        P.rad[slot] = make_float4(0, 0, 0, last_pdf);
-        return kRetire;
-    }
-    if (smp.transmitted && s.thickness > 0) {
-        uint32_t boundary = 0;
-        if (s.front) {
-            if (medium >= 0) boundary = kErrNested;
-            else {
-                medium = int32_t(s.material);
-                const DevMaterial& m = sc.materials[s.material];
-                sigma = V3{m.sigma[0], m.sigma[1], m.sigma[2]};
-            }
-        } else {
-            if (medium < 0) boundary = kErrExitWithoutEntry;
-            else {
-                medium = -1;
-                sigma = V3{0, 0, 0};
-            }
-        }
-        if (boundary) {
-            sh.emit = false;
-            if (!precise) return kReplay;  // precision replay of the whole sample
-            report_error(ctl, ji, key, boundary, depth, tri_id, s.material, hit.t, s.position, ray_o,
-                         ray_d);
-            P.rad[slot] = make_float4(0, 0, 0, last_pdf);
-            return kRetire;
-        }
-        eta_scale *= smp.eta * smp.eta;
-    }
-    V3 no, nd, nlo{0, 0, 0};
-    float ntmax = kInf;
-    if (precise) {
-        no = s.position;
-        nd = smp.direction;
-        nlo = s.position_low;
-    } else {
-        const PrimAux& ax = sc.aux[prim];
-        const uint32_t err =
-            offset_ray(s.position, s.ng, V3{ax.ung[0], ax.ung[1], ax.ung[2]}, smp.direction, kInf, no, nd, ntmax);
-        if (err) {
-            report_error(ctl, ji, key, err, depth, tri_id, s.material, sf::zero(0), s.position, ray_o, ray_d);
-            sh.emit = false;
-            P.rad[slot] = make_float4(0, 0, 0, last_pdf);
-            return kRetire;
-        }
-    }
-    uint32_t action = kContinue;
-    if (J.rr && depth >= 3) {
-        const float survive = clampf(max_component(T) * eta_scale, 0.05f, 0.95f);
-        if (rng.uniform() >= survive) action = kRetire;
-        else T = T * (1 / survive);
-    }
-    P.rng[slot] = rng.state;
-    P.thr[slot] = make_float4(T.x, T.y, T.z, eta_scale);
-    P.rad[slot] = make_float4(L.x, L.y, L.z, smp.pdf);
-    P.lastpos[slot] = make_float4(s.position.x, s.position.y, s.position.z, 0);
-    P.absorb[slot] = make_float4(sigma.x, sigma.y, sigma.z, 0);
-    P.medium[slot] = medium;
-    P.ray_o[slot] = make_float4(no.x, no.y, no.z, 0.0f);
-    P.ray_d[slot] = make_float4(nd.x, nd.y, nd.z, ntmax);
-    P.ray_lo[slot] = make_float4(nlo.x, nlo.y, nlo.z, 0);
-    P.prev[slot] = precise ? tri_id : kNoTriangle;
-    P.flags[slot] = (depth + 1) | (smp.delta ? kLastDelta : 0u) | (precise ? kPrecise : 0u);
-    return action;
-}
-
 __global__ void __launch_bounds__(256) k_shade(PathSoA P, ShadowSoA S, const DevJob* jobs, const uint32_t* queue,
                                                uint32_t* next_queue, uint32_t* retire, int parity, Control* ctl) {
     const uint32_t n = ctl->qcount[parity];
@@ -388,16 +204,28 @@ __global__ void __launch_bounds__(256) k_shade(PathSoA P, ShadowSoA S, const Dev
         sh.emit = false;
         if (live) {
             path = queue[i];
-            action = shade_path(P, jobs, path, sh, ctl);
+            const uint32_t ji = P.job[path];
+            const DevJob& J = jobs[ji];
+            const uint64_t key = uint64_t(P.pixel[path]) * uint64_t(J.spp) + P.sample[path];
+            const uint32_t prim = P.hit[path];
+            PathState st = load_state(P, path);
+            if (prim == kNoTriangle - 1) {
+                report_error(ctl, ji, key, kErrStack, st.flags & kDepthMask, 0, 0, sf::zero(0), st.o, st.o, st.d);
+                st.L = v3(0);
+                action = kRetire;
+            } else {
+                ErrorOut err;
+                action = bounce(J, st, prim, sh, err);
+                if (err.code)
+                    report_error(ctl, ji, key, err.code, err.depth, err.triangle, err.material, err.t, err.point,
+                                 err.origin, err.direction);
+            }
             if (action == kReplay) {
-                const uint32_t ji = P.job[path];
-                const DevJob& J = jobs[ji];
-                const uint64_t npix = uint64_t(J.width) * uint64_t(J.height);
-                const uint64_t local = uint64_t(P.sample[path]) * npix + P.pixel[path];
                 atomicAdd(&ctl->replays, 1ull);
-                init_path(P, path, J, ji, local, 1u, ctl);
+                begin_sample(J, P.pixel[path], P.sample[path], true, st);
                 action = kContinue;
             }
+            store_state(P, path, st);
         }
         const uint32_t spos = wave_append(&ctl->scount, live && sh.emit);
         if (live && sh.emit) {
@@ -456,7 +284,7 @@ __global__ void __launch_bounds__(256) k_retire(PathSoA P, const DevJob* jobs, u
             const float4 L = P.rad[path];
             const V3 l = xyz(L);
             const uint64_t npix = uint64_t(J.width) * uint64_t(J.height);
-            if (!finite(l) || min_component(l) < 0) {
+            if (!radiance_ok(l)) {
                 report_error(ctl, ji, uint64_t(P.pixel[path]) * uint64_t(J.spp) + P.sample[path], kErrBadRadiance,
                              0, 0, 0, sf::zero(0), V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0});
             }
@@ -466,7 +294,7 @@ __global__ void __launch_bounds__(256) k_retire(PathSoA P, const DevJob* jobs, u
         bool queued = false;
         if (live && item < ctl->total) {
             const uint32_t j = find_job(jobs, njobs, item);
-            queued = init_path(P, path, jobs[j], j, item - jobs[j].item_base, 0, ctl);
+            queued = init_path(P, path, jobs[j], j, item - jobs[j].item_base, false, ctl);
         }
         if (live) atomicAdd(&ctl->retired, 1ull);
         const uint32_t qpos = wave_append(&ctl->qcount[parity ^ 1], queued);

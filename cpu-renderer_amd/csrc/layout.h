@@ -14,6 +14,13 @@
 
 #if defined(__HIPCC__) || defined(__HIP__)
 #include <hip/hip_runtime.h>
+#else
+// Plain C++ builds of the device logic (CPU emulation in tests) use this
+// stand-in for HIP's vector type.
+struct alignas(16) float4 {
+    float x, y, z, w;
+};
+inline float4 make_float4(float x, float y, float z, float w) { return float4{x, y, z, w}; }
 #endif
 
 namespace cr {

@@ -1,0 +1,359 @@
+// TEST INFRASTRUCTURE: CPU emulation of the gfx950 path.
+//
+// Compiles the device-side sources (path.h, geom.h, shade.h, sf64.h) and the
+// host scene preparation (host_scene.cpp) with a plain C++ compiler and runs
+// the exact per-path functions the kernels run: begin_sample -> [traverse
+// closest -> bounce -> traverse any for the shadow ray] ... -> accumulate in
+// emulated binary64.  Used by the CPU test-suite to check the device logic
+// against the oracle without a GPU, and to reproduce device behaviour on the
+// host when a GPU result needs explaining.
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../cpu-renderer_amd/csrc/host_scene.h"
+#include "../../cpu-renderer_amd/csrc/path.h"
+
+using namespace cr;
+
+namespace {
+
+thread_local std::string lastError;
+
+struct EmuScene {
+    HostScene hs;
+    WideBvh bvh;
+    std::vector<PrimRec> prims;
+    std::vector<float4> png;
+    std::vector<PrimAux> aux;
+    std::vector<DevMaterial> mats;
+    std::vector<uint32_t> ranks, tri_to_prim;
+    SceneLights lights;
+    DevScene dev{};
+};
+
+struct EmuEnv {
+    HostEnv he;
+    std::vector<float4> texels;
+    DevEnv dev{};
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* emu_last_error(void) { return lastError.c_str(); }
+
+void* emu_scene_create(const crg_scene_desc* d) {
+    try {
+        auto s = std::make_unique<EmuScene>();
+        s->hs = finalize_scene(*d);
+        s->bvh = build_wide_bvh(s->hs);
+        const size_t n = s->hs.tris.size();
+        s->tri_to_prim.assign(n, 0);
+        for (size_t i = 0; i < n; ++i) s->tri_to_prim[s->bvh.order[i]] = uint32_t(i);
+        s->prims.resize(n);
+        s->png.resize(n);
+        s->aux.resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            const HTriangle& t = s->hs.tris[s->bvh.order[i]];
+            const crg_material& m = s->hs.materials[t.material];
+            uint32_t flags = 0;
+            if (!m.double_sided && m.transmission <= 0) flags |= kPrimCull;
+            if (m.alpha_mode == 1 && !(m.base_color[3] * 1.0f >= m.alpha_cutoff)) flags |= kPrimNeverHit;
+            s->prims[i] = PrimRec{t.p[0].x, t.p[0].y, t.p[0].z, s->bvh.order[i], t.p[1].x, t.p[1].y, t.p[1].z, flags,
+                                  t.p[2].x, t.p[2].y, t.p[2].z, t.material};
+            s->png[i] = make_float4(t.ng.x, t.ng.y, t.ng.z, 0);
+            PrimAux& a = s->aux[i];
+            const HVec3* nv[3] = {&t.n[0], &t.n[1], &t.n[2]};
+            float* dst[3] = {a.n0, a.n1, a.n2};
+            for (int k = 0; k < 3; ++k) {
+                dst[k][0] = nv[k]->x;
+                dst[k][1] = nv[k]->y;
+                dst[k][2] = nv[k]->z;
+            }
+            a.ng[0] = t.ng.x; a.ng[1] = t.ng.y; a.ng[2] = t.ng.z;
+            a.ung[0] = t.ung.x; a.ung[1] = t.ung.y; a.ung[2] = t.ung.z;
+            a.area = t.area;
+        }
+        s->lights = scene_lights(s->hs);
+        s->ranks = light_ranks(s->lights, s->tri_to_prim, n);
+        s->mats = device_materials(s->hs);
+        s->dev = DevScene{s->bvh.nodes.data(), s->prims.data(), s->png.data(), s->aux.data(), s->mats.data(),
+                          s->ranks.data(), s->lights.punctual.data(), uint32_t(s->bvh.nodes.size()), uint32_t(n),
+                          uint32_t(s->lights.punctual.size()), 0};
+        return s.release();
+    } catch (const std::exception& e) {
+        lastError = e.what();
+        return nullptr;
+    }
+}
+
+void emu_scene_destroy(void* s) { delete static_cast<EmuScene*>(s); }
+uint64_t emu_scene_triangles(void* s) { return static_cast<EmuScene*>(s)->hs.tris.size(); }
+uint64_t emu_scene_wide_nodes(void* s) { return static_cast<EmuScene*>(s)->bvh.nodes.size(); }
+
+// Structural check of the 4-wide tree: every primitive in exactly one leaf,
+// every child box encloses its subtree, and leaves respect the encoding.
+int emu_scene_check(void* sp) {
+    const EmuScene& s = *static_cast<EmuScene*>(sp);
+    const auto& nodes = s.bvh.nodes;
+    const size_t n = s.prims.size();
+    if (n == 0) return nodes.empty() ? 0 : 1;
+    std::vector<int> seen(n, 0);
+    struct Item {
+        uint32_t ref;
+        float lo[3], hi[3];
+    };
+    std::vector<Item> stack{{0, {-INFINITY, -INFINITY, -INFINITY}, {INFINITY, INFINITY, INFINITY}}};
+    while (!stack.empty()) {
+        const Item it = stack.back();
+        stack.pop_back();
+        if (ref_is_leaf(it.ref)) {
+            for (uint32_t i = leaf_first(it.ref); i < leaf_first(it.ref) + leaf_count(it.ref); ++i) {
+                if (i >= n) return 2;
+                ++seen[i];
+                const PrimRec& p = s.prims[i];
+                const float xs[3] = {p.ax, p.bx, p.cx}, ys[3] = {p.ay, p.by, p.cy}, zs[3] = {p.az, p.bz, p.cz};
+                for (int k = 0; k < 3; ++k)
+                    if (xs[k] < it.lo[0] || xs[k] > it.hi[0] || ys[k] < it.lo[1] || ys[k] > it.hi[1] ||
+                        zs[k] < it.lo[2] || zs[k] > it.hi[2])
+                        return 3;
+            }
+            continue;
+        }
+        if (it.ref >= nodes.size()) return 4;
+        const WNode& w = nodes[it.ref];
+        for (int c = 0; c < 4; ++c) {
+            if (w.child[c] == kEmptyChild) continue;
+            Item ch{w.child[c], {w.lox[c], w.loy[c], w.loz[c]}, {w.hix[c], w.hiy[c], w.hiz[c]}};
+            for (int a = 0; a < 3; ++a)
+                if (ch.lo[a] < it.lo[a] || ch.hi[a] > it.hi[a]) return 5;
+            stack.push_back(ch);
+        }
+    }
+    for (size_t i = 0; i < n; ++i)
+        if (seen[i] != 1) return 6;
+    return 0;
+}
+
+void* emu_env_create(const crg_env_desc* d) {
+    try {
+        auto e = std::make_unique<EmuEnv>();
+        e->he = build_env(*d);
+        DevEnv& v = e->dev;
+        v = DevEnv{};
+        v.width = e->he.width;
+        v.height = e->he.height;
+        v.rotation = e->he.rotation;
+        v.intensity = e->he.intensity;
+        std::memcpy(v.constant, e->he.constant, sizeof(v.constant));
+        v.is_image = e->he.is_image;
+        if (e->he.is_image) {
+            const size_t n = size_t(v.width) * v.height;
+            e->texels.resize(n);
+            for (size_t i = 0; i < n; ++i)
+                e->texels[i] = make_float4(e->he.texels[4 * i], e->he.texels[4 * i + 1], e->he.texels[4 * i + 2], 1);
+            v.texels = e->texels.data();
+            v.pdf = e->he.pdf.data();
+            v.threshold = e->he.threshold.data();
+            v.guide = e->he.guide.data();
+            v.rowcos = e->he.rowcos.data();
+        }
+        return e.release();
+    } catch (const std::exception& ex) {
+        lastError = ex.what();
+        return nullptr;
+    }
+}
+void emu_env_destroy(void* e) { delete static_cast<EmuEnv*>(e); }
+
+// Exhaustive check of the integer inversion table: for every 24-bit draw k the
+// device texel must equal the reference's upper_bound over its binary64 CDF
+// (reference src/render.cpp:73-75).  Returns the number of mismatching k.
+uint64_t emu_env_verify_inversion(void* ep) {
+    const EmuEnv& e = *static_cast<EmuEnv*>(ep);
+    if (!e.he.is_image) return 0;
+    const std::vector<double>& cdf = e.he.cdf;
+    uint64_t bad = 0;
+    for (uint32_t k = 0; k < (1u << 24); ++k) {
+        const double pick = double(float(k) * (1.0f / 16777216.0f)) * e.he.weight_sum;
+        const size_t ref = std::min(size_t(std::upper_bound(cdf.begin(), cdf.end(), pick) - cdf.begin()), cdf.size() - 1);
+        if (env_invert(e.dev, k) != ref) ++bad;
+    }
+    return bad;
+}
+void emu_env_sample(void* e, uint64_t* rng, float* dir, float* pdf) {
+    Rng r{*rng};
+    const V3 d = env_sample(static_cast<EmuEnv*>(e)->dev, r, *pdf);
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+    *rng = r.state;
+}
+float emu_env_pdf(void* e, const float* d) { return env_pdf(static_cast<EmuEnv*>(e)->dev, V3{d[0], d[1], d[2]}); }
+void emu_env_radiance(void* e, const float* d, float* rgb) {
+    const V3 r = env_radiance(static_cast<EmuEnv*>(e)->dev, V3{d[0], d[1], d[2]});
+    rgb[0] = r.x; rgb[1] = r.y; rgb[2] = r.z;
+}
+
+int emu_trace(void* sp, const crg_ray* rays, uint64_t n, crg_hit* hits, int any, uint64_t* fallbacks) {
+    const EmuScene& s = *static_cast<EmuScene*>(sp);
+    uint64_t fbt = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const crg_ray& r = rays[i];
+        RayQuery q;
+        setup_query(q, V3{r.origin[0], r.origin[1], r.origin[2]}, V3{r.direction[0], r.direction[1], r.direction[2]},
+                    r.tmin, r.tmax, V3{r.origin_low[0], r.origin_low[1], r.origin_low[2]}, r.ignore);
+        uint32_t fb = 0;
+        ClosestResult b = any ? traverse<true>(s.dev, q, fb) : traverse<false>(s.dev, q, fb);
+        fbt += fb;
+        if (b.overflow) {
+            lastError = "BVH traversal exceeded its bounded stack";
+            return 1;
+        }
+        crg_hit& h = hits[i];
+        h.hit = b.prim != kNoTriangle;
+        h.triangle = h.hit ? b.id : kNoTriangle;
+        h.t = h.u = h.v = 0;
+        if (h.hit) {
+            if (!b.exact) {
+                const ExactHit e = exact_of(s.dev, q, b.prim);
+                b.t = e.t;
+                b.u = e.u;
+                b.v = e.v;
+            }
+            std::memcpy(&h.t, &b.t.b, 8);
+            std::memcpy(&h.u, &b.u.b, 8);
+            std::memcpy(&h.v, &b.v.b, 8);
+        }
+    }
+    if (fallbacks) *fallbacks = fbt;
+    return 0;
+}
+
+// Device camera / offset / BSDF pieces for golden comparisons.
+int emu_camera_generate(const crg_camera* c, float px, float py, int w, int h, crg_ray* out) {
+    try {
+        const DevCamera cam = prepare_camera(*c, w, h);
+        V3 o, d;
+        float tmin, tmax;
+        if (!camera_ray(cam, px, py, o, d, tmin, tmax)) throw std::invalid_argument("Unrepresentable perspective ray");
+        std::memset(out, 0, sizeof(*out));
+        out->origin[0] = o.x; out->origin[1] = o.y; out->origin[2] = o.z;
+        out->direction[0] = d.x; out->direction[1] = d.y; out->direction[2] = d.z;
+        out->tmin = tmin;
+        out->tmax = tmax;
+        out->ignore = kNoTriangle;
+        return 0;
+    } catch (const std::exception& e) {
+        lastError = e.what();
+        return 1;
+    }
+}
+
+int emu_render(void* sp, void* ep, const crg_camera* camera, const crg_options* o, float* film, crg_stats* stats) {
+    const EmuScene& s = *static_cast<EmuScene*>(sp);
+    const EmuEnv& env = *static_cast<EmuEnv*>(ep);
+    try {
+        validate_options(*o);
+        DevJob J{};
+        J.scene = &s.dev;
+        J.env = &env.dev;
+        const LightTable lt = build_light_table(s.lights, s.tri_to_prim, env.he.integral);
+        J.lights = DevLights{lt.entries.data(), uint32_t(lt.entries.size()), lt.env_probability};
+        J.cam = prepare_camera(*camera, o->width, o->height);
+        J.width = o->width;
+        J.height = o->height;
+        J.spp = o->samples;
+        J.bounces = o->bounces;
+        J.seed = o->seed;
+        J.nee = o->next_event_estimation;
+        J.rr = o->russian_roulette;
+        const uint32_t npix = uint32_t(o->width) * uint32_t(o->height);
+        std::atomic<uint32_t> next{0};
+        std::atomic<uint64_t> replays{0}, ext{0}, shd{0};
+        std::atomic<uint64_t> best_key{~0ull};
+        std::string err_text;
+        std::mutex mu;
+        auto work = [&] {
+            for (;;) {
+                const uint32_t p = next.fetch_add(1);
+                if (p >= npix) break;
+                sf::f64 acc[3] = {sf::zero(0), sf::zero(0), sf::zero(0)};
+                for (int smp = 0; smp < o->samples; ++smp) {
+                    PathState st;
+                    bool ok = begin_sample(J, p, uint32_t(smp), false, st);
+                    ErrorOut err{};
+                    uint32_t code = ok ? 0 : uint32_t(kErrCamera);
+                    while (ok) {
+                        RayQuery q;
+                        setup_query(q, st.o, st.d, st.tmin, st.tmax, st.lo, st.prev);
+                        uint32_t fb = 0;
+                        const ClosestResult c = traverse<false>(s.dev, q, fb);
+                        ++ext;
+                        if (c.overflow) {
+                            code = kErrStack;
+                            break;
+                        }
+                        ShadowOut sh;
+                        const uint32_t a = bounce(J, st, c.prim, sh, err);
+                        if (err.code) {
+                            code = err.code;
+                            break;
+                        }
+                        if (sh.emit) {
+                            RayQuery sq;
+                            setup_query(sq, sh.o, sh.d, 0, sh.tmax, sh.lo, sh.ignore);
+                            const ClosestResult b = traverse<true>(s.dev, sq, fb);
+                            ++shd;
+                            if (b.prim == kNoTriangle) st.L = st.L + sh.c;
+                        }
+                        if (a == kReplay) {
+                            ++replays;
+                            begin_sample(J, p, uint32_t(smp), true, st);
+                            continue;
+                        }
+                        if (a == kRetire) break;
+                    }
+                    if (!code && !radiance_ok(st.L)) code = kErrBadRadiance;
+                    if (code) {
+                        const uint64_t key = uint64_t(p) * uint64_t(o->samples) + uint64_t(smp);
+                        std::lock_guard<std::mutex> lk(mu);
+                        if (key < best_key.load()) {
+                            best_key = key;
+                            err_text = "Pixel (" + std::to_string(p % uint32_t(o->width)) + "," +
+                                       std::to_string(p / uint32_t(o->width)) + "), sample " + std::to_string(smp) +
+                                       ": code " + std::to_string(code);
+                        }
+                    }
+                    acc[0] = sf::add(acc[0], sf::from_float(st.L.x));
+                    acc[1] = sf::add(acc[1], sf::from_float(st.L.y));
+                    acc[2] = sf::add(acc[2], sf::from_float(st.L.z));
+                }
+                const sf::f64 n = sf::from_i32(o->samples);
+                for (int k = 0; k < 3; ++k) film[3 * size_t(p) + k] = sf::to_float(sf::div(acc[k], n));
+            }
+        };
+        const int workers = int(std::max(1u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (int i = 0; i < workers; ++i) pool.emplace_back(work);
+        for (auto& t : pool) t.join();
+        if (!err_text.empty()) throw std::runtime_error(err_text);
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->camera_samples = uint64_t(npix) * o->samples;
+            stats->boundary_retries = replays.load();
+            stats->extend_rays = ext.load();
+            stats->shadow_rays = shd.load();
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        lastError = e.what();
+        return 1;
+    }
+}
+}
