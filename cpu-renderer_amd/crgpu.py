@@ -53,6 +53,8 @@ def load_library(path: Optional[pathlib.Path] = None) -> C.CDLL:
         "crg_render": (i32, [vp, u64, u64, C.POINTER(abi.Camera), C.POINTER(abi.Options), C.POINTER(C.c_float),
                              C.POINTER(abi.Stats)]),
         "crg_trace_rays": (i32, [vp, u64, vp, u64, vp, i32]),
+        "crg_set_profiling": (i32, [vp, i32]),
+        "crg_kernel_times": (i32, [vp, C.POINTER(abi.KernelTime), i32, C.POINTER(i32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -74,6 +76,11 @@ class RenderResult:
     extend_rays: int
     shadow_rays: int
     iterations: int
+    extend_nodes: int = 0
+    extend_tris: int = 0
+    shadow_nodes: int = 0
+    shadow_tris: int = 0
+    exact_fallbacks: int = 0
 
 
 class Context:
@@ -129,7 +136,17 @@ class Context:
         self._check(self.lib.crg_render(self.handle, scene_id, env_id, C.byref(camera), C.byref(options),
                                         film.ctypes.data_as(C.POINTER(C.c_float)), C.byref(stats)))
         return RenderResult(film, stats.seconds, stats.workers, stats.camera_samples, stats.boundary_retries,
-                            stats.extend_rays, stats.shadow_rays, stats.iterations)
+                            stats.extend_rays, stats.shadow_rays, stats.iterations, stats.extend_nodes,
+                            stats.extend_tris, stats.shadow_nodes, stats.shadow_tris, stats.exact_fallbacks)
+
+    def set_profiling(self, enable: bool):
+        self._check(self.lib.crg_set_profiling(self.handle, int(bool(enable))))
+
+    def kernel_times(self) -> dict:
+        arr = (abi.KernelTime * 16)()
+        n = C.c_int()
+        self._check(self.lib.crg_kernel_times(self.handle, arr, 16, C.byref(n)))
+        return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(min(n.value, 16))}
 
     def trace(self, scene_id: int, rays: np.ndarray, any_hit: bool = False) -> np.ndarray:
         rays = np.ascontiguousarray(rays, abi.RAY_DTYPE)

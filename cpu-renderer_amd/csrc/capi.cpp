@@ -127,8 +127,55 @@ const char* cause_of(uint32_t code) {
 
 }  // namespace
 
+// Event-pair timing of kernel launches on the context stream.
+struct Profiler {
+    static constexpr int kSlots = kStageCount + 2;  // + seed, accumulate
+    static constexpr int kPairs = 256;
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    int stage_of[kPairs] = {};
+    int used = 0;
+    double total_ms[kSlots] = {};
+    uint64_t launches[kSlots] = {};
+
+    ~Profiler() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    void begin(int slot, hipStream_t st) {
+        if (!on) return;
+        if (ev.empty()) {
+            ev.resize(2 * kPairs);
+            for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+        }
+        if (used == kPairs) drain(st);
+        stage_of[used] = slot;
+        check(hipEventRecord(ev[2 * used], st), "hipEventRecord");
+    }
+    void end(hipStream_t st) {
+        if (!on) return;
+        check(hipEventRecord(ev[2 * used + 1], st), "hipEventRecord");
+        ++used;
+    }
+    void drain(hipStream_t st) {
+        if (!used) return;
+        check(hipStreamSynchronize(st), "profiler sync");
+        for (int i = 0; i < used; ++i) {
+            float ms = 0;
+            check(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]), "hipEventElapsedTime");
+            total_ms[stage_of[i]] += ms;
+            ++launches[stage_of[i]];
+        }
+        used = 0;
+    }
+    static const char* name(int slot) {
+        if (slot < kStageCount) return kStageNames[slot];
+        return slot == kStageCount ? "k_seed" : "k_accumulate";
+    }
+};
+
 struct crg_ctx {
     int device = 0;
+    Profiler prof;
     hipStream_t stream = nullptr;
     std::string error;
     std::mutex mu;
@@ -430,29 +477,40 @@ int crg_render(crg_ctx* ctx, uint64_t scene_id, uint64_t env_id, const crg_camer
         Control* dctl = ctx->ctl.as<Control>();
         const DevJob* djobs = ctx->jobs.as<DevJob>();
         const auto start = std::chrono::steady_clock::now();
+        Profiler& prof = ctx->prof;
+        prof.begin(kStageCount, st);
         check(launch_seed(ctx->pool.P, djobs, 1, pool, ctx->pool.q[0], dctl, st), "seed launch");
+        prof.end(st);
         int parity = 0, iterations = 0;
         Control head{};
         const int batch = 4;
         for (;;) {
             for (int k = 0; k < batch; ++k) {
-                check(launch_iteration(ctx->pool.P, ctx->pool.S, djobs, 1, ctx->pool.q, ctx->pool.retire.as<uint32_t>(),
-                                       parity, dctl, ctx->grid, st),
-                      "iteration launch");
+                for (int stage = 0; stage < kStageCount; ++stage) {
+                    prof.begin(stage, st);
+                    check(launch_stage(Stage(stage), ctx->pool.P, ctx->pool.S, djobs, 1, ctx->pool.q,
+                                       ctx->pool.retire.as<uint32_t>(), parity, dctl, ctx->grid, st),
+                          "iteration launch");
+                    prof.end(st);
+                }
                 parity ^= 1;
                 ++iterations;
             }
             check(hipMemcpyAsync(&head, dctl, offsetof(Control, errors), hipMemcpyDeviceToHost, st), "control read");
             check(hipStreamSynchronize(st), "render iteration");
+            prof.drain(st);
             if (head.issued >= head.total && head.qcount[parity] == 0) break;
             // Every path finishes within 2*(bounces+1) iterations of its start (one
             // precision replay at most), so this bound only trips on a logic error.
             const uint64_t bound = (job.items / pool + 2) * uint64_t(2 * (o.bounces + 2)) + 64;
             if (uint64_t(iterations) > bound) throw std::logic_error("render did not converge");
         }
+        prof.begin(kStageCount + 1, st);
         check(launch_accumulate(djobs, 0, npix, ctx->film.as<float>(), st), "accumulate launch");
+        prof.end(st);
         check(hipMemcpyAsync(film_rgb, ctx->film.p, npix * 3 * sizeof(float), hipMemcpyDeviceToHost, st), "film read");
         check(hipStreamSynchronize(st), "render");
+        prof.drain(st);
         const double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
         check(hipMemcpy(&head, dctl, sizeof(Control), hipMemcpyDeviceToHost), "control read");
         if (head.err_count) {
@@ -471,6 +529,34 @@ int crg_render(crg_ctx* ctx, uint64_t scene_id, uint64_t env_id, const crg_camer
             stats->shadow_rays = head.shadow_rays;
             stats->workers = 0;
             stats->iterations = iterations;
+            stats->extend_nodes = head.extend_nodes;
+            stats->extend_tris = head.extend_tris;
+            stats->shadow_nodes = head.shadow_nodes;
+            stats->shadow_tris = head.shadow_tris;
+            stats->exact_fallbacks = head.fallbacks;
+        }
+    });
+}
+
+int crg_set_profiling(crg_ctx* ctx, int enable) {
+    return guarded(ctx, [&] {
+        ctx->prof.on = enable != 0;
+        if (enable) {
+            std::fill(std::begin(ctx->prof.total_ms), std::end(ctx->prof.total_ms), 0.0);
+            std::fill(std::begin(ctx->prof.launches), std::end(ctx->prof.launches), 0ull);
+        }
+    });
+}
+
+int crg_kernel_times(crg_ctx* ctx, crg_kernel_time* out, int max, int* count) {
+    return guarded(ctx, [&] {
+        const int n = Profiler::kSlots;
+        if (count) *count = n;
+        for (int i = 0; i < n && i < max && out; ++i) {
+            std::memset(&out[i], 0, sizeof(out[i]));
+            std::strncpy(out[i].name, Profiler::name(i), sizeof(out[i].name) - 1);
+            out[i].launches = ctx->prof.launches[i];
+            out[i].total_ms = ctx->prof.total_ms[i];
         }
     });
 }

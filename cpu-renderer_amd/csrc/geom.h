@@ -241,13 +241,19 @@ CR_HDI void take_exact(ClosestResult& best, uint32_t prim, uint32_t id, const Ex
     best.thi = next_up(tf);
 }
 
+// Per-query work counters (wide nodes visited, triangles tested, exact fallbacks).
+struct TraceCount {
+    uint32_t nodes, tris, fallbacks;
+};
+
 template <bool ANY>
 CR_HD inline bool leaf_prims(const DevScene& s, RayQuery& q, uint32_t first, uint32_t count, ClosestResult& best,
-                             uint32_t& fb) {
+                             TraceCount& tc) {
     for (uint32_t i = first; i < first + count; ++i) {
         const PrimRec p = s.prims[i];
         if (p.id == q.ignore) continue;
         const V3 va{p.ax, p.ay, p.az}, vb{p.bx, p.by, p.bz}, vc{p.cx, p.cy, p.cz};
+        ++tc.tris;
         float clo = 0, chi = 0;
         const int fr = fast_triangle(q, va, vb, vc, clo, chi);
         if (fr == kReject) continue;
@@ -272,7 +278,7 @@ CR_HD inline bool leaf_prims(const DevScene& s, RayQuery& q, uint32_t first, uin
             }
         }
         // Undecided in FP32: decide with the binary64 emulation.
-        ++fb;
+        ++tc.fallbacks;
         if (!ANY && !best.exact) {
             const PrimRec bp = s.prims[best.prim];
             const ExactHit e = exact_triangle(q, V3{bp.ax, bp.ay, bp.az}, V3{bp.bx, bp.by, bp.bz},
@@ -294,7 +300,7 @@ CR_HD inline bool leaf_prims(const DevScene& s, RayQuery& q, uint32_t first, uin
 
 // Bvh::intersect on the 4-wide tree (reference src/geometry.cpp:554-639).
 template <bool ANY>
-CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, uint32_t& fallbacks) {
+CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, TraceCount& tc) {
     ClosestResult best;
     best.prim = kNoTriangle;
     best.id = 0;
@@ -303,7 +309,6 @@ CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, uint32_t& fa
     best.t = sf::from_float(q.tmax);
     best.u = best.v = sf::zero(0);
     best.tlo = best.thi = q.tmax;
-    uint32_t fb = 0;
     if (!q.valid || s.node_count == 0 || q.tmax < q.tmin) return best;
     StackEntry stack[kStackSize];
     int sp = 0;
@@ -311,6 +316,7 @@ CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, uint32_t& fa
     for (;;) {
         if (!ref_is_leaf(ref)) {
             const WNode& n = s.nodes[ref];
+            ++tc.nodes;
             float nears[4];
             uint32_t mask = node_hit(q, n, best.thi, nears);
             int order[4];
@@ -339,7 +345,7 @@ CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, uint32_t& fa
                 continue;
             }
         } else {
-            if (leaf_prims<ANY>(s, q, leaf_first(ref), leaf_count(ref), best, fb)) break;
+            if (leaf_prims<ANY>(s, q, leaf_first(ref), leaf_count(ref), best, tc)) break;
         }
         // pop the next entry that may still hold a nearer hit
         bool found = false;
@@ -352,7 +358,6 @@ CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, uint32_t& fa
         }
         if (!found) break;
     }
-    fallbacks += fb;
     return best;
 }
 

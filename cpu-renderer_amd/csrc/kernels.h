@@ -24,9 +24,13 @@ hipError_t launch_trace_rays(const DevScene* scene, const RayIn* rays, uint64_t 
                              Control* ctl, hipStream_t st);
 hipError_t launch_seed(const PathSoA& P, const DevJob* jobs, uint32_t njobs, uint32_t pool, uint32_t* q0,
                        Control* ctl, hipStream_t st);
-hipError_t launch_iteration(const PathSoA& P, const ShadowSoA& S, const DevJob* jobs, uint32_t njobs,
-                            uint32_t* const q[2], uint32_t* retire, int parity, Control* ctl, unsigned grid,
-                            hipStream_t st);
+// The five kernels of one wavefront iteration, in launch order.
+enum Stage { kStageExtend = 0, kStageShade, kStageShadow, kStageRetire, kStageReset, kStageCount };
+constexpr const char* kStageNames[kStageCount] = {"k_extend", "k_shade", "k_shadow", "k_retire", "k_reset"};
+
+hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const DevJob* jobs, uint32_t njobs,
+                        uint32_t* const q[2], uint32_t* retire, int parity, Control* ctl, unsigned grid,
+                        hipStream_t st);
 hipError_t launch_accumulate(const DevJob* jobs, uint32_t job, uint64_t npix, float* film, hipStream_t st);
 
 }  // namespace cr

@@ -42,6 +42,13 @@ __device__ inline unsigned long long wave_claim(unsigned long long* counter, boo
     return base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
+// Sum a per-lane counter over the wave and add it once to `dst`.
+__device__ inline void wave_add(unsigned long long* dst, uint32_t v) {
+    unsigned long long s = v;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (__lane_id() == 0 && s) atomicAdd(dst, s);
+}
+
 __device__ inline void report_error(Control* ctl, uint32_t job, uint64_t key, uint32_t code, uint32_t depth,
                                     uint32_t tri, uint32_t material, sf::f64 t, V3 point, V3 origin, V3 dir) {
     const uint32_t slot = atomicAdd(&ctl->err_count, 1u);
@@ -72,8 +79,8 @@ __global__ void __launch_bounds__(256) k_trace_rays(const DevScene* scene, const
     RayQuery q;
     setup_query(q, V3{r.o[0], r.o[1], r.o[2]}, V3{r.d[0], r.d[1], r.d[2]}, r.tmin, r.tmax,
                 V3{r.lo[0], r.lo[1], r.lo[2]}, r.ignore);
-    uint32_t fb = 0;
-    ClosestResult best = any ? traverse<true>(s, q, fb) : traverse<false>(s, q, fb);
+    TraceCount tc{0, 0, 0};
+    ClosestResult best = any ? traverse<true>(s, q, tc) : traverse<false>(s, q, tc);
     HitOut h;
     h.hit = best.prim != kNoTriangle ? 1u : 0u;
     h.triangle = h.hit ? best.id : kNoTriangle;
@@ -91,7 +98,7 @@ __global__ void __launch_bounds__(256) k_trace_rays(const DevScene* scene, const
         h.v = best.v.b;
     }
     hits[i] = h;
-    if (fb) atomicAdd(&ctl->fallbacks, (unsigned long long)fb);
+    if (tc.fallbacks) atomicAdd(&ctl->fallbacks, (unsigned long long)tc.fallbacks);
 }
 
 // ------------------------------------------------------------------ path state
@@ -176,17 +183,19 @@ __global__ void __launch_bounds__(256) k_extend(PathSoA P, const DevJob* jobs, c
                                                 Control* ctl) {
     const uint32_t n = ctl->qcount[parity];
     const uint32_t stride = gridDim.x * blockDim.x;
-    uint32_t fb = 0;
+    TraceCount tc{0, 0, 0};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint32_t path = queue[i];
         const DevScene& s = *jobs[P.job[path]].scene;
         const float4 ro = P.ray_o[path], rd = P.ray_d[path], rl = P.ray_lo[path];
         RayQuery q;
         setup_query(q, xyz(ro), xyz(rd), ro.w, rd.w, xyz(rl), P.prev[path]);
-        const ClosestResult best = traverse<false>(s, q, fb);
+        const ClosestResult best = traverse<false>(s, q, tc);
         P.hit[path] = best.overflow ? kNoTriangle - 1 : best.prim;
     }
-    if (fb) atomicAdd(&ctl->fallbacks, (unsigned long long)fb);
+    wave_add(&ctl->extend_nodes, tc.nodes);
+    wave_add(&ctl->extend_tris, tc.tris);
+    wave_add(&ctl->fallbacks, tc.fallbacks);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->extend_rays, (unsigned long long)n);
 }
 
@@ -246,14 +255,14 @@ __global__ void __launch_bounds__(256) k_shade(PathSoA P, ShadowSoA S, const Dev
 __global__ void __launch_bounds__(256) k_shadow(PathSoA P, ShadowSoA S, const DevJob* jobs, Control* ctl) {
     const uint32_t n = ctl->scount;
     const uint32_t stride = gridDim.x * blockDim.x;
-    uint32_t fb = 0;
+    TraceCount tc{0, 0, 0};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint32_t path = S.path[i];
         const DevScene& s = *jobs[P.job[path]].scene;
         const float4 so = S.o[i], sd = S.d[i], sl = S.lo[i];
         RayQuery q;
         setup_query(q, xyz(so), xyz(sd), so.w, sd.w, xyz(sl), __float_as_uint(sl.w));
-        const ClosestResult b = traverse<true>(s, q, fb);
+        const ClosestResult b = traverse<true>(s, q, tc);
         if (b.prim == kNoTriangle && !b.overflow) {
             const float4 c = S.c[i];
             float4 r = P.rad[path];
@@ -263,7 +272,9 @@ __global__ void __launch_bounds__(256) k_shadow(PathSoA P, ShadowSoA S, const De
             P.rad[path] = r;
         }
     }
-    if (fb) atomicAdd(&ctl->fallbacks, (unsigned long long)fb);
+    wave_add(&ctl->shadow_nodes, tc.nodes);
+    wave_add(&ctl->shadow_tris, tc.tris);
+    wave_add(&ctl->fallbacks, tc.fallbacks);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->shadow_rays, (unsigned long long)n);
 }
 
@@ -344,13 +355,28 @@ hipError_t launch_seed(const PathSoA& P, const DevJob* jobs, uint32_t njobs, uin
     return hipGetLastError();
 }
 
-hipError_t launch_iteration(const PathSoA& P, const ShadowSoA& S, const DevJob* jobs, uint32_t njobs, uint32_t* const q[2],
-                            uint32_t* retire, int parity, Control* ctl, unsigned grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_extend, dim3(grid), dim3(256), 0, st, P, jobs, q[parity], parity, ctl);
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, st, P, S, jobs, q[parity], q[parity ^ 1], retire, parity, ctl);
-    hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(256), 0, st, P, S, jobs, ctl);
-    hipLaunchKernelGGL(k_retire, dim3(grid), dim3(256), 0, st, P, jobs, njobs, retire, q[parity ^ 1], parity, ctl);
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, parity, ctl);
+hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const DevJob* jobs, uint32_t njobs,
+                        uint32_t* const q[2], uint32_t* retire, int parity, Control* ctl, unsigned grid,
+                        hipStream_t st) {
+    switch (stage) {
+        case kStageExtend:
+            hipLaunchKernelGGL(k_extend, dim3(grid), dim3(256), 0, st, P, jobs, q[parity], parity, ctl);
+            break;
+        case kStageShade:
+            hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, st, P, S, jobs, q[parity], q[parity ^ 1], retire,
+                               parity, ctl);
+            break;
+        case kStageShadow:
+            hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(256), 0, st, P, S, jobs, ctl);
+            break;
+        case kStageRetire:
+            hipLaunchKernelGGL(k_retire, dim3(grid), dim3(256), 0, st, P, jobs, njobs, retire, q[parity ^ 1], parity,
+                               ctl);
+            break;
+        default:
+            hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, parity, ctl);
+            break;
+    }
     return hipGetLastError();
 }
 

@@ -136,6 +136,11 @@ typedef struct crg_stats {
     uint64_t shadow_rays;      /* any-hit queries                              */
     int32_t workers;           /* 0 = device                                   */
     int32_t iterations;        /* wavefront iterations executed                */
+    uint64_t extend_nodes;     /* 4-wide nodes visited by closest-hit queries  */
+    uint64_t extend_tris;      /* triangles tested by closest-hit queries      */
+    uint64_t shadow_nodes;     /* 4-wide nodes visited by any-hit queries      */
+    uint64_t shadow_tris;      /* triangles tested by any-hit queries          */
+    uint64_t exact_fallbacks;  /* triangle tests decided in emulated binary64  */
 } crg_stats;
 
 /*
@@ -196,6 +201,20 @@ int crg_render(crg_ctx* ctx, uint64_t scene_id, uint64_t env_id, const crg_camer
  * any != 0 selects visibility queries (reference src/geometry.cpp:608). */
 int crg_trace_rays(crg_ctx* ctx, uint64_t scene_id, const crg_ray* rays, uint64_t count,
                    crg_hit* hits, int any);
+
+/* ---- measurement ------------------------------------------------------- */
+/* Per-kernel device time, measured with HIP events recorded on the context's
+ * stream around every launch while profiling is enabled.  No counterpart in
+ * the reference (it has no device); used by the benchmark's roofline figures. */
+typedef struct crg_kernel_time {
+    char name[32];
+    uint64_t launches;
+    double total_ms;
+} crg_kernel_time;
+
+int crg_set_profiling(crg_ctx* ctx, int enable);
+/* Copies up to `max` entries; *count receives the number of kernels tracked. */
+int crg_kernel_times(crg_ctx* ctx, crg_kernel_time* out, int max, int* count);
 
 #ifdef __cplusplus
 }

@@ -208,9 +208,9 @@ int emu_trace(void* sp, const crg_ray* rays, uint64_t n, crg_hit* hits, int any,
         RayQuery q;
         setup_query(q, V3{r.origin[0], r.origin[1], r.origin[2]}, V3{r.direction[0], r.direction[1], r.direction[2]},
                     r.tmin, r.tmax, V3{r.origin_low[0], r.origin_low[1], r.origin_low[2]}, r.ignore);
-        uint32_t fb = 0;
-        ClosestResult b = any ? traverse<true>(s.dev, q, fb) : traverse<false>(s.dev, q, fb);
-        fbt += fb;
+        TraceCount tc{0, 0, 0};
+        ClosestResult b = any ? traverse<true>(s.dev, q, tc) : traverse<false>(s.dev, q, tc);
+        fbt += tc.fallbacks;
         if (b.overflow) {
             lastError = "BVH traversal exceeded its bounded stack";
             return 1;
@@ -292,8 +292,8 @@ int emu_render(void* sp, void* ep, const crg_camera* camera, const crg_options* 
                     while (ok) {
                         RayQuery q;
                         setup_query(q, st.o, st.d, st.tmin, st.tmax, st.lo, st.prev);
-                        uint32_t fb = 0;
-                        const ClosestResult c = traverse<false>(s.dev, q, fb);
+                        TraceCount tc{0, 0, 0};
+                        const ClosestResult c = traverse<false>(s.dev, q, tc);
                         ++ext;
                         if (c.overflow) {
                             code = kErrStack;
@@ -308,7 +308,7 @@ int emu_render(void* sp, void* ep, const crg_camera* camera, const crg_options* 
                         if (sh.emit) {
                             RayQuery sq;
                             setup_query(sq, sh.o, sh.d, 0, sh.tmax, sh.lo, sh.ignore);
-                            const ClosestResult b = traverse<true>(s.dev, sq, fb);
+                            const ClosestResult b = traverse<true>(s.dev, sq, tc);
                             ++shd;
                             if (b.prim == kNoTriangle) st.L = st.L + sh.c;
                         }
