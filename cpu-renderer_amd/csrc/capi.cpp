@@ -87,10 +87,10 @@ struct Pool {
     uint32_t size = 0;
     DevBuf ray_o, ray_d, ray_lo, thr, rad, lastpos, absorb, rng, job, pixel, sample, medium, flags, prev, hit;
     DevBuf sh_path, sh_o, sh_d, sh_lo, sh_c;
-    DevBuf q0, q1, retire;
+    DevBuf q0, q1, retire, xq, xsq;
     PathSoA P{};
     ShadowSoA S{};
-    uint32_t* q[2] = {nullptr, nullptr};
+    Queues Q{};
 
     void ensure(uint32_t n) {
         if (n <= size) return;
@@ -99,31 +99,18 @@ struct Pool {
         absorb.alloc(f4); rng.alloc(size_t(n) * 8); job.alloc(u32); pixel.alloc(u32); sample.alloc(u32);
         medium.alloc(u32); flags.alloc(u32); prev.alloc(u32); hit.alloc(u32);
         sh_path.alloc(u32); sh_o.alloc(f4); sh_d.alloc(f4); sh_lo.alloc(f4); sh_c.alloc(f4);
-        q0.alloc(u32); q1.alloc(u32); retire.alloc(u32);
+        q0.alloc(u32); q1.alloc(u32); retire.alloc(u32); xq.alloc(u32); xsq.alloc(u32);
         P = PathSoA{ray_o.as<float4>(), ray_d.as<float4>(), ray_lo.as<float4>(), thr.as<float4>(), rad.as<float4>(),
                     lastpos.as<float4>(), absorb.as<float4>(), rng.as<uint64_t>(), job.as<uint32_t>(),
                     pixel.as<uint32_t>(), sample.as<uint32_t>(), medium.as<int32_t>(), flags.as<uint32_t>(),
                     prev.as<uint32_t>(), hit.as<uint32_t>()};
         S = ShadowSoA{sh_path.as<uint32_t>(), sh_o.as<float4>(), sh_d.as<float4>(), sh_lo.as<float4>(),
                       sh_c.as<float4>()};
-        q[0] = q0.as<uint32_t>();
-        q[1] = q1.as<uint32_t>();
+        Q = Queues{{q0.as<uint32_t>(), q1.as<uint32_t>()}, retire.as<uint32_t>(), xq.as<uint32_t>(),
+                   xsq.as<uint32_t>()};
         size = n;
     }
 };
-
-const char* cause_of(uint32_t code) {
-    switch (code) {
-        case kErrNonFiniteThroughput: return "Non-finite path throughput";
-        case kErrBadRadiance: return "Non-finite or negative path radiance";
-        case kErrNested: return "Nested/intersecting refractive volumes are unsupported; separate the closed meshes";
-        case kErrInvalidSecondary: return "Invalid secondary ray";
-        case kErrOffsetOverflow: return "Secondary ray offset exceeds finite coordinates";
-        case kErrCamera: return "Unrepresentable perspective ray";
-        case kErrStack: return "BVH traversal exceeded its bounded stack";
-        default: return "Refractive exit without entry; check outward normals and keep the camera outside closed volumes";
-    }
-}
 
 }  // namespace
 
@@ -214,23 +201,6 @@ int guarded(crg_ctx* ctx, F&& f) {
         ctx->error = e.what();
         return CRG_EINVAL;
     }
-}
-
-std::string format_error(const ErrorRec& e, int width, int spp) {
-    const uint64_t pixel = e.key / uint64_t(spp), sample = e.key % uint64_t(spp);
-    std::ostringstream m;
-    m << "Pixel (" << pixel % uint64_t(width) << "," << pixel / uint64_t(width) << "), sample " << sample << ": "
-      << cause_of(e.code);
-    if (e.code == kErrExitWithoutEntry) {
-        double t;
-        std::memcpy(&t, &e.tbits, 8);
-        m.precision(9);
-        m << "; bounce " << e.depth << ", triangle " << e.triangle << ", material " << e.material << ", t " << t
-          << ", point (" << e.point[0] << ',' << e.point[1] << ',' << e.point[2] << "), origin (" << e.origin[0] << ','
-          << e.origin[1] << ',' << e.origin[2] << "), direction (" << e.direction[0] << ',' << e.direction[1] << ','
-          << e.direction[2] << ')';
-    }
-    return m.str();
 }
 
 }  // namespace
@@ -479,7 +449,7 @@ int crg_render(crg_ctx* ctx, uint64_t scene_id, uint64_t env_id, const crg_camer
         const auto start = std::chrono::steady_clock::now();
         Profiler& prof = ctx->prof;
         prof.begin(kStageCount, st);
-        check(launch_seed(ctx->pool.P, djobs, 1, pool, ctx->pool.q[0], dctl, st), "seed launch");
+        check(launch_seed(ctx->pool.P, djobs, 1, pool, ctx->pool.Q.q[0], dctl, st), "seed launch");
         prof.end(st);
         int parity = 0, iterations = 0;
         Control head{};
@@ -488,8 +458,8 @@ int crg_render(crg_ctx* ctx, uint64_t scene_id, uint64_t env_id, const crg_camer
             for (int k = 0; k < batch; ++k) {
                 for (int stage = 0; stage < kStageCount; ++stage) {
                     prof.begin(stage, st);
-                    check(launch_stage(Stage(stage), ctx->pool.P, ctx->pool.S, djobs, 1, ctx->pool.q,
-                                       ctx->pool.retire.as<uint32_t>(), parity, dctl, ctx->grid, st),
+                    check(launch_stage(Stage(stage), ctx->pool.P, ctx->pool.S, djobs, 1, ctx->pool.Q, parity, dctl,
+                                       ctx->grid, st),
                           "iteration launch");
                     prof.end(st);
                 }
@@ -518,7 +488,7 @@ int crg_render(crg_ctx* ctx, uint64_t scene_id, uint64_t env_id, const crg_camer
             const ErrorRec* best = &head.errors[0];
             for (uint32_t i = 1; i < n; ++i)
                 if (head.errors[i].key < best->key) best = &head.errors[i];
-            throw PathError(format_error(*best, o.width, o.samples));
+            throw PathError(format_path_error(*best, o.width, o.samples));
         }
         if (stats) {
             std::memset(stats, 0, sizeof(*stats));

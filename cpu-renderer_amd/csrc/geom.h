@@ -28,7 +28,7 @@ struct RayQuery {
     float sx, sy, sz;
     float inv[3];
     float slack;    // absolute widening of box intervals for split origins
-    float ovec;     // |olow|_1 * 2: absolute error of vertex offsets in FP32
+    float ovec;     // 2.02*|olow|_inf: bound on the split-origin part of FP32 vertex offsets
     bool precise;   // origin_low != 0
     bool valid;
     bool have_exact;
@@ -76,7 +76,7 @@ CR_HDI void setup_query(RayQuery& q, V3 o, V3 d, float tmin, float tmax, V3 olow
         if (q.precise && c != 0 && finite1(iv)) slack = fmaxf(slack, fabsf(at(olow, a) * iv) * 2.0f);
     }
     q.slack = slack;
-    q.ovec = (fabsf(olow.x) + fabsf(olow.y) + fabsf(olow.z)) * 2.0f;
+    q.ovec = fmaxf(fabsf(olow.x), fmaxf(fabsf(olow.y), fabsf(olow.z))) * 2.02f;
 }
 
 CR_HDI void ensure_exact_shear(RayQuery& q) {
@@ -134,6 +134,20 @@ enum FastResult { kReject = 0, kAccept = 1, kUnsure = 2 };
 
 // FP32 watertight test with a forward error bound.  On kAccept, [tlo, thi]
 // encloses the reference's binary64 t and that t lies certainly in [tmin, tmax].
+//
+// Bound derivation (u = 2^-24; exact quantities are the real-number values of
+// the reference's formulas on the same float inputs; the reference's own
+// binary64 rounding is ~2^-29 of every term below and is covered by the
+// safety factors):
+//   A = V - O            |Af - A|     <= 1.01u|Af| + |olow|        per component
+//   x = A_kx - s*A_kz    |xf - x|     <= E_v = 1.02u(|A_kx|+|A_ky|+3|A_kz|+|x|+|y|) + 2|olow|_inf
+//   w = cx*by - cy*bx    |wf - w|     <= (|cx|+|cy|)E_b + (|bx|+|by|)E_c + 2E_bE_c + 1.01u(|cx by|+|cy bx|+|wf|)
+//   det = w0+w1+w2       |detf - det| <= sum E_w + 2.02u sum|w|
+//   z_v = sz*A_v,kz      |zf - z|     <= 1.01|sz| eA_kz + 2.02u|zf|
+//   N = sum w_v z_v      |Nf - N|     <= sum(|w|E_z + |z|E_w + E_wE_z) + 3.03u sum|w z|
+//   t = N/det            |tf - t|     <= (E_N + |tf|E_det)/(|detf| - E_det) + 1.01u|tf|
+// A sign is certain when |wf| exceeds its bound; the decision then equals the
+// reference's because the binary64 value lies within the same interval.
 CR_HDI int fast_triangle(const RayQuery& q, V3 va, V3 vb, V3 vc, float& tlo, float& thi) {
     const V3 a = va - q.o, b = vb - q.o, c = vc - q.o;
     const float az = at(a, q.kz), bz = at(b, q.kz), cz = at(c, q.kz);
@@ -143,35 +157,40 @@ CR_HDI int fast_triangle(const RayQuery& q, V3 va, V3 vb, V3 vc, float& tlo, flo
     const float w0 = cx * by - cy * bx;
     const float w1 = ax * cy - ay * cx;
     const float w2 = bx * ay - by * ax;
-    // Magnitude bound for every sheared coordinate (|s| <= 1 for sx, sy).
-    const float ma = fabsf(a.x) + fabsf(a.y) + fabsf(a.z);
-    const float mb = fabsf(b.x) + fabsf(b.y) + fabsf(b.z);
-    const float mc = fabsf(c.x) + fabsf(c.y) + fabsf(c.z);
-    const float m = fmaxf(ma, fmaxf(mb, mc)) + q.ovec;
-    const float ev = 8.0f * kU * m + q.ovec;            // error of each sheared coordinate
-    const float ew = (2.0f * m * ev + ev * ev + 4.0f * kU * m * m) * 1.25f;  // error of each w
-    if (!finite1(ew) || !finite1(w0) || !finite1(w1) || !finite1(w2)) return kUnsure;
-    const bool p0 = w0 > ew, p1 = w1 > ew, p2 = w2 > ew;
-    const bool n0 = w0 < -ew, n1 = w1 < -ew, n2 = w2 < -ew;
+    const float k1 = 1.02f * kU;
+    const float ea = k1 * ((fabsf(at(a, q.kx)) + fabsf(at(a, q.ky))) + 3.0f * fabsf(az) + (fabsf(ax) + fabsf(ay))) + q.ovec;
+    const float eb = k1 * ((fabsf(at(b, q.kx)) + fabsf(at(b, q.ky))) + 3.0f * fabsf(bz) + (fabsf(bx) + fabsf(by))) + q.ovec;
+    const float ec = k1 * ((fabsf(at(c, q.kx)) + fabsf(at(c, q.ky))) + 3.0f * fabsf(cz) + (fabsf(cx) + fabsf(cy))) + q.ovec;
+    const float ma = fabsf(ax) + fabsf(ay), mb = fabsf(bx) + fabsf(by), mc = fabsf(cx) + fabsf(cy);
+    const float k2 = 1.01f * kU;
+    const float e0 = (mc * eb + mb * ec + 2.0f * eb * ec + k2 * (fabsf(cx * by) + fabsf(cy * bx) + fabsf(w0))) * 1.02f;
+    const float e1 = (ma * ec + mc * ea + 2.0f * ec * ea + k2 * (fabsf(ax * cy) + fabsf(ay * cx) + fabsf(w1))) * 1.02f;
+    const float e2 = (mb * ea + ma * eb + 2.0f * ea * eb + k2 * (fabsf(bx * ay) + fabsf(by * ax) + fabsf(w2))) * 1.02f;
+    if (!finite1(e0) || !finite1(e1) || !finite1(e2) || !finite1(w0) || !finite1(w1) || !finite1(w2)) return kUnsure;
+    const bool p0 = w0 > e0, p1 = w1 > e1, p2 = w2 > e2;
+    const bool n0 = w0 < -e0, n1 = w1 < -e1, n2 = w2 < -e2;
     if ((p0 || p1 || p2) && (n0 || n1 || n2)) return kReject;  // certainly mixed signs
-    const bool allpos = p0 && p1 && p2, allneg = n0 && n1 && n2;
-    if (!allpos && !allneg) return kUnsure;
+    if (!((p0 && p1 && p2) || (n0 && n1 && n2))) return kUnsure;
+    const float wsum = fabsf(w0) + fabsf(w1) + fabsf(w2);
     const float det = (w0 + w1) + w2;
-    const float edet = 3.0f * ew + 2.0f * kU * fabsf(det);
+    const float edet = (e0 + e1) + e2 + 2.02f * kU * wsum;
     const float adet = fabsf(det);
     if (!(adet > 2.0f * edet)) return kUnsure;
     const float za = q.sz * az, zb = q.sz * bz, zc = q.sz * cz;
     const float num = (w0 * za + w1 * zb) + w2 * zc;
-    const float ez = 3.0f * kU * m * fabsf(q.sz) + fabsf(q.sz) * q.ovec;  // error of each sz*A_kz
-    const float zmax = fmaxf(fabsf(za), fmaxf(fabsf(zb), fabsf(zc)));
-    const float wsum = fabsf(w0) + fabsf(w1) + fabsf(w2);
-    const float enum_ = (wsum * ez + 3.0f * ew * (zmax + ez)) + 4.0f * kU * wsum * zmax;
+    const float asz = fabsf(q.sz) * 1.01f;
+    const float eza = asz * (k2 * fabsf(az) + q.ovec) + 2.02f * kU * fabsf(za);
+    const float ezb = asz * (k2 * fabsf(bz) + q.ovec) + 2.02f * kU * fabsf(zb);
+    const float ezc = asz * (k2 * fabsf(cz) + q.ovec) + 2.02f * kU * fabsf(zc);
+    const float enu = ((fabsf(w0) * eza + fabsf(za) * e0 + e0 * eza) + (fabsf(w1) * ezb + fabsf(zb) * e1 + e1 * ezb) +
+                       (fabsf(w2) * ezc + fabsf(zc) * e2 + e2 * ezc)) +
+                      3.03f * kU * ((fabsf(w0 * za) + fabsf(w1 * zb)) + fabsf(w2 * zc));
     const float t = num / det;
-    if (!finite1(t) || !finite1(enum_)) return kUnsure;
-    const float et = ((enum_ + fabsf(t) * edet) / (adet - edet) + 2.0f * kU * fabsf(t)) * 1.5f + 1e-37f;
+    if (!finite1(t) || !finite1(enu)) return kUnsure;
+    const float et = ((enu + fabsf(t) * edet * 1.01f) / (adet - edet) + 1.01f * kU * fabsf(t)) * 1.05f + 1e-37f;
     tlo = t - et;
     thi = t + et;
-    if (!(tlo == tlo) || !(thi == thi) || !finite1(thi) || !finite1(tlo)) return kUnsure;
+    if (!finite1(thi) || !finite1(tlo)) return kUnsure;
     if (tlo > q.tmax || thi < q.tmin) return kReject;
     if (tlo < q.tmin || thi > q.tmax) return kUnsure;
     return kAccept;
@@ -221,8 +240,9 @@ struct StackEntry {
 struct ClosestResult {
     uint32_t prim;  // leaf-order primitive, kNoTriangle on miss
     uint32_t id;    // finalized triangle index
-    bool exact;     // t,u,v hold the binary64 values
-    bool overflow;  // traversal stack exhausted (reported as an error)
+    bool exact;       // t,u,v hold the binary64 values
+    bool overflow;    // traversal stack exhausted (reported as an error)
+    bool need_exact;  // FP32-only traversal met an undecided test: rerun with EXACT
     sf::f64 t, u, v;
     float tlo, thi;  // FP32 enclosure of the binary64 t (or the query limit)
 };
@@ -246,7 +266,10 @@ struct TraceCount {
     uint32_t nodes, tris, fallbacks;
 };
 
-template <bool ANY>
+// Tests a leaf's primitives.  Returns true when traversal can stop (any-hit
+// found, or an FP32-only traversal needs the exact rerun).  EXACT=false
+// compiles no binary64 emulation at all: undecided tests flag need_exact.
+template <bool ANY, bool EXACT>
 CR_HD inline bool leaf_prims(const DevScene& s, RayQuery& q, uint32_t first, uint32_t count, ClosestResult& best,
                              TraceCount& tc) {
     for (uint32_t i = first; i < first + count; ++i) {
@@ -278,6 +301,11 @@ CR_HD inline bool leaf_prims(const DevScene& s, RayQuery& q, uint32_t first, uin
             }
         }
         // Undecided in FP32: decide with the binary64 emulation.
+        if constexpr (!EXACT) {
+            best.need_exact = true;
+            if (!ANY) return true;  // any-hit keeps looking for a certain blocker
+            continue;
+        } else {
         ++tc.fallbacks;
         if (!ANY && !best.exact) {
             const PrimRec bp = s.prims[best.prim];
@@ -294,18 +322,21 @@ CR_HD inline bool leaf_prims(const DevScene& s, RayQuery& q, uint32_t first, uin
         }
         if (best.prim == kNoTriangle || sf::lt(e.t, best.t) || (sf::eq(e.t, best.t) && p.id < best.id))
             take_exact(best, i, p.id, e);
+        }
     }
     return false;
 }
 
 // Bvh::intersect on the 4-wide tree (reference src/geometry.cpp:554-639).
-template <bool ANY>
+// EXACT=false is the FP32-only fast path (see leaf_prims).
+template <bool ANY, bool EXACT = true>
 CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, TraceCount& tc) {
     ClosestResult best;
     best.prim = kNoTriangle;
     best.id = 0;
     best.exact = true;
     best.overflow = false;
+    best.need_exact = false;
     best.t = sf::from_float(q.tmax);
     best.u = best.v = sf::zero(0);
     best.tlo = best.thi = q.tmax;
@@ -345,7 +376,7 @@ CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, TraceCount& 
                 continue;
             }
         } else {
-            if (leaf_prims<ANY>(s, q, leaf_first(ref), leaf_count(ref), best, tc)) break;
+            if (leaf_prims<ANY, EXACT>(s, q, leaf_first(ref), leaf_count(ref), best, tc)) break;
         }
         // pop the next entry that may still hold a nearer hit
         bool found = false;

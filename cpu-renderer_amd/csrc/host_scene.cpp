@@ -10,6 +10,7 @@
 #include <future>
 #include <limits>
 #include <numeric>
+#include <sstream>
 
 namespace cr {
 namespace {
@@ -559,6 +560,35 @@ DevCamera prepare_camera(const crg_camera& c, int width, int height) {
     cam.width = width;
     cam.height = height;
     return cam;
+}
+
+std::string format_path_error(const ErrorRec& e, int width, int spp) {
+    const uint64_t pixel = e.key / uint64_t(spp), sample = e.key % uint64_t(spp);
+    const char* cause = "";
+    switch (e.code) {
+        case kErrNonFiniteThroughput: cause = "Non-finite path throughput"; break;
+        case kErrBadRadiance: cause = "Non-finite or negative path radiance"; break;
+        case kErrNested: cause = "Nested/intersecting refractive volumes are unsupported; separate the closed meshes"; break;
+        case kErrInvalidSecondary: cause = "Invalid secondary ray"; break;
+        case kErrOffsetOverflow: cause = "Secondary ray offset exceeds finite coordinates"; break;
+        case kErrCamera: cause = "Unrepresentable perspective ray"; break;
+        case kErrStack: cause = "BVH traversal exceeded its bounded stack"; break;
+        default:
+            cause = "Refractive exit without entry; check outward normals and keep the camera outside closed volumes";
+    }
+    std::ostringstream m;
+    m << "Pixel (" << pixel % uint64_t(width) << "," << pixel / uint64_t(width) << "), sample " << sample << ": "
+      << cause;
+    if (e.code == kErrExitWithoutEntry) {
+        double t;
+        std::memcpy(&t, &e.tbits, 8);
+        m.precision(9);
+        m << "; bounce " << e.depth << ", triangle " << e.triangle << ", material " << e.material << ", t " << t
+          << ", point (" << e.point[0] << ',' << e.point[1] << ',' << e.point[2] << "), origin (" << e.origin[0] << ','
+          << e.origin[1] << ',' << e.origin[2] << "), direction (" << e.direction[0] << ',' << e.direction[1] << ','
+          << e.direction[2] << ')';
+    }
+    return m.str();
 }
 
 void validate_options(const crg_options& o) {

@@ -91,6 +91,10 @@ DevCamera prepare_camera(const crg_camera& c, int width, int height);
 // RenderOptions::validate (reference src/render.cpp:97-104).
 void validate_options(const crg_options& o);
 
+// "Pixel (x,y), sample s: <cause>" for a device error record, worded like the
+// reference's rethrown exceptions (reference src/render.cpp:437-438, :310-323).
+std::string format_path_error(const ErrorRec& e, int width, int spp);
+
 // Material table with precomputed absorption (reference src/bsdf.cpp:63-68).
 std::vector<DevMaterial> device_materials(const HostScene& s);
 

@@ -24,13 +24,30 @@ hipError_t launch_trace_rays(const DevScene* scene, const RayIn* rays, uint64_t 
                              Control* ctl, hipStream_t st);
 hipError_t launch_seed(const PathSoA& P, const DevJob* jobs, uint32_t njobs, uint32_t pool, uint32_t* q0,
                        Control* ctl, hipStream_t st);
-// The five kernels of one wavefront iteration, in launch order.
-enum Stage { kStageExtend = 0, kStageShade, kStageShadow, kStageRetire, kStageReset, kStageCount };
-constexpr const char* kStageNames[kStageCount] = {"k_extend", "k_shade", "k_shadow", "k_retire", "k_reset"};
+// The kernels of one wavefront iteration, in launch order.
+enum Stage {
+    kStageExtend = 0,
+    kStageExtendExact,
+    kStageShade,
+    kStageShadow,
+    kStageShadowExact,
+    kStageRetire,
+    kStageReset,
+    kStageCount
+};
+constexpr const char* kStageNames[kStageCount] = {"k_extend", "k_extend_exact", "k_shade", "k_shadow",
+                                                  "k_shadow_exact", "k_retire", "k_reset"};
+
+// Path-id queues of the wavefront loop.
+struct Queues {
+    uint32_t* q[2];   // extend queues (current / next iteration)
+    uint32_t* retire; // finished paths
+    uint32_t* xq;     // extend rays needing the exact traversal
+    uint32_t* xsq;    // shadow records needing the exact traversal
+};
 
 hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const DevJob* jobs, uint32_t njobs,
-                        uint32_t* const q[2], uint32_t* retire, int parity, Control* ctl, unsigned grid,
-                        hipStream_t st);
+                        const Queues& Q, int parity, Control* ctl, unsigned grid, hipStream_t st);
 hipError_t launch_accumulate(const DevJob* jobs, uint32_t job, uint64_t npix, float* film, hipStream_t st);
 
 }  // namespace cr

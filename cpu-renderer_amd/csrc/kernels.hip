@@ -179,24 +179,50 @@ __global__ void __launch_bounds__(256) k_seed(PathSoA P, const DevJob* jobs, uin
 }
 
 // ------------------------------------------------------------------ extend
-__global__ void __launch_bounds__(256) k_extend(PathSoA P, const DevJob* jobs, const uint32_t* queue, int parity,
-                                                Control* ctl) {
+// FP32-only closest hit; rays with an undecided triangle go to the exact queue.
+__global__ void __launch_bounds__(256) k_extend(PathSoA P, const DevJob* jobs, const uint32_t* queue, uint32_t* xq,
+                                                int parity, Control* ctl) {
     const uint32_t n = ctl->qcount[parity];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t rounds = (n + stride - 1) / stride;
+    TraceCount tc{0, 0, 0};
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t r = 0; r < rounds; ++r, i += stride) {
+        bool exact = false;
+        uint32_t path = 0;
+        if (i < n) {
+            path = queue[i];
+            const DevScene& s = *jobs[P.job[path]].scene;
+            const float4 ro = P.ray_o[path], rd = P.ray_d[path], rl = P.ray_lo[path];
+            RayQuery q;
+            setup_query(q, xyz(ro), xyz(rd), ro.w, rd.w, xyz(rl), P.prev[path]);
+            const ClosestResult best = traverse<false, false>(s, q, tc);
+            exact = best.need_exact && !best.overflow;
+            P.hit[path] = best.overflow ? kNoTriangle - 1 : best.prim;
+        }
+        const uint32_t pos = wave_append(&ctl->xcount, exact);
+        if (exact) xq[pos] = path;
+    }
+    wave_add(&ctl->extend_nodes, tc.nodes);
+    wave_add(&ctl->extend_tris, tc.tris);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->extend_rays, (unsigned long long)n);
+}
+
+// Closest hit with binary64-exact resolution of undecided tests.
+__global__ void __launch_bounds__(256) k_extend_exact(PathSoA P, const DevJob* jobs, const uint32_t* xq, Control* ctl) {
+    const uint32_t n = ctl->xcount;
     const uint32_t stride = gridDim.x * blockDim.x;
     TraceCount tc{0, 0, 0};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint32_t path = queue[i];
+        const uint32_t path = xq[i];
         const DevScene& s = *jobs[P.job[path]].scene;
         const float4 ro = P.ray_o[path], rd = P.ray_d[path], rl = P.ray_lo[path];
         RayQuery q;
         setup_query(q, xyz(ro), xyz(rd), ro.w, rd.w, xyz(rl), P.prev[path]);
-        const ClosestResult best = traverse<false>(s, q, tc);
+        const ClosestResult best = traverse<false, true>(s, q, tc);
         P.hit[path] = best.overflow ? kNoTriangle - 1 : best.prim;
     }
-    wave_add(&ctl->extend_nodes, tc.nodes);
-    wave_add(&ctl->extend_tris, tc.tris);
     wave_add(&ctl->fallbacks, tc.fallbacks);
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->extend_rays, (unsigned long long)n);
 }
 
 // ------------------------------------------------------------------ shade
@@ -252,30 +278,63 @@ __global__ void __launch_bounds__(256) k_shade(PathSoA P, ShadowSoA S, const Dev
 }
 
 // ------------------------------------------------------------------ shadow
-__global__ void __launch_bounds__(256) k_shadow(PathSoA P, ShadowSoA S, const DevJob* jobs, Control* ctl) {
+__device__ inline void add_contribution(const PathSoA& P, const ShadowSoA& S, uint32_t i) {
+    const uint32_t path = S.path[i];
+    const float4 c = S.c[i];
+    float4 r = P.rad[path];
+    r.x = r.x + c.x;
+    r.y = r.y + c.y;
+    r.z = r.z + c.z;
+    P.rad[path] = r;
+}
+
+__device__ inline void shadow_query(RayQuery& q, const ShadowSoA& S, uint32_t i) {
+    const float4 so = S.o[i], sd = S.d[i], sl = S.lo[i];
+    setup_query(q, xyz(so), xyz(sd), so.w, sd.w, xyz(sl), __float_as_uint(sl.w));
+}
+
+// FP32-only visibility; undecided segments without a certain blocker go to the exact queue.
+__global__ void __launch_bounds__(256) k_shadow(PathSoA P, ShadowSoA S, const DevJob* jobs, uint32_t* xsq,
+                                                Control* ctl) {
     const uint32_t n = ctl->scount;
     const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t rounds = (n + stride - 1) / stride;
     TraceCount tc{0, 0, 0};
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint32_t path = S.path[i];
-        const DevScene& s = *jobs[P.job[path]].scene;
-        const float4 so = S.o[i], sd = S.d[i], sl = S.lo[i];
-        RayQuery q;
-        setup_query(q, xyz(so), xyz(sd), so.w, sd.w, xyz(sl), __float_as_uint(sl.w));
-        const ClosestResult b = traverse<true>(s, q, tc);
-        if (b.prim == kNoTriangle && !b.overflow) {
-            const float4 c = S.c[i];
-            float4 r = P.rad[path];
-            r.x = r.x + c.x;
-            r.y = r.y + c.y;
-            r.z = r.z + c.z;
-            P.rad[path] = r;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t r = 0; r < rounds; ++r, i += stride) {
+        bool exact = false;
+        if (i < n) {
+            const DevScene& s = *jobs[P.job[S.path[i]]].scene;
+            RayQuery q;
+            shadow_query(q, S, i);
+            const ClosestResult b = traverse<true, false>(s, q, tc);
+            if (b.prim == kNoTriangle && !b.overflow) {
+                if (b.need_exact) exact = true;
+                else add_contribution(P, S, i);
+            }
         }
+        const uint32_t pos = wave_append(&ctl->xscount, exact);
+        if (exact) xsq[pos] = i;
     }
     wave_add(&ctl->shadow_nodes, tc.nodes);
     wave_add(&ctl->shadow_tris, tc.tris);
-    wave_add(&ctl->fallbacks, tc.fallbacks);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->shadow_rays, (unsigned long long)n);
+}
+
+__global__ void __launch_bounds__(256) k_shadow_exact(PathSoA P, ShadowSoA S, const DevJob* jobs, const uint32_t* xsq,
+                                                      Control* ctl) {
+    const uint32_t n = ctl->xscount;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    TraceCount tc{0, 0, 0};
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        const uint32_t i = xsq[j];
+        const DevScene& s = *jobs[P.job[S.path[i]]].scene;
+        RayQuery q;
+        shadow_query(q, S, i);
+        const ClosestResult b = traverse<true, true>(s, q, tc);
+        if (b.prim == kNoTriangle && !b.overflow) add_contribution(P, S, i);
+    }
+    wave_add(&ctl->fallbacks, tc.fallbacks);
 }
 
 // ------------------------------------------------------------------ retire + regenerate
@@ -317,6 +376,8 @@ __global__ void k_reset(int parity, Control* ctl) {
     ctl->qcount[parity] = 0;
     ctl->scount = 0;
     ctl->rcount = 0;
+    ctl->xcount = 0;
+    ctl->xscount = 0;
 }
 
 // ------------------------------------------------------------------ accumulate
@@ -356,22 +417,27 @@ hipError_t launch_seed(const PathSoA& P, const DevJob* jobs, uint32_t njobs, uin
 }
 
 hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const DevJob* jobs, uint32_t njobs,
-                        uint32_t* const q[2], uint32_t* retire, int parity, Control* ctl, unsigned grid,
-                        hipStream_t st) {
+                        const Queues& Q, int parity, Control* ctl, unsigned grid, hipStream_t st) {
     switch (stage) {
         case kStageExtend:
-            hipLaunchKernelGGL(k_extend, dim3(grid), dim3(256), 0, st, P, jobs, q[parity], parity, ctl);
+            hipLaunchKernelGGL(k_extend, dim3(grid), dim3(256), 0, st, P, jobs, Q.q[parity], Q.xq, parity, ctl);
+            break;
+        case kStageExtendExact:
+            hipLaunchKernelGGL(k_extend_exact, dim3(grid / 4 + 1), dim3(256), 0, st, P, jobs, Q.xq, ctl);
             break;
         case kStageShade:
-            hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, st, P, S, jobs, q[parity], q[parity ^ 1], retire,
-                               parity, ctl);
+            hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, st, P, S, jobs, Q.q[parity], Q.q[parity ^ 1],
+                               Q.retire, parity, ctl);
             break;
         case kStageShadow:
-            hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(256), 0, st, P, S, jobs, ctl);
+            hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(256), 0, st, P, S, jobs, Q.xsq, ctl);
+            break;
+        case kStageShadowExact:
+            hipLaunchKernelGGL(k_shadow_exact, dim3(grid / 4 + 1), dim3(256), 0, st, P, S, jobs, Q.xsq, ctl);
             break;
         case kStageRetire:
-            hipLaunchKernelGGL(k_retire, dim3(grid), dim3(256), 0, st, P, jobs, njobs, retire, q[parity ^ 1], parity,
-                               ctl);
+            hipLaunchKernelGGL(k_retire, dim3(grid), dim3(256), 0, st, P, jobs, njobs, Q.retire, Q.q[parity ^ 1],
+                               parity, ctl);
             break;
         default:
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, parity, ctl);

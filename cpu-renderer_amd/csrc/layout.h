@@ -197,7 +197,7 @@ constexpr int kMaxErrors = 64;
 struct Control {
     uint32_t qcount[2];
     uint32_t scount, rcount;
-    uint32_t fetch[2];
+    uint32_t xcount, xscount;  // extend / shadow rays needing the exact traversal
     uint32_t err_count;
     uint32_t pad0;
     unsigned long long issued;

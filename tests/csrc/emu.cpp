@@ -209,8 +209,15 @@ int emu_trace(void* sp, const crg_ray* rays, uint64_t n, crg_hit* hits, int any,
         setup_query(q, V3{r.origin[0], r.origin[1], r.origin[2]}, V3{r.direction[0], r.direction[1], r.direction[2]},
                     r.tmin, r.tmax, V3{r.origin_low[0], r.origin_low[1], r.origin_low[2]}, r.ignore);
         TraceCount tc{0, 0, 0};
-        ClosestResult b = any ? traverse<true>(s.dev, q, tc) : traverse<false>(s.dev, q, tc);
-        fbt += tc.fallbacks;
+        // FP32-only pass first, exact rerun only when flagged (the kernels' order).
+        ClosestResult b = any ? traverse<true, false>(s.dev, q, tc) : traverse<false, false>(s.dev, q, tc);
+        const bool rerun = b.need_exact && (!any || b.prim == kNoTriangle);
+        if (rerun) {
+            setup_query(q, V3{r.origin[0], r.origin[1], r.origin[2]}, V3{r.direction[0], r.direction[1], r.direction[2]},
+                        r.tmin, r.tmax, V3{r.origin_low[0], r.origin_low[1], r.origin_low[2]}, r.ignore);
+            b = any ? traverse<true, true>(s.dev, q, tc) : traverse<false, true>(s.dev, q, tc);
+        }
+        fbt += rerun ? 1 : 0;
         if (b.overflow) {
             lastError = "BVH traversal exceeded its bounded stack";
             return 1;
@@ -293,7 +300,13 @@ int emu_render(void* sp, void* ep, const crg_camera* camera, const crg_options* 
                         RayQuery q;
                         setup_query(q, st.o, st.d, st.tmin, st.tmax, st.lo, st.prev);
                         TraceCount tc{0, 0, 0};
-                        const ClosestResult c = traverse<false>(s.dev, q, tc);
+                        // Same two-phase order as k_extend / k_extend_exact.
+                        ClosestResult c = traverse<false, false>(s.dev, q, tc);
+                        if (c.need_exact && !c.overflow) {
+                            RayQuery qx;
+                            setup_query(qx, st.o, st.d, st.tmin, st.tmax, st.lo, st.prev);
+                            c = traverse<false, true>(s.dev, qx, tc);
+                        }
                         ++ext;
                         if (c.overflow) {
                             code = kErrStack;
@@ -308,7 +321,12 @@ int emu_render(void* sp, void* ep, const crg_camera* camera, const crg_options* 
                         if (sh.emit) {
                             RayQuery sq;
                             setup_query(sq, sh.o, sh.d, 0, sh.tmax, sh.lo, sh.ignore);
-                            const ClosestResult b = traverse<true>(s.dev, sq, tc);
+                            ClosestResult b = traverse<true, false>(s.dev, sq, tc);
+                            if (b.prim == kNoTriangle && b.need_exact) {
+                                RayQuery sx;
+                                setup_query(sx, sh.o, sh.d, 0, sh.tmax, sh.lo, sh.ignore);
+                                b = traverse<true, true>(s.dev, sx, tc);
+                            }
                             ++shd;
                             if (b.prim == kNoTriangle) st.L = st.L + sh.c;
                         }
@@ -325,9 +343,21 @@ int emu_render(void* sp, void* ep, const crg_camera* camera, const crg_options* 
                         std::lock_guard<std::mutex> lk(mu);
                         if (key < best_key.load()) {
                             best_key = key;
-                            err_text = "Pixel (" + std::to_string(p % uint32_t(o->width)) + "," +
-                                       std::to_string(p / uint32_t(o->width)) + "), sample " + std::to_string(smp) +
-                                       ": code " + std::to_string(code);
+                            ErrorRec rec{};
+                            rec.key = key;
+                            rec.code = code;
+                            rec.depth = err.depth;
+                            rec.triangle = err.triangle;
+                            rec.material = err.material;
+                            rec.tbits = err.t.b;
+                            const V3 vs[3] = {err.point, err.origin, err.direction};
+                            float* dst[3] = {rec.point, rec.origin, rec.direction};
+                            for (int k = 0; k < 3; ++k) {
+                                dst[k][0] = vs[k].x;
+                                dst[k][1] = vs[k].y;
+                                dst[k][2] = vs[k].z;
+                            }
+                            err_text = format_path_error(rec, o->width, o->samples);
                         }
                     }
                     acc[0] = sf::add(acc[0], sf::from_float(st.L.x));
