@@ -11,6 +11,7 @@
 // inputs.  The same source compiles for the host, where tests/test_sf64*.py
 // checks it against native double on random and edge-case operands.
 #pragma once
+#include <math.h>
 #include <stdint.h>
 
 #ifndef CR_HD
@@ -169,7 +170,18 @@ CR_HD CR_NOINLINE inline f64 mul(f64 a, f64 b) {
     return round_pack(s, e, sig);
 }
 
-CR_HD CR_NOINLINE inline f64 div(f64 a, f64 b) {
+// Integer -> float conversions used for quotient-digit estimates.
+CR_HDI float u64_to_f(uint64_t x) { return float(x); }
+CR_HDI float i64_to_f(int64_t x) { return float(x); }
+
+// Correctly rounded quotient.  With A in [B, 2B) and B in [2^52, 2^53), the
+// 63-bit quotient Q = floor(A*2^62/B) is produced as three signed digits
+// (23, 23 and 16 bits) from a float reciprocal estimate; each partial
+// remainder R_k = R_{k-1}*2^s - Q_k*B is computed exactly modulo 2^64 (its
+// true value is below 2^58 in magnitude, so wrapping arithmetic is exact),
+// and a final loop brings the remainder into [0, B).  Digit errors are <= 4,
+// <= 17 and <= 2 units, so the loop runs at most a few times.
+CR_HD inline f64 div(f64 a, f64 b) {
     if (is_nan(a) || is_nan(b)) return qnan();
     const uint32_t s = sign_of(a) ^ sign_of(b);
     if (is_inf(a)) return is_inf(b) ? qnan() : inf(s);
@@ -185,21 +197,45 @@ CR_HD CR_NOINLINE inline f64 div(f64 a, f64 b) {
         ma <<= 1;
         --e;
     }
-    // Restoring division: 63 quotient bits with the leading one at bit 62.
-    uint64_t q = 0, r = ma;
-    for (int i = 0; i < 63; ++i) {
-        q <<= 1;
-        if (r >= mb) {
-            r -= mb;
-            q |= 1;
-        }
-        r <<= 1;
+    const float rb = 1.0f / u64_to_f(mb);  // ~2^-53, relative error <= 2^-23
+    // digit 1 (weight 2^39): ~A*2^23/B
+    const int64_t q1 = int64_t(u64_to_f(ma) * (rb * 8388608.0f));
+    int64_t r = int64_t((ma << 23) - uint64_t(q1) * mb);
+    // digit 2 (weight 2^16)
+    const int64_t q2 = int64_t(i64_to_f(r) * (rb * 8388608.0f));
+    r = int64_t((uint64_t(r) << 23) - uint64_t(q2) * mb);
+    // digit 3 (weight 1)
+    int64_t q3 = int64_t(i64_to_f(r) * (rb * 65536.0f));
+    r = int64_t((uint64_t(r) << 16) - uint64_t(q3) * mb);
+    while (r < 0) {
+        --q3;
+        r += int64_t(mb);
     }
-    q |= uint64_t(r != 0);
-    return round_pack(s, e, q);
+    while (r >= int64_t(mb)) {
+        ++q3;
+        r -= int64_t(mb);
+    }
+    const uint64_t q = (uint64_t(q1) << 39) + (uint64_t(q2) << 16) + uint64_t(q3);
+    return round_pack(s, e, q | uint64_t(r != 0));
 }
 
-CR_HD CR_NOINLINE inline f64 sqrt(f64 a) {
+// 128-bit helpers for the square root.
+struct U128 {
+    uint64_t hi, lo;
+};
+CR_HDI U128 sq128(uint64_t x) { return U128{mul_hi(x, x), x * x}; }
+// Signed difference N - X of two 128-bit values whose difference fits in 127 bits.
+CR_HDI void sub128(U128 n, U128 x, int64_t& hi, uint64_t& lo) {
+    lo = n.lo - x.lo;
+    hi = int64_t(n.hi - x.hi - (n.lo < x.lo ? 1 : 0));
+}
+CR_HDI float i128_to_f(int64_t hi, uint64_t lo) { return float(hi) * 18446744073709551616.0f + float(lo); }
+
+// Correctly rounded square root.  For m in [2^52, 2^54) the 55-bit integer root
+// R = floor(sqrt(m*2^56)) is reached from a float sqrt estimate (error < 2^32)
+// by two Newton corrections R += (N - R^2)/(2R) evaluated with float digit
+// estimates on the exact 128-bit remainder, then a final +-1 adjustment.
+CR_HD inline f64 sqrt(f64 a) {
     if (is_nan(a)) return qnan();
     if (is_zero(a)) return a;
     if (sign_of(a)) return qnan();
@@ -211,25 +247,35 @@ CR_HD CR_NOINLINE inline f64 sqrt(f64 a) {
         m <<= 1;
         --e;
     }
-    // m in [2^52, 2^54).  Radicand N = m * 2^56 in [2^108, 2^110); its integer
-    // square root has 55 bits.  Digit-by-digit over the two 64-bit words.
-    const uint64_t nhi = m >> 8, nlo = m << 56;
-    uint64_t root = 0, rem = 0;
-    for (int i = 54; i >= 0; --i) {
-        const int bit = 2 * i;
-        uint64_t two;
-        if (bit >= 64) two = (nhi >> (bit - 64)) & 3;
-        else two = (nlo >> bit) & 3;
-        rem = (rem << 2) | two;
-        const uint64_t trial = (root << 2) | 1;
-        root <<= 1;
-        if (rem >= trial) {
-            rem -= trial;
-            root |= 1;
-        }
+    const U128 n{m >> 8, m << 56};  // N = m * 2^56
+    uint64_t root = uint64_t(sqrtf(u64_to_f(m))) << 28;
+    for (int it = 0; it < 2; ++it) {
+        int64_t dh;
+        uint64_t dl;
+        sub128(n, sq128(root), dh, dl);
+        const float corr = i128_to_f(dh, dl) / (2.0f * u64_to_f(root));
+        root = uint64_t(int64_t(root) + int64_t(corr));
     }
-    // root in [2^54, 2^55): move the leading one to bit 62, jam the remainder.
-    const uint64_t sig = (root << 8) | uint64_t(rem != 0);
+    for (;;) {  // root^2 <= N < (root+1)^2
+        int64_t dh;
+        uint64_t dl;
+        sub128(n, sq128(root), dh, dl);
+        if (dh < 0) {
+            --root;
+            continue;
+        }
+        const uint64_t r1 = root + 1;
+        sub128(n, sq128(r1), dh, dl);
+        if (dh >= 0) {  // (root+1)^2 <= N
+            ++root;
+            continue;
+        }
+        break;
+    }
+    int64_t dh;
+    uint64_t dl;
+    sub128(n, sq128(root), dh, dl);
+    const uint64_t sig = (root << 8) | uint64_t(dh != 0 || dl != 0);
     return round_pack(0, e / 2, sig);
 }
 

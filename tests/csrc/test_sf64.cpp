@@ -63,6 +63,24 @@ int main(int argc, char** argv) {
         const int32_t ii = int32_t(g());
         if (!same(double(ii), sf::from_i32(ii))) report("from_i32", ii, 0, ii, sf::from_i32(ii));
     }
+    // Division / square-root edge families: exact quotients, near-equal
+    // operands, perfect squares and their neighbours.
+    for (long i = 0; i < n / 4; ++i) {
+        const double b = std::ldexp(1.0 + double(g() >> 12) * 0x1p-52, int(g() % 200) - 100);
+        const double q = std::ldexp(double(g() >> (11 + g() % 40)), -int(g() % 30));
+        const double a = q * b;  // often exactly representable
+        const double near = b * (1.0 + std::ldexp(double(int(g() % 9) - 4), -52));
+        const double pairs[3][2] = {{a, b}, {near, b}, {b, near}};
+        for (const auto& pq : pairs) {
+            sf::f64 x{db(pq[0])}, y{db(pq[1])};
+            if (!same(pq[0] / pq[1], sf::div(x, y))) report("div-edge", pq[0], pq[1], pq[0] / pq[1], sf::div(x, y));
+        }
+        const double r = std::ldexp(double((g() >> 38) | 1), int(g() % 100) - 50);
+        for (double v : {r * r, std::nextafter(r * r, 0.0), std::nextafter(r * r, INFINITY)}) {
+            sf::f64 x{db(v)};
+            if (!same(std::sqrt(v), sf::sqrt(x))) report("sqrt-edge", v, 0, std::sqrt(v), sf::sqrt(x));
+        }
+    }
     std::printf("sf64: %ld random cases, %d failures\n", n, failures);
     return failures ? 1 : 0;
 }
