@@ -156,6 +156,7 @@ def main():
         "mrays_per_s": round(rays / elapsed / 1e6, 2),
         "rays_per_sample": round((res.extend_rays + res.shadow_rays) / samples, 3),
         "exact_fallbacks_per_mray": round(pr.exact_fallbacks / max(1, pr.extend_rays + pr.shadow_rays) * 1e6, 2),
+        "exact_rays_per_mray": round(pr.exact_rays / max(1, pr.extend_rays + pr.shadow_rays) * 1e6, 2),
         "roofline": roofline,
         "kernels_ms": kernels,
         "cpu_baseline": cpu,

@@ -113,7 +113,8 @@ class Stats(C.Structure):
     _fields_ = [("seconds", C.c_double), ("camera_samples", C.c_uint64), ("boundary_retries", C.c_uint64),
                 ("extend_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("workers", C.c_int32),
                 ("iterations", C.c_int32), ("extend_nodes", C.c_uint64), ("extend_tris", C.c_uint64),
-                ("shadow_nodes", C.c_uint64), ("shadow_tris", C.c_uint64), ("exact_fallbacks", C.c_uint64)]
+                ("shadow_nodes", C.c_uint64), ("shadow_tris", C.c_uint64), ("exact_fallbacks", C.c_uint64),
+                ("exact_rays", C.c_uint64)]
 
 
 class KernelTime(C.Structure):

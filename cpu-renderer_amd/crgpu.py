@@ -81,6 +81,7 @@ class RenderResult:
     shadow_nodes: int = 0
     shadow_tris: int = 0
     exact_fallbacks: int = 0
+    exact_rays: int = 0
 
 
 class Context:
@@ -137,7 +138,8 @@ class Context:
                                         film.ctypes.data_as(C.POINTER(C.c_float)), C.byref(stats)))
         return RenderResult(film, stats.seconds, stats.workers, stats.camera_samples, stats.boundary_retries,
                             stats.extend_rays, stats.shadow_rays, stats.iterations, stats.extend_nodes,
-                            stats.extend_tris, stats.shadow_nodes, stats.shadow_tris, stats.exact_fallbacks)
+                            stats.extend_tris, stats.shadow_nodes, stats.shadow_tris, stats.exact_fallbacks,
+                            stats.exact_rays)
 
     def set_profiling(self, enable: bool):
         self._check(self.lib.crg_set_profiling(self.handle, int(bool(enable))))

@@ -504,6 +504,7 @@ int crg_render(crg_ctx* ctx, uint64_t scene_id, uint64_t env_id, const crg_camer
             stats->shadow_nodes = head.shadow_nodes;
             stats->shadow_tris = head.shadow_tris;
             stats->exact_fallbacks = head.fallbacks;
+            stats->exact_rays = head.exact_rays;
         }
     });
 }

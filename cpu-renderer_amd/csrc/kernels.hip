@@ -178,6 +178,16 @@ __global__ void __launch_bounds__(256) k_seed(PathSoA P, const DevJob* jobs, uin
     if (queued) q0[pos] = slot;
 }
 
+// Exact-queue indexing: consecutive entries go to different waves, so a wave
+// holds as few undecided rays as the grid allows.  Emulated binary64 runs per
+// lane with divergent control flow; packing those rays into few waves would
+// serialise them.
+__device__ inline uint32_t spread_waves() { return gridDim.x * (blockDim.x / 64); }
+__device__ inline uint32_t spread_first() {
+    return (blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) + spread_waves() * (threadIdx.x % 64);
+}
+__device__ inline uint32_t spread_stride() { return spread_waves() * 64; }
+
 // ------------------------------------------------------------------ extend
 // FP32-only closest hit; rays with an undecided triangle go to the exact queue.
 __global__ void __launch_bounds__(256) k_extend(PathSoA P, const DevJob* jobs, const uint32_t* queue, uint32_t* xq,
@@ -211,9 +221,8 @@ __global__ void __launch_bounds__(256) k_extend(PathSoA P, const DevJob* jobs, c
 // Closest hit with binary64-exact resolution of undecided tests.
 __global__ void __launch_bounds__(256) k_extend_exact(PathSoA P, const DevJob* jobs, const uint32_t* xq, Control* ctl) {
     const uint32_t n = ctl->xcount;
-    const uint32_t stride = gridDim.x * blockDim.x;
     TraceCount tc{0, 0, 0};
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    for (uint32_t i = spread_first(); i < n; i += spread_stride()) {
         const uint32_t path = xq[i];
         const DevScene& s = *jobs[P.job[path]].scene;
         const float4 ro = P.ray_o[path], rd = P.ray_d[path], rl = P.ray_lo[path];
@@ -223,6 +232,7 @@ __global__ void __launch_bounds__(256) k_extend_exact(PathSoA P, const DevJob* j
         P.hit[path] = best.overflow ? kNoTriangle - 1 : best.prim;
     }
     wave_add(&ctl->fallbacks, tc.fallbacks);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->exact_rays, (unsigned long long)n);
 }
 
 // ------------------------------------------------------------------ shade
@@ -324,9 +334,8 @@ __global__ void __launch_bounds__(256) k_shadow(PathSoA P, ShadowSoA S, const De
 __global__ void __launch_bounds__(256) k_shadow_exact(PathSoA P, ShadowSoA S, const DevJob* jobs, const uint32_t* xsq,
                                                       Control* ctl) {
     const uint32_t n = ctl->xscount;
-    const uint32_t stride = gridDim.x * blockDim.x;
     TraceCount tc{0, 0, 0};
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    for (uint32_t j = spread_first(); j < n; j += spread_stride()) {
         const uint32_t i = xsq[j];
         const DevScene& s = *jobs[P.job[S.path[i]]].scene;
         RayQuery q;
@@ -335,6 +344,7 @@ __global__ void __launch_bounds__(256) k_shadow_exact(PathSoA P, ShadowSoA S, co
         if (b.prim == kNoTriangle && !b.overflow) add_contribution(P, S, i);
     }
     wave_add(&ctl->fallbacks, tc.fallbacks);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->exact_rays, (unsigned long long)n);
 }
 
 // ------------------------------------------------------------------ retire + regenerate
@@ -423,7 +433,7 @@ hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const
             hipLaunchKernelGGL(k_extend, dim3(grid), dim3(256), 0, st, P, jobs, Q.q[parity], Q.xq, parity, ctl);
             break;
         case kStageExtendExact:
-            hipLaunchKernelGGL(k_extend_exact, dim3(grid / 4 + 1), dim3(256), 0, st, P, jobs, Q.xq, ctl);
+            hipLaunchKernelGGL(k_extend_exact, dim3(grid), dim3(256), 0, st, P, jobs, Q.xq, ctl);
             break;
         case kStageShade:
             hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, st, P, S, jobs, Q.q[parity], Q.q[parity ^ 1],
@@ -433,7 +443,7 @@ hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const
             hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(256), 0, st, P, S, jobs, Q.xsq, ctl);
             break;
         case kStageShadowExact:
-            hipLaunchKernelGGL(k_shadow_exact, dim3(grid / 4 + 1), dim3(256), 0, st, P, S, jobs, Q.xsq, ctl);
+            hipLaunchKernelGGL(k_shadow_exact, dim3(grid), dim3(256), 0, st, P, S, jobs, Q.xsq, ctl);
             break;
         case kStageRetire:
             hipLaunchKernelGGL(k_retire, dim3(grid), dim3(256), 0, st, P, jobs, njobs, Q.retire, Q.q[parity ^ 1],

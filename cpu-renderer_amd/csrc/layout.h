@@ -207,6 +207,7 @@ struct Control {
     unsigned long long extend_rays;
     unsigned long long shadow_rays;
     unsigned long long fallbacks;
+    unsigned long long exact_rays;
     unsigned long long extend_nodes, extend_tris, shadow_nodes, shadow_tris;
     ErrorRec errors[kMaxErrors];
 };

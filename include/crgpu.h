@@ -141,6 +141,7 @@ typedef struct crg_stats {
     uint64_t shadow_nodes;     /* 4-wide nodes visited by any-hit queries      */
     uint64_t shadow_tris;      /* triangles tested by any-hit queries          */
     uint64_t exact_fallbacks;  /* triangle tests decided in emulated binary64  */
+    uint64_t exact_rays;       /* queries rerun by the exact traversal         */
 } crg_stats;
 
 /*
