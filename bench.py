@@ -14,6 +14,17 @@ a profiled render (HIP events on the library's stream); and the reference CPU
 renderer (compiled from the reference's own sources, or the CPU oracle when
 that build is absent) timed on this host's cores on a bounded sample of the
 same workload (full frame, fewer samples per pixel).
+
+Two systems forms are measured against the plain one-render-at-a-time path on
+the same request mix (films compared bit for bit):
+  batched: 64 seeded requests (terrain / spheres / soup of 1k..2M triangles,
+           128^2..1024^2 px, 4..256 spp, 8..16 bounces) rendered in one shared
+           wavefront after autotuning the launch configuration;
+  served:  200 requests arriving as a Poisson stream at 20 req/s over those 64
+           resident scenes (sizes drawn from the same ranges), continuously
+           batched; p50/p99 time-to-image and aggregate Msamples/s.  The plain
+           path's latencies are those of a FIFO single-render server with the
+           measured plain render times under the same arrivals.
 """
 from __future__ import annotations
 
@@ -23,6 +34,7 @@ import json
 import os
 import pathlib
 import sys
+import threading
 import time
 
 ROOT = pathlib.Path(__file__).resolve().parent
@@ -82,6 +94,154 @@ def cpu_baseline(w, budget_s: float, threads: int) -> dict:
                       f"GPU run's first {spp} samples), render() time excluding BVH build"}
 
 
+def _pct(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q))
+
+
+def quality(ctx) -> dict:
+    """relMSE against the reference's converged film of the box-and-spheres workload
+    (128x128, 4096 spp, other seed, binary64 accumulation): the spp at which the GPU
+    render reaches relMSE <= 1e-3 and the throughput there."""
+    import numpy as np
+    from cr_amd import scenes
+    f = ROOT / "tests" / "golden" / "converged_c1.npz"
+    if not f.exists():
+        return None
+    g = np.load(f, allow_pickle=False)
+    width, height, spp_ref, seed_ref = (int(x) for x in g["meta"])
+    ref = g["film"].astype(np.float64)
+    w = scenes.config1(width=width, height=height)
+    sid, _ = ctx.upload_scene(w.scene)
+    eid, _ = ctx.upload_env(w.env)
+    ladder, hit = {}, None
+    try:
+        for spp in (64, 256, 1024, 2048, 4096):
+            r = ctx.render(sid, eid, w.camera, w.options(samples=spp, seed=0))
+            img = r.pixels.astype(np.float64)
+            e = float(np.mean((img - ref) ** 2 / (ref * ref + 1e-2)))
+            ladder[spp] = round(e, 7)
+            if e <= 1e-3 and hit is None:
+                hit = (spp, width * height * spp / r.seconds / 1e6, (r.extend_rays + r.shadow_rays) / r.seconds / 1e6)
+    finally:
+        ctx.release_scene(sid)
+        ctx.release_env(eid)
+    return {"workload": f"config1 box+spheres {width}x{height}", "reference": f"{spp_ref} spp seed {seed_ref}",
+            "relmse_by_spp": ladder, "spp_at_relmse_1e-3": hit[0] if hit else None,
+            "msamples_per_s_at_target": round(hit[1], 3) if hit else None,
+            "mrays_per_s_at_target": round(hit[2], 2) if hit else None}
+
+
+def batched_form(ctx, count: int, log_sky) -> dict:
+    """64-request batched mix: plain sequential renders vs one shared batch."""
+    import numpy as np
+    from cr_amd import scenes
+    t0 = time.time()
+    sky = scenes.synthetic_sky()
+    mix = [scenes.batch_request(1000 + i, sky=sky) for i in range(count)]
+    gen_s = time.time() - t0
+    env_id, _ = ctx.upload_env(mix[0].env)  # one shared sky
+    res = []
+    build_s = 0.0
+    for w in mix:
+        sid, info = ctx.upload_scene(w.scene)
+        build_s += info.build_seconds + info.upload_seconds
+        res.append((w, sid))
+    tris = sum(int(w.scene.triangle_count) for w in mix)
+    samples = sum(w.camera_samples for w in mix)
+    log(f"batched: {count} scenes, {tris} tris generated in {gen_s:.1f}s, built+uploaded in {build_s:.1f}s, "
+        f"{samples / 1e6:.1f} Msamples")
+    # Autotune on the median-size request at reduced spp (results never depend on it).
+    med = sorted(res, key=lambda r: r[0].width * r[0].height)[count // 2]
+    tuned, rate = ctx.autotune(med[1], env_id, med[0].camera, med[0].options(samples=4))
+    log(f"batched: autotuned {tuned.as_dict()} ({rate / 1e6:.1f} Msamples/s on the sample)")
+    ctx.render(res[0][1], env_id, res[0][0].camera, res[0][0].options(samples=1))  # warm
+    t = time.perf_counter()
+    plain = [ctx.render(sid, env_id, w.camera, w.options()) for w, sid in res]
+    plain_s = time.perf_counter() - t
+    log(f"batched: plain one-at-a-time {plain_s:.2f}s")
+    t = time.perf_counter()
+    batch = ctx.render_batch([(sid, env_id, w.camera, w.options()) for w, sid in res])
+    batch_s = time.perf_counter() - t
+    log(f"batched: one shared wavefront {batch_s:.2f}s")
+    same = sum(1 for a, b in zip(plain, batch)
+               if not isinstance(b, Exception) and np.array_equal(a.pixels.view(np.uint32), b.pixels.view(np.uint32)))
+    rays = sum(b.extend_rays + b.shadow_rays for b in batch if not isinstance(b, Exception))
+    out = {"requests": count, "triangles_total": tris, "msamples": round(samples / 1e6, 2),
+           "msamples_per_s": round(samples / batch_s / 1e6, 3),
+           "plain_msamples_per_s": round(samples / plain_s / 1e6, 3),
+           "vs_plain": round(plain_s / batch_s, 3), "mrays_per_s": round(rays / batch_s / 1e6, 2),
+           "seconds": round(batch_s, 3), "plain_seconds": round(plain_s, 3),
+           "films_bit_identical_to_plain": f"{same}/{count}", "tuning": tuned.as_dict(),
+           "host_build_seconds": round(build_s, 2)}
+    return out, res, env_id
+
+
+def served_form(ctx, catalog, env_id, count: int, rate: float) -> dict:
+    """Poisson request stream over resident scenes, continuously batched."""
+    import numpy as np
+    from cr_amd import scenes
+    arrivals, reqs = scenes.served_stream(count, rate, len(catalog), seed=0)
+    jobs = []
+    for k, side, spp, bounces in reqs:
+        w, sid = catalog[k]
+        jobs.append((sid, w.camera, w.options(width=side, height=side, samples=spp, bounces=bounces)))
+    samples = sum(o.width * o.height * o.samples for _, _, o in jobs)
+    log(f"served: {count} requests at {rate} req/s, {samples / 1e6:.1f} Msamples")
+    # plain: one render at a time, back to back
+    t = time.perf_counter()
+    plain_times, plain_films = [], []
+    for sid, cam, opt in jobs:
+        r = ctx.render(sid, env_id, cam, opt)
+        plain_times.append(r.seconds)
+        plain_films.append(r.pixels)
+    plain_s = time.perf_counter() - t
+    log(f"served: plain one-at-a-time {plain_s:.2f}s")
+    # FIFO single-render server with the plain service times under the same arrivals
+    free_at, plain_lat = 0.0, []
+    for a, s_ in zip(arrivals, plain_times):
+        start = max(a, free_at)
+        free_at = start + s_
+        plain_lat.append(free_at - a)
+    # served: submit at arrival times from a thread, collect in order
+    tickets = [None] * count
+    t_start = time.perf_counter()
+
+    def submitter():
+        for i, (sid, cam, opt) in enumerate(jobs):
+            delay = t_start + arrivals[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets[i] = ctx.submit(sid, env_id, cam, opt)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    lat, same, last_done = [], 0, 0.0
+    for i in range(count):
+        while tickets[i] is None:
+            time.sleep(0.001)
+        tk, film = tickets[i]
+        r = ctx.wait(tk, film)
+        lat.append(r.latency_seconds)
+        last_done = max(last_done, time.perf_counter() - t_start)
+        if np.array_equal(film.view(np.uint32), plain_films[i].view(np.uint32)):
+            same += 1
+        plain_films[i] = None
+        if (i + 1) % 50 == 0:
+            log(f"served: {i + 1}/{count} done")
+    th.join()
+    served_s = last_done
+    return {"requests": count, "rate_req_per_s": rate, "msamples": round(samples / 1e6, 2),
+            "msamples_per_s": round(samples / served_s / 1e6, 3),
+            "plain_msamples_per_s": round(samples / plain_s / 1e6, 3),
+            "vs_plain": round((samples / served_s) / (samples / plain_s), 3),
+            "p50_s": round(_pct(lat, 50), 3), "p99_s": round(_pct(lat, 99), 3),
+            "plain_p50_s": round(_pct(plain_lat, 50), 3), "plain_p99_s": round(_pct(plain_lat, 99), 3),
+            "makespan_s": round(served_s, 3), "plain_seconds": round(plain_s, 3),
+            "films_bit_identical_to_plain": f"{same}/{count}",
+            "scenes": "64 resident (host BVH build outside the latency)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +250,10 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline rendering")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="plain,batched,served", help="comma list of plain,batched,served")
+    ap.add_argument("--batch", type=int, default=64, help="requests in the batched mix")
+    ap.add_argument("--served", type=int, default=200, help="requests in the served stream")
+    ap.add_argument("--rate", type=float, default=20.0, help="served arrival rate (req/s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
@@ -97,6 +261,7 @@ def main():
         sys.exit(2)
     _package()
     from cr_amd import crgpu
+    forms = set(args.forms.split(","))
 
     t0 = time.time()
     w = workload(args.workload)
@@ -135,12 +300,27 @@ def main():
     bytes_per_ray = 48 + 128 * n_node + 48 * n_tri
     achieved = bytes_per_ray * pr.extend_rays / (ext_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": _traffic(w.name),
                 "kernel": "k_extend", "avg_launch_ms": round(ext_ms / max(1, ext_launches), 4),
                 "launches": ext_launches, "bytes_per_ray": round(bytes_per_ray, 1),
                 "nodes_per_ray": round(n_node, 2), "tris_per_ray": round(n_tri, 2)}
     kernels = {k: {"launches": v[0], "ms": round(v[1], 2)} for k, v in kt.items()}
     log(f"kernels: {kernels}")
+    ctx.release_scene(sid)
+    ctx.release_env(eid)
+
+    qual = quality(ctx)
+    log(f"quality: {qual}")
+
+    batched = served = None
+    if "batched" in forms or "served" in forms:
+        batched, catalog, env_id = batched_form(ctx, args.batch, None)
+        log(f"batched: {batched}")
+        if "served" in forms:
+            served = served_form(ctx, catalog, env_id, args.served, args.rate)
+            log(f"served: {served}")
+        if "batched" not in forms:
+            batched = None
 
     cpu = None
     if not args.no_cpu:
@@ -159,10 +339,29 @@ def main():
         "exact_rays_per_mray": round(pr.exact_rays / max(1, pr.extend_rays + pr.shadow_rays) * 1e6, 2),
         "roofline": roofline,
         "kernels_ms": kernels,
+        "quality": qual,
+        "batched": batched,
+        "served": served,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
     }
     print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def _traffic(workload_name: str):
+    """HBM bytes per k_extend launch from the committed rocprofv3 counter summary, if any
+    (a line `bytes_per_launch: N` in profiles/traffic_k_extend_<workload>.md)."""
+    f = ROOT / "profiles" / f"traffic_k_extend_{workload_name}.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.strip().startswith("bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
 
 
 if __name__ == "__main__":

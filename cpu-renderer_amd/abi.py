@@ -114,7 +114,27 @@ class Stats(C.Structure):
                 ("extend_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("workers", C.c_int32),
                 ("iterations", C.c_int32), ("extend_nodes", C.c_uint64), ("extend_tris", C.c_uint64),
                 ("shadow_nodes", C.c_uint64), ("shadow_tris", C.c_uint64), ("exact_fallbacks", C.c_uint64),
-                ("exact_rays", C.c_uint64)]
+                ("exact_rays", C.c_uint64), ("queue_seconds", C.c_double), ("latency_seconds", C.c_double)]
+
+
+class Request(C.Structure):
+    """crg_request: one render of the batched / served forms."""
+    _fields_ = [("scene", C.c_uint64), ("env", C.c_uint64), ("camera", Camera), ("options", Options),
+                ("film_rgb", C.POINTER(C.c_float))]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("pool_paths", C.c_uint32), ("workgroup", C.c_uint32), ("groups_per_cu", C.c_uint32),
+                ("iterations_per_graph", C.c_uint32), ("bvh_width", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class EngineInfo(C.Structure):
+    _fields_ = [("budget_bytes", C.c_uint64), ("resident_bytes", C.c_uint64), ("inflight_bytes", C.c_uint64),
+                ("inflight", C.c_uint32), ("pending", C.c_uint32), ("completed", C.c_uint64),
+                ("steps", C.c_uint64)]
 
 
 class KernelTime(C.Structure):

@@ -23,7 +23,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
 
-UNITS = [("kernels.hip", []), ("capi.cpp", ["--offload-host-only"]), ("host_scene.cpp", ["--offload-host-only"])]
+UNITS = [("kernels.hip", []), ("capi.cpp", ["--offload-host-only"]), ("engine.cpp", ["--offload-host-only"]),
+         ("host_scene.cpp", ["--offload-host-only"])]
 
 
 def _run(cmd):

@@ -53,6 +53,14 @@ def load_library(path: Optional[pathlib.Path] = None) -> C.CDLL:
         "crg_render": (i32, [vp, u64, u64, C.POINTER(abi.Camera), C.POINTER(abi.Options), C.POINTER(C.c_float),
                              C.POINTER(abi.Stats)]),
         "crg_trace_rays": (i32, [vp, u64, vp, u64, vp, i32]),
+        "crg_render_batch": (i32, [vp, C.POINTER(abi.Request), C.c_uint32, C.POINTER(i32), C.POINTER(abi.Stats)]),
+        "crg_submit": (i32, [vp, C.POINTER(abi.Request), C.POINTER(u64)]),
+        "crg_wait": (i32, [vp, u64, C.c_double, C.POINTER(abi.Stats)]),
+        "crg_set_memory_budget": (i32, [vp, u64]),
+        "crg_engine_info_get": (i32, [vp, C.POINTER(abi.EngineInfo)]),
+        "crg_get_tuning": (i32, [vp, C.POINTER(abi.Tuning)]),
+        "crg_set_tuning": (i32, [vp, C.POINTER(abi.Tuning)]),
+        "crg_autotune": (i32, [vp, C.POINTER(abi.Request), C.POINTER(abi.Tuning), C.POINTER(C.c_double)]),
         "crg_set_profiling": (i32, [vp, i32]),
         "crg_kernel_times": (i32, [vp, C.POINTER(abi.KernelTime), i32, C.POINTER(i32)]),
     }
@@ -82,6 +90,15 @@ class RenderResult:
     shadow_tris: int = 0
     exact_fallbacks: int = 0
     exact_rays: int = 0
+    queue_seconds: float = 0.0
+    latency_seconds: float = 0.0
+
+    @staticmethod
+    def from_stats(film: np.ndarray, stats: abi.Stats) -> "RenderResult":
+        return RenderResult(film, stats.seconds, stats.workers, stats.camera_samples, stats.boundary_retries,
+                            stats.extend_rays, stats.shadow_rays, stats.iterations, stats.extend_nodes,
+                            stats.extend_tris, stats.shadow_nodes, stats.shadow_tris, stats.exact_fallbacks,
+                            stats.exact_rays, stats.queue_seconds, stats.latency_seconds)
 
 
 class Context:
@@ -136,10 +153,74 @@ class Context:
         stats = abi.Stats()
         self._check(self.lib.crg_render(self.handle, scene_id, env_id, C.byref(camera), C.byref(options),
                                         film.ctypes.data_as(C.POINTER(C.c_float)), C.byref(stats)))
-        return RenderResult(film, stats.seconds, stats.workers, stats.camera_samples, stats.boundary_retries,
-                            stats.extend_rays, stats.shadow_rays, stats.iterations, stats.extend_nodes,
-                            stats.extend_tris, stats.shadow_nodes, stats.shadow_tris, stats.exact_fallbacks,
-                            stats.exact_rays)
+        return RenderResult.from_stats(film, stats)
+
+    # ---- batched and served forms -------------------------------------------------
+    @staticmethod
+    def request(scene_id: int, env_id: int, camera: abi.Camera, options: abi.Options):
+        """(crg_request, film) with a freshly allocated film the request writes into."""
+        film = np.zeros((options.height, options.width, 3), np.float32)
+        req = abi.Request(scene_id, env_id, camera, options, film.ctypes.data_as(C.POINTER(C.c_float)))
+        return req, film
+
+    def render_batch(self, requests) -> list:
+        """Render [(scene_id, env_id, camera, options), ...] together; returns RenderResults
+        (or CrgError instances for requests that failed)."""
+        made = [self.request(*r) for r in requests]
+        arr = (abi.Request * len(made))(*[m[0] for m in made])
+        status = (C.c_int * len(made))()
+        stats = (abi.Stats * len(made))()
+        rc = self.lib.crg_render_batch(self.handle, arr, len(made), status, stats)
+        if rc != abi.CRG_OK and not any(status[i] != abi.CRG_OK for i in range(len(made))):
+            raise CrgError(rc, self.lib.crg_last_error(self.handle).decode())
+        first = self.lib.crg_last_error(self.handle).decode() if rc != abi.CRG_OK else ""
+        out = []
+        for i, (_, film) in enumerate(made):
+            if status[i] == abi.CRG_OK:
+                out.append(RenderResult.from_stats(film, stats[i]))
+            else:
+                msg = first if first.startswith(f"request {i}:") else f"request {i} failed (status {status[i]})"
+                out.append(CrgError(status[i], msg))
+        return out
+
+    def submit(self, scene_id: int, env_id: int, camera: abi.Camera, options: abi.Options):
+        """Queue a request; returns (ticket, film).  The film array is written when the request completes."""
+        req, film = self.request(scene_id, env_id, camera, options)
+        ticket = C.c_uint64()
+        self._check(self.lib.crg_submit(self.handle, C.byref(req), C.byref(ticket)))
+        return ticket.value, film
+
+    def wait(self, ticket: int, film: np.ndarray, timeout: float = -1.0) -> Optional[RenderResult]:
+        """RenderResult when done, None if still in flight after `timeout` seconds."""
+        stats = abi.Stats()
+        rc = self.lib.crg_wait(self.handle, ticket, float(timeout), C.byref(stats))
+        if rc == abi.CRG_EBUSY:
+            return None
+        self._check(rc)
+        return RenderResult.from_stats(film, stats)
+
+    def set_memory_budget(self, nbytes: int):
+        self._check(self.lib.crg_set_memory_budget(self.handle, int(nbytes)))
+
+    def engine_info(self) -> abi.EngineInfo:
+        info = abi.EngineInfo()
+        self._check(self.lib.crg_engine_info_get(self.handle, C.byref(info)))
+        return info
+
+    def get_tuning(self) -> abi.Tuning:
+        t = abi.Tuning()
+        self._check(self.lib.crg_get_tuning(self.handle, C.byref(t)))
+        return t
+
+    def set_tuning(self, tuning: abi.Tuning):
+        self._check(self.lib.crg_set_tuning(self.handle, C.byref(tuning)))
+
+    def autotune(self, scene_id: int, env_id: int, camera: abi.Camera, options: abi.Options):
+        """Pick the fastest launch configuration on a sample request; returns (Tuning, samples/s)."""
+        req, _film = self.request(scene_id, env_id, camera, options)
+        t, rate = abi.Tuning(), C.c_double()
+        self._check(self.lib.crg_autotune(self.handle, C.byref(req), C.byref(t), C.byref(rate)))
+        return t, rate.value
 
     def set_profiling(self, enable: bool):
         self._check(self.lib.crg_set_profiling(self.handle, int(bool(enable))))

@@ -1,9 +1,10 @@
 // capi.cpp — the crgpu C ABI (include/crgpu.h) over the gfx950 kernels.
 //
-// A context owns one HIP stream on one device, the resident scenes and
-// environments uploaded to it, and a pool of path slots that every render on
-// the context reuses.  Nothing here throws across the ABI: failures become a
-// status code plus crg_last_error() text in the reference's wording.
+// A context owns, on one device, the resident scenes and environments, an
+// upload / ray-query stream, and the wavefront engine (engine.h) that every
+// render request — plain, batched or served — runs on.  Nothing here throws
+// across the ABI: failures become a status code plus crg_last_error() text in
+// the reference's wording.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,197 +18,103 @@
 #include <vector>
 
 #include "../../include/crgpu.h"
+#include "device_buf.h"
+#include "engine.h"
 #include "host_scene.h"
 #include "kernels.h"
 
 using namespace cr;
 
-namespace {
-
-struct HipError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-// A transport failure inside a render (reference src/render.cpp:429-451).
-struct PathError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-void check(hipError_t e, const char* what) {
-    if (e != hipSuccess) {
-        if (e == hipErrorOutOfMemory) throw std::bad_alloc();
-        throw HipError(std::string(what) + ": " + hipGetErrorString(e));
-    }
-}
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    void alloc(size_t n) {
-        if (n <= bytes && p) return;
-        release();
-        check(hipMalloc(&p, std::max<size_t>(n, 16)), "hipMalloc");
-        bytes = std::max<size_t>(n, 16);
-    }
-    template <class T>
-    T* as() const { return static_cast<T*>(p); }
-};
-
-template <class T>
-void upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
-    b.alloc(v.size() * sizeof(T));
-    if (!v.empty()) check(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st), "upload");
-}
-
-struct SceneRes {
-    DevBuf nodes, prims, prim_ng, aux, materials, light_rank, punctual, desc;
-    DevScene host{};
-    SceneLights lights;
-    std::vector<uint32_t> tri_to_prim;
-    std::vector<uint32_t> prim_to_tri;
-    uint64_t triangles = 0, bytes = 0;
-};
-
-struct EnvRes {
-    DevBuf texels, pdf, threshold, guide, rowcos, desc;
-    DevEnv host{};
-    double integral = 0;
-    uint64_t bytes = 0;
-};
-
-struct Pool {
-    uint32_t size = 0;
-    DevBuf ray_o, ray_d, ray_lo, thr, rad, lastpos, absorb, rng, job, pixel, sample, medium, flags, prev, hit;
-    DevBuf sh_path, sh_o, sh_d, sh_lo, sh_c;
-    DevBuf q0, q1, retire, xq, xsq;
-    PathSoA P{};
-    ShadowSoA S{};
-    Queues Q{};
-
-    void ensure(uint32_t n) {
-        if (n <= size) return;
-        const size_t f4 = size_t(n) * sizeof(float4), u32 = size_t(n) * 4;
-        ray_o.alloc(f4); ray_d.alloc(f4); ray_lo.alloc(f4); thr.alloc(f4); rad.alloc(f4); lastpos.alloc(f4);
-        absorb.alloc(f4); rng.alloc(size_t(n) * 8); job.alloc(u32); pixel.alloc(u32); sample.alloc(u32);
-        medium.alloc(u32); flags.alloc(u32); prev.alloc(u32); hit.alloc(u32);
-        sh_path.alloc(u32); sh_o.alloc(f4); sh_d.alloc(f4); sh_lo.alloc(f4); sh_c.alloc(f4);
-        q0.alloc(u32); q1.alloc(u32); retire.alloc(u32); xq.alloc(u32); xsq.alloc(u32);
-        P = PathSoA{ray_o.as<float4>(), ray_d.as<float4>(), ray_lo.as<float4>(), thr.as<float4>(), rad.as<float4>(),
-                    lastpos.as<float4>(), absorb.as<float4>(), rng.as<uint64_t>(), job.as<uint32_t>(),
-                    pixel.as<uint32_t>(), sample.as<uint32_t>(), medium.as<int32_t>(), flags.as<uint32_t>(),
-                    prev.as<uint32_t>(), hit.as<uint32_t>()};
-        S = ShadowSoA{sh_path.as<uint32_t>(), sh_o.as<float4>(), sh_d.as<float4>(), sh_lo.as<float4>(),
-                      sh_c.as<float4>()};
-        Q = Queues{{q0.as<uint32_t>(), q1.as<uint32_t>()}, retire.as<uint32_t>(), xq.as<uint32_t>(),
-                   xsq.as<uint32_t>()};
-        size = n;
-    }
-};
-
-}  // namespace
-
-// Event-pair timing of kernel launches on the context stream.
-struct Profiler {
-    static constexpr int kSlots = kStageCount + 2;  // + seed, accumulate
-    static constexpr int kPairs = 256;
-    bool on = false;
-    std::vector<hipEvent_t> ev;
-    int stage_of[kPairs] = {};
-    int used = 0;
-    double total_ms[kSlots] = {};
-    uint64_t launches[kSlots] = {};
-
-    ~Profiler() {
-        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-    }
-    void begin(int slot, hipStream_t st) {
-        if (!on) return;
-        if (ev.empty()) {
-            ev.resize(2 * kPairs);
-            for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
-        }
-        if (used == kPairs) drain(st);
-        stage_of[used] = slot;
-        check(hipEventRecord(ev[2 * used], st), "hipEventRecord");
-    }
-    void end(hipStream_t st) {
-        if (!on) return;
-        check(hipEventRecord(ev[2 * used + 1], st), "hipEventRecord");
-        ++used;
-    }
-    void drain(hipStream_t st) {
-        if (!used) return;
-        check(hipStreamSynchronize(st), "profiler sync");
-        for (int i = 0; i < used; ++i) {
-            float ms = 0;
-            check(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]), "hipEventElapsedTime");
-            total_ms[stage_of[i]] += ms;
-            ++launches[stage_of[i]];
-        }
-        used = 0;
-    }
-    static const char* name(int slot) {
-        if (slot < kStageCount) return kStageNames[slot];
-        return slot == kStageCount ? "k_seed" : "k_accumulate";
-    }
-};
-
 struct crg_ctx {
     int device = 0;
-    Profiler prof;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // uploads and ray queries
     std::string error;
-    std::mutex mu;
+    std::mutex mu;                 // handles, error text, the upload stream
     uint64_t next_id = 1;
-    std::map<uint64_t, std::unique_ptr<SceneRes>> scenes;
-    std::map<uint64_t, std::unique_ptr<EnvRes>> envs;
-    Pool pool;
-    DevBuf ctl, jobs, lights, samples, film, rays, hits;
-    uint32_t pool_size = 1u << 22;  // 4M in-flight paths
-    unsigned grid = 0;
+    std::map<uint64_t, std::shared_ptr<SceneRes>> scenes;
+    std::map<uint64_t, std::shared_ptr<EnvRes>> envs;
+    std::unique_ptr<Engine> engine;
+    DevBuf ctl, rays, hits;
 };
 
 namespace {
+
+// Runs f() and maps exceptions to status codes; the failure text is kept per
+// context.  f() takes the context lock itself where it touches shared state.
+// A request's own failure status, carried out of guarded() with its text.
+struct StatusError : std::runtime_error {
+    int code;
+    StatusError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
 
 template <class F>
 int guarded(crg_ctx* ctx, F&& f) {
     if (!ctx) return CRG_EINVAL;
-    std::lock_guard<std::mutex> lock(ctx->mu);
+    int rc = CRG_OK;
+    std::string msg;
     try {
         f();
-        ctx->error.clear();
-        return CRG_OK;
+    } catch (const StatusError& e) {
+        rc = e.code;
+        msg = e.what();
     } catch (const std::bad_alloc&) {
-        ctx->error = "out of memory";
-        return CRG_ENOMEM;
+        rc = CRG_ENOMEM;
+        msg = "out of memory";
     } catch (const HipError& e) {
-        ctx->error = e.what();
-        return CRG_EDEVICE;
+        rc = CRG_EDEVICE;
+        msg = e.what();
     } catch (const PathError& e) {
-        ctx->error = e.what();
-        return CRG_EPATH;
+        rc = CRG_EPATH;
+        msg = e.what();
     } catch (const std::out_of_range& e) {
-        ctx->error = e.what();
-        return CRG_ENOTFOUND;
+        rc = CRG_ENOTFOUND;
+        msg = e.what();
     } catch (const std::exception& e) {
-        ctx->error = e.what();
-        return CRG_EINVAL;
+        rc = CRG_EINVAL;
+        msg = e.what();
     }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->error = msg;
+    return rc;
+}
+
+// Request from C arguments: option/camera validation and the light table,
+// on the calling thread (reference src/render.cpp:356-372).
+Request make_request(crg_ctx* ctx, const crg_request& r) {
+    if (!r.film_rgb) throw std::invalid_argument("null film buffer");
+    validate_options(r.options);
+    Request q;
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        auto si = ctx->scenes.find(r.scene);
+        if (si == ctx->scenes.end()) throw std::out_of_range("unknown scene handle");
+        auto ei = ctx->envs.find(r.env);
+        if (ei == ctx->envs.end()) throw std::out_of_range("unknown environment handle");
+        q.scene = si->second;
+        q.env = ei->second;
+    }
+    q.options = r.options;
+    q.camera = prepare_camera(r.camera, r.options.width, r.options.height);
+    q.lights = build_light_table(q.scene->lights, q.scene->tri_to_prim, q.env->integral);
+    q.film = r.film_rgb;
+    return q;
+}
+
+// CRG_OK or CRG_EBUSY; a failed request throws StatusError.
+int status_of_wait(Engine& eng, uint64_t ticket, double timeout, crg_stats* stats) {
+    std::string err;
+    crg_stats st{};
+    const int rc = eng.wait(ticket, timeout, &st, &err);
+    if (stats && rc != CRG_EBUSY) *stats = st;
+    if (rc != CRG_OK && rc != CRG_EBUSY) throw StatusError(rc, err);
+    return rc;
 }
 
 }  // namespace
 
 extern "C" {
 
-const char* crg_version(void) { return "crgpu 0.1 gfx950 wavefront (binary64-exact predicates, no FP64 instructions)"; }
+const char* crg_version(void) { return "crgpu 0.2 gfx950 wavefront (binary64-exact predicates, no FP64 instructions)"; }
 
 int crg_context_create(int device, crg_ctx** out) {
     if (!out) return CRG_EINVAL;
@@ -220,11 +127,15 @@ int crg_context_create(int device, crg_ctx** out) {
     auto* ctx = new (std::nothrow) crg_ctx();
     if (!ctx) return CRG_ENOMEM;
     ctx->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    try {
+        check(hipSetDevice(device), "hipSetDevice");
+        check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+        ctx->engine = std::make_unique<Engine>(device, unsigned(prop.multiProcessorCount));
+    } catch (const std::exception&) {
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return CRG_EDEVICE;
     }
-    ctx->grid = unsigned(prop.multiProcessorCount) * 8;
     *out = ctx;
     return CRG_OK;
 }
@@ -232,6 +143,7 @@ int crg_context_create(int device, crg_ctx** out) {
 void crg_context_destroy(crg_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
+    ctx->engine.reset();  // finishes admitted requests, fails queued ones
     (void)hipStreamSynchronize(ctx->stream);
     ctx->scenes.clear();
     ctx->envs.clear();
@@ -247,6 +159,8 @@ int crg_scene_upload(crg_ctx* ctx, const crg_scene_desc* desc, uint64_t* scene_i
         if (desc->triangle_count && (!desc->positions || !desc->material_ids))
             throw std::invalid_argument("scene descriptor without positions or material ids");
         check(hipSetDevice(ctx->device), "hipSetDevice");
+        // Host preparation (finalize, SAH build, flattening) runs without the
+        // context lock, so it overlaps renders in flight.
         const auto t0 = std::chrono::steady_clock::now();
         HostScene hs = finalize_scene(*desc);
         for (size_t i = 0; i < hs.materials.size(); ++i) {
@@ -258,7 +172,7 @@ int crg_scene_upload(crg_ctx* ctx, const crg_scene_desc* desc, uint64_t* scene_i
                                                 " uses a texture map; the device path supports factor-only materials");
         }
         WideBvh bvh = build_wide_bvh(hs);
-        auto res = std::make_unique<SceneRes>();
+        auto res = std::make_shared<SceneRes>();
         const size_t n = hs.tris.size();
         res->triangles = n;
         res->prim_to_tri = bvh.order;
@@ -292,6 +206,7 @@ int crg_scene_upload(crg_ctx* ctx, const crg_scene_desc* desc, uint64_t* scene_i
         const std::vector<uint32_t> ranks = light_ranks(res->lights, res->tri_to_prim, n);
         const std::vector<DevMaterial> mats = device_materials(hs);
         const auto t1 = std::chrono::steady_clock::now();
+        std::lock_guard<std::mutex> lock(ctx->mu);
         hipStream_t st = ctx->stream;
         upload(res->nodes, bvh.nodes, st);
         upload(res->prims, prims, st);
@@ -318,6 +233,7 @@ int crg_scene_upload(crg_ctx* ctx, const crg_scene_desc* desc, uint64_t* scene_i
             info->build_seconds = std::chrono::duration<double>(t1 - t0).count();
             info->upload_seconds = std::chrono::duration<double>(t2 - t1).count();
         }
+        ctx->engine->add_resident(int64_t(res->bytes));
         const uint64_t id = ctx->next_id++;
         ctx->scenes[id] = std::move(res);
         *scene_id = id;
@@ -326,8 +242,12 @@ int crg_scene_upload(crg_ctx* ctx, const crg_scene_desc* desc, uint64_t* scene_i
 
 int crg_scene_release(crg_ctx* ctx, uint64_t scene_id) {
     return guarded(ctx, [&] {
-        (void)hipStreamSynchronize(ctx->stream);
-        if (!ctx->scenes.erase(scene_id)) throw std::out_of_range("unknown scene handle");
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        auto it = ctx->scenes.find(scene_id);
+        if (it == ctx->scenes.end()) throw std::out_of_range("unknown scene handle");
+        // Requests in flight keep their own reference; memory goes with the last.
+        ctx->engine->add_resident(-int64_t(it->second->bytes));
+        ctx->scenes.erase(it);
     });
 }
 
@@ -336,7 +256,8 @@ int crg_env_upload(crg_ctx* ctx, const crg_env_desc* desc, uint64_t* env_id, uin
         if (!desc || !env_id) throw std::invalid_argument("null environment descriptor");
         check(hipSetDevice(ctx->device), "hipSetDevice");
         HostEnv he = build_env(*desc);
-        auto res = std::make_unique<EnvRes>();
+        auto res = std::make_shared<EnvRes>();
+        std::lock_guard<std::mutex> lock(ctx->mu);
         hipStream_t st = ctx->stream;
         res->integral = he.integral;
         DevEnv& d = res->host;
@@ -365,6 +286,7 @@ int crg_env_upload(crg_ctx* ctx, const crg_env_desc* desc, uint64_t* env_id, uin
         res->bytes = res->texels.bytes + res->pdf.bytes + res->threshold.bytes + res->guide.bytes + res->rowcos.bytes +
                      res->desc.bytes;
         if (device_bytes) *device_bytes = res->bytes;
+        ctx->engine->add_resident(int64_t(res->bytes));
         const uint64_t id = ctx->next_id++;
         ctx->envs[id] = std::move(res);
         *env_id = id;
@@ -373,13 +295,17 @@ int crg_env_upload(crg_ctx* ctx, const crg_env_desc* desc, uint64_t* env_id, uin
 
 int crg_env_release(crg_ctx* ctx, uint64_t env_id) {
     return guarded(ctx, [&] {
-        (void)hipStreamSynchronize(ctx->stream);
-        if (!ctx->envs.erase(env_id)) throw std::out_of_range("unknown environment handle");
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        auto it = ctx->envs.find(env_id);
+        if (it == ctx->envs.end()) throw std::out_of_range("unknown environment handle");
+        ctx->engine->add_resident(-int64_t(it->second->bytes));
+        ctx->envs.erase(it);
     });
 }
 
 int crg_trace_rays(crg_ctx* ctx, uint64_t scene_id, const crg_ray* rays, uint64_t count, crg_hit* hits, int any) {
     return guarded(ctx, [&] {
+        std::lock_guard<std::mutex> lock(ctx->mu);
         auto it = ctx->scenes.find(scene_id);
         if (it == ctx->scenes.end()) throw std::out_of_range("unknown scene handle");
         if (count && (!rays || !hits)) throw std::invalid_argument("null ray or hit array");
@@ -402,134 +328,149 @@ int crg_trace_rays(crg_ctx* ctx, uint64_t scene_id, const crg_ray* rays, uint64_
 
 int crg_render(crg_ctx* ctx, uint64_t scene_id, uint64_t env_id, const crg_camera* camera, const crg_options* options,
                float* film_rgb, crg_stats* stats) {
-    return guarded(ctx, [&] {
+    int rc = CRG_OK;
+    const int g = guarded(ctx, [&] {
         if (!camera || !options || !film_rgb) throw std::invalid_argument("null render argument");
-        const crg_options o = *options;
-        validate_options(o);
-        auto si = ctx->scenes.find(scene_id);
-        if (si == ctx->scenes.end()) throw std::out_of_range("unknown scene handle");
-        auto ei = ctx->envs.find(env_id);
-        if (ei == ctx->envs.end()) throw std::out_of_range("unknown environment handle");
-        SceneRes& sc = *si->second;
-        EnvRes& env = *ei->second;
-        check(hipSetDevice(ctx->device), "hipSetDevice");
-        hipStream_t st = ctx->stream;
-        DevJob job{};
-        job.cam = prepare_camera(*camera, o.width, o.height);
-        const LightTable lt = build_light_table(sc.lights, sc.tri_to_prim, env.integral);
-        upload(ctx->lights, lt.entries, st);
-        job.scene = sc.desc.as<DevScene>();
-        job.env = env.desc.as<DevEnv>();
-        job.lights = DevLights{ctx->lights.as<DevLightEntry>(), uint32_t(lt.entries.size()), lt.env_probability};
-        job.width = o.width;
-        job.height = o.height;
-        job.spp = o.samples;
-        job.bounces = o.bounces;
-        job.seed = o.seed;
-        job.nee = o.next_event_estimation;
-        job.rr = o.russian_roulette;
-        const uint64_t npix = uint64_t(o.width) * uint64_t(o.height);
-        job.items = npix * uint64_t(o.samples);
-        job.item_base = 0;
-        ctx->samples.alloc(job.items * sizeof(float4));
-        job.samples = ctx->samples.as<float4>();
-        ctx->jobs.alloc(sizeof(DevJob));
-        check(hipMemcpyAsync(ctx->jobs.p, &job, sizeof(DevJob), hipMemcpyHostToDevice, st), "job upload");
-        const uint32_t pool = uint32_t(std::min<uint64_t>(ctx->pool_size, job.items));
-        ctx->pool.ensure(pool);
-        ctx->ctl.alloc(sizeof(Control));
-        ctx->film.alloc(npix * 3 * sizeof(float));
-        Control init{};
-        init.total = job.items;
-        init.issued = pool;
-        check(hipMemcpyAsync(ctx->ctl.p, &init, sizeof(Control), hipMemcpyHostToDevice, st), "control upload");
-        check(hipStreamSynchronize(st), "render setup");
-        Control* dctl = ctx->ctl.as<Control>();
-        const DevJob* djobs = ctx->jobs.as<DevJob>();
-        const auto start = std::chrono::steady_clock::now();
-        Profiler& prof = ctx->prof;
-        prof.begin(kStageCount, st);
-        check(launch_seed(ctx->pool.P, djobs, 1, pool, ctx->pool.Q.q[0], dctl, st), "seed launch");
-        prof.end(st);
-        int parity = 0, iterations = 0;
-        Control head{};
-        const int batch = 4;
-        for (;;) {
-            for (int k = 0; k < batch; ++k) {
-                for (int stage = 0; stage < kStageCount; ++stage) {
-                    prof.begin(stage, st);
-                    check(launch_stage(Stage(stage), ctx->pool.P, ctx->pool.S, djobs, 1, ctx->pool.Q, parity, dctl,
-                                       ctx->grid, st),
-                          "iteration launch");
-                    prof.end(st);
-                }
-                parity ^= 1;
-                ++iterations;
+        crg_request r{};
+        r.scene = scene_id;
+        r.env = env_id;
+        r.camera = *camera;
+        r.options = *options;
+        r.film_rgb = film_rgb;
+        const uint64_t t = ctx->engine->submit(make_request(ctx, r));
+        rc = status_of_wait(*ctx->engine, t, -1.0, stats);
+    });
+    return g != CRG_OK ? g : rc;
+}
+
+int crg_render_batch(crg_ctx* ctx, const crg_request* requests, uint32_t count, int* status, crg_stats* stats) {
+    int first = CRG_OK;
+    std::string first_msg;
+    const int g = guarded(ctx, [&] {
+        if (count && !requests) throw std::invalid_argument("null request array");
+        std::vector<uint64_t> tickets(count, 0);
+        std::vector<int> rcs(count, CRG_OK);
+        std::vector<std::string> msgs(count);
+        for (uint32_t i = 0; i < count; ++i) {
+            try {
+                tickets[i] = ctx->engine->submit(make_request(ctx, requests[i]));
+            } catch (const std::bad_alloc&) {
+                rcs[i] = CRG_ENOMEM;
+                msgs[i] = "out of memory";
+            } catch (const std::out_of_range& e) {
+                rcs[i] = CRG_ENOTFOUND;
+                msgs[i] = e.what();
+            } catch (const std::exception& e) {
+                rcs[i] = CRG_EINVAL;
+                msgs[i] = e.what();
             }
-            check(hipMemcpyAsync(&head, dctl, offsetof(Control, errors), hipMemcpyDeviceToHost, st), "control read");
-            check(hipStreamSynchronize(st), "render iteration");
-            prof.drain(st);
-            if (head.issued >= head.total && head.qcount[parity] == 0) break;
-            // Every path finishes within 2*(bounces+1) iterations of its start (one
-            // precision replay at most), so this bound only trips on a logic error.
-            const uint64_t bound = (job.items / pool + 2) * uint64_t(2 * (o.bounces + 2)) + 64;
-            if (uint64_t(iterations) > bound) throw std::logic_error("render did not converge");
         }
-        prof.begin(kStageCount + 1, st);
-        check(launch_accumulate(djobs, 0, npix, ctx->film.as<float>(), st), "accumulate launch");
-        prof.end(st);
-        check(hipMemcpyAsync(film_rgb, ctx->film.p, npix * 3 * sizeof(float), hipMemcpyDeviceToHost, st), "film read");
-        check(hipStreamSynchronize(st), "render");
-        prof.drain(st);
-        const double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
-        check(hipMemcpy(&head, dctl, sizeof(Control), hipMemcpyDeviceToHost), "control read");
-        if (head.err_count) {
-            const uint32_t n = std::min<uint32_t>(head.err_count, kMaxErrors);
-            const ErrorRec* best = &head.errors[0];
-            for (uint32_t i = 1; i < n; ++i)
-                if (head.errors[i].key < best->key) best = &head.errors[i];
-            throw PathError(format_path_error(*best, o.width, o.samples));
+        for (uint32_t i = 0; i < count; ++i) {
+            crg_stats st{};
+            if (tickets[i]) rcs[i] = ctx->engine->wait(tickets[i], -1.0, &st, &msgs[i]);
+            if (stats) stats[i] = st;
+            if (status) status[i] = rcs[i];
+            if (rcs[i] != CRG_OK && first == CRG_OK) {
+                first = rcs[i];
+                first_msg = "request " + std::to_string(i) + ": " + msgs[i];
+            }
         }
-        if (stats) {
-            std::memset(stats, 0, sizeof(*stats));
-            stats->seconds = seconds;
-            stats->camera_samples = job.items;
-            stats->boundary_retries = head.replays;
-            stats->extend_rays = head.extend_rays;
-            stats->shadow_rays = head.shadow_rays;
-            stats->workers = 0;
-            stats->iterations = iterations;
-            stats->extend_nodes = head.extend_nodes;
-            stats->extend_tris = head.extend_tris;
-            stats->shadow_nodes = head.shadow_nodes;
-            stats->shadow_tris = head.shadow_tris;
-            stats->exact_fallbacks = head.fallbacks;
-            stats->exact_rays = head.exact_rays;
-        }
+    });
+    if (g != CRG_OK) return g;
+    if (first != CRG_OK) {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        ctx->error = first_msg;
+    }
+    return first;
+}
+
+int crg_submit(crg_ctx* ctx, const crg_request* request, uint64_t* ticket) {
+    return guarded(ctx, [&] {
+        if (!request || !ticket) throw std::invalid_argument("null request");
+        *ticket = ctx->engine->submit(make_request(ctx, *request));
+    });
+}
+
+int crg_wait(crg_ctx* ctx, uint64_t ticket, double timeout_seconds, crg_stats* stats) {
+    int rc = CRG_OK;
+    const int g = guarded(ctx, [&] { rc = status_of_wait(*ctx->engine, ticket, timeout_seconds, stats); });
+    return g != CRG_OK ? g : rc;
+}
+
+int crg_set_memory_budget(crg_ctx* ctx, uint64_t bytes) {
+    return guarded(ctx, [&] { ctx->engine->set_budget(bytes); });
+}
+
+int crg_engine_info_get(crg_ctx* ctx, crg_engine_info* info) {
+    return guarded(ctx, [&] {
+        if (!info) throw std::invalid_argument("null info");
+        *info = ctx->engine->info();
+    });
+}
+
+int crg_get_tuning(crg_ctx* ctx, crg_tuning* out) {
+    return guarded(ctx, [&] {
+        if (!out) throw std::invalid_argument("null tuning");
+        *out = ctx->engine->tuning();
+    });
+}
+
+int crg_set_tuning(crg_ctx* ctx, const crg_tuning* tuning) {
+    return guarded(ctx, [&] {
+        if (!tuning) throw std::invalid_argument("null tuning");
+        ctx->engine->set_tuning(*tuning);
+    });
+}
+
+int crg_autotune(crg_ctx* ctx, const crg_request* sample, crg_tuning* out, double* samples_per_second) {
+    return guarded(ctx, [&] {
+        if (!sample) throw std::invalid_argument("null sample request");
+        Engine& eng = *ctx->engine;
+        const crg_tuning start = eng.tuning();
+        const uint64_t items = uint64_t(sample->options.width) * sample->options.height * sample->options.samples;
+        auto time_of = [&](const crg_tuning& t) {
+            eng.set_tuning(t);
+            double best = 1e300;
+            for (int rep = 0; rep < 2; ++rep) {  // the first run also absorbs (re)configuration
+                const auto t0 = std::chrono::steady_clock::now();
+                std::string err;
+                const int rc = eng.wait(eng.submit(make_request(ctx, *sample)), -1.0, nullptr, &err);
+                if (rc != CRG_OK) throw std::runtime_error("autotune sample render failed: " + err);
+                best = std::min(best, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+            }
+            return best;
+        };
+        crg_tuning best = start;
+        double best_s = time_of(best);
+        auto try_field = [&](uint32_t crg_tuning::*field, std::initializer_list<uint32_t> values) {
+            for (uint32_t v : values) {
+                crg_tuning t = best;
+                t.*field = v;
+                if (std::memcmp(&t, &best, sizeof(t)) == 0) continue;
+                if (field == &crg_tuning::pool_paths && uint64_t(v) > 2 * items && v > (1u << 16)) continue;
+                const double s = time_of(t);
+                if (s < best_s * 0.98) {
+                    best_s = s;
+                    best = t;
+                }
+            }
+        };
+        try_field(&crg_tuning::pool_paths, {1u << 20, 1u << 21, 1u << 22, 1u << 23});
+        try_field(&crg_tuning::groups_per_cu, {4, 8, 16});
+        try_field(&crg_tuning::workgroup, {64, 128, 256});
+        try_field(&crg_tuning::iterations_per_graph, {2, 4, 8});
+        eng.set_tuning(best);
+        if (out) *out = best;
+        if (samples_per_second) *samples_per_second = double(items) / best_s;
     });
 }
 
 int crg_set_profiling(crg_ctx* ctx, int enable) {
-    return guarded(ctx, [&] {
-        ctx->prof.on = enable != 0;
-        if (enable) {
-            std::fill(std::begin(ctx->prof.total_ms), std::end(ctx->prof.total_ms), 0.0);
-            std::fill(std::begin(ctx->prof.launches), std::end(ctx->prof.launches), 0ull);
-        }
-    });
+    return guarded(ctx, [&] { ctx->engine->set_profiling(enable != 0); });
 }
 
 int crg_kernel_times(crg_ctx* ctx, crg_kernel_time* out, int max, int* count) {
-    return guarded(ctx, [&] {
-        const int n = Profiler::kSlots;
-        if (count) *count = n;
-        for (int i = 0; i < n && i < max && out; ++i) {
-            std::memset(&out[i], 0, sizeof(out[i]));
-            std::strncpy(out[i].name, Profiler::name(i), sizeof(out[i].name) - 1);
-            out[i].launches = ctx->prof.launches[i];
-            out[i].total_ms = ctx->prof.total_ms[i];
-        }
-    });
+    return guarded(ctx, [&] { ctx->engine->kernel_times(out, max, count); });
 }
 
 }  // extern "C"

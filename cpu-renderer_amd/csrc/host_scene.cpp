@@ -573,6 +573,7 @@ std::string format_path_error(const ErrorRec& e, int width, int spp) {
         case kErrOffsetOverflow: cause = "Secondary ray offset exceeds finite coordinates"; break;
         case kErrCamera: cause = "Unrepresentable perspective ray"; break;
         case kErrStack: cause = "BVH traversal exceeded its bounded stack"; break;
+        case kErrInternal: cause = "Work item outside its render request"; break;
         default:
             cause = "Refractive exit without entry; check outward normals and keep the camera outside closed volumes";
     }

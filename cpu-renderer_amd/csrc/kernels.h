@@ -22,8 +22,6 @@ static_assert(sizeof(HitOut) == 32, "crg_hit layout");
 
 hipError_t launch_trace_rays(const DevScene* scene, const RayIn* rays, uint64_t n, HitOut* hits, int any,
                              Control* ctl, hipStream_t st);
-hipError_t launch_seed(const PathSoA& P, const DevJob* jobs, uint32_t njobs, uint32_t pool, uint32_t* q0,
-                       Control* ctl, hipStream_t st);
 // The kernels of one wavefront iteration, in launch order.
 enum Stage {
     kStageExtend = 0,
@@ -33,10 +31,12 @@ enum Stage {
     kStageShadowExact,
     kStageRetire,
     kStageReset,
+    kStageRefill,
     kStageCount
 };
-constexpr const char* kStageNames[kStageCount] = {"k_extend", "k_extend_exact", "k_shade", "k_shadow",
-                                                  "k_shadow_exact", "k_retire", "k_reset"};
+constexpr const char* kStageNames[kStageCount] = {"k_extend",       "k_extend_exact", "k_shade",
+                                                  "k_shadow",       "k_shadow_exact", "k_retire",
+                                                  "k_reset",        "k_refill"};
 
 // Path-id queues of the wavefront loop.
 struct Queues {
@@ -44,10 +44,21 @@ struct Queues {
     uint32_t* retire; // finished paths
     uint32_t* xq;     // extend rays needing the exact traversal
     uint32_t* xsq;    // shadow records needing the exact traversal
+    uint32_t* freeq;  // stack of idle path slots
 };
 
-hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const DevJob* jobs, uint32_t njobs,
-                        const Queues& Q, int parity, Control* ctl, unsigned grid, hipStream_t st);
+// Launch shape of the wavefront kernels (persistent-style grids that loop
+// over their queue).
+struct LaunchShape {
+    unsigned grid;   // workgroups
+    unsigned block;  // threads per workgroup: 64, 128 or 256
+};
+
+hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const DevJob* jobs, const Queues& Q,
+                        int parity, Control* ctl, JobCounters* cnt, LaunchShape shape, hipStream_t st);
+hipError_t launch_pool_init(uint32_t* freeq, uint32_t pool, Control* ctl, hipStream_t st);
+hipError_t launch_admit(Control* ctl, JobCounters* cnt, uint64_t total, uint32_t seq_lo, uint32_t seq_hi, uint32_t slot,
+                        hipStream_t st);
 hipError_t launch_accumulate(const DevJob* jobs, uint32_t job, uint64_t npix, float* film, hipStream_t st);
 
 }  // namespace cr

@@ -49,11 +49,28 @@ __device__ inline void wave_add(unsigned long long* dst, uint32_t v) {
     if (__lane_id() == 0 && s) atomicAdd(dst, s);
 }
 
+// Sum `v` per distinct `key` over the wave; one atomic per key into arr[key].
+// The loop is wave-uniform and runs once per distinct key (usually once).
+__device__ inline void wave_add_keyed(unsigned long long* arr, uint32_t key, uint32_t v) {
+    unsigned long long pending = __ballot(v != 0);
+    const int lane = __lane_id();
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const uint32_t k = __shfl(key, leader);
+        const bool mine = ((pending >> lane) & 1ull) && key == k;
+        pending &= ~__ballot(mine);
+        unsigned long long s = mine ? v : 0;
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == leader) atomicAdd(&arr[k], s);
+    }
+}
+
+// Errors go to a ring the host drains after every step; a step that raises
+// more than kMaxErrors errors keeps the last kMaxErrors records.
 __device__ inline void report_error(Control* ctl, uint32_t job, uint64_t key, uint32_t code, uint32_t depth,
                                     uint32_t tri, uint32_t material, sf::f64 t, V3 point, V3 origin, V3 dir) {
-    const uint32_t slot = atomicAdd(&ctl->err_count, 1u);
-    if (slot >= uint32_t(kMaxErrors)) return;
-    ErrorRec& e = ctl->errors[slot];
+    const uint32_t n = atomicAdd(&ctl->err_count, 1u);
+    ErrorRec& e = ctl->errors[n % uint32_t(kMaxErrors)];
     e.key = key;
     e.job = job;
     e.code = code;
@@ -102,14 +119,16 @@ __global__ void __launch_bounds__(256) k_trace_rays(const DevScene* scene, const
 }
 
 // ------------------------------------------------------------------ path state
-__device__ inline uint32_t find_job(const DevJob* jobs, uint32_t njobs, uint64_t item) {
-    uint32_t lo = 0, hi = njobs;
+// Slot of the job owning global work item `item`: the last live sequence
+// number whose item_base is <= item (item_base grows with the sequence).
+__device__ inline uint32_t find_job(const DevJob* jobs, const Control* ctl, uint64_t item) {
+    uint32_t lo = ctl->seq_lo, hi = ctl->seq_hi;
     while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (jobs[mid].item_base <= item) lo = mid;
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (jobs[mid % kJobSlots].item_base <= item) lo = mid;
         else hi = mid;
     }
-    return lo;
+    return lo % kJobSlots;
 }
 
 __device__ inline PathState load_state(const PathSoA& P, uint32_t slot) {
@@ -156,6 +175,12 @@ __device__ inline bool init_path(const PathSoA& P, uint32_t slot, const DevJob& 
     const uint32_t pixel = uint32_t(local - uint64_t(sample) * npix);
     PathState st;
     const bool ok = begin_sample(J, pixel, sample, precise, st);
+    if (!ok) {  // retired by the next shade with zero radiance
+        st.flags |= kFailed;
+        st.L = V3{0, 0, 0};
+        st.tmin = 0;
+        st.tmax = -1;
+    }
     P.job[slot] = ji;
     P.pixel[slot] = pixel;
     P.sample[slot] = sample;
@@ -166,16 +191,58 @@ __device__ inline bool init_path(const PathSoA& P, uint32_t slot, const DevJob& 
     return ok;
 }
 
-__global__ void __launch_bounds__(256) k_seed(PathSoA P, const DevJob* jobs, uint32_t njobs, uint32_t pool,
-                                              uint32_t* q0, Control* ctl) {
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    bool queued = false;
-    if (slot < pool && uint64_t(slot) < ctl->total) {
-        const uint32_t j = find_job(jobs, njobs, slot);
-        queued = init_path(P, slot, jobs[j], j, uint64_t(slot) - jobs[j].item_base, false, ctl);
+// Start work item `item` in path slot `path` (a failed camera ray is still
+// queued: shade retires it with zero radiance after the error is recorded).
+__device__ inline void start_item(const PathSoA& P, const DevJob* jobs, uint32_t path, uint64_t item,
+                                  Control* ctl) {
+    const uint32_t job = find_job(jobs, ctl, item);
+    const DevJob& J = jobs[job];
+    if (item >= J.item_base && item - J.item_base < J.items) {
+        init_path(P, path, J, job, item - J.item_base, false, ctl);
+        return;
     }
-    const uint32_t pos = wave_append(&ctl->qcount[0], queued);
-    if (queued) q0[pos] = slot;
+    // Unreachable with a consistent job table; kept so that a bookkeeping bug
+    // surfaces as an error instead of a write outside a sample buffer.
+    report_error(ctl, job, 0, kErrInternal, 0, 0, 0, sf::zero(0), V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0});
+    P.job[path] = job;
+    P.pixel[path] = 0;
+    P.sample[path] = uint32_t(J.spp);  // outside the sample buffer: never stored
+    P.flags[path] = kFailed;
+    P.rad[path] = make_float4(0, 0, 0, 0);
+    P.ray_o[path] = make_float4(0, 0, 0, 0);
+    P.ray_d[path] = make_float4(0, 0, 1, -1);
+    P.ray_lo[path] = make_float4(0, 0, 0, 0);
+    P.prev[path] = kNoTriangle;
+}
+
+// ------------------------------------------------------------------ extend
+// FP32-only closest hit; rays with an undecided triangle go to the exact queue.
+__global__ void __launch_bounds__(256) k_extend(PathSoA P, const DevJob* jobs, const uint32_t* queue, uint32_t* xq,
+                                                int parity, Control* ctl, JobCounters* cnt) {
+    const uint32_t n = ctl->qcount[parity];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t rounds = (n + stride - 1) / stride;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t r = 0; r < rounds; ++r, i += stride) {
+        bool exact = false;
+        uint32_t path = 0, job = 0;
+        TraceCount tc{0, 0, 0};
+        if (i < n) {
+            path = queue[i];
+            job = P.job[path];
+            const DevScene& s = *jobs[job].scene;
+            const float4 ro = P.ray_o[path], rd = P.ray_d[path], rl = P.ray_lo[path];
+            RayQuery q;
+            setup_query(q, xyz(ro), xyz(rd), ro.w, rd.w, xyz(rl), P.prev[path]);
+            const ClosestResult best = traverse<false, false>(s, q, tc);
+            exact = best.need_exact && !best.overflow;
+            P.hit[path] = best.overflow ? kNoTriangle - 1 : best.prim;
+        }
+        const uint32_t pos = wave_append(&ctl->xcount, exact);
+        if (exact) xq[pos] = path;
+        wave_add_keyed(cnt->extend_nodes, job, tc.nodes);
+        wave_add_keyed(cnt->extend_tris, job, tc.tris);
+    }
 }
 
 // Exact-queue indexing: consecutive entries go to different waves, so a wave
@@ -188,73 +255,49 @@ __device__ inline uint32_t spread_first() {
 }
 __device__ inline uint32_t spread_stride() { return spread_waves() * 64; }
 
-// ------------------------------------------------------------------ extend
-// FP32-only closest hit; rays with an undecided triangle go to the exact queue.
-__global__ void __launch_bounds__(256) k_extend(PathSoA P, const DevJob* jobs, const uint32_t* queue, uint32_t* xq,
-                                                int parity, Control* ctl) {
-    const uint32_t n = ctl->qcount[parity];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t rounds = (n + stride - 1) / stride;
-    TraceCount tc{0, 0, 0};
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint32_t r = 0; r < rounds; ++r, i += stride) {
-        bool exact = false;
-        uint32_t path = 0;
-        if (i < n) {
-            path = queue[i];
-            const DevScene& s = *jobs[P.job[path]].scene;
-            const float4 ro = P.ray_o[path], rd = P.ray_d[path], rl = P.ray_lo[path];
-            RayQuery q;
-            setup_query(q, xyz(ro), xyz(rd), ro.w, rd.w, xyz(rl), P.prev[path]);
-            const ClosestResult best = traverse<false, false>(s, q, tc);
-            exact = best.need_exact && !best.overflow;
-            P.hit[path] = best.overflow ? kNoTriangle - 1 : best.prim;
-        }
-        const uint32_t pos = wave_append(&ctl->xcount, exact);
-        if (exact) xq[pos] = path;
-    }
-    wave_add(&ctl->extend_nodes, tc.nodes);
-    wave_add(&ctl->extend_tris, tc.tris);
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->extend_rays, (unsigned long long)n);
-}
-
 // Closest hit with binary64-exact resolution of undecided tests.
-__global__ void __launch_bounds__(256) k_extend_exact(PathSoA P, const DevJob* jobs, const uint32_t* xq, Control* ctl) {
+__global__ void __launch_bounds__(256) k_extend_exact(PathSoA P, const DevJob* jobs, const uint32_t* xq, Control* ctl,
+                                                      JobCounters* cnt) {
     const uint32_t n = ctl->xcount;
-    TraceCount tc{0, 0, 0};
     for (uint32_t i = spread_first(); i < n; i += spread_stride()) {
         const uint32_t path = xq[i];
-        const DevScene& s = *jobs[P.job[path]].scene;
+        const uint32_t job = P.job[path];
+        const DevScene& s = *jobs[job].scene;
         const float4 ro = P.ray_o[path], rd = P.ray_d[path], rl = P.ray_lo[path];
         RayQuery q;
         setup_query(q, xyz(ro), xyz(rd), ro.w, rd.w, xyz(rl), P.prev[path]);
+        TraceCount tc{0, 0, 0};
         const ClosestResult best = traverse<false, true>(s, q, tc);
         P.hit[path] = best.overflow ? kNoTriangle - 1 : best.prim;
+        atomicAdd(&cnt->fallbacks[job], (unsigned long long)tc.fallbacks);
+        atomicAdd(&cnt->exact_rays[job], 1ull);
     }
-    wave_add(&ctl->fallbacks, tc.fallbacks);
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->exact_rays, (unsigned long long)n);
 }
 
 // ------------------------------------------------------------------ shade
 __global__ void __launch_bounds__(256) k_shade(PathSoA P, ShadowSoA S, const DevJob* jobs, const uint32_t* queue,
-                                               uint32_t* next_queue, uint32_t* retire, int parity, Control* ctl) {
+                                               uint32_t* next_queue, uint32_t* retire, int parity, Control* ctl,
+                                               JobCounters* cnt) {
     const uint32_t n = ctl->qcount[parity];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t rounds = (n + stride - 1) / stride;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t r = 0; r < rounds; ++r, i += stride) {
         const bool live = i < n;
-        uint32_t path = 0, action = kRetire;
+        uint32_t path = 0, action = kRetire, ji = 0;
+        bool replayed = false;
         ShadowOut sh;
         sh.emit = false;
         if (live) {
             path = queue[i];
-            const uint32_t ji = P.job[path];
+            ji = P.job[path];
             const DevJob& J = jobs[ji];
             const uint64_t key = uint64_t(P.pixel[path]) * uint64_t(J.spp) + P.sample[path];
             const uint32_t prim = P.hit[path];
             PathState st = load_state(P, path);
-            if (prim == kNoTriangle - 1) {
+            if (st.flags & kFailed) {
+                action = kRetire;
+            } else if (prim == kNoTriangle - 1) {
                 report_error(ctl, ji, key, kErrStack, st.flags & kDepthMask, 0, 0, sf::zero(0), st.o, st.o, st.d);
                 st.L = v3(0);
                 action = kRetire;
@@ -266,12 +309,15 @@ __global__ void __launch_bounds__(256) k_shade(PathSoA P, ShadowSoA S, const Dev
                                  err.origin, err.direction);
             }
             if (action == kReplay) {
-                atomicAdd(&ctl->replays, 1ull);
+                replayed = true;
                 begin_sample(J, P.pixel[path], P.sample[path], true, st);
                 action = kContinue;
             }
             store_state(P, path, st);
         }
+        wave_add_keyed(cnt->extend_rays, ji, live ? 1u : 0u);
+        wave_add_keyed(cnt->shadow_rays, ji, live && sh.emit ? 1u : 0u);
+        wave_add_keyed(cnt->replays, ji, replayed ? 1u : 0u);
         const uint32_t spos = wave_append(&ctl->scount, live && sh.emit);
         if (live && sh.emit) {
             S.path[spos] = path;
@@ -305,16 +351,18 @@ __device__ inline void shadow_query(RayQuery& q, const ShadowSoA& S, uint32_t i)
 
 // FP32-only visibility; undecided segments without a certain blocker go to the exact queue.
 __global__ void __launch_bounds__(256) k_shadow(PathSoA P, ShadowSoA S, const DevJob* jobs, uint32_t* xsq,
-                                                Control* ctl) {
+                                                Control* ctl, JobCounters* cnt) {
     const uint32_t n = ctl->scount;
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t rounds = (n + stride - 1) / stride;
-    TraceCount tc{0, 0, 0};
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t r = 0; r < rounds; ++r, i += stride) {
         bool exact = false;
+        uint32_t job = 0;
+        TraceCount tc{0, 0, 0};
         if (i < n) {
-            const DevScene& s = *jobs[P.job[S.path[i]]].scene;
+            job = P.job[S.path[i]];
+            const DevScene& s = *jobs[job].scene;
             RayQuery q;
             shadow_query(q, S, i);
             const ClosestResult b = traverse<true, false>(s, q, tc);
@@ -325,41 +373,44 @@ __global__ void __launch_bounds__(256) k_shadow(PathSoA P, ShadowSoA S, const De
         }
         const uint32_t pos = wave_append(&ctl->xscount, exact);
         if (exact) xsq[pos] = i;
+        wave_add_keyed(cnt->shadow_nodes, job, tc.nodes);
+        wave_add_keyed(cnt->shadow_tris, job, tc.tris);
     }
-    wave_add(&ctl->shadow_nodes, tc.nodes);
-    wave_add(&ctl->shadow_tris, tc.tris);
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->shadow_rays, (unsigned long long)n);
 }
 
 __global__ void __launch_bounds__(256) k_shadow_exact(PathSoA P, ShadowSoA S, const DevJob* jobs, const uint32_t* xsq,
-                                                      Control* ctl) {
+                                                      Control* ctl, JobCounters* cnt) {
     const uint32_t n = ctl->xscount;
-    TraceCount tc{0, 0, 0};
     for (uint32_t j = spread_first(); j < n; j += spread_stride()) {
         const uint32_t i = xsq[j];
-        const DevScene& s = *jobs[P.job[S.path[i]]].scene;
+        const uint32_t job = P.job[S.path[i]];
+        const DevScene& s = *jobs[job].scene;
         RayQuery q;
         shadow_query(q, S, i);
+        TraceCount tc{0, 0, 0};
         const ClosestResult b = traverse<true, true>(s, q, tc);
         if (b.prim == kNoTriangle && !b.overflow) add_contribution(P, S, i);
+        atomicAdd(&cnt->fallbacks[job], (unsigned long long)tc.fallbacks);
+        atomicAdd(&cnt->exact_rays[job], 1ull);
     }
-    wave_add(&ctl->fallbacks, tc.fallbacks);
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctl->exact_rays, (unsigned long long)n);
 }
 
 // ------------------------------------------------------------------ retire + regenerate
-__global__ void __launch_bounds__(256) k_retire(PathSoA P, const DevJob* jobs, uint32_t njobs, const uint32_t* retire,
-                                                uint32_t* next_queue, int parity, Control* ctl) {
+// Finished paths store their sample; each slot then claims the next work item
+// or, when none is left, goes onto the free stack.
+__global__ void __launch_bounds__(256) k_retire(PathSoA P, const DevJob* jobs, const uint32_t* retire,
+                                                uint32_t* next_queue, uint32_t* freeq, int parity, Control* ctl,
+                                                JobCounters* cnt) {
     const uint32_t n = ctl->rcount;
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t rounds = (n + stride - 1) / stride;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t r = 0; r < rounds; ++r, i += stride) {
         const bool live = i < n;
-        uint32_t path = 0;
+        uint32_t path = 0, ji = 0;
         if (live) {
             path = retire[i];
-            const uint32_t ji = P.job[path];
+            ji = P.job[path];
             const DevJob& J = jobs[ji];
             const float4 L = P.rad[path];
             const V3 l = xyz(L);
@@ -368,26 +419,94 @@ __global__ void __launch_bounds__(256) k_retire(PathSoA P, const DevJob* jobs, u
                 report_error(ctl, ji, uint64_t(P.pixel[path]) * uint64_t(J.spp) + P.sample[path], kErrBadRadiance,
                              0, 0, 0, sf::zero(0), V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0});
             }
-            J.samples[uint64_t(P.sample[path]) * npix + P.pixel[path]] = make_float4(l.x, l.y, l.z, 0);
+            const uint64_t at = uint64_t(P.sample[path]) * npix + P.pixel[path];
+            if (at < J.items) J.samples[at] = make_float4(l.x, l.y, l.z, 0);
         }
+        wave_add_keyed(cnt->retired, ji, live ? 1u : 0u);
         const unsigned long long item = wave_claim(&ctl->issued, live);
-        bool queued = false;
-        if (live && item < ctl->total) {
-            const uint32_t j = find_job(jobs, njobs, item);
-            queued = init_path(P, path, jobs[j], j, item - jobs[j].item_base, false, ctl);
-        }
-        if (live) atomicAdd(&ctl->retired, 1ull);
+        const bool queued = live && item < ctl->total;
+        if (queued) start_item(P, jobs, path, item, ctl);
         const uint32_t qpos = wave_append(&ctl->qcount[parity ^ 1], queued);
         if (queued) next_queue[qpos] = path;
+        const bool idle = live && !queued;
+        const uint32_t fpos = wave_append(&ctl->fcount, idle);
+        if (idle) freeq[fpos] = path;
     }
 }
 
+// End of an iteration: clear the consumed counters and reserve the work items
+// that idle slots start in the following refill.  Claims past `total` made by
+// k_retire were not used (those slots went idle) and are given back here.
 __global__ void k_reset(int parity, Control* ctl) {
     ctl->qcount[parity] = 0;
     ctl->scount = 0;
     ctl->rcount = 0;
     ctl->xcount = 0;
     ctl->xscount = 0;
+    unsigned long long issued = ctl->issued;
+    const unsigned long long total = ctl->total;
+    if (issued > total) issued = total;
+    const unsigned long long avail = total - issued;
+    const uint32_t take = avail < ctl->fcount ? uint32_t(avail) : ctl->fcount;
+    ctl->refill_base = issued;
+    ctl->refill_n = take;
+    ctl->fcount -= take;
+    ctl->refill_from = ctl->fcount;
+    ctl->issued = issued + take;
+}
+
+// Idle slots start the items reserved by k_reset; they join the next extend queue.
+__global__ void __launch_bounds__(256) k_refill(PathSoA P, const DevJob* jobs, const uint32_t* freeq,
+                                                uint32_t* next_queue, int parity, Control* ctl) {
+    const uint32_t n = ctl->refill_n;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t rounds = (n + stride - 1) / stride;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t r = 0; r < rounds; ++r, i += stride) {
+        const bool live = i < n;
+        uint32_t path = 0;
+        if (live) {
+            path = freeq[ctl->refill_from + i];
+            start_item(P, jobs, path, ctl->refill_base + i, ctl);
+        }
+        const uint32_t qpos = wave_append(&ctl->qcount[parity ^ 1], live);
+        if (live) next_queue[qpos] = path;
+    }
+}
+
+// Host-side admission: new total and live sequence range; the admitted job's
+// slot starts with zero counters.
+__global__ void k_admit(Control* ctl, JobCounters* cnt, unsigned long long total, uint32_t seq_lo, uint32_t seq_hi,
+                        uint32_t slot) {
+    ctl->total = total;
+    ctl->seq_lo = seq_lo;
+    ctl->seq_hi = seq_hi;
+    cnt->retired[slot] = 0;
+    cnt->replays[slot] = 0;
+    cnt->extend_rays[slot] = 0;
+    cnt->shadow_rays[slot] = 0;
+    cnt->extend_nodes[slot] = 0;
+    cnt->extend_tris[slot] = 0;
+    cnt->shadow_nodes[slot] = 0;
+    cnt->shadow_tris[slot] = 0;
+    cnt->fallbacks[slot] = 0;
+    cnt->exact_rays[slot] = 0;
+}
+
+// Initial control state for a pool of `pool` idle slots (all on the free stack).
+__global__ void __launch_bounds__(256) k_pool_init(uint32_t* freeq, uint32_t pool, Control* ctl) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < pool) freeq[i] = i;
+    if (i == 0) {
+        ctl->qcount[0] = ctl->qcount[1] = 0;
+        ctl->scount = ctl->rcount = ctl->xcount = ctl->xscount = 0;
+        ctl->fcount = pool;
+        ctl->refill_n = ctl->refill_from = 0;
+        ctl->seq_lo = ctl->seq_hi = 0;
+        ctl->err_count = 0;
+        ctl->issued = ctl->total = ctl->refill_base = 0;
+        ctl->fallbacks = 0;
+    }
 }
 
 // ------------------------------------------------------------------ accumulate
@@ -420,39 +539,47 @@ hipError_t launch_trace_rays(const DevScene* scene, const RayIn* rays, uint64_t 
     return hipGetLastError();
 }
 
-hipError_t launch_seed(const PathSoA& P, const DevJob* jobs, uint32_t njobs, uint32_t pool, uint32_t* q0, Control* ctl,
-                       hipStream_t st) {
-    hipLaunchKernelGGL(k_seed, dim3(blocks_for(pool, 256)), dim3(256), 0, st, P, jobs, njobs, pool, q0, ctl);
+hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const DevJob* jobs, const Queues& Q,
+                        int parity, Control* ctl, JobCounters* cnt, LaunchShape shape, hipStream_t st) {
+    const dim3 g(shape.grid), b(shape.block);
+    switch (stage) {
+        case kStageExtend:
+            hipLaunchKernelGGL(k_extend, g, b, 0, st, P, jobs, Q.q[parity], Q.xq, parity, ctl, cnt);
+            break;
+        case kStageExtendExact:
+            hipLaunchKernelGGL(k_extend_exact, g, b, 0, st, P, jobs, Q.xq, ctl, cnt);
+            break;
+        case kStageShade:
+            hipLaunchKernelGGL(k_shade, g, b, 0, st, P, S, jobs, Q.q[parity], Q.q[parity ^ 1], Q.retire, parity, ctl,
+                               cnt);
+            break;
+        case kStageShadow:
+            hipLaunchKernelGGL(k_shadow, g, b, 0, st, P, S, jobs, Q.xsq, ctl, cnt);
+            break;
+        case kStageShadowExact:
+            hipLaunchKernelGGL(k_shadow_exact, g, b, 0, st, P, S, jobs, Q.xsq, ctl, cnt);
+            break;
+        case kStageRetire:
+            hipLaunchKernelGGL(k_retire, g, b, 0, st, P, jobs, Q.retire, Q.q[parity ^ 1], Q.freeq, parity, ctl, cnt);
+            break;
+        case kStageReset:
+            hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, parity, ctl);
+            break;
+        default:
+            hipLaunchKernelGGL(k_refill, g, b, 0, st, P, jobs, Q.freeq, Q.q[parity ^ 1], parity, ctl);
+            break;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const DevJob* jobs, uint32_t njobs,
-                        const Queues& Q, int parity, Control* ctl, unsigned grid, hipStream_t st) {
-    switch (stage) {
-        case kStageExtend:
-            hipLaunchKernelGGL(k_extend, dim3(grid), dim3(256), 0, st, P, jobs, Q.q[parity], Q.xq, parity, ctl);
-            break;
-        case kStageExtendExact:
-            hipLaunchKernelGGL(k_extend_exact, dim3(grid), dim3(256), 0, st, P, jobs, Q.xq, ctl);
-            break;
-        case kStageShade:
-            hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, st, P, S, jobs, Q.q[parity], Q.q[parity ^ 1],
-                               Q.retire, parity, ctl);
-            break;
-        case kStageShadow:
-            hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(256), 0, st, P, S, jobs, Q.xsq, ctl);
-            break;
-        case kStageShadowExact:
-            hipLaunchKernelGGL(k_shadow_exact, dim3(grid), dim3(256), 0, st, P, S, jobs, Q.xsq, ctl);
-            break;
-        case kStageRetire:
-            hipLaunchKernelGGL(k_retire, dim3(grid), dim3(256), 0, st, P, jobs, njobs, Q.retire, Q.q[parity ^ 1],
-                               parity, ctl);
-            break;
-        default:
-            hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, parity, ctl);
-            break;
-    }
+hipError_t launch_pool_init(uint32_t* freeq, uint32_t pool, Control* ctl, hipStream_t st) {
+    hipLaunchKernelGGL(k_pool_init, dim3(blocks_for(pool, 256)), dim3(256), 0, st, freeq, pool, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_admit(Control* ctl, JobCounters* cnt, uint64_t total, uint32_t seq_lo, uint32_t seq_hi, uint32_t slot,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_admit, dim3(1), dim3(1), 0, st, ctl, cnt, (unsigned long long)total, seq_lo, seq_hi, slot);
     return hipGetLastError();
 }
 

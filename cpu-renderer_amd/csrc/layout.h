@@ -146,6 +146,7 @@ enum PathFlag : uint32_t {
     kDepthMask = 0xffu,
     kLastDelta = 1u << 8,
     kPrecise = 1u << 9,
+    kFailed = 1u << 10,  // camera ray failed: retire with zero radiance
 };
 
 struct PathSoA {
@@ -190,26 +191,45 @@ enum ErrorCode : uint32_t {
     kErrOffsetOverflow = 6,
     kErrCamera = 7,
     kErrStack = 8,
+    kErrInternal = 9,  // work item outside the job it maps to (engine bookkeeping)
 };
 
-constexpr int kMaxErrors = 64;
+constexpr int kMaxErrors = 64;  // error ring (records of one host step)
 
+// Device job table: job sequence number s lives in slot s % kJobSlots; the
+// live sequence range [seq_lo, seq_hi) is ordered by item_base.
+constexpr uint32_t kJobSlots = 1024;
+
+// Wavefront control block.  Queue counters are per iteration; issued/total
+// count global work items (camera samples) over every job ever admitted.
 struct Control {
     uint32_t qcount[2];
     uint32_t scount, rcount;
     uint32_t xcount, xscount;  // extend / shadow rays needing the exact traversal
-    uint32_t err_count;
-    uint32_t pad0;
+    uint32_t fcount;           // idle path slots on the free stack
+    uint32_t refill_n;         // free slots the next refill starts
+    uint32_t refill_from;      // free-stack index of the first of them
+    uint32_t seq_lo, seq_hi;   // live job sequence range
+    uint32_t err_count;        // monotonic; records go to errors[n % kMaxErrors]
     unsigned long long issued;
     unsigned long long total;
-    unsigned long long retired;
-    unsigned long long replays;
-    unsigned long long extend_rays;
-    unsigned long long shadow_rays;
-    unsigned long long fallbacks;
-    unsigned long long exact_rays;
-    unsigned long long extend_nodes, extend_tris, shadow_nodes, shadow_tris;
+    unsigned long long refill_base;  // first item of the refill range
+    unsigned long long fallbacks;    // exact triangle tests of k_trace_rays
     ErrorRec errors[kMaxErrors];
+};
+
+// Per-job counters, one array per quantity, indexed by job slot.
+struct JobCounters {
+    unsigned long long retired[kJobSlots];
+    unsigned long long replays[kJobSlots];
+    unsigned long long extend_rays[kJobSlots];
+    unsigned long long shadow_rays[kJobSlots];
+    unsigned long long extend_nodes[kJobSlots];
+    unsigned long long extend_tris[kJobSlots];
+    unsigned long long shadow_nodes[kJobSlots];
+    unsigned long long shadow_tris[kJobSlots];
+    unsigned long long fallbacks[kJobSlots];
+    unsigned long long exact_rays[kJobSlots];
 };
 
 }  // namespace cr

@@ -421,6 +421,23 @@ def batch_request(seed: int, sky: Optional[np.ndarray] = None, max_triangles: in
                     bounces, seed)
 
 
+def served_stream(count: int = 200, rate: float = 20.0, catalog: int = 64, seed: int = 0):
+    """Poisson arrivals (rate req/s) and per-request (catalog scene index, side, spp, bounces)
+    of the served mix; sizes follow batch_request's ranges (128^2..1024^2 px, 4..256 spp,
+    8..16 bounces)."""
+    rng = np.random.Generator(np.random.PCG64(seed + 9000))
+    gaps = rng.exponential(1.0 / rate, count)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    reqs = []
+    for _ in range(count):
+        k = int(rng.integers(0, catalog))
+        side = int(rng.integers(128, 1025))
+        spp = max(4, min(256, _loguniform_int(rng, 4, 256)))
+        bounces = int(rng.integers(8, 17))
+        reqs.append((k, side, spp, bounces))
+    return arrivals, reqs
+
+
 def random_rays(scene: abi.SceneData, count: int, seed: int, spread: float = 1.5) -> np.ndarray:
     """Ray set in the style of the reference BVH test (tests/test_geometry.cpp:533-568):
     random origins around the scene bounds, every other ray aimed at a point of a

@@ -37,7 +37,7 @@ extern "C" {
 #define CRG_EPATH     3  /* non-finite/negative transport or replay failure  */
 #define CRG_EDEVICE   4  /* HIP runtime failure or no usable gfx950 device   */
 #define CRG_ENOTFOUND 5  /* unknown scene / environment / request handle     */
-#define CRG_EBUSY     6  /* admission refused: footprint over budget         */
+#define CRG_EBUSY     6  /* request still in flight (crg_wait timed out)    */
 
 /* ---- scene model ------------------------------------------------------- */
 /* Texture binding: texture = -1 means "no map" (reference src/scene.h:179-182). */
@@ -142,6 +142,8 @@ typedef struct crg_stats {
     uint64_t shadow_tris;      /* triangles tested by any-hit queries          */
     uint64_t exact_fallbacks;  /* triangle tests decided in emulated binary64  */
     uint64_t exact_rays;       /* queries rerun by the exact traversal         */
+    double queue_seconds;      /* submission -> admission (batched / served)   */
+    double latency_seconds;    /* submission -> film written (time to image)   */
 } crg_stats;
 
 /*
@@ -202,6 +204,73 @@ int crg_render(crg_ctx* ctx, uint64_t scene_id, uint64_t env_id, const crg_camer
  * any != 0 selects visibility queries (reference src/geometry.cpp:608). */
 int crg_trace_rays(crg_ctx* ctx, uint64_t scene_id, const crg_ray* rays, uint64_t count,
                    crg_hit* hits, int any);
+
+/* ---- batched and served forms ------------------------------------------
+ * Requests over resident scenes share one wavefront path pool: their camera
+ * samples are work items of one global stream, so bounces of renders of
+ * different scenes and sizes run in the same kernel launches, and a request
+ * admitted while others are in flight starts in the slots that finished paths
+ * free up (continuous batching).  Each request's film is bit-identical to its
+ * plain crg_render result.  The reference renders one call at a time
+ * (reference src/render.cpp:352-456); these entry points have no counterpart
+ * there and exist for callers that render many requests.
+ */
+typedef struct crg_request {
+    uint64_t scene;        /* resident scene handle                            */
+    uint64_t env;          /* resident environment handle                      */
+    crg_camera camera;
+    crg_options options;
+    float* film_rgb;       /* width*height*3 floats, owned by the caller until
+                              the request completes                            */
+} crg_request;
+
+/* Render `count` requests together; returns when all are done.  status[i] and
+ * stats[i] (either may be NULL) receive each request's own outcome; the return
+ * value is CRG_OK when every request succeeded, else the first failing
+ * request's status (crg_last_error names it). */
+int crg_render_batch(crg_ctx* ctx, const crg_request* requests, uint32_t count, int* status, crg_stats* stats);
+
+/* Asynchronous submission (serving).  The request is validated now and
+ * admitted as soon as its device footprint (sample buffer, film, light table)
+ * fits the memory budget next to the resident scenes and the requests in
+ * flight; a request that could never fit fails with CRG_ENOMEM. */
+int crg_submit(crg_ctx* ctx, const crg_request* request, uint64_t* ticket);
+/* Wait up to timeout_seconds (< 0: forever) for a submitted request.  Returns
+ * CRG_EBUSY if it is still in flight, else the request's own status (its film
+ * is written on CRG_OK); the ticket is consumed unless CRG_EBUSY. */
+int crg_wait(crg_ctx* ctx, uint64_t ticket, double timeout_seconds, crg_stats* stats);
+
+/* Device memory budget for admission (default 200 GiB, capped by the device). */
+int crg_set_memory_budget(crg_ctx* ctx, uint64_t bytes);
+
+typedef struct crg_engine_info {
+    uint64_t budget_bytes;     /* admission budget                              */
+    uint64_t resident_bytes;   /* resident scenes + environments                */
+    uint64_t inflight_bytes;   /* path pool + admitted requests                 */
+    uint32_t inflight;         /* admitted requests not yet complete            */
+    uint32_t pending;          /* submitted requests waiting for admission      */
+    uint64_t completed;        /* requests finished since context creation      */
+    uint64_t steps;            /* host steps (graph launches)                   */
+} crg_engine_info;
+int crg_engine_info_get(crg_ctx* ctx, crg_engine_info* info);
+
+/* ---- launch configuration and autotuning --------------------------------
+ * Results never depend on these choices (every path's result is independent of
+ * the slot, wave and iteration that ran it); only speed does. */
+typedef struct crg_tuning {
+    uint32_t pool_paths;            /* path slots in flight (queue capacity)     */
+    uint32_t workgroup;             /* threads per workgroup: 64, 128 or 256     */
+    uint32_t groups_per_cu;         /* workgroups per compute unit (grid size)   */
+    uint32_t iterations_per_graph;  /* wavefront iterations per hipGraph launch
+                                       (even, >= 2)                              */
+    uint32_t bvh_width;             /* children per BVH node (4)                 */
+} crg_tuning;
+int crg_get_tuning(crg_ctx* ctx, crg_tuning* out);
+/* Waits for the requests in flight, then applies the configuration. */
+int crg_set_tuning(crg_ctx* ctx, const crg_tuning* tuning);
+/* Times short renders of `sample` under candidate configurations, applies the
+ * fastest and returns it (with its measured rate in *samples_per_second). */
+int crg_autotune(crg_ctx* ctx, const crg_request* sample, crg_tuning* out, double* samples_per_second);
 
 /* ---- measurement ------------------------------------------------------- */
 /* Per-kernel device time, measured with HIP events recorded on the context's

@@ -57,10 +57,13 @@ def test_struct_layouts_match_ctypes(tmp_path):
               "crg_image": abi.Image, "crg_texture": abi.Texture, "crg_scene_desc": abi.SceneDesc,
               "crg_camera": abi.Camera, "crg_env_desc": abi.EnvDesc, "crg_options": abi.Options,
               "crg_stats": abi.Stats, "crg_ray": abi.Ray, "crg_hit": abi.Hit, "crg_scene_info": abi.SceneInfo,
-              "crg_kernel_time": abi.KernelTime}
+              "crg_kernel_time": abi.KernelTime, "crg_request": abi.Request, "crg_tuning": abi.Tuning,
+              "crg_engine_info": abi.EngineInfo}
     for name, cls in mirror.items():
         assert C.sizeof(cls) == sizes[name], name
     assert abi.Stats.exact_fallbacks.offset == sizes["crg_stats.exact_fallbacks"]
+    assert abi.Stats.latency_seconds.offset == sizes["crg_stats.latency_seconds"]
+    assert abi.Request.film_rgb.offset == sizes["crg_request.film_rgb"]
     assert abi.Options.seed.offset == sizes["crg_options.seed"]
 
 
