@@ -254,7 +254,13 @@ def main():
     ap.add_argument("--batch", type=int, default=64, help="requests in the batched mix")
     ap.add_argument("--served", type=int, default=200, help="requests in the served stream")
     ap.add_argument("--rate", type=float, default=20.0, help="served arrival rate (req/s)")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="one render of the workload (kernels launched one by one) plus a 1 GiB device copy, "
+                         "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs")
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe(args.workload)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
                           "n_gpus": args.gpus}))
@@ -346,6 +352,32 @@ def main():
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
     }
     print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def traffic_probe(name: str):
+    """Counter-collection workload: the copy kernel of known size calibrates the
+    HBM byte counters (bytes read = bytes written = 1 GiB)."""
+    _package()
+    import torch
+    from cr_amd import crgpu
+    w = workload(name)
+    ctx = crgpu.Context(0)
+    sid, _ = ctx.upload_scene(w.scene)
+    eid, _ = ctx.upload_env(w.env)
+    ctx.set_profiling(True)  # individual launches (no graph) so every dispatch is counted
+    r = ctx.render(sid, eid, w.camera, w.options())
+    ctx.set_profiling(False)
+    kt = ctx.kernel_times()
+    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": w.name, "extend_rays": r.extend_rays, "extend_nodes": r.extend_nodes,
+                      "extend_tris": r.extend_tris, "k_extend_launches": kt["k_extend"][0],
+                      "copy_bytes": 1 << 30}), flush=True)
     ctx.close()
 
 

@@ -187,6 +187,8 @@ int crg_scene_upload(crg_ctx* ctx, const crg_scene_desc* desc, uint64_t* scene_i
             uint32_t flags = 0;
             if (!m.double_sided && m.transmission <= 0) flags |= kPrimCull;
             if (m.alpha_mode == 1 && !(m.base_color[3] * 1.0f >= m.alpha_cutoff)) flags |= kPrimNeverHit;
+            const uint32_t cls = m.transmission > 0 ? kShadeTransmissive : m.metallic > 0 ? kShadeMetal : kShadeOpaque;
+            flags |= cls << kPrimClassShift;
             prims[i] = PrimRec{t.p[0].x, t.p[0].y, t.p[0].z, bvh.order[i], t.p[1].x, t.p[1].y, t.p[1].z, flags,
                                t.p[2].x, t.p[2].y, t.p[2].z, t.material};
             png[i] = make_float4(t.ng.x, t.ng.y, t.ng.z, 0);

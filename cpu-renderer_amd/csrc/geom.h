@@ -327,24 +327,34 @@ CR_HD inline bool leaf_prims(const DevScene& s, RayQuery& q, uint32_t first, uin
     return false;
 }
 
-// Bvh::intersect on the 4-wide tree (reference src/geometry.cpp:554-639).
-// EXACT=false is the FP32-only fast path (see leaf_prims).
+// Bvh::intersect on the 4-wide tree (reference src/geometry.cpp:554-639) as a
+// resumable traversal: step() performs one node or leaf visit, so a
+// persistent kernel can interleave several queries per lane.  EXACT=false is
+// the FP32-only fast path (see leaf_prims).
 template <bool ANY, bool EXACT = true>
-CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, TraceCount& tc) {
+struct Traversal {
     ClosestResult best;
-    best.prim = kNoTriangle;
-    best.id = 0;
-    best.exact = true;
-    best.overflow = false;
-    best.need_exact = false;
-    best.t = sf::from_float(q.tmax);
-    best.u = best.v = sf::zero(0);
-    best.tlo = best.thi = q.tmax;
-    if (!q.valid || s.node_count == 0 || q.tmax < q.tmin) return best;
     StackEntry stack[kStackSize];
-    int sp = 0;
-    uint32_t ref = 0;
-    for (;;) {
+    int sp;
+    uint32_t ref;
+
+    // Returns true when there is nothing to traverse (invalid query, empty tree).
+    CR_HDI bool begin(const DevScene& s, const RayQuery& q) {
+        best.prim = kNoTriangle;
+        best.id = 0;
+        best.exact = true;
+        best.overflow = false;
+        best.need_exact = false;
+        best.t = sf::from_float(q.tmax);
+        best.u = best.v = sf::zero(0);
+        best.tlo = best.thi = q.tmax;
+        sp = 0;
+        ref = 0;
+        return !q.valid || s.node_count == 0 || q.tmax < q.tmin;
+    }
+
+    // One visit; returns true when the query is finished (result in best).
+    CR_HDI bool step(const DevScene& s, RayQuery& q, TraceCount& tc) {
         if (!ref_is_leaf(ref)) {
             const WNode& n = s.nodes[ref];
             ++tc.nodes;
@@ -365,7 +375,7 @@ CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, TraceCount& 
             if (cnt > 0) {
                 if (sp + cnt - 1 > kStackSize) {
                     best.overflow = true;
-                    break;
+                    return true;
                 }
                 for (int k = 0; k < cnt - 1; ++k) {
                     stack[sp].ref = n.child[order[k]];
@@ -373,23 +383,29 @@ CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, TraceCount& 
                     ++sp;
                 }
                 ref = n.child[order[cnt - 1]];
-                continue;
+                return false;
             }
-        } else {
-            if (leaf_prims<ANY, EXACT>(s, q, leaf_first(ref), leaf_count(ref), best, tc)) break;
+        } else if (leaf_prims<ANY, EXACT>(s, q, leaf_first(ref), leaf_count(ref), best, tc)) {
+            return true;
         }
         // pop the next entry that may still hold a nearer hit
-        bool found = false;
         while (sp > 0) {
             const StackEntry e = stack[--sp];
             if (!ANY && e.near > best.thi) continue;
             ref = e.ref;
-            found = true;
-            break;
+            return false;
         }
-        if (!found) break;
+        return true;
     }
-    return best;
+};
+
+template <bool ANY, bool EXACT = true>
+CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, TraceCount& tc) {
+    Traversal<ANY, EXACT> tr;
+    if (!tr.begin(s, q))
+        while (!tr.step(s, q, tc)) {
+        }
+    return tr.best;
 }
 
 // Exact (t, u, v) of a known hit primitive.

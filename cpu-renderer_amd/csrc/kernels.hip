@@ -17,6 +17,17 @@
 #include "kernels.h"
 #include "path.h"
 
+// Minimum workgroups of 256 lanes per compute unit the compiler must fit
+// (register budget = occupancy): traversal kernels are latency bound and keep
+// their stacks in scratch anyway; shading needs registers for the emulated
+// binary64 arithmetic.
+#ifndef CR_TRAVERSE_MIN_BLOCKS
+#define CR_TRAVERSE_MIN_BLOCKS 6
+#endif
+#ifndef CR_SHADE_MIN_BLOCKS
+#define CR_SHADE_MIN_BLOCKS 2
+#endif
+
 namespace cr {
 
 // ------------------------------------------------------------------ helpers
@@ -216,32 +227,95 @@ __device__ inline void start_item(const PathSoA& P, const DevJob* jobs, uint32_t
 }
 
 // ------------------------------------------------------------------ extend
-// FP32-only closest hit; rays with an undecided triangle go to the exact queue.
-__global__ void __launch_bounds__(256) k_extend(PathSoA P, const DevJob* jobs, const uint32_t* queue, uint32_t* xq,
-                                                int parity, Control* ctl, JobCounters* cnt) {
+// Persistent FP32-only closest hit.  Each lane runs one query at a time as a
+// resumable traversal; every few visits, when a quarter of the wave is idle,
+// the idle lanes claim the next queue entries together (one atomic per wave),
+// so a wave's time follows the mean query length rather than its longest
+// query.  Rays with an undecided triangle go to the exact queue.
+constexpr int kVisitsPerRefill = 8;
+constexpr int kRefillBelow = 48;  // refill when at most this many lanes are busy
+
+__device__ inline uint32_t wave_claim32(uint32_t* counter, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    if (!mask) return 0xffffffffu;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+    base = __shfl(base, leader);
+    return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+}
+
+// Shading class of a closest-hit result (misses and failed queries: class 0).
+__device__ inline uint32_t shade_class(const DevScene& s, uint32_t prim) {
+    if (prim >= kNoTriangle - 1) return kShadeMiss;
+    return (s.prims[prim].flags >> kPrimClassShift) & 3u;
+}
+
+// Queue `path` for shading in its class (a per-class counting sort).
+__device__ inline void append_shade(Control* ctl, uint32_t* shq, bool pred, uint32_t cls, uint32_t path) {
+    const uint32_t pool = ctl->pool;
+#pragma unroll
+    for (uint32_t c = 0; c < kShadeClasses; ++c) {
+        const bool mine = pred && cls == c;
+        const uint32_t pos = wave_append(&ctl->shcount[c], mine);
+        if (mine) shq[c * pool + pos] = path;
+    }
+}
+
+__global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_extend(PathSoA P, const DevJob* jobs, const uint32_t* queue, uint32_t* xq,
+                                                uint32_t* shq, int parity, Control* ctl, JobCounters* cnt) {
     const uint32_t n = ctl->qcount[parity];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t rounds = (n + stride - 1) / stride;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint32_t r = 0; r < rounds; ++r, i += stride) {
-        bool exact = false;
-        uint32_t path = 0, job = 0;
-        TraceCount tc{0, 0, 0};
-        if (i < n) {
-            path = queue[i];
-            job = P.job[path];
-            const DevScene& s = *jobs[job].scene;
-            const float4 ro = P.ray_o[path], rd = P.ray_d[path], rl = P.ray_lo[path];
-            RayQuery q;
-            setup_query(q, xyz(ro), xyz(rd), ro.w, rd.w, xyz(rl), P.prev[path]);
-            const ClosestResult best = traverse<false, false>(s, q, tc);
-            exact = best.need_exact && !best.overflow;
-            P.hit[path] = best.overflow ? kNoTriangle - 1 : best.prim;
+    Traversal<false, false> tr;
+    RayQuery q;
+    const DevScene* sc = nullptr;
+    uint32_t path = 0, job = 0;
+    bool busy = false, drained = false, done_now = false;
+    TraceCount tc{0, 0, 0};
+    for (;;) {
+        if (!drained && __popcll(__ballot(busy)) <= kRefillBelow) {
+            const uint32_t idx = wave_claim32(&ctl->ehead, !busy);
+            if (__ballot(!busy && idx >= n)) drained = true;
+            if (!busy && idx < n) {
+                path = queue[idx];
+                job = P.job[path];
+                sc = jobs[job].scene;
+                const float4 ro = P.ray_o[path], rd = P.ray_d[path], rl = P.ray_lo[path];
+                setup_query(q, xyz(ro), xyz(rd), ro.w, rd.w, xyz(rl), P.prev[path]);
+                tc = TraceCount{0, 0, 0};
+                busy = !tr.begin(*sc, q);
+                if (!busy) {
+                    P.hit[path] = kNoTriangle;
+                    tr.best.overflow = false;
+                    tr.best.need_exact = false;
+                    tr.best.prim = kNoTriangle;
+                    done_now = true;
+                }
+            }
+        }
+        if (!__ballot(busy || done_now)) {
+            if (drained) break;
+            continue;
+        }
+        bool fin = done_now;
+        done_now = false;
+        for (int k = 0; k < kVisitsPerRefill && busy; ++k)
+            if (tr.step(*sc, q, tc)) {
+                busy = false;
+                fin = true;
+            }
+        const bool exact = fin && tr.best.need_exact && !tr.best.overflow;
+        uint32_t hit = kNoTriangle, cls = 0;
+        if (fin) {
+            hit = tr.best.overflow ? kNoTriangle - 1 : tr.best.prim;
+            P.hit[path] = hit;
+            if (!exact) cls = shade_class(*sc, hit);
         }
         const uint32_t pos = wave_append(&ctl->xcount, exact);
         if (exact) xq[pos] = path;
-        wave_add_keyed(cnt->extend_nodes, job, tc.nodes);
-        wave_add_keyed(cnt->extend_tris, job, tc.tris);
+        append_shade(ctl, shq, fin && !exact, cls, path);
+        wave_add_keyed(cnt->extend_nodes, job, fin ? tc.nodes : 0u);
+        wave_add_keyed(cnt->extend_tris, job, fin ? tc.tris : 0u);
     }
 }
 
@@ -256,29 +330,43 @@ __device__ inline uint32_t spread_first() {
 __device__ inline uint32_t spread_stride() { return spread_waves() * 64; }
 
 // Closest hit with binary64-exact resolution of undecided tests.
-__global__ void __launch_bounds__(256) k_extend_exact(PathSoA P, const DevJob* jobs, const uint32_t* xq, Control* ctl,
-                                                      JobCounters* cnt) {
+__global__ void __launch_bounds__(256) k_extend_exact(PathSoA P, const DevJob* jobs, const uint32_t* xq, uint32_t* shq,
+                                                      Control* ctl, JobCounters* cnt) {
     const uint32_t n = ctl->xcount;
-    for (uint32_t i = spread_first(); i < n; i += spread_stride()) {
-        const uint32_t path = xq[i];
-        const uint32_t job = P.job[path];
-        const DevScene& s = *jobs[job].scene;
-        const float4 ro = P.ray_o[path], rd = P.ray_d[path], rl = P.ray_lo[path];
-        RayQuery q;
-        setup_query(q, xyz(ro), xyz(rd), ro.w, rd.w, xyz(rl), P.prev[path]);
-        TraceCount tc{0, 0, 0};
-        const ClosestResult best = traverse<false, true>(s, q, tc);
-        P.hit[path] = best.overflow ? kNoTriangle - 1 : best.prim;
-        atomicAdd(&cnt->fallbacks[job], (unsigned long long)tc.fallbacks);
-        atomicAdd(&cnt->exact_rays[job], 1ull);
+    const uint32_t stride = spread_stride();
+    const uint32_t rounds = (n + stride - 1) / stride;
+    uint32_t i = spread_first();
+    for (uint32_t r = 0; r < rounds; ++r, i += stride) {
+        const bool live = i < n;
+        uint32_t path = 0, cls = 0;
+        if (live) {
+            path = xq[i];
+            const uint32_t job = P.job[path];
+            const DevScene& s = *jobs[job].scene;
+            const float4 ro = P.ray_o[path], rd = P.ray_d[path], rl = P.ray_lo[path];
+            RayQuery q;
+            setup_query(q, xyz(ro), xyz(rd), ro.w, rd.w, xyz(rl), P.prev[path]);
+            TraceCount tc{0, 0, 0};
+            const ClosestResult best = traverse<false, true>(s, q, tc);
+            const uint32_t hit = best.overflow ? kNoTriangle - 1 : best.prim;
+            P.hit[path] = hit;
+            cls = shade_class(s, hit);
+            atomicAdd(&cnt->fallbacks[job], (unsigned long long)tc.fallbacks);
+            atomicAdd(&cnt->exact_rays[job], 1ull);
+        }
+        append_shade(ctl, shq, live, cls, path);
     }
 }
 
 // ------------------------------------------------------------------ shade
-__global__ void __launch_bounds__(256) k_shade(PathSoA P, ShadowSoA S, const DevJob* jobs, const uint32_t* queue,
-                                               uint32_t* next_queue, uint32_t* retire, int parity, Control* ctl,
-                                               JobCounters* cnt) {
-    const uint32_t n = ctl->qcount[parity];
+__global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, ShadowSoA S, const DevJob* jobs,
+                                                                  const uint32_t* shq, uint32_t* next_queue,
+                                                                  uint32_t* retire, int parity, Control* ctl,
+                                                                  JobCounters* cnt) {
+    // The class queues are read back to back: consecutive lanes shade the same class.
+    const uint32_t c0 = ctl->shcount[0], c1 = c0 + ctl->shcount[1], c2 = c1 + ctl->shcount[2];
+    const uint32_t n = c2 + ctl->shcount[3];
+    const uint32_t pool = ctl->pool;
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t rounds = (n + stride - 1) / stride;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -289,7 +377,9 @@ __global__ void __launch_bounds__(256) k_shade(PathSoA P, ShadowSoA S, const Dev
         ShadowOut sh;
         sh.emit = false;
         if (live) {
-            path = queue[i];
+            const uint32_t c = i < c0 ? 0u : i < c1 ? 1u : i < c2 ? 2u : 3u;
+            const uint32_t base = c == 0 ? 0u : c == 1 ? c0 : c == 2 ? c1 : c2;
+            path = shq[c * pool + (i - base)];
             ji = P.job[path];
             const DevJob& J = jobs[ji];
             const uint64_t key = uint64_t(P.pixel[path]) * uint64_t(J.spp) + P.sample[path];
@@ -349,32 +439,50 @@ __device__ inline void shadow_query(RayQuery& q, const ShadowSoA& S, uint32_t i)
     setup_query(q, xyz(so), xyz(sd), so.w, sd.w, xyz(sl), __float_as_uint(sl.w));
 }
 
-// FP32-only visibility; undecided segments without a certain blocker go to the exact queue.
-__global__ void __launch_bounds__(256) k_shadow(PathSoA P, ShadowSoA S, const DevJob* jobs, uint32_t* xsq,
+// Persistent FP32-only visibility (same scheme as k_extend); undecided
+// segments without a certain blocker go to the exact queue.
+__global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_shadow(PathSoA P, ShadowSoA S, const DevJob* jobs, uint32_t* xsq,
                                                 Control* ctl, JobCounters* cnt) {
     const uint32_t n = ctl->scount;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t rounds = (n + stride - 1) / stride;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint32_t r = 0; r < rounds; ++r, i += stride) {
-        bool exact = false;
-        uint32_t job = 0;
-        TraceCount tc{0, 0, 0};
-        if (i < n) {
-            job = P.job[S.path[i]];
-            const DevScene& s = *jobs[job].scene;
-            RayQuery q;
-            shadow_query(q, S, i);
-            const ClosestResult b = traverse<true, false>(s, q, tc);
-            if (b.prim == kNoTriangle && !b.overflow) {
-                if (b.need_exact) exact = true;
-                else add_contribution(P, S, i);
+    Traversal<true, false> tr;
+    RayQuery q;
+    const DevScene* sc = nullptr;
+    uint32_t i = 0, job = 0;
+    bool busy = false, drained = false;
+    TraceCount tc{0, 0, 0};
+    for (;;) {
+        bool fin = false;
+        if (!drained && __popcll(__ballot(busy)) <= kRefillBelow) {
+            const uint32_t idx = wave_claim32(&ctl->shead, !busy);
+            if (__ballot(!busy && idx >= n)) drained = true;
+            if (!busy && idx < n) {
+                i = idx;
+                job = P.job[S.path[i]];
+                sc = jobs[job].scene;
+                shadow_query(q, S, i);
+                tc = TraceCount{0, 0, 0};
+                busy = !tr.begin(*sc, q);
+                fin = !busy;  // nothing to traverse: unoccluded
             }
+        }
+        if (!__ballot(busy || fin)) {
+            if (drained) break;
+            continue;
+        }
+        for (int k = 0; k < kVisitsPerRefill && busy; ++k)
+            if (tr.step(*sc, q, tc)) {
+                busy = false;
+                fin = true;
+            }
+        bool exact = false;
+        if (fin && tr.best.prim == kNoTriangle && !tr.best.overflow) {
+            if (tr.best.need_exact) exact = true;
+            else add_contribution(P, S, i);
         }
         const uint32_t pos = wave_append(&ctl->xscount, exact);
         if (exact) xsq[pos] = i;
-        wave_add_keyed(cnt->shadow_nodes, job, tc.nodes);
-        wave_add_keyed(cnt->shadow_tris, job, tc.tris);
+        wave_add_keyed(cnt->shadow_nodes, job, fin ? tc.nodes : 0u);
+        wave_add_keyed(cnt->shadow_tris, job, fin ? tc.tris : 0u);
     }
 }
 
@@ -443,6 +551,9 @@ __global__ void k_reset(int parity, Control* ctl) {
     ctl->rcount = 0;
     ctl->xcount = 0;
     ctl->xscount = 0;
+    ctl->ehead = 0;
+    ctl->shead = 0;
+    for (uint32_t c = 0; c < kShadeClasses; ++c) ctl->shcount[c] = 0;
     unsigned long long issued = ctl->issued;
     const unsigned long long total = ctl->total;
     if (issued > total) issued = total;
@@ -504,6 +615,9 @@ __global__ void __launch_bounds__(256) k_pool_init(uint32_t* freeq, uint32_t poo
         ctl->refill_n = ctl->refill_from = 0;
         ctl->seq_lo = ctl->seq_hi = 0;
         ctl->err_count = 0;
+        ctl->ehead = ctl->shead = 0;
+        ctl->pool = pool;
+        for (uint32_t c = 0; c < kShadeClasses; ++c) ctl->shcount[c] = 0;
         ctl->issued = ctl->total = ctl->refill_base = 0;
         ctl->fallbacks = 0;
     }
@@ -544,14 +658,13 @@ hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const
     const dim3 g(shape.grid), b(shape.block);
     switch (stage) {
         case kStageExtend:
-            hipLaunchKernelGGL(k_extend, g, b, 0, st, P, jobs, Q.q[parity], Q.xq, parity, ctl, cnt);
+            hipLaunchKernelGGL(k_extend, g, b, 0, st, P, jobs, Q.q[parity], Q.xq, Q.shq, parity, ctl, cnt);
             break;
         case kStageExtendExact:
-            hipLaunchKernelGGL(k_extend_exact, g, b, 0, st, P, jobs, Q.xq, ctl, cnt);
+            hipLaunchKernelGGL(k_extend_exact, g, b, 0, st, P, jobs, Q.xq, Q.shq, ctl, cnt);
             break;
         case kStageShade:
-            hipLaunchKernelGGL(k_shade, g, b, 0, st, P, S, jobs, Q.q[parity], Q.q[parity ^ 1], Q.retire, parity, ctl,
-                               cnt);
+            hipLaunchKernelGGL(k_shade, g, b, 0, st, P, S, jobs, Q.shq, Q.q[parity ^ 1], Q.retire, parity, ctl, cnt);
             break;
         case kStageShadow:
             hipLaunchKernelGGL(k_shadow, g, b, 0, st, P, S, jobs, Q.xsq, ctl, cnt);

@@ -46,7 +46,18 @@ static_assert(sizeof(WNode) == 128, "WNode must be one 128-byte line");
 enum PrimFlags : uint32_t {
     kPrimCull = 1u,        // single-sided opaque: reject when dot(Ng, d) >= 0
     kPrimNeverHit = 2u,    // MASK material whose constant alpha is below its cutoff
-    kPrimMaskTexture = 4u  // MASK material with a base-colour texture (unsupported on device)
+    kPrimMaskTexture = 4u,  // MASK material with a base-colour texture (unsupported on device)
+    kPrimClassShift = 8u,   // bits 8..9: shading class (scheduling only, never results)
+};
+
+// Shading classes: hits are queued per class so that a wave shades one kind of
+// surface (or only misses) at a time.
+enum ShadeClass : uint32_t {
+    kShadeMiss = 0,        // environment / no hit / failed query
+    kShadeOpaque = 1,      // opaque, non-metal
+    kShadeMetal = 2,       // opaque, metallic > 0
+    kShadeTransmissive = 3,
+    kShadeClasses = 4,
 };
 
 struct alignas(16) PrimRec {
@@ -211,6 +222,10 @@ struct Control {
     uint32_t refill_from;      // free-stack index of the first of them
     uint32_t seq_lo, seq_hi;   // live job sequence range
     uint32_t err_count;        // monotonic; records go to errors[n % kMaxErrors]
+    uint32_t ehead, shead;     // next unclaimed entry of the extend / shadow queue
+    uint32_t pool;             // path slots (capacity of every queue)
+    uint32_t shcount[kShadeClasses];  // shade queue entries per class
+    uint32_t pad1[3];
     unsigned long long issued;
     unsigned long long total;
     unsigned long long refill_base;  // first item of the refill range
