@@ -17,8 +17,8 @@ constexpr int kStallSteps = 96;
 
 uint64_t pool_bytes_for(uint32_t paths) {
     // 7 float4 + rng + 8 u32 of path state, 4 float4 + u32 of shadow records,
-    // 6 u32 queues + the per-class shade queues.
-    return uint64_t(paths) * (7 * 16 + 8 + 8 * 4 + 4 * 16 + 4 + 6 * 4 + 4 * kShadeClasses);
+    // 6 u32 queues.
+    return uint64_t(paths) * (7 * 16 + 8 + 8 * 4 + 4 * 16 + 4 + 6 * 4);
 }
 
 double seconds_between(Clock::time_point a, Clock::time_point b) {
@@ -283,9 +283,8 @@ void Engine::configure() {
                  flags_.as<uint32_t>(), prev_.as<uint32_t>(), hit_.as<uint32_t>()};
     S_ = ShadowSoA{sh_path_.as<uint32_t>(), sh_o_.as<float4>(), sh_d_.as<float4>(), sh_lo_.as<float4>(),
                    sh_c_.as<float4>()};
-    shq_.alloc(u32 * kShadeClasses);
     Q_ = Queues{{q0_.as<uint32_t>(), q1_.as<uint32_t>()}, retire_.as<uint32_t>(), xq_.as<uint32_t>(),
-                xsq_.as<uint32_t>(), freeq_.as<uint32_t>(), shq_.as<uint32_t>()};
+                xsq_.as<uint32_t>(), freeq_.as<uint32_t>()};
     ctl_.alloc(sizeof(Control));
     cnt_.alloc(sizeof(JobCounters));
     jobs_.alloc(sizeof(DevJob) * kJobSlots);

@@ -45,7 +45,6 @@ struct Queues {
     uint32_t* xq;     // extend rays needing the exact traversal
     uint32_t* xsq;    // shadow records needing the exact traversal
     uint32_t* freeq;  // stack of idle path slots
-    uint32_t* shq;    // shade queues, one per shading class (kShadeClasses x pool)
 };
 
 // Launch shape of the wavefront kernels (persistent-style grids that loop

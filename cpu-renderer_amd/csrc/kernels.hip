@@ -246,31 +246,27 @@ __device__ inline uint32_t wave_claim32(uint32_t* counter, bool pred) {
     return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
 }
 
-// Shading class of a closest-hit result (misses and failed queries: class 0).
-__device__ inline uint32_t shade_class(const DevScene& s, uint32_t prim) {
-    if (prim >= kNoTriangle - 1) return kShadeMiss;
-    return (s.prims[prim].flags >> kPrimClassShift) & 3u;
-}
+// Closest-hit word stored per path: primitive index in bits 0..29 (or one of
+// two codes), shading class in bits 30..31 so that shade can group paths by
+// class without touching the scene.
+constexpr uint32_t kHitPrimMask = 0x3fffffffu;
+constexpr uint32_t kHitMiss = 0x3fffffffu;      // no hit: environment
+constexpr uint32_t kHitOverflow = 0x3ffffffeu;  // traversal stack exhausted
 
-// Queue `path` for shading in its class (a per-class counting sort).
-__device__ inline void append_shade(Control* ctl, uint32_t* shq, bool pred, uint32_t cls, uint32_t path) {
-    const uint32_t pool = ctl->pool;
-#pragma unroll
-    for (uint32_t c = 0; c < kShadeClasses; ++c) {
-        const bool mine = pred && cls == c;
-        const uint32_t pos = wave_append(&ctl->shcount[c], mine);
-        if (mine) shq[c * pool + pos] = path;
-    }
+__device__ inline uint32_t encode_hit(const DevScene& s, const ClosestResult& b) {
+    if (b.overflow) return kHitOverflow;
+    if (b.prim == kNoTriangle) return kHitMiss;
+    return b.prim | (((s.prims[b.prim].flags >> kPrimClassShift) & 3u) << 30);
 }
 
 __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_extend(PathSoA P, const DevJob* jobs, const uint32_t* queue, uint32_t* xq,
-                                                uint32_t* shq, int parity, Control* ctl, JobCounters* cnt) {
+                                                int parity, Control* ctl, JobCounters* cnt) {
     const uint32_t n = ctl->qcount[parity];
     Traversal<false, false> tr;
     RayQuery q;
     const DevScene* sc = nullptr;
     uint32_t path = 0, job = 0;
-    bool busy = false, drained = false, done_now = false;
+    bool busy = false, drained = false;
     TraceCount tc{0, 0, 0};
     for (;;) {
         if (!drained && __popcll(__ballot(busy)) <= kRefillBelow) {
@@ -284,36 +280,23 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_extend(PathSoA 
                 setup_query(q, xyz(ro), xyz(rd), ro.w, rd.w, xyz(rl), P.prev[path]);
                 tc = TraceCount{0, 0, 0};
                 busy = !tr.begin(*sc, q);
-                if (!busy) {
-                    P.hit[path] = kNoTriangle;
-                    tr.best.overflow = false;
-                    tr.best.need_exact = false;
-                    tr.best.prim = kNoTriangle;
-                    done_now = true;
-                }
+                if (!busy) P.hit[path] = kHitMiss;
             }
         }
-        if (!__ballot(busy || done_now)) {
+        if (!__ballot(busy)) {
             if (drained) break;
             continue;
         }
-        bool fin = done_now;
-        done_now = false;
+        bool fin = false;
         for (int k = 0; k < kVisitsPerRefill && busy; ++k)
             if (tr.step(*sc, q, tc)) {
                 busy = false;
                 fin = true;
             }
         const bool exact = fin && tr.best.need_exact && !tr.best.overflow;
-        uint32_t hit = kNoTriangle, cls = 0;
-        if (fin) {
-            hit = tr.best.overflow ? kNoTriangle - 1 : tr.best.prim;
-            P.hit[path] = hit;
-            if (!exact) cls = shade_class(*sc, hit);
-        }
+        if (fin && !exact) P.hit[path] = encode_hit(*sc, tr.best);
         const uint32_t pos = wave_append(&ctl->xcount, exact);
         if (exact) xq[pos] = path;
-        append_shade(ctl, shq, fin && !exact, cls, path);
         wave_add_keyed(cnt->extend_nodes, job, fin ? tc.nodes : 0u);
         wave_add_keyed(cnt->extend_tris, job, fin ? tc.tris : 0u);
     }
@@ -330,96 +313,107 @@ __device__ inline uint32_t spread_first() {
 __device__ inline uint32_t spread_stride() { return spread_waves() * 64; }
 
 // Closest hit with binary64-exact resolution of undecided tests.
-__global__ void __launch_bounds__(256) k_extend_exact(PathSoA P, const DevJob* jobs, const uint32_t* xq, uint32_t* shq,
-                                                      Control* ctl, JobCounters* cnt) {
+__global__ void __launch_bounds__(256) k_extend_exact(PathSoA P, const DevJob* jobs, const uint32_t* xq, Control* ctl,
+                                                      JobCounters* cnt) {
     const uint32_t n = ctl->xcount;
-    const uint32_t stride = spread_stride();
-    const uint32_t rounds = (n + stride - 1) / stride;
-    uint32_t i = spread_first();
-    for (uint32_t r = 0; r < rounds; ++r, i += stride) {
-        const bool live = i < n;
-        uint32_t path = 0, cls = 0;
-        if (live) {
-            path = xq[i];
-            const uint32_t job = P.job[path];
-            const DevScene& s = *jobs[job].scene;
-            const float4 ro = P.ray_o[path], rd = P.ray_d[path], rl = P.ray_lo[path];
-            RayQuery q;
-            setup_query(q, xyz(ro), xyz(rd), ro.w, rd.w, xyz(rl), P.prev[path]);
-            TraceCount tc{0, 0, 0};
-            const ClosestResult best = traverse<false, true>(s, q, tc);
-            const uint32_t hit = best.overflow ? kNoTriangle - 1 : best.prim;
-            P.hit[path] = hit;
-            cls = shade_class(s, hit);
-            atomicAdd(&cnt->fallbacks[job], (unsigned long long)tc.fallbacks);
-            atomicAdd(&cnt->exact_rays[job], 1ull);
-        }
-        append_shade(ctl, shq, live, cls, path);
+    for (uint32_t i = spread_first(); i < n; i += spread_stride()) {
+        const uint32_t path = xq[i];
+        const uint32_t job = P.job[path];
+        const DevScene& s = *jobs[job].scene;
+        const float4 ro = P.ray_o[path], rd = P.ray_d[path], rl = P.ray_lo[path];
+        RayQuery q;
+        setup_query(q, xyz(ro), xyz(rd), ro.w, rd.w, xyz(rl), P.prev[path]);
+        TraceCount tc{0, 0, 0};
+        const ClosestResult best = traverse<false, true>(s, q, tc);
+        P.hit[path] = encode_hit(s, best);
+        atomicAdd(&cnt->fallbacks[job], (unsigned long long)tc.fallbacks);
+        atomicAdd(&cnt->exact_rays[job], 1ull);
     }
 }
 
 // ------------------------------------------------------------------ shade
+// Shade in chunks of kShadeChunk queue entries per workgroup: the chunk is
+// first counting-sorted by shading class in LDS, so each wave shades one kind
+// of surface (or only misses) instead of paying for every branch.
+constexpr uint32_t kShadeChunk = 1024;
+
 __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, ShadowSoA S, const DevJob* jobs,
-                                                                  const uint32_t* shq, uint32_t* next_queue,
+                                                                  const uint32_t* queue, uint32_t* next_queue,
                                                                   uint32_t* retire, int parity, Control* ctl,
                                                                   JobCounters* cnt) {
-    // The class queues are read back to back: consecutive lanes shade the same class.
-    const uint32_t c0 = ctl->shcount[0], c1 = c0 + ctl->shcount[1], c2 = c1 + ctl->shcount[2];
-    const uint32_t n = c2 + ctl->shcount[3];
-    const uint32_t pool = ctl->pool;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t rounds = (n + stride - 1) / stride;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint32_t r = 0; r < rounds; ++r, i += stride) {
-        const bool live = i < n;
-        uint32_t path = 0, action = kRetire, ji = 0;
-        bool replayed = false;
-        ShadowOut sh;
-        sh.emit = false;
-        if (live) {
-            const uint32_t c = i < c0 ? 0u : i < c1 ? 1u : i < c2 ? 2u : 3u;
-            const uint32_t base = c == 0 ? 0u : c == 1 ? c0 : c == 2 ? c1 : c2;
-            path = shq[c * pool + (i - base)];
-            ji = P.job[path];
-            const DevJob& J = jobs[ji];
-            const uint64_t key = uint64_t(P.pixel[path]) * uint64_t(J.spp) + P.sample[path];
-            const uint32_t prim = P.hit[path];
-            PathState st = load_state(P, path);
-            if (st.flags & kFailed) {
-                action = kRetire;
-            } else if (prim == kNoTriangle - 1) {
-                report_error(ctl, ji, key, kErrStack, st.flags & kDepthMask, 0, 0, sf::zero(0), st.o, st.o, st.d);
-                st.L = v3(0);
-                action = kRetire;
-            } else {
-                ErrorOut err;
-                action = bounce(J, st, prim, sh, err);
-                if (err.code)
-                    report_error(ctl, ji, key, err.code, err.depth, err.triangle, err.material, err.t, err.point,
-                                 err.origin, err.direction);
-            }
-            if (action == kReplay) {
-                replayed = true;
-                begin_sample(J, P.pixel[path], P.sample[path], true, st);
-                action = kContinue;
-            }
-            store_state(P, path, st);
+    __shared__ uint32_t stage[kShadeChunk], sorted[kShadeChunk];
+    __shared__ uint32_t counts[kShadeClasses], cursor[kShadeClasses];
+    const uint32_t n = ctl->qcount[parity];
+    const uint32_t per = kShadeChunk / blockDim.x;  // entries per thread and chunk
+    for (uint32_t base = blockIdx.x * kShadeChunk; base < n; base += gridDim.x * kShadeChunk) {
+        const uint32_t m = min(kShadeChunk, n - base);
+        if (threadIdx.x < kShadeClasses) counts[threadIdx.x] = cursor[threadIdx.x] = 0;
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {  // path ids < 2^25: class fits above
+            const uint32_t path = queue[base + k];
+            const uint32_t c = P.hit[path] >> 30;
+            stage[k] = path | (c << 30);
+            atomicAdd(&counts[c], 1u);
         }
-        wave_add_keyed(cnt->extend_rays, ji, live ? 1u : 0u);
-        wave_add_keyed(cnt->shadow_rays, ji, live && sh.emit ? 1u : 0u);
-        wave_add_keyed(cnt->replays, ji, replayed ? 1u : 0u);
-        const uint32_t spos = wave_append(&ctl->scount, live && sh.emit);
-        if (live && sh.emit) {
-            S.path[spos] = path;
-            S.o[spos] = make_float4(sh.o.x, sh.o.y, sh.o.z, 0.0f);
-            S.d[spos] = make_float4(sh.d.x, sh.d.y, sh.d.z, sh.tmax);
-            S.lo[spos] = make_float4(sh.lo.x, sh.lo.y, sh.lo.z, __uint_as_float(sh.ignore));
-            S.c[spos] = make_float4(sh.c.x, sh.c.y, sh.c.z, 0);
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
+            const uint32_t v = stage[k], c = v >> 30;
+            uint32_t off = 0;
+            for (uint32_t j = 0; j < c; ++j) off += counts[j];
+            sorted[off + atomicAdd(&cursor[c], 1u)] = v & kHitPrimMask;
         }
-        const uint32_t qpos = wave_append(&ctl->qcount[parity ^ 1], live && action == kContinue);
-        if (live && action == kContinue) next_queue[qpos] = path;
-        const uint32_t rpos = wave_append(&ctl->rcount, live && action == kRetire);
-        if (live && action == kRetire) retire[rpos] = path;
+        __syncthreads();
+        for (uint32_t r = 0; r < per; ++r) {
+            const uint32_t k = r * blockDim.x + threadIdx.x;
+            const bool live = k < m;
+            uint32_t path = 0, action = kRetire, ji = 0;
+            bool replayed = false;
+            ShadowOut sh;
+            sh.emit = false;
+            if (live) {
+                path = sorted[k];
+                ji = P.job[path];
+                const DevJob& J = jobs[ji];
+                const uint64_t key = uint64_t(P.pixel[path]) * uint64_t(J.spp) + P.sample[path];
+                const uint32_t hit = P.hit[path] & kHitPrimMask;
+                PathState st = load_state(P, path);
+                if (st.flags & kFailed) {
+                    action = kRetire;
+                } else if (hit == kHitOverflow) {
+                    report_error(ctl, ji, key, kErrStack, st.flags & kDepthMask, 0, 0, sf::zero(0), st.o, st.o, st.d);
+                    st.L = v3(0);
+                    action = kRetire;
+                } else {
+                    ErrorOut err;
+                    action = bounce(J, st, hit == kHitMiss ? kNoTriangle : hit, sh, err);
+                    if (err.code)
+                        report_error(ctl, ji, key, err.code, err.depth, err.triangle, err.material, err.t, err.point,
+                                     err.origin, err.direction);
+                }
+                if (action == kReplay) {
+                    replayed = true;
+                    begin_sample(J, P.pixel[path], P.sample[path], true, st);
+                    action = kContinue;
+                }
+                store_state(P, path, st);
+            }
+            wave_add_keyed(cnt->extend_rays, ji, live ? 1u : 0u);
+            wave_add_keyed(cnt->shadow_rays, ji, live && sh.emit ? 1u : 0u);
+            wave_add_keyed(cnt->replays, ji, replayed ? 1u : 0u);
+            const uint32_t spos = wave_append(&ctl->scount, live && sh.emit);
+            if (live && sh.emit) {
+                S.path[spos] = path;
+                S.o[spos] = make_float4(sh.o.x, sh.o.y, sh.o.z, 0.0f);
+                S.d[spos] = make_float4(sh.d.x, sh.d.y, sh.d.z, sh.tmax);
+                S.lo[spos] = make_float4(sh.lo.x, sh.lo.y, sh.lo.z, __uint_as_float(sh.ignore));
+                S.c[spos] = make_float4(sh.c.x, sh.c.y, sh.c.z, 0);
+            }
+            const uint32_t qpos = wave_append(&ctl->qcount[parity ^ 1], live && action == kContinue);
+            if (live && action == kContinue) next_queue[qpos] = path;
+            const uint32_t rpos = wave_append(&ctl->rcount, live && action == kRetire);
+            if (live && action == kRetire) retire[rpos] = path;
+        }
+        __syncthreads();  // `sorted` is rewritten by the next chunk
     }
 }
 
@@ -553,7 +547,6 @@ __global__ void k_reset(int parity, Control* ctl) {
     ctl->xscount = 0;
     ctl->ehead = 0;
     ctl->shead = 0;
-    for (uint32_t c = 0; c < kShadeClasses; ++c) ctl->shcount[c] = 0;
     unsigned long long issued = ctl->issued;
     const unsigned long long total = ctl->total;
     if (issued > total) issued = total;
@@ -617,7 +610,6 @@ __global__ void __launch_bounds__(256) k_pool_init(uint32_t* freeq, uint32_t poo
         ctl->err_count = 0;
         ctl->ehead = ctl->shead = 0;
         ctl->pool = pool;
-        for (uint32_t c = 0; c < kShadeClasses; ++c) ctl->shcount[c] = 0;
         ctl->issued = ctl->total = ctl->refill_base = 0;
         ctl->fallbacks = 0;
     }
@@ -658,13 +650,14 @@ hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const
     const dim3 g(shape.grid), b(shape.block);
     switch (stage) {
         case kStageExtend:
-            hipLaunchKernelGGL(k_extend, g, b, 0, st, P, jobs, Q.q[parity], Q.xq, Q.shq, parity, ctl, cnt);
+            hipLaunchKernelGGL(k_extend, g, b, 0, st, P, jobs, Q.q[parity], Q.xq, parity, ctl, cnt);
             break;
         case kStageExtendExact:
-            hipLaunchKernelGGL(k_extend_exact, g, b, 0, st, P, jobs, Q.xq, Q.shq, ctl, cnt);
+            hipLaunchKernelGGL(k_extend_exact, g, b, 0, st, P, jobs, Q.xq, ctl, cnt);
             break;
         case kStageShade:
-            hipLaunchKernelGGL(k_shade, g, b, 0, st, P, S, jobs, Q.shq, Q.q[parity ^ 1], Q.retire, parity, ctl, cnt);
+            hipLaunchKernelGGL(k_shade, g, b, 0, st, P, S, jobs, Q.q[parity], Q.q[parity ^ 1], Q.retire, parity, ctl,
+                               cnt);
             break;
         case kStageShadow:
             hipLaunchKernelGGL(k_shadow, g, b, 0, st, P, S, jobs, Q.xsq, ctl, cnt);

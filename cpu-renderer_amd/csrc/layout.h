@@ -50,8 +50,8 @@ enum PrimFlags : uint32_t {
     kPrimClassShift = 8u,   // bits 8..9: shading class (scheduling only, never results)
 };
 
-// Shading classes: hits are queued per class so that a wave shades one kind of
-// surface (or only misses) at a time.
+// Shading classes: shade groups the paths of a chunk by class so that a wave
+// shades one kind of surface (or only misses) at a time.
 enum ShadeClass : uint32_t {
     kShadeMiss = 0,        // environment / no hit / failed query
     kShadeOpaque = 1,      // opaque, non-metal
@@ -224,8 +224,6 @@ struct Control {
     uint32_t err_count;        // monotonic; records go to errors[n % kMaxErrors]
     uint32_t ehead, shead;     // next unclaimed entry of the extend / shadow queue
     uint32_t pool;             // path slots (capacity of every queue)
-    uint32_t shcount[kShadeClasses];  // shade queue entries per class
-    uint32_t pad1[3];
     unsigned long long issued;
     unsigned long long total;
     unsigned long long refill_base;  // first item of the refill range
