@@ -331,12 +331,59 @@ CR_HD inline bool leaf_prims(const DevScene& s, RayQuery& q, uint32_t first, uin
 // resumable traversal: step() performs one node or leaf visit, so a
 // persistent kernel can interleave several queries per lane.  EXACT=false is
 // the FP32-only fast path (see leaf_prims).
-template <bool ANY, bool EXACT = true>
+// Traversal stack = a small ring of the newest entries (in LDS on the device:
+// `LdsRing`; a private array on the host: `LocalRing`) on top of a private
+// overflow array holding the older entries.  One LIFO stack: when the ring is
+// full, its oldest entry moves to the overflow array; pops take the ring
+// first.  Entries are {child ref, entry distance}.
+struct LocalRing {
+    static constexpr int kSize = 8;
+    uint32_t ref[kSize];
+    float near[kSize];
+    CR_HDI void put(int i, uint32_t r, float n) {
+        ref[i] = r;
+        near[i] = n;
+    }
+    CR_HDI void get(int i, uint32_t& r, float& n) const {
+        r = ref[i];
+        n = near[i];
+    }
+};
+
+template <bool ANY, bool EXACT = true, class Ring = LocalRing>
 struct Traversal {
     ClosestResult best;
-    StackEntry stack[kStackSize];
-    int sp;
+    StackEntry stack[kStackSize];  // overflow: entries older than the ring's
+    Ring ring;
+    int sp;    // entries in `stack`
+    int head;  // ring slot of the next push
+    int nr;    // entries in the ring
     uint32_t ref;
+
+    CR_HDI void push(uint32_t r, float n) {
+        if (nr == Ring::kSize) {  // ring full: its oldest entry (at head) moves to the overflow array
+            ring.get(head, stack[sp].ref, stack[sp].near);
+            ++sp;
+            --nr;
+        }
+        ring.put(head, r, n);
+        head = head + 1 == Ring::kSize ? 0 : head + 1;
+        ++nr;
+    }
+
+    CR_HDI bool pop(StackEntry& e) {
+        if (nr > 0) {
+            head = head == 0 ? Ring::kSize - 1 : head - 1;
+            ring.get(head, e.ref, e.near);
+            --nr;
+            return true;
+        }
+        if (sp > 0) {
+            e = stack[--sp];
+            return true;
+        }
+        return false;
+    }
 
     // Returns true when there is nothing to traverse (invalid query, empty tree).
     CR_HDI bool begin(const DevScene& s, const RayQuery& q) {
@@ -349,6 +396,8 @@ struct Traversal {
         best.u = best.v = sf::zero(0);
         best.tlo = best.thi = q.tmax;
         sp = 0;
+        head = 0;
+        nr = 0;
         ref = 0;
         return !q.valid || s.node_count == 0 || q.tmax < q.tmin;
     }
@@ -373,15 +422,11 @@ struct Traversal {
                 order[k] = c;  // descending near: order[cnt-1] is the nearest
             }
             if (cnt > 0) {
-                if (sp + cnt - 1 > kStackSize) {
+                if (sp + nr + cnt - 1 > kStackSize) {
                     best.overflow = true;
                     return true;
                 }
-                for (int k = 0; k < cnt - 1; ++k) {
-                    stack[sp].ref = n.child[order[k]];
-                    stack[sp].near = nears[order[k]];
-                    ++sp;
-                }
+                for (int k = 0; k < cnt - 1; ++k) push(n.child[order[k]], nears[order[k]]);
                 ref = n.child[order[cnt - 1]];
                 return false;
             }
@@ -389,8 +434,8 @@ struct Traversal {
             return true;
         }
         // pop the next entry that may still hold a nearer hit
-        while (sp > 0) {
-            const StackEntry e = stack[--sp];
+        StackEntry e;
+        while (pop(e)) {
             if (!ANY && e.near > best.thi) continue;
             ref = e.ref;
             return false;

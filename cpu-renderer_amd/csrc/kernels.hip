@@ -235,6 +235,29 @@ __device__ inline void start_item(const PathSoA& P, const DevJob* jobs, uint32_t
 constexpr int kVisitsPerRefill = 8;
 constexpr int kRefillBelow = 48;  // refill when at most this many lanes are busy
 
+// Newest traversal stack entries of each lane in LDS (lane-minor layout, so a
+// wave's accesses to one ring slot are conflict-free); older entries overflow
+// to the lane's private array.
+#ifndef CR_LDS_RING
+#define CR_LDS_RING 12
+#endif
+template <int N>
+struct LdsRing {
+    static constexpr int kSize = N;
+    uint32_t* ref;  // this lane's column: ref[i * stride]
+    float* near;
+    uint32_t stride;
+    __device__ void put(int i, uint32_t r, float n) {
+        ref[i * stride] = r;
+        near[i * stride] = n;
+    }
+    __device__ void get(int i, uint32_t& r, float& n) const {
+        r = ref[i * stride];
+        n = near[i * stride];
+    }
+};
+using TraverseRing = LdsRing<CR_LDS_RING>;
+
 __device__ inline uint32_t wave_claim32(uint32_t* counter, bool pred) {
     const unsigned long long mask = __ballot(pred);
     if (!mask) return 0xffffffffu;
@@ -262,7 +285,10 @@ __device__ inline uint32_t encode_hit(const DevScene& s, const ClosestResult& b)
 __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_extend(PathSoA P, const DevJob* jobs, const uint32_t* queue, uint32_t* xq,
                                                 int parity, Control* ctl, JobCounters* cnt) {
     const uint32_t n = ctl->qcount[parity];
-    Traversal<false, false> tr;
+    __shared__ uint32_t ring_ref[CR_LDS_RING * 256];
+    __shared__ float ring_near[CR_LDS_RING * 256];
+    Traversal<false, false, TraverseRing> tr;
+    tr.ring = TraverseRing{ring_ref + threadIdx.x, ring_near + threadIdx.x, blockDim.x};
     RayQuery q;
     const DevScene* sc = nullptr;
     uint32_t path = 0, job = 0;
@@ -438,7 +464,10 @@ __device__ inline void shadow_query(RayQuery& q, const ShadowSoA& S, uint32_t i)
 __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_shadow(PathSoA P, ShadowSoA S, const DevJob* jobs, uint32_t* xsq,
                                                 Control* ctl, JobCounters* cnt) {
     const uint32_t n = ctl->scount;
-    Traversal<true, false> tr;
+    __shared__ uint32_t ring_ref[CR_LDS_RING * 256];
+    __shared__ float ring_near[CR_LDS_RING * 256];
+    Traversal<true, false, TraverseRing> tr;
+    tr.ring = TraverseRing{ring_ref + threadIdx.x, ring_near + threadIdx.x, blockDim.x};
     RayQuery q;
     const DevScene* sc = nullptr;
     uint32_t i = 0, job = 0;
