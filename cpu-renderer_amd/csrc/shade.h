@@ -89,6 +89,23 @@ CR_HDI void material_into(const DevMaterial& m, Surf& s) {
     s.double_sided = m.double_sided != 0;
 }
 
+// surfaceAt from a position and interpolated normal already rounded to float
+// (the filtered path of fastsurf.h); no split origin.
+CR_HDI Surf surface_from(const DevScene& sc, uint32_t prim, V3 ray_dir, V3 position, V3 normal) {
+    const PrimRec p = sc.prims[prim];
+    const PrimAux& ax = sc.aux[prim];
+    Surf s;
+    s.position = position;
+    s.position_low = V3{0, 0, 0};
+    s.ng = V3{ax.ng[0], ax.ng[1], ax.ng[2]};
+    s.front = dot_sign_exact(ray_dir, s.ng) < 0;
+    s.normal = normal;
+    if (dot(s.normal, s.ng) < 0) s.normal = -s.normal;
+    s.material = p.material;
+    material_into(sc.materials[p.material], s);
+    return s;
+}
+
 CR_HDI Surf surface_at(const DevScene& sc, uint32_t prim, V3 ray_dir, sf::f64 hu, sf::f64 hv) {
     const PrimRec p = sc.prims[prim];
     const PrimAux& ax = sc.aux[prim];
