@@ -16,9 +16,8 @@ constexpr uint32_t kMinPool = 1u << 12, kMaxPool = 1u << 25;
 constexpr int kStallSteps = 96;
 
 uint64_t pool_bytes_for(uint32_t paths) {
-    // 7 float4 + rng + 8 u32 of path state, 4 float4 + u32 of shadow records,
-    // 6 u32 queues.
-    return uint64_t(paths) * (7 * 16 + 8 + 8 * 4 + 4 * 16 + 4 + 6 * 4);
+    // ray + state records, 4 float4 + 2 u32 of shadow records, 6 u32 queues.
+    return uint64_t(paths) * (sizeof(PathRay) + sizeof(PathRec) + 4 * 16 + 2 * 4 + 6 * 4);
 }
 
 double seconds_between(Clock::time_point a, Clock::time_point b) {
@@ -271,18 +270,13 @@ void Engine::configure() {
     }
     const uint32_t n = tune_.pool_paths;
     const size_t f4 = size_t(n) * sizeof(float4), u32 = size_t(n) * 4;
-    for (DevBuf* b : {&ray_o_, &ray_d_, &ray_lo_, &thr_, &rad_, &lastpos_, &absorb_, &sh_o_, &sh_d_, &sh_lo_, &sh_c_})
-        b->alloc(f4);
-    rng_.alloc(size_t(n) * 8);
-    for (DevBuf* b : {&job_, &pixel_, &sample_, &medium_, &flags_, &prev_, &hit_, &sh_path_, &q0_, &q1_, &retire_,
-                      &xq_, &xsq_, &freeq_})
-        b->alloc(u32);
-    P_ = PathSoA{ray_o_.as<float4>(), ray_d_.as<float4>(), ray_lo_.as<float4>(), thr_.as<float4>(),
-                 rad_.as<float4>(),   lastpos_.as<float4>(), absorb_.as<float4>(), rng_.as<uint64_t>(),
-                 job_.as<uint32_t>(), pixel_.as<uint32_t>(), sample_.as<uint32_t>(), medium_.as<int32_t>(),
-                 flags_.as<uint32_t>(), prev_.as<uint32_t>(), hit_.as<uint32_t>()};
-    S_ = ShadowSoA{sh_path_.as<uint32_t>(), sh_o_.as<float4>(), sh_d_.as<float4>(), sh_lo_.as<float4>(),
-                   sh_c_.as<float4>()};
+    ray_.alloc(size_t(n) * sizeof(PathRay));
+    rec_.alloc(size_t(n) * sizeof(PathRec));
+    for (DevBuf* b : {&sh_o_, &sh_d_, &sh_lo_, &sh_c_}) b->alloc(f4);
+    for (DevBuf* b : {&sh_path_, &sh_job_, &q0_, &q1_, &retire_, &xq_, &xsq_, &freeq_}) b->alloc(u32);
+    P_ = PathSoA{ray_.as<PathRay>(), rec_.as<PathRec>()};
+    S_ = ShadowSoA{sh_path_.as<uint32_t>(), sh_job_.as<uint32_t>(), sh_o_.as<float4>(), sh_d_.as<float4>(),
+                   sh_lo_.as<float4>(), sh_c_.as<float4>()};
     Q_ = Queues{{q0_.as<uint32_t>(), q1_.as<uint32_t>()}, retire_.as<uint32_t>(), xq_.as<uint32_t>(),
                 xsq_.as<uint32_t>(), freeq_.as<uint32_t>()};
     ctl_.alloc(sizeof(Control));

@@ -141,9 +141,8 @@ private:
     Profiler prof_;
 
     // device state
-    DevBuf ray_o_, ray_d_, ray_lo_, thr_, rad_, lastpos_, absorb_, rng_, job_, pixel_, sample_, medium_, flags_, prev_,
-        hit_;
-    DevBuf sh_path_, sh_o_, sh_d_, sh_lo_, sh_c_;
+    DevBuf ray_, rec_;
+    DevBuf sh_path_, sh_job_, sh_o_, sh_d_, sh_lo_, sh_c_;
     DevBuf q0_, q1_, retire_, xq_, xsq_, freeq_;
     DevBuf ctl_, cnt_, jobs_;
     PathSoA P_{};

@@ -94,7 +94,7 @@ struct ExactHit {
 };
 
 // The reference's intersectTriangle in emulated binary64, same expression order.
-CR_HD inline ExactHit exact_triangle(RayQuery& q, V3 va, V3 vb, V3 vc, sf::f64 limit_in) {
+CR_OUTLINE ExactHit exact_triangle(RayQuery& q, V3 va, V3 vb, V3 vc, sf::f64 limit_in) {
     ExactHit r;
     r.hit = false;
     if (!q.valid) return r;

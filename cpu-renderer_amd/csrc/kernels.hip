@@ -144,8 +144,10 @@ __device__ inline uint32_t find_job(const DevJob* jobs, const Control* ctl, uint
 
 __device__ inline PathState load_state(const PathSoA& P, uint32_t slot) {
     PathState st;
-    const float4 ro = P.ray_o[slot], rd = P.ray_d[slot], rl = P.ray_lo[slot];
-    const float4 th = P.thr[slot], ra = P.rad[slot], lp = P.lastpos[slot], ab = P.absorb[slot];
+    const PathRay& r = P.ray[slot];
+    const PathRec& c = P.rec[slot];
+    const float4 ro = r.o, rd = r.d, rl = r.lo;
+    const float4 th = c.thr, ra = c.rad, lp = c.lastpos, ab = c.absorb;
     st.o = xyz(ro);
     st.tmin = ro.w;
     st.d = xyz(rd);
@@ -157,25 +159,29 @@ __device__ inline PathState load_state(const PathSoA& P, uint32_t slot) {
     st.last_pdf = ra.w;
     st.last_pos = xyz(lp);
     st.sigma = xyz(ab);
-    st.medium = P.medium[slot];
-    st.flags = P.flags[slot];
-    st.prev = P.prev[slot];
-    st.rng = P.rng[slot];
+    st.medium = c.medium;
+    st.flags = r.flags;
+    st.prev = r.prev;
+    st.rng = c.rng;
     return st;
 }
 
+// Writes the ray record (except job and hit) and the state record (except
+// pixel and sample).
 __device__ inline void store_state(const PathSoA& P, uint32_t slot, const PathState& st) {
-    P.ray_o[slot] = make_float4(st.o.x, st.o.y, st.o.z, st.tmin);
-    P.ray_d[slot] = make_float4(st.d.x, st.d.y, st.d.z, st.tmax);
-    P.ray_lo[slot] = make_float4(st.lo.x, st.lo.y, st.lo.z, 0);
-    P.thr[slot] = make_float4(st.T.x, st.T.y, st.T.z, st.eta_scale);
-    P.rad[slot] = make_float4(st.L.x, st.L.y, st.L.z, st.last_pdf);
-    P.lastpos[slot] = make_float4(st.last_pos.x, st.last_pos.y, st.last_pos.z, 0);
-    P.absorb[slot] = make_float4(st.sigma.x, st.sigma.y, st.sigma.z, 0);
-    P.medium[slot] = st.medium;
-    P.flags[slot] = st.flags;
-    P.prev[slot] = st.prev;
-    P.rng[slot] = st.rng;
+    PathRay& r = P.ray[slot];
+    PathRec& c = P.rec[slot];
+    r.o = make_float4(st.o.x, st.o.y, st.o.z, st.tmin);
+    r.d = make_float4(st.d.x, st.d.y, st.d.z, st.tmax);
+    r.lo = make_float4(st.lo.x, st.lo.y, st.lo.z, 0);
+    r.prev = st.prev;
+    r.flags = st.flags;
+    c.thr = make_float4(st.T.x, st.T.y, st.T.z, st.eta_scale);
+    c.rad = make_float4(st.L.x, st.L.y, st.L.z, st.last_pdf);
+    c.lastpos = make_float4(st.last_pos.x, st.last_pos.y, st.last_pos.z, 0);
+    c.absorb = make_float4(st.sigma.x, st.sigma.y, st.sigma.z, 0);
+    c.medium = st.medium;
+    c.rng = st.rng;
 }
 
 // Camera sample `local` of job `ji` into `slot`; false if the camera failed.
@@ -192,9 +198,9 @@ __device__ inline bool init_path(const PathSoA& P, uint32_t slot, const DevJob& 
         st.tmin = 0;
         st.tmax = -1;
     }
-    P.job[slot] = ji;
-    P.pixel[slot] = pixel;
-    P.sample[slot] = sample;
+    P.ray[slot].job = ji;
+    P.rec[slot].pixel = pixel;
+    P.rec[slot].sample = sample;
     store_state(P, slot, st);
     if (!ok)
         report_error(ctl, ji, uint64_t(pixel) * uint64_t(J.spp) + sample, kErrCamera, 0, 0, 0, sf::zero(0),
@@ -215,15 +221,16 @@ __device__ inline void start_item(const PathSoA& P, const DevJob* jobs, uint32_t
     // Unreachable with a consistent job table; kept so that a bookkeeping bug
     // surfaces as an error instead of a write outside a sample buffer.
     report_error(ctl, job, 0, kErrInternal, 0, 0, 0, sf::zero(0), V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0});
-    P.job[path] = job;
-    P.pixel[path] = 0;
-    P.sample[path] = uint32_t(J.spp);  // outside the sample buffer: never stored
-    P.flags[path] = kFailed;
-    P.rad[path] = make_float4(0, 0, 0, 0);
-    P.ray_o[path] = make_float4(0, 0, 0, 0);
-    P.ray_d[path] = make_float4(0, 0, 1, -1);
-    P.ray_lo[path] = make_float4(0, 0, 0, 0);
-    P.prev[path] = kNoTriangle;
+    PathRay& r = P.ray[path];
+    r.job = job;
+    r.flags = kFailed;
+    r.o = make_float4(0, 0, 0, 0);
+    r.d = make_float4(0, 0, 1, -1);
+    r.lo = make_float4(0, 0, 0, 0);
+    r.prev = kNoTriangle;
+    P.rec[path].pixel = 0;
+    P.rec[path].sample = uint32_t(J.spp);  // outside the sample buffer: never stored
+    P.rec[path].rad = make_float4(0, 0, 0, 0);
 }
 
 // ------------------------------------------------------------------ extend
@@ -300,13 +307,13 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_extend(PathSoA 
             if (__ballot(!busy && idx >= n)) drained = true;
             if (!busy && idx < n) {
                 path = queue[idx];
-                job = P.job[path];
+                const PathRay r = P.ray[path];
+                job = r.job;
                 sc = jobs[job].scene;
-                const float4 ro = P.ray_o[path], rd = P.ray_d[path], rl = P.ray_lo[path];
-                setup_query(q, xyz(ro), xyz(rd), ro.w, rd.w, xyz(rl), P.prev[path]);
+                setup_query(q, xyz(r.o), xyz(r.d), r.o.w, r.d.w, xyz(r.lo), r.prev);
                 tc = TraceCount{0, 0, 0};
                 busy = !tr.begin(*sc, q);
-                if (!busy) P.hit[path] = kHitMiss;
+                if (!busy) P.ray[path].hit = kHitMiss;
             }
         }
         if (!__ballot(busy)) {
@@ -320,7 +327,7 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_extend(PathSoA 
                 fin = true;
             }
         const bool exact = fin && tr.best.need_exact && !tr.best.overflow;
-        if (fin && !exact) P.hit[path] = encode_hit(*sc, tr.best);
+        if (fin && !exact) P.ray[path].hit = encode_hit(*sc, tr.best);
         const uint32_t pos = wave_append(&ctl->xcount, exact);
         if (exact) xq[pos] = path;
         wave_add_keyed(cnt->extend_nodes, job, fin ? tc.nodes : 0u);
@@ -344,14 +351,14 @@ __global__ void __launch_bounds__(256) k_extend_exact(PathSoA P, const DevJob* j
     const uint32_t n = ctl->xcount;
     for (uint32_t i = spread_first(); i < n; i += spread_stride()) {
         const uint32_t path = xq[i];
-        const uint32_t job = P.job[path];
+        const PathRay r = P.ray[path];
+        const uint32_t job = r.job;
         const DevScene& s = *jobs[job].scene;
-        const float4 ro = P.ray_o[path], rd = P.ray_d[path], rl = P.ray_lo[path];
         RayQuery q;
-        setup_query(q, xyz(ro), xyz(rd), ro.w, rd.w, xyz(rl), P.prev[path]);
+        setup_query(q, xyz(r.o), xyz(r.d), r.o.w, r.d.w, xyz(r.lo), r.prev);
         TraceCount tc{0, 0, 0};
         const ClosestResult best = traverse<false, true>(s, q, tc);
-        P.hit[path] = encode_hit(s, best);
+        P.ray[path].hit = encode_hit(s, best);
         atomicAdd(&cnt->fallbacks[job], (unsigned long long)tc.fallbacks);
         atomicAdd(&cnt->exact_rays[job], 1ull);
     }
@@ -377,7 +384,7 @@ __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, S
         __syncthreads();
         for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {  // path ids < 2^25: class fits above
             const uint32_t path = queue[base + k];
-            const uint32_t c = P.hit[path] >> 30;
+            const uint32_t c = P.ray[path].hit >> 30;
             stage[k] = path | (c << 30);
             atomicAdd(&counts[c], 1u);
         }
@@ -398,10 +405,12 @@ __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, S
             sh.emit = false;
             if (live) {
                 path = sorted[k];
-                ji = P.job[path];
+                const uint32_t hit_word = P.ray[path].hit;
+                ji = P.ray[path].job;
                 const DevJob& J = jobs[ji];
-                const uint64_t key = uint64_t(P.pixel[path]) * uint64_t(J.spp) + P.sample[path];
-                const uint32_t hit = P.hit[path] & kHitPrimMask;
+                const uint32_t pixel = P.rec[path].pixel, sample = P.rec[path].sample;
+                const uint64_t key = uint64_t(pixel) * uint64_t(J.spp) + sample;
+                const uint32_t hit = hit_word & kHitPrimMask;
                 PathState st = load_state(P, path);
                 if (st.flags & kFailed) {
                     action = kRetire;
@@ -418,7 +427,7 @@ __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, S
                 }
                 if (action == kReplay) {
                     replayed = true;
-                    begin_sample(J, P.pixel[path], P.sample[path], true, st);
+                    begin_sample(J, pixel, sample, true, st);
                     action = kContinue;
                 }
                 store_state(P, path, st);
@@ -429,6 +438,7 @@ __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, S
             const uint32_t spos = wave_append(&ctl->scount, live && sh.emit);
             if (live && sh.emit) {
                 S.path[spos] = path;
+                S.job[spos] = ji;
                 S.o[spos] = make_float4(sh.o.x, sh.o.y, sh.o.z, 0.0f);
                 S.d[spos] = make_float4(sh.d.x, sh.d.y, sh.d.z, sh.tmax);
                 S.lo[spos] = make_float4(sh.lo.x, sh.lo.y, sh.lo.z, __uint_as_float(sh.ignore));
@@ -447,11 +457,11 @@ __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, S
 __device__ inline void add_contribution(const PathSoA& P, const ShadowSoA& S, uint32_t i) {
     const uint32_t path = S.path[i];
     const float4 c = S.c[i];
-    float4 r = P.rad[path];
+    float4 r = P.rec[path].rad;
     r.x = r.x + c.x;
     r.y = r.y + c.y;
     r.z = r.z + c.z;
-    P.rad[path] = r;
+    P.rec[path].rad = r;
 }
 
 __device__ inline void shadow_query(RayQuery& q, const ShadowSoA& S, uint32_t i) {
@@ -480,7 +490,7 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_shadow(PathSoA 
             if (__ballot(!busy && idx >= n)) drained = true;
             if (!busy && idx < n) {
                 i = idx;
-                job = P.job[S.path[i]];
+                job = S.job[i];
                 sc = jobs[job].scene;
                 shadow_query(q, S, i);
                 tc = TraceCount{0, 0, 0};
@@ -514,7 +524,7 @@ __global__ void __launch_bounds__(256) k_shadow_exact(PathSoA P, ShadowSoA S, co
     const uint32_t n = ctl->xscount;
     for (uint32_t j = spread_first(); j < n; j += spread_stride()) {
         const uint32_t i = xsq[j];
-        const uint32_t job = P.job[S.path[i]];
+        const uint32_t job = S.job[i];
         const DevScene& s = *jobs[job].scene;
         RayQuery q;
         shadow_query(q, S, i);
@@ -541,16 +551,17 @@ __global__ void __launch_bounds__(256) k_retire(PathSoA P, const DevJob* jobs, c
         uint32_t path = 0, ji = 0;
         if (live) {
             path = retire[i];
-            ji = P.job[path];
+            ji = P.ray[path].job;
             const DevJob& J = jobs[ji];
-            const float4 L = P.rad[path];
-            const V3 l = xyz(L);
+            const PathRec& c = P.rec[path];
+            const V3 l = xyz(c.rad);
+            const uint32_t pixel = c.pixel, sample = c.sample;
             const uint64_t npix = uint64_t(J.width) * uint64_t(J.height);
             if (!radiance_ok(l)) {
-                report_error(ctl, ji, uint64_t(P.pixel[path]) * uint64_t(J.spp) + P.sample[path], kErrBadRadiance,
-                             0, 0, 0, sf::zero(0), V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0});
+                report_error(ctl, ji, uint64_t(pixel) * uint64_t(J.spp) + sample, kErrBadRadiance, 0, 0, 0,
+                             sf::zero(0), V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0});
             }
-            const uint64_t at = uint64_t(P.sample[path]) * npix + P.pixel[path];
+            const uint64_t at = uint64_t(sample) * npix + pixel;
             if (at < J.items) J.samples[at] = make_float4(l.x, l.y, l.z, 0);
         }
         wave_add_keyed(cnt->retired, ji, live ? 1u : 0u);

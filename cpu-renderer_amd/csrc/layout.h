@@ -160,26 +160,41 @@ enum PathFlag : uint32_t {
     kFailed = 1u << 10,  // camera ray failed: retire with zero radiance
 };
 
+// Per-path state as two aligned records.  Kernels reach paths through
+// shuffled queues, so a field-per-array layout costs one cache line per field
+// per path; a record costs one line.  The ray record is all the traversal
+// kernels read.
+struct alignas(64) PathRay {
+    float4 o;        // origin, tmin
+    float4 d;        // direction, tmax
+    float4 lo;       // originLow, (unused)
+    uint32_t prev;   // departed triangle id (precise replay) or kNoTriangle
+    uint32_t job;    // job-table slot
+    uint32_t hit;    // encoded closest hit of the last extend
+    uint32_t flags;  // depth | kLastDelta | kPrecise | kFailed
+};
+static_assert(sizeof(PathRay) == 64, "PathRay is one 64-byte record");
+
+struct alignas(128) PathRec {
+    float4 thr;      // throughput, etaScale
+    float4 rad;      // radiance, lastPdf
+    float4 lastpos;  // lastPosition
+    float4 absorb;   // active medium absorption
+    uint64_t rng;
+    uint32_t pixel, sample;
+    int32_t medium;
+    uint32_t pad[9];
+};
+static_assert(sizeof(PathRec) == 128, "PathRec is one 128-byte record");
+
 struct PathSoA {
-    float4* ray_o;    // origin, tmin
-    float4* ray_d;    // direction, tmax
-    float4* ray_lo;   // originLow, (unused)
-    float4* thr;      // throughput, etaScale
-    float4* rad;      // radiance, lastPdf
-    float4* lastpos;  // lastPosition
-    float4* absorb;   // active medium absorption
-    uint64_t* rng;
-    uint32_t* job;
-    uint32_t* pixel;
-    uint32_t* sample;
-    int32_t* medium;
-    uint32_t* flags;
-    uint32_t* prev;  // departed triangle id (precise replay) or kNoTriangle
-    uint32_t* hit;   // hit prim or kNoTriangle
+    PathRay* ray;
+    PathRec* rec;
 };
 
 struct ShadowSoA {
     uint32_t* path;
+    uint32_t* job;  // job of the emitting path (read by the traversal without a gather)
     float4* o;   // origin, tmin
     float4* d;   // direction, tmax
     float4* lo;  // originLow, ignore (as bits)

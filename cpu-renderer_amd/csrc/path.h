@@ -7,7 +7,6 @@
 // back; the same functions compiled for the CPU drive the emulation used by
 // the CPU test-suite, so the device logic is checked without a GPU.
 #pragma once
-#include "fastsurf.h"
 #include "geom.h"
 #include "shade.h"
 
@@ -101,27 +100,9 @@ CR_HDI uint32_t bounce(const DevJob& J, PathState& st, uint32_t prim, ShadowOut&
     RayQuery q;
     setup_query(q, ray_o, ray_d, st.tmin, st.tmax, st.lo, st.prev);
     const uint32_t tri_id = sc.prims[prim].id;
-    // Float results of the hit's binary64 reconstruction: filtered double-float
-    // evaluation when certain (no split origin), exact emulation otherwise.
-    ExactHit hit;
-    hit.t = sf::zero(0);
-    Surf s;
-    float tf = 0;
-    bool fast = false;
-    if (!precise) {
-        const PrimRec pr = sc.prims[prim];
-        const PrimAux& ax = sc.aux[prim];
-        V3 fp, fn;
-        fast = fast_hit_surface(q, V3{pr.ax, pr.ay, pr.az}, V3{pr.bx, pr.by, pr.bz}, V3{pr.cx, pr.cy, pr.cz},
-                                V3{ax.n0[0], ax.n0[1], ax.n0[2]}, V3{ax.n1[0], ax.n1[1], ax.n1[2]},
-                                V3{ax.n2[0], ax.n2[1], ax.n2[2]}, fp, fn, tf);
-        if (fast) s = surface_from(sc, prim, ray_d, fp, fn);
-    }
-    if (!fast) {
-        hit = exact_of(sc, q, prim);
-        tf = sf::to_float(hit.t);
-        s = surface_at(sc, prim, ray_d, hit.u, hit.v);
-    }
+    const ExactHit hit = exact_of(sc, q, prim);
+    const float tf = sf::to_float(hit.t);
+    const Surf s = surface_at(sc, prim, ray_d, hit.u, hit.v);
     if (st.medium >= 0) T = T * beer_lambert(st.sigma, tf);
     if ((s.front || s.double_sided) && max_component(s.emission) > 0) {
         const float w = last_delta || !nee

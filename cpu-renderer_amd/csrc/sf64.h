@@ -26,6 +26,13 @@
 #endif
 #endif
 
+// Large routines called from several sites.  Inlined: keeping one out-of-line
+// copy of each (smaller kernels) measured slower on gfx950 (call overhead
+// outweighs the instruction-cache gain for the shading kernels).
+#ifndef CR_OUTLINE
+#define CR_OUTLINE CR_HD inline
+#endif
+
 namespace sf {
 
 struct f64 {
@@ -235,7 +242,7 @@ CR_HDI float i128_to_f(int64_t hi, uint64_t lo) { return float(hi) * 18446744073
 // R = floor(sqrt(m*2^56)) is reached from a float sqrt estimate (error < 2^32)
 // by two Newton corrections R += (N - R^2)/(2R) evaluated with float digit
 // estimates on the exact 128-bit remainder, then a final +-1 adjustment.
-CR_HD inline f64 sqrt(f64 a) {
+CR_OUTLINE f64 sqrt(f64 a) {
     if (is_nan(a)) return qnan();
     if (is_zero(a)) return a;
     if (sign_of(a)) return qnan();

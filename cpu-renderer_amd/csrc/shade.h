@@ -16,7 +16,7 @@
 namespace cr {
 
 // ---------------------------------------------------------------- camera
-CR_HDI bool camera_ray(const DevCamera& c, float px, float py, V3& o, V3& d, float& tmin, float& tmax) {
+CR_OUTLINE bool camera_ray(const DevCamera& c, float px, float py, V3& o, V3& d, float& tmin, float& tmax) {
     const sf::f64 one = sf::one(), two = sf::two();
     const sf::f64 x = sf::mul(
         sf::mul(sf::sub(sf::div(sf::mul(two, sf::from_float(px)), sf::from_i32(c.width)), one), c.ratio),
@@ -38,7 +38,7 @@ CR_HDI bool camera_ray(const DevCamera& c, float px, float py, V3& o, V3& d, flo
 // ---------------------------------------------------------------- offsetRay
 // `ung` is unit(geometricNormal) (binary64 normalization done once per triangle
 // on the host); `ng` is the stored geometric normal used for the validity test.
-CR_HDI uint32_t offset_ray(V3 p, V3 ng, V3 ung, V3 direction, float max_distance, V3& origin, V3& dir,
+CR_OUTLINE uint32_t offset_ray(V3 p, V3 ng, V3 ung, V3 direction, float max_distance, V3& origin, V3& dir,
                            float& tmax) {
     if (!finite(p) || !finite(ng) || !finite(direction) || !nonzero(ng) || !nonzero(direction) ||
         max_distance != max_distance || max_distance < 0)
@@ -89,24 +89,7 @@ CR_HDI void material_into(const DevMaterial& m, Surf& s) {
     s.double_sided = m.double_sided != 0;
 }
 
-// surfaceAt from a position and interpolated normal already rounded to float
-// (the filtered path of fastsurf.h); no split origin.
-CR_HDI Surf surface_from(const DevScene& sc, uint32_t prim, V3 ray_dir, V3 position, V3 normal) {
-    const PrimRec p = sc.prims[prim];
-    const PrimAux& ax = sc.aux[prim];
-    Surf s;
-    s.position = position;
-    s.position_low = V3{0, 0, 0};
-    s.ng = V3{ax.ng[0], ax.ng[1], ax.ng[2]};
-    s.front = dot_sign_exact(ray_dir, s.ng) < 0;
-    s.normal = normal;
-    if (dot(s.normal, s.ng) < 0) s.normal = -s.normal;
-    s.material = p.material;
-    material_into(sc.materials[p.material], s);
-    return s;
-}
-
-CR_HDI Surf surface_at(const DevScene& sc, uint32_t prim, V3 ray_dir, sf::f64 hu, sf::f64 hv) {
+CR_OUTLINE Surf surface_at(const DevScene& sc, uint32_t prim, V3 ray_dir, sf::f64 hu, sf::f64 hv) {
     const PrimRec p = sc.prims[prim];
     const PrimAux& ax = sc.aux[prim];
     Surf s;
@@ -284,7 +267,7 @@ CR_HDI bool correct_side(const Surf& s, V3 wo, V3 wi, bool transmitted) {
     return transmitted ? d < 0 : d > 0;
 }
 
-CR_HDI BsdfEval evaluate_bsdf(const Surf& s, V3 wo, V3 wi) {
+CR_OUTLINE BsdfEval evaluate_bsdf(const Surf& s, V3 wo, V3 wi) {
     Bsdf c;
     bsdf_init(c, s, wo);
     const V3 local = to_local(c.frame, wi);
@@ -303,7 +286,7 @@ CR_HDI bool sample_ok(const BsdfSample& b) {
     return max_component(b.weight) > 0 && finite(b.weight) && finite(b.direction);
 }
 
-CR_HDI BsdfSample sample_bsdf(const Surf& s, V3 wo, Rng& rng) {
+CR_OUTLINE BsdfSample sample_bsdf(const Surf& s, V3 wo, Rng& rng) {
     Bsdf c;
     bsdf_init(c, s, wo);
     BsdfSample out = empty_sample();
@@ -373,7 +356,7 @@ CR_HDI V3 env_texel(const DevEnv& e, int ix, int iy) {
     return V3{t.x, t.y, t.z};
 }
 
-CR_HDI V3 env_radiance(const DevEnv& e, V3 d) {
+CR_OUTLINE V3 env_radiance(const DevEnv& e, V3 d) {
     if (!e.is_image) return V3{e.constant[0], e.constant[1], e.constant[2]};
     float u, v;
     env_uv(e, d, u, v);
@@ -385,7 +368,7 @@ CR_HDI V3 env_radiance(const DevEnv& e, V3 d) {
            e.intensity;
 }
 
-CR_HDI float env_pdf(const DevEnv& e, V3 d) {
+CR_OUTLINE float env_pdf(const DevEnv& e, V3 d) {
     if (!e.is_image) return 1 / (4 * kPi);
     float u, v;
     env_uv(e, d, u, v);
@@ -409,7 +392,7 @@ CR_HDI uint32_t env_invert(const DevEnv& e, uint32_t k) {
     return lo < n ? lo : n - 1;
 }
 
-CR_HDI V3 env_sample(const DevEnv& e, Rng& rng, float& density) {
+CR_OUTLINE V3 env_sample(const DevEnv& e, Rng& rng, float& density) {
     if (!e.is_image) {
         const float y = 1 - 2 * rng.uniform(), phi = 2 * kPi * rng.uniform(), r = sqrtf(fmax_(0.0f, 1 - y * y));
         density = 1 / (4 * kPi);
@@ -453,7 +436,7 @@ CR_HDI float triangle_pdf(const DevScene& sc, const DevLights& lt, uint32_t prim
 
 CR_HDI float environment_pdf(const DevLights& lt, const DevEnv& e, V3 d) { return lt.env_probability * env_pdf(e, d); }
 
-CR_HDI LightSample sample_light(const DevScene& sc, const DevLights& lt, const DevEnv& env, V3 from, Rng& rng) {
+CR_OUTLINE LightSample sample_light(const DevScene& sc, const DevLights& lt, const DevEnv& env, V3 from, Rng& rng) {
     LightSample out = no_light();
     if (lt.count == 0) return out;
     const uint32_t k = rng.bits24();

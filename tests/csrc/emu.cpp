@@ -245,79 +245,6 @@ int emu_trace(void* sp, const crg_ray* rays, uint64_t n, crg_hit* hits, int any,
 }
 
 // Device camera / offset / BSDF pieces for golden comparisons.
-// Randomized cross-check of the filtered hit reconstruction (fastsurf.h)
-// against the exact emulation: for random triangles and rays that hit them
-// (several magnitude families, axis-aligned faces with zero coordinates,
-// hits near edges), every result the filter certifies must equal the exact
-// position, interpolated normal and float(t) bit for bit.  Returns the number
-// of mismatches; *tested counts hits, *accepted the certified ones.
-uint64_t emu_fastsurf_check(uint64_t n, uint64_t seed, uint64_t* tested, uint64_t* accepted) {
-    using namespace cr;
-    std::mt19937_64 g(seed);
-    auto uni = [&] { return double(g() >> 11) * 0x1p-53; };
-    auto sym = [&] { return 2.0 * uni() - 1.0; };
-    auto unit3 = [&] {
-        for (;;) {
-            const double x = sym(), y = sym(), z = sym(), l = std::sqrt(x * x + y * y + z * z);
-            if (l > 0.1 && l <= 1) return V3{float(x / l), float(y / l), float(z / l)};
-        }
-    };
-    uint64_t bad = 0, t_count = 0, a_count = 0;
-    for (uint64_t i = 0; i < n; ++i) {
-        const int fam = int(i % 6);
-        const double scale = fam == 1 ? 1e3 : fam == 2 ? 1e-3 : fam == 3 ? 3e4 : 1.0;
-        const double size = scale * std::exp(std::log(1e-4) * uni());
-        V3 cen{float(sym() * scale), float(sym() * scale), float(sym() * scale)};
-        V3 v[3], nn[3];
-        for (int k = 0; k < 3; ++k) {
-            v[k] = V3{float(cen.x + sym() * size), float(cen.y + sym() * size), float(cen.z + sym() * size)};
-            nn[k] = unit3();
-        }
-        if (fam == 4) {  // axis-aligned face at a coordinate of zero with axis normals
-            const int ax = int(g() % 3);
-            for (int k = 0; k < 3; ++k) {
-                float* c = ax == 0 ? &v[k].x : ax == 1 ? &v[k].y : &v[k].z;
-                *c = 0.0f;
-                nn[k] = V3{ax == 0 ? 1.0f : 0.0f, ax == 1 ? 1.0f : 0.0f, ax == 2 ? 1.0f : 0.0f};
-            }
-        }
-        double b1 = uni(), b2 = uni();
-        if (b1 + b2 > 1) {
-            b1 = 1 - b1;
-            b2 = 1 - b2;
-        }
-        if (fam == 5) b1 = std::ldexp(uni(), -int(g() % 40));  // near the edge opposite b
-        const double hx = v[0].x * (1 - b1 - b2) + v[1].x * b1 + v[2].x * b2;
-        const double hy = v[0].y * (1 - b1 - b2) + v[1].y * b1 + v[2].y * b2;
-        const double hz = v[0].z * (1 - b1 - b2) + v[1].z * b1 + v[2].z * b2;
-        const V3 d = unit3();
-        const double dist = size * std::exp(std::log(1e3) * uni());
-        const V3 o{float(hx - d.x * dist), float(hy - d.y * dist), float(hz - d.z * dist)};
-        RayQuery q;
-        setup_query(q, o, d, 0.0f, kInf, V3{0, 0, 0}, kNoTriangle);
-        const ExactHit e = exact_triangle(q, v[0], v[1], v[2], sf::inf(0));
-        if (!e.hit) continue;
-        ++t_count;
-        const sf::f64 w = sf::sub(sf::sub(sf::one(), e.u), e.v);
-        const D3 P = dadd(dadd(dscale(widen(v[0]), w), dscale(widen(v[1]), e.u)), dscale(widen(v[2]), e.v));
-        const D3 N = dadd(dadd(dscale(widen(nn[0]), w), dscale(widen(nn[1]), e.u)), dscale(widen(nn[2]), e.v));
-        const V3 xp = narrow(P);
-        const V3 ng{1, 0, 0};
-        const V3 xn = dunit(N, ng);
-        const float xt = sf::to_float(e.t);
-        V3 fp, fn;
-        float ft;
-        if (!fast_hit_surface(q, v[0], v[1], v[2], nn[0], nn[1], nn[2], fp, fn, ft)) continue;
-        ++a_count;
-        if (std::memcmp(&fp, &xp, sizeof(V3)) || std::memcmp(&fn, &xn, sizeof(V3)) ||
-            sf::fbits(ft) != sf::fbits(xt))
-            ++bad;
-    }
-    *tested = t_count;
-    *accepted = a_count;
-    return bad;
-}
-
 int emu_camera_generate(const crg_camera* c, float px, float py, int w, int h, crg_ray* out) {
     try {
         const DevCamera cam = prepare_camera(*c, w, h);

@@ -48,17 +48,10 @@ class Emu:
                                               C.POINTER(abi.Ray)]),
             "emu_render": (C.c_int, [vp, vp, C.POINTER(abi.Camera), C.POINTER(abi.Options), fp,
                                      C.POINTER(abi.Stats)]),
-            "emu_fastsurf_check": (u64, [u64, u64, C.POINTER(u64), C.POINTER(u64)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(self.lib, name)
             f.restype, f.argtypes = res, args
-
-    def fastsurf_check(self, n: int, seed: int):
-        """(mismatches, tested, accepted) of the filtered hit reconstruction vs the exact one."""
-        t, a = C.c_uint64(), C.c_uint64()
-        bad = self.lib.emu_fastsurf_check(n, seed, C.byref(t), C.byref(a))
-        return bad, t.value, a.value
 
     def error(self):
         return self.lib.emu_last_error().decode()

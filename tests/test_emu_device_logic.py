@@ -104,3 +104,4 @@ def test_emu_render(emu, name):
     else:
         assert ok.all()
     assert stats.boundary_retries == int(g[f"{name}/retries"])
+
