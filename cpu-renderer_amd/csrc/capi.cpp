@@ -163,74 +163,56 @@ int crg_scene_upload(crg_ctx* ctx, const crg_scene_desc* desc, uint64_t* scene_i
         // context lock, so it overlaps renders in flight.
         const auto t0 = std::chrono::steady_clock::now();
         HostScene hs = finalize_scene(*desc);
-        for (size_t i = 0; i < hs.materials.size(); ++i) {
-            const crg_material& m = hs.materials[i];
-            for (const crg_texture_slot* s :
-                 {&m.base_color_tex, &m.metallic_roughness_tex, &m.normal_tex, &m.emissive_tex, &m.transmission_tex})
-                if (s->texture >= 0)
-                    throw std::invalid_argument("Material " + std::to_string(i) +
-                                                " uses a texture map; the device path supports factor-only materials");
-        }
-        WideBvh bvh = build_wide_bvh(hs);
+        SceneArrays arr = scene_arrays(hs);
         auto res = std::make_shared<SceneRes>();
         const size_t n = hs.tris.size();
         res->triangles = n;
-        res->prim_to_tri = bvh.order;
-        res->tri_to_prim.assign(n, 0);
-        for (size_t i = 0; i < n; ++i) res->tri_to_prim[bvh.order[i]] = uint32_t(i);
-        std::vector<PrimRec> prims(n);
-        std::vector<float4> png(n);
-        std::vector<PrimAux> aux(n);
-        for (size_t i = 0; i < n; ++i) {
-            const HTriangle& t = hs.tris[bvh.order[i]];
-            const crg_material& m = hs.materials[t.material];
-            uint32_t flags = 0;
-            if (!m.double_sided && m.transmission <= 0) flags |= kPrimCull;
-            if (m.alpha_mode == 1 && !(m.base_color[3] * 1.0f >= m.alpha_cutoff)) flags |= kPrimNeverHit;
-            const uint32_t cls = m.transmission > 0 ? kShadeTransmissive : m.metallic > 0 ? kShadeMetal : kShadeOpaque;
-            flags |= cls << kPrimClassShift;
-            prims[i] = PrimRec{t.p[0].x, t.p[0].y, t.p[0].z, bvh.order[i], t.p[1].x, t.p[1].y, t.p[1].z, flags,
-                               t.p[2].x, t.p[2].y, t.p[2].z, t.material};
-            png[i] = make_float4(t.ng.x, t.ng.y, t.ng.z, 0);
-            PrimAux& a = aux[i];
-            const HVec3* nv[3] = {&t.n[0], &t.n[1], &t.n[2]};
-            float* dst[3] = {a.n0, a.n1, a.n2};
-            for (int k = 0; k < 3; ++k) {
-                dst[k][0] = nv[k]->x;
-                dst[k][1] = nv[k]->y;
-                dst[k][2] = nv[k]->z;
-            }
-            a.ng[0] = t.ng.x; a.ng[1] = t.ng.y; a.ng[2] = t.ng.z;
-            a.ung[0] = t.ung.x; a.ung[1] = t.ung.y; a.ung[2] = t.ung.z;
-            a.area = t.area;
-        }
+        res->prim_to_tri = arr.bvh.order;
+        res->tri_to_prim = arr.tri_to_prim;
         res->lights = scene_lights(hs);
         const std::vector<uint32_t> ranks = light_ranks(res->lights, res->tri_to_prim, n);
-        const std::vector<DevMaterial> mats = device_materials(hs);
         const auto t1 = std::chrono::steady_clock::now();
         std::lock_guard<std::mutex> lock(ctx->mu);
         hipStream_t st = ctx->stream;
-        upload(res->nodes, bvh.nodes, st);
-        upload(res->prims, prims, st);
-        upload(res->prim_ng, png, st);
-        upload(res->aux, aux, st);
-        upload(res->materials, mats, st);
+        upload(res->nodes, arr.bvh.nodes, st);
+        upload(res->prims, arr.prims, st);
+        upload(res->prim_ng, arr.prim_ng, st);
+        upload(res->aux, arr.aux, st);
+        upload(res->materials, arr.materials, st);
         upload(res->light_rank, ranks, st);
         upload(res->punctual, res->lights.punctual, st);
-        res->host = DevScene{res->nodes.as<WNode>(), res->prims.as<PrimRec>(), res->prim_ng.as<float4>(),
-                             res->aux.as<PrimAux>(), res->materials.as<DevMaterial>(), res->light_rank.as<uint32_t>(),
-                             res->punctual.as<DevPunctual>(), uint32_t(bvh.nodes.size()), uint32_t(n),
-                             uint32_t(res->lights.punctual.size()), 0};
+        upload(res->uvs, arr.uvs, st);
+        upload(res->tangents, arr.tangents, st);
+        upload(res->texels, arr.texels, st);
+        upload(res->images, arr.images, st);
+        upload(res->textures, arr.textures, st);
+        res->host = DevScene{res->nodes.as<WNode>(),
+                             res->prims.as<PrimRec>(),
+                             res->prim_ng.as<float4>(),
+                             res->aux.as<PrimAux>(),
+                             res->materials.as<DevMaterial>(),
+                             res->light_rank.as<uint32_t>(),
+                             res->punctual.as<DevPunctual>(),
+                             arr.uvs.empty() ? nullptr : res->uvs.as<float4>(),
+                             arr.tangents.empty() ? nullptr : res->tangents.as<float4>(),
+                             res->texels.as<float4>(),
+                             res->images.as<DevImage>(),
+                             res->textures.as<DevTexture>(),
+                             uint32_t(arr.bvh.nodes.size()),
+                             uint32_t(n),
+                             uint32_t(res->lights.punctual.size()),
+                             0};
         res->desc.alloc(sizeof(DevScene));
         check(hipMemcpyAsync(res->desc.p, &res->host, sizeof(DevScene), hipMemcpyHostToDevice, st), "upload");
         check(hipStreamSynchronize(st), "scene upload");
         const auto t2 = std::chrono::steady_clock::now();
         res->bytes = res->nodes.bytes + res->prims.bytes + res->prim_ng.bytes + res->aux.bytes + res->materials.bytes +
-                     res->light_rank.bytes + res->punctual.bytes + res->desc.bytes;
+                     res->light_rank.bytes + res->punctual.bytes + res->uvs.bytes + res->tangents.bytes +
+                     res->texels.bytes + res->images.bytes + res->textures.bytes + res->desc.bytes;
         if (info) {
             info->triangles = n;
-            info->bvh_nodes = bvh.binary_nodes;
-            info->wide_nodes = bvh.nodes.size();
+            info->bvh_nodes = arr.bvh.binary_nodes;
+            info->wide_nodes = arr.bvh.nodes.size();
             info->device_bytes = res->bytes;
             info->build_seconds = std::chrono::duration<double>(t1 - t0).count();
             info->upload_seconds = std::chrono::duration<double>(t2 - t1).count();
@@ -323,8 +305,10 @@ int crg_trace_rays(crg_ctx* ctx, uint64_t scene_id, const crg_ray* rays, uint64_
               "trace launch");
         if (count) check(hipMemcpyAsync(hits, ctx->hits.p, count * sizeof(crg_hit), hipMemcpyDeviceToHost, st), "copy");
         check(hipStreamSynchronize(st), "trace");
-        for (uint64_t i = 0; i < count; ++i)
+        for (uint64_t i = 0; i < count; ++i) {
             if (hits[i].hit == 2u) throw std::logic_error("BVH traversal exceeded its bounded stack");
+            if (hits[i].hit == 3u) throw std::runtime_error("Non-finite texture coordinates or transform");
+        }
     });
 }
 

@@ -35,7 +35,7 @@ namespace cr {
 using Clock = std::chrono::steady_clock;
 
 struct SceneRes {
-    DevBuf nodes, prims, prim_ng, aux, materials, light_rank, punctual, desc;
+    DevBuf nodes, prims, prim_ng, aux, materials, light_rank, punctual, uvs, tangents, texels, images, textures, desc;
     DevScene host{};
     SceneLights lights;
     std::vector<uint32_t> tri_to_prim;

@@ -15,6 +15,7 @@
 // only over-include.
 #pragma once
 #include "layout.h"
+#include "texture.h"
 
 namespace cr {
 
@@ -242,6 +243,7 @@ struct ClosestResult {
     uint32_t id;    // finalized triangle index
     bool exact;       // t,u,v hold the binary64 values
     bool overflow;    // traversal stack exhausted (reported as an error)
+    bool tex_error;   // alpha-mask texture coordinates not finite (reported as an error)
     bool need_exact;  // FP32-only traversal met an undecided test: rerun with EXACT
     sf::f64 t, u, v;
     float tlo, thi;  // FP32 enclosure of the binary64 t (or the query limit)
@@ -280,7 +282,7 @@ CR_HD inline bool leaf_prims(const DevScene& s, RayQuery& q, uint32_t first, uin
         float clo = 0, chi = 0;
         const int fr = fast_triangle(q, va, vb, vc, clo, chi);
         if (fr == kReject) continue;
-        if (fr == kAccept) {
+        if (fr == kAccept && !(p.flags & kPrimMaskTexture)) {
             if (!ANY && clo > best.thi) continue;  // certainly beyond the current limit
             if (!prim_accepts(s, i, p.flags, q.d)) continue;
             if (ANY) {
@@ -300,7 +302,9 @@ CR_HD inline bool leaf_prims(const DevScene& s, RayQuery& q, uint32_t first, uin
                 continue;
             }
         }
-        // Undecided in FP32: decide with the binary64 emulation.
+        if (fr == kAccept && !ANY && clo > best.thi) continue;
+        // Undecided in FP32, or an alpha mask that needs the binary64
+        // barycentrics: decide with the binary64 emulation.
         if constexpr (!EXACT) {
             best.need_exact = true;
             if (!ANY) return true;  // any-hit keeps looking for a certain blocker
@@ -316,6 +320,15 @@ CR_HD inline bool leaf_prims(const DevScene& s, RayQuery& q, uint32_t first, uin
         const ExactHit e = exact_triangle(q, va, vb, vc, ANY ? sf::inf(0) : best.t);
         if (!e.hit) continue;
         if (!prim_accepts(s, i, p.flags, q.d)) continue;
+        if (p.flags & kPrimMaskTexture) {
+            bool bad = false;
+            const bool keep = mask_accepts(s, i, p.material, e.u, e.v, bad);
+            if (bad) {
+                best.tex_error = true;
+                return true;
+            }
+            if (!keep) continue;
+        }
         if (ANY) {
             take_exact(best, i, p.id, e);
             return true;
@@ -391,6 +404,7 @@ struct Traversal {
         best.id = 0;
         best.exact = true;
         best.overflow = false;
+        best.tex_error = false;
         best.need_exact = false;
         best.t = sf::from_float(q.tmax);
         best.u = best.v = sf::zero(0);

@@ -43,6 +43,24 @@ HVec3 orthogonal_of(DV v, HVec3 n, HVec3 fallback = {0, 0, 0}) {
     if (ddot_(proj, proj) <= ddot_(v, v) * tol * tol) return fallback;
     return unit_of(proj, fallback);
 }
+HVec3 hcross(HVec3 a, HVec3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+bool nonzero3_(HVec3 v) { return v.x != 0 || v.y != 0 || v.z != 0; }
+
+// UV-to-position differential directions (reference src/geometry.cpp:98-116).
+bool uv_directions(const HTriangle& t, int set, DV& tangent, DV& bitangent) {
+    const double du1 = double(t.uv[1][2 * set]) - t.uv[0][2 * set];
+    const double dv1 = double(t.uv[1][2 * set + 1]) - t.uv[0][2 * set + 1];
+    const double du2 = double(t.uv[2][2 * set]) - t.uv[0][2 * set];
+    const double dv2 = double(t.uv[2][2 * set + 1]) - t.uv[0][2 * set + 1];
+    const double det = du1 * dv2 - dv1 * du2;
+    if (det == 0) return false;
+    const DV e1 = dv(t.p[1]) - dv(t.p[0]), e2 = dv(t.p[2]) - dv(t.p[0]);
+    const double sign = det < 0 ? -1 : 1;
+    tangent = (e1 * dv2 - e2 * dv1) * sign;
+    bitangent = (e2 * du1 - e1 * du2) * sign;
+    return nonzero3_(unit_of(tangent)) && nonzero3_(unit_of(bitangent));
+}
+
 bool fin(float v) { return std::isfinite(v); }
 bool fin3(const float* v) { return fin(v[0]) && fin(v[1]) && fin(v[2]); }
 bool nonzero3(HVec3 v) { return v.x != 0 || v.y != 0 || v.z != 0; }
@@ -64,6 +82,16 @@ HostScene finalize_scene(const crg_scene_desc& d) {
         for (size_t k = 0; k < n; ++k)
             if (!fin(im.rgba[k])) throw std::runtime_error("Image " + std::to_string(i) + " contains non-finite pixels");
     }
+    for (uint32_t i = 0; i < d.image_count; ++i) {
+        const crg_image& im = d.images[i];
+        HImage h;
+        h.width = im.width;
+        h.height = im.height;
+        h.texels.resize(size_t(im.width) * size_t(im.height));
+        for (size_t k = 0; k < h.texels.size(); ++k)
+            h.texels[k] = make_float4(im.rgba[4 * k], im.rgba[4 * k + 1], im.rgba[4 * k + 2], im.rgba[4 * k + 3]);
+        s.images.push_back(std::move(h));
+    }
     for (uint32_t i = 0; i < d.texture_count; ++i) {
         const crg_texture& t = d.textures[i];
         if (t.image < 0 || uint32_t(t.image) >= d.image_count)
@@ -71,6 +99,7 @@ HostScene finalize_scene(const crg_scene_desc& d) {
         auto ok = [](int w) { return w == 0 || w == 1 || w == 2; };
         if (!ok(t.wrap_s) || !ok(t.wrap_t))
             throw std::runtime_error("Texture " + std::to_string(i) + " has an invalid wrap mode");
+        s.textures.push_back(t);
     }
     for (size_t i = 0; i < s.materials.size(); ++i) {
         const crg_material& m = s.materials[i];
@@ -110,13 +139,17 @@ HostScene finalize_scene(const crg_scene_desc& d) {
         if (!ok) throw std::runtime_error("Triangle " + std::to_string(i) + " contains non-finite vertex data");
     }
     s.tris.reserve(n);
-    size_t degenerate = 0, first_degenerate = 0, generated = 0;
+    size_t degenerate = 0, first_degenerate = 0, generated = 0, fallback_tangents = 0;
     for (uint64_t i = 0; i < n; ++i) {
         HTriangle t;
         for (int k = 0; k < 3; ++k) {
             t.p[k] = {d.positions[9 * i + 3 * k], d.positions[9 * i + 3 * k + 1], d.positions[9 * i + 3 * k + 2]};
             t.n[k] = d.normals ? HVec3{d.normals[9 * i + 3 * k], d.normals[9 * i + 3 * k + 1], d.normals[9 * i + 3 * k + 2]}
                                : HVec3{0, 0, 0};
+            for (int q = 0; q < 4; ++q) {
+                t.tan[k][q] = d.tangents ? d.tangents[12 * i + 4 * k + q] : (q == 3 ? 1.0f : 0.0f);
+                t.uv[k][q] = d.uvs ? d.uvs[12 * i + 4 * k + q] : 0.0f;
+            }
         }
         t.material = d.material_ids[i];
         const DV normal = dcross(dv(t.p[1]) - dv(t.p[0]), dv(t.p[2]) - dv(t.p[0]));
@@ -130,15 +163,41 @@ HostScene finalize_scene(const crg_scene_desc& d) {
             continue;
         }
         t.ng = unit_of(normal);
+        // Tangent frames (reference src/geometry.cpp:365-388): vertex tangents
+        // orthogonalized against the repaired normal, else UV-derived in the
+        // normal map's UV set, else an orthogonal basis.
+        const crg_material& mat = s.materials[t.material];
+        const int set = mat.normal_tex.texture >= 0 ? mat.normal_tex.texcoord : 0;
+        DV ut{0, 0, 0}, ub{0, 0, 0};
+        const bool usable = uv_directions(t, set, ut, ub);
         for (int k = 0; k < 3; ++k) {
             HVec3 nn = unit_of(dv(t.n[k]));
+            float hand = t.tan[k][3] < 0 ? -1.0f : 1.0f;
             if (!nonzero3(nn) || fdot(nn, t.ng) == 0) {
                 nn = t.ng;
                 ++generated;
             } else if (fdot(nn, t.ng) < 0) {
                 nn = {-nn.x, -nn.y, -nn.z};
+                hand = -hand;
+            }
+            HVec3 tg = orthogonal_of(DV{t.tan[k][0], t.tan[k][1], t.tan[k][2]}, nn);
+            if (!nonzero3(tg)) {
+                if (usable) tg = orthogonal_of(ut, nn);
+                if (nonzero3(tg)) {
+                    const HVec3 c = hcross(nn, tg);
+                    hand = ddot_(dv(c), ub) < 0 ? -1.0f : 1.0f;
+                } else {
+                    const V3 fx = make_frame(V3{nn.x, nn.y, nn.z}).x;
+                    tg = {fx.x, fx.y, fx.z};
+                    hand = 1;
+                    if (mat.normal_tex.texture >= 0) ++fallback_tangents;
+                }
             }
             t.n[k] = nn;
+            t.tan[k][0] = tg.x;
+            t.tan[k][1] = tg.y;
+            t.tan[k][2] = tg.z;
+            t.tan[k][3] = hand;
         }
         t.ung = unit_of(dv(t.ng));
         s.tris.push_back(t);
@@ -149,6 +208,9 @@ HostScene finalize_scene(const crg_scene_desc& d) {
     if (generated)
         s.warnings.push_back("Generated flat normals for " + std::to_string(generated) +
                              " missing or tangential vertex normals");
+    if (fallback_tangents)
+        s.warnings.push_back("Used orthogonal fallback tangents for " + std::to_string(fallback_tangents) +
+                             " normal-mapped vertices with degenerate UVs");
     return s;
 }
 
@@ -574,6 +636,7 @@ std::string format_path_error(const ErrorRec& e, int width, int spp) {
         case kErrCamera: cause = "Unrepresentable perspective ray"; break;
         case kErrStack: cause = "BVH traversal exceeded its bounded stack"; break;
         case kErrInternal: cause = "Work item outside its render request"; break;
+        case kErrTexture: cause = "Non-finite texture coordinates or transform"; break;
         default:
             cause = "Refractive exit without entry; check outward normals and keep the camera outside closed volumes";
     }
@@ -624,9 +687,82 @@ std::vector<DevMaterial> device_materials(const HostScene& s) {
             }
         }
         d.double_sided = m.double_sided ? 1u : 0u;
+        d.normal_scale = m.normal_scale;
+        d.alpha_cutoff = m.alpha_cutoff;
+        const crg_texture_slot* src[5] = {&m.base_color_tex, &m.metallic_roughness_tex, &m.normal_tex,
+                                          &m.emissive_tex, &m.transmission_tex};
+        DevSlot* dst[5] = {&d.base_tex, &d.mr_tex, &d.normal_tex, &d.emissive_tex, &d.transmission_tex};
+        for (int k = 0; k < 5; ++k) {
+            DevSlot& o = *dst[k];
+            o.texture = src[k]->texture;
+            o.texcoord = src[k]->texcoord;
+            o.offset[0] = src[k]->offset[0];
+            o.offset[1] = src[k]->offset[1];
+            o.scale[0] = src[k]->scale[0];
+            o.scale[1] = src[k]->scale[1];
+            const double c = std::cos(double(src[k]->rotation)), sn = std::sin(double(src[k]->rotation));
+            std::memcpy(&o.cos_bits, &c, 8);
+            std::memcpy(&o.sin_bits, &sn, 8);
+            if (o.texture >= 0) d.maps = 1;
+        }
         out.push_back(d);
     }
     return out;
+}
+
+SceneArrays scene_arrays(const HostScene& hs) {
+    SceneArrays a;
+    a.bvh = build_wide_bvh(hs);
+    const size_t n = hs.tris.size();
+    a.tri_to_prim.assign(n, 0);
+    for (size_t i = 0; i < n; ++i) a.tri_to_prim[a.bvh.order[i]] = uint32_t(i);
+    a.materials = device_materials(hs);
+    bool any_map = false, any_normal_map = false;
+    for (const crg_material& m : hs.materials) {
+        for (const crg_texture_slot* sl :
+             {&m.base_color_tex, &m.metallic_roughness_tex, &m.normal_tex, &m.emissive_tex, &m.transmission_tex})
+            any_map = any_map || sl->texture >= 0;
+        any_normal_map = any_normal_map || m.normal_tex.texture >= 0;
+    }
+    a.prims.resize(n);
+    a.prim_ng.resize(n);
+    a.aux.resize(n);
+    if (any_map) a.uvs.resize(3 * n);
+    if (any_normal_map) a.tangents.resize(3 * n);
+    for (size_t i = 0; i < n; ++i) {
+        const HTriangle& t = hs.tris[a.bvh.order[i]];
+        const crg_material& m = hs.materials[t.material];
+        uint32_t flags = 0;
+        if (!m.double_sided && m.transmission <= 0) flags |= kPrimCull;
+        if (m.alpha_mode == 1) {
+            if (m.base_color_tex.texture >= 0) flags |= kPrimMaskTexture;
+            else if (!(m.base_color[3] * 1.0f >= m.alpha_cutoff)) flags |= kPrimNeverHit;
+        }
+        const uint32_t cls = m.transmission > 0 ? kShadeTransmissive : m.metallic > 0 ? kShadeMetal : kShadeOpaque;
+        flags |= cls << kPrimClassShift;
+        a.prims[i] = PrimRec{t.p[0].x, t.p[0].y, t.p[0].z, a.bvh.order[i], t.p[1].x, t.p[1].y, t.p[1].z, flags,
+                             t.p[2].x, t.p[2].y, t.p[2].z, t.material};
+        a.prim_ng[i] = make_float4(t.ng.x, t.ng.y, t.ng.z, 0);
+        PrimAux& x = a.aux[i];
+        const HVec3* nv[3] = {&t.n[0], &t.n[1], &t.n[2]};
+        float* dst[3] = {x.n0, x.n1, x.n2};
+        for (int k = 0; k < 3; ++k) {
+            dst[k][0] = nv[k]->x;
+            dst[k][1] = nv[k]->y;
+            dst[k][2] = nv[k]->z;
+            if (any_map) a.uvs[3 * i + k] = make_float4(t.uv[k][0], t.uv[k][1], t.uv[k][2], t.uv[k][3]);
+            if (any_normal_map) a.tangents[3 * i + k] = make_float4(t.tan[k][0], t.tan[k][1], t.tan[k][2], t.tan[k][3]);
+        }
+        x.ng[0] = t.ng.x; x.ng[1] = t.ng.y; x.ng[2] = t.ng.z;
+        x.ung[0] = t.ung.x; x.ung[1] = t.ung.y; x.ung[2] = t.ung.z;
+        x.area = t.area;
+    }
+    for (const HImage& im : hs.images) {
+        a.images.push_back(DevImage{uint64_t(a.texels.size()), im.width, im.height});
+        a.texels.insert(a.texels.end(), im.texels.begin(), im.texels.end());
+    }
+    for (const crg_texture& t : hs.textures) a.textures.push_back(DevTexture{t.image, t.wrap_s, t.wrap_t, t.linear});
+    return a;
 }
 
 }  // namespace cr

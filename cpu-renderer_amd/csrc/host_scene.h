@@ -21,16 +21,25 @@ struct HVec3 {
 struct HTriangle {
     HVec3 p[3];
     HVec3 n[3];
+    float tan[3][4];  // finalized tangent.xyz and handedness per vertex
+    float uv[3][4];   // uv0.xy, uv1.xy per vertex
     uint32_t material;
     HVec3 ng;   // geometric normal (finalize)
     HVec3 ung;  // unit(ng) in binary64 (offsetRay)
     float area;
 };
 
+struct HImage {
+    int width = 0, height = 0;
+    std::vector<float4> texels;
+};
+
 struct HostScene {
     std::vector<HTriangle> tris;
     std::vector<crg_material> materials;
     std::vector<crg_light> lights;
+    std::vector<HImage> images;
+    std::vector<crg_texture> textures;
     std::vector<std::string> warnings;
 };
 
@@ -95,7 +104,24 @@ void validate_options(const crg_options& o);
 // reference's rethrown exceptions (reference src/render.cpp:437-438, :310-323).
 std::string format_path_error(const ErrorRec& e, int width, int spp);
 
-// Material table with precomputed absorption (reference src/bsdf.cpp:63-68).
+// Material table with precomputed absorption (reference src/bsdf.cpp:63-68)
+// and texture slots with their rotation's binary64 cos/sin.
 std::vector<DevMaterial> device_materials(const HostScene& s);
+
+// Everything the device reads for one scene, in BVH leaf order.
+struct SceneArrays {
+    WideBvh bvh;
+    std::vector<uint32_t> tri_to_prim;
+    std::vector<PrimRec> prims;
+    std::vector<float4> prim_ng;
+    std::vector<PrimAux> aux;
+    std::vector<DevMaterial> materials;
+    std::vector<float4> uvs;       // empty unless a material samples a texture
+    std::vector<float4> tangents;  // empty unless a material has a normal map
+    std::vector<float4> texels;
+    std::vector<DevImage> images;
+    std::vector<DevTexture> textures;
+};
+SceneArrays scene_arrays(const HostScene& hs);
 
 }  // namespace cr

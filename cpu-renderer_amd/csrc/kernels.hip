@@ -114,6 +114,7 @@ __global__ void __launch_bounds__(256) k_trace_rays(const DevScene* scene, const
     h.triangle = h.hit ? best.id : kNoTriangle;
     h.t = h.u = h.v = 0;
     if (best.overflow) h.hit = 2u;
+    if (best.tex_error) h.hit = 3u;
     if (h.hit == 1u) {
         if (!best.exact) {
             const ExactHit e = exact_of(s, q, best.prim);
@@ -282,9 +283,11 @@ __device__ inline uint32_t wave_claim32(uint32_t* counter, bool pred) {
 constexpr uint32_t kHitPrimMask = 0x3fffffffu;
 constexpr uint32_t kHitMiss = 0x3fffffffu;      // no hit: environment
 constexpr uint32_t kHitOverflow = 0x3ffffffeu;  // traversal stack exhausted
+constexpr uint32_t kHitTexture = 0x3ffffffdu;   // alpha-mask texture coordinates not finite
 
 __device__ inline uint32_t encode_hit(const DevScene& s, const ClosestResult& b) {
     if (b.overflow) return kHitOverflow;
+    if (b.tex_error) return kHitTexture;
     if (b.prim == kNoTriangle) return kHitMiss;
     return b.prim | (((s.prims[b.prim].flags >> kPrimClassShift) & 3u) << 30);
 }
@@ -414,8 +417,9 @@ __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, S
                 PathState st = load_state(P, path);
                 if (st.flags & kFailed) {
                     action = kRetire;
-                } else if (hit == kHitOverflow) {
-                    report_error(ctl, ji, key, kErrStack, st.flags & kDepthMask, 0, 0, sf::zero(0), st.o, st.o, st.d);
+                } else if (hit == kHitOverflow || hit == kHitTexture) {
+                    report_error(ctl, ji, key, hit == kHitOverflow ? kErrStack : kErrTexture, st.flags & kDepthMask, 0,
+                                 0, sf::zero(0), st.o, st.o, st.d);
                     st.L = v3(0);
                     action = kRetire;
                 } else {
@@ -530,7 +534,13 @@ __global__ void __launch_bounds__(256) k_shadow_exact(PathSoA P, ShadowSoA S, co
         shadow_query(q, S, i);
         TraceCount tc{0, 0, 0};
         const ClosestResult b = traverse<true, true>(s, q, tc);
-        if (b.prim == kNoTriangle && !b.overflow) add_contribution(P, S, i);
+        if (b.tex_error) {
+            const uint32_t path = S.path[i];
+            const uint64_t key = uint64_t(P.rec[path].pixel) * uint64_t(jobs[job].spp) + P.rec[path].sample;
+            report_error(ctl, job, key, kErrTexture, 0, 0, 0, sf::zero(0), q.o, q.o, q.d);
+        } else if (b.prim == kNoTriangle && !b.overflow) {
+            add_contribution(P, S, i);
+        }
         atomicAdd(&cnt->fallbacks[job], (unsigned long long)tc.fallbacks);
         atomicAdd(&cnt->exact_rays[job], 1ull);
     }

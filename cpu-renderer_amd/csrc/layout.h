@@ -46,7 +46,7 @@ static_assert(sizeof(WNode) == 128, "WNode must be one 128-byte line");
 enum PrimFlags : uint32_t {
     kPrimCull = 1u,        // single-sided opaque: reject when dot(Ng, d) >= 0
     kPrimNeverHit = 2u,    // MASK material whose constant alpha is below its cutoff
-    kPrimMaskTexture = 4u,  // MASK material with a base-colour texture (unsupported on device)
+    kPrimMaskTexture = 4u,  // MASK material with a base-colour texture: alpha sampled at the exact hit
     kPrimClassShift = 8u,   // bits 8..9: shading class (scheduling only, never results)
 };
 
@@ -78,6 +78,25 @@ struct alignas(16) PrimAux {
 };
 static_assert(sizeof(PrimAux) == 64, "PrimAux is one 64-byte record");
 
+// Texture binding (reference TextureSlot, src/scene.h:181-184) with the
+// binary64 cos/sin of its rotation computed on the host (std::cos/std::sin on
+// the same value the reference converts).
+struct DevSlot {
+    int32_t texture;   // -1: no map
+    int32_t texcoord;  // UV set 0 or 1
+    float offset[2], scale[2];
+    uint64_t cos_bits, sin_bits;
+};
+
+struct DevImage {
+    uint64_t offset;  // first texel in the scene's texel array
+    int32_t width, height;
+};
+
+struct DevTexture {
+    int32_t image, wrap_s, wrap_t, linear;  // wrap: 0 repeat, 1 clamp, 2 mirror
+};
+
 struct DevMaterial {
     float base[4];
     float metallic, roughness;
@@ -86,7 +105,10 @@ struct DevMaterial {
     float att_color[3];
     float sigma[3];  // absorptionCoefficient(att_color, att_distance), evaluated on the host
     uint32_t double_sided;
-    uint32_t pad;
+    uint32_t maps;   // any slot bound (surfaceAt samples textures)
+    float normal_scale;
+    float alpha_cutoff;  // MASK materials
+    DevSlot base_tex, mr_tex, normal_tex, emissive_tex, transmission_tex;
 };
 
 struct DevLightEntry {
@@ -110,6 +132,11 @@ struct DevScene {
     const DevMaterial* materials;
     const uint32_t* light_rank;   // per prim: entry index in the light table, or kNoTriangle
     const DevPunctual* punctual;
+    const float4* uvs;       // per prim 3 x (uv0.xy, uv1.xy) when a material samples a texture
+    const float4* tangents;  // per prim 3 x (tangent.xyz, handedness) when a material has a normal map
+    const float4* texels;    // all images, row-major
+    const DevImage* images;
+    const DevTexture* textures;
     uint32_t node_count, prim_count, punctual_count, pad;
 };
 
@@ -218,6 +245,7 @@ enum ErrorCode : uint32_t {
     kErrCamera = 7,
     kErrStack = 8,
     kErrInternal = 9,  // work item outside the job it maps to (engine bookkeeping)
+    kErrTexture = 10,  // non-finite texture coordinates
 };
 
 constexpr int kMaxErrors = 64;  // error ring (records of one host step)

@@ -103,6 +103,11 @@ CR_HDI uint32_t bounce(const DevJob& J, PathState& st, uint32_t prim, ShadowOut&
     const ExactHit hit = exact_of(sc, q, prim);
     const float tf = sf::to_float(hit.t);
     const Surf s = surface_at(sc, prim, ray_d, hit.u, hit.v);
+    if (s.texture_error) {
+        set_error(err, kErrTexture, depth, tri_id, s.material, sf::zero(0), s.position, ray_o, ray_d);
+        st.L = v3(0);
+        return kRetire;
+    }
     if (st.medium >= 0) T = T * beer_lambert(st.sigma, tf);
     if ((s.front || s.double_sided) && max_component(s.emission) > 0) {
         const float w = last_delta || !nee
@@ -118,6 +123,11 @@ CR_HDI uint32_t bounce(const DevJob& J, PathState& st, uint32_t prim, ShadowOut&
     Rng rng{st.rng};
     if (nee) {
         const LightSample light = sample_light(sc, J.lights, env, s.position, rng);
+        if (light.tex_error) {
+            set_error(err, kErrTexture, depth, tri_id, s.material, sf::zero(0), s.position, ray_o, ray_d);
+            st.L = v3(0);
+            return kRetire;
+        }
         const bool enclosed_infinite = st.medium >= 0 && s.thickness <= 0 && !finite1(light.distance);
         if (!enclosed_infinite && light.pdf > 0 && max_component(light.radiance) > 0) {
             const BsdfEval bsdf = evaluate_bsdf(s, wo, light.direction);

@@ -393,4 +393,42 @@ CR_HDI f64 from_i32(int32_t v) {
 CR_HDI f64 one() { return f64{0x3ff0000000000000ull}; }
 CR_HDI f64 two() { return f64{0x4000000000000000ull}; }
 
+// floor() of a binary64 value (exact; signed zeros and integers pass through).
+CR_HDI f64 floor_(f64 a) {
+    if (!is_finite(a) || is_zero(a)) return a;
+    const int32_t ex = exp_of(a) - 1023;
+    if (ex >= 52) return a;
+    if (ex < 0) return sign_of(a) ? f64{0xbff0000000000000ull} : zero(0);  // (-1, 0) -> -1, (0, 1) -> +0
+    const uint64_t frac = (uint64_t(1) << (52 - ex)) - 1;
+    if ((a.b & frac) == 0) return a;
+    const f64 t{a.b & ~frac};  // toward zero
+    return sign_of(a) ? sub(t, one()) : t;
+}
+
+// fmod(a, 2.0) (exact; the result has a's sign and |result| < 2).
+CR_HDI f64 fmod2(f64 a) {
+    if (!is_finite(a)) return qnan();
+    if (is_zero(a)) return a;
+    const int32_t ex = exp_of(a) - 1023;
+    if (ex < 1) return a;                     // |a| < 2
+    if (ex >= 53) return zero(sign_of(a));    // an even integer
+    const f64 half{a.b - (uint64_t(1) << 52)};  // a / 2 (exact: a is normal with ex >= 1)
+    const int32_t hx = ex - 1;
+    const uint64_t frac = hx >= 52 ? 0 : (uint64_t(1) << (52 - hx)) - 1;
+    const f64 q{half.b & ~frac};                // trunc(a / 2)
+    const f64 r = sub(a, f64{q.b + (uint64_t(1) << 52)});  // a - 2*trunc(a/2), exact
+    return is_zero(r) ? zero(sign_of(a)) : r;
+}
+
+// Integral binary64 value with |a| < 2^62 to int64.
+CR_HDI int64_t to_i64(f64 a) {
+    if (is_zero(a)) return 0;
+    int32_t e;
+    uint64_t m;
+    unpack(a, e, m);
+    const uint64_t mag = e >= 52 ? m << (e - 52) : m >> (52 - e);
+    return sign_of(a) ? -int64_t(mag) : int64_t(mag);
+}
+
+
 }  // namespace sf

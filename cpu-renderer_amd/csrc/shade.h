@@ -12,6 +12,7 @@
 //   triangle_pdf    Lights::trianglePdf     reference src/render.cpp:165-170
 #pragma once
 #include "layout.h"
+#include "texture.h"
 
 namespace cr {
 
@@ -74,6 +75,7 @@ struct Surf {
     float metallic, roughness, transmission, ior, thickness, att_distance;
     uint32_t material;
     bool front, double_sided;
+    bool texture_error;  // non-finite texture coordinates (reported like the reference's exception)
 };
 
 CR_HDI void material_into(const DevMaterial& m, Surf& s) {
@@ -87,6 +89,63 @@ CR_HDI void material_into(const DevMaterial& m, Surf& s) {
     s.att_distance = m.att_distance;
     s.att_color = V3{m.att_color[0], m.att_color[1], m.att_color[2]};
     s.double_sided = m.double_sided != 0;
+    s.texture_error = false;
+}
+
+// orthogonalized() (reference src/geometry.cpp:40-47) in emulated binary64.
+CR_HDI V3 orthogonalize(const D3& v, V3 n, V3 fallback) {
+    const D3 dn = widen(n);
+    const sf::f64 nn = ddot(dn, dn);
+    if (!sf::gt(nn, sf::zero(0))) return fallback;
+    const D3 proj = dsub(v, dscale(dn, sf::div(ddot(v, dn), nn)));
+    const sf::f64 tol{0x3ea0000000000000ull};  // 4 * FLT_EPSILON = 2^-21
+    if (sf::le(ddot(proj, proj), sf::mul(sf::mul(ddot(v, v), tol), tol))) return fallback;
+    return dunit(proj, fallback);
+}
+
+// Material maps and normal mapping of surfaceAt (reference src/geometry.cpp:420-457).
+// The tangent frame after mapping is not consumed by the BSDF (it shades in
+// Frame(normal)), so only the mapped normal is produced.
+CR_OUTLINE void apply_maps(const DevScene& sc, const DevMaterial& m, uint32_t prim, sf::f64 w, sf::f64 hu,
+                           sf::f64 hv, Surf& s) {
+    bool bad = false;
+    const UvPair uv = interp_uv(sc, prim, hu, hv);
+    const float4 base = sample_tex(sc, m.base_tex, uv, true, bad);
+    s.base = V3{m.base[0], m.base[1], m.base[2]} * V3{base.x, base.y, base.z};
+    const float4 mr = sample_tex(sc, m.mr_tex, uv, false, bad);
+    s.metallic = clampf(m.metallic * mr.z, 0.0f, 1.0f);
+    s.roughness = clampf(m.roughness * mr.y, 0.0f, 1.0f);
+    const float4 em = sample_tex(sc, m.emissive_tex, uv, true, bad);
+    s.emission = V3{m.emission[0], m.emission[1], m.emission[2]} * V3{em.x, em.y, em.z};
+    s.transmission = clampf(m.transmission * sample_tex(sc, m.transmission_tex, uv, false, bad).x, 0.0f, 1.0f);
+    if (m.normal_tex.texture >= 0) {
+        const float4 t0 = sc.tangents[3 * prim], t1 = sc.tangents[3 * prim + 1], t2 = sc.tangents[3 * prim + 2];
+        const D3 ti = dadd(dadd(dscale(widen(V3{t0.x, t0.y, t0.z}), w), dscale(widen(V3{t1.x, t1.y, t1.z}), hu)),
+                           dscale(widen(V3{t2.x, t2.y, t2.z}), hv));
+        V3 tangent = orthogonalize(ti, s.normal, make_frame(s.normal).x);
+        const sf::f64 hsum = sf::add(sf::add(sf::mul(w, sf::from_float(t0.w)), sf::mul(hu, sf::from_float(t1.w))),
+                                     sf::mul(hv, sf::from_float(t2.w)));
+        float hand = sf::lt(hsum, sf::zero(0)) ? -1.0f : 1.0f;
+        V3 bitangent = cross(s.normal, tangent) * hand;
+        const DevSlot& sl = m.normal_tex;
+        if (sl.scale[0] != 0 && sl.scale[1] != 0) {
+            const sf::f64 c{sl.cos_bits}, sn{sl.sin_bits};
+            const D3 tt = dsub(dscale(widen(tangent), sf::div(c, sf::from_float(sl.scale[0]))),
+                               dscale(widen(bitangent), sf::div(sn, sf::from_float(sl.scale[1]))));
+            tangent = orthogonalize(tt, s.normal, tangent);
+            if ((sl.scale[0] < 0) != (sl.scale[1] < 0)) hand = -hand;
+            bitangent = cross(s.normal, tangent) * hand;
+        }
+        const float4 tx = sample_tex(sc, sl, uv, false, bad);
+        const sf::f64 one = sf::one(), two = sf::two(), ns = sf::from_float(m.normal_scale);
+        const sf::f64 kx = sf::mul(sf::sub(sf::mul(two, sf::from_float(tx.x)), one), ns);
+        const sf::f64 ky = sf::mul(sf::sub(sf::mul(two, sf::from_float(tx.y)), one), ns);
+        const sf::f64 kz = sf::sub(sf::mul(two, sf::from_float(tx.z)), one);
+        const D3 mapped = dadd(dadd(dscale(widen(tangent), kx), dscale(widen(bitangent), ky)), dscale(widen(s.normal), kz));
+        s.normal = dunit(mapped, s.normal);
+        if (dot(s.normal, s.ng) < 0) s.normal = -s.normal;
+    }
+    s.texture_error = bad;
 }
 
 CR_OUTLINE Surf surface_at(const DevScene& sc, uint32_t prim, V3 ray_dir, sf::f64 hu, sf::f64 hv) {
@@ -108,7 +167,9 @@ CR_OUTLINE Surf surface_at(const DevScene& sc, uint32_t prim, V3 ray_dir, sf::f6
     s.normal = dunit(N, s.ng);
     if (dot(s.normal, s.ng) < 0) s.normal = -s.normal;
     s.material = p.material;
-    material_into(sc.materials[p.material], s);
+    const DevMaterial& m = sc.materials[p.material];
+    material_into(m, s);
+    if (m.maps) apply_maps(sc, m, prim, w, hu, hv, s);
     return s;
 }
 
@@ -413,6 +474,7 @@ struct LightSample {
     V3 direction, radiance, target, target_normal, target_unormal;
     float distance, pdf;
     bool delta;
+    bool tex_error;  // the emitter's textures read non-finite coordinates
 };
 
 CR_HDI LightSample no_light() {
@@ -421,6 +483,7 @@ CR_HDI LightSample no_light() {
     l.distance = kInf;
     l.pdf = 0;
     l.delta = false;
+    l.tex_error = false;
     return l;
 }
 
@@ -500,9 +563,27 @@ CR_OUTLINE LightSample sample_light(const DevScene& sc, const DevLights& lt, con
     const int side = dot_sign_exact(out.target_normal, out.direction);
     if ((p.flags & kPrimCull) && side >= 0) return no_light();
     const DevMaterial& m = sc.materials[p.material];
+    // accepts() and surfaceAt() at the sampled barycentrics (u1, u2).
+    V3 emission{m.emission[0], m.emission[1], m.emission[2]};
+    if (m.maps) {
+        const sf::f64 hu = sf::from_float(u1), hv = sf::from_float(u2);
+        const UvPair uv = interp_uv(sc, prim, hu, hv);
+        bool bad = slot_uv_bad(m.base_tex, uv);
+        if (!bad && (p.flags & kPrimMaskTexture) && !mask_accepts(sc, prim, p.material, hu, hv, bad))
+            return no_light();
+        bad = bad || slot_uv_bad(m.mr_tex, uv) || slot_uv_bad(m.emissive_tex, uv) ||
+              slot_uv_bad(m.transmission_tex, uv) || slot_uv_bad(m.normal_tex, uv);
+        if (bad) {
+            LightSample e = no_light();
+            e.tex_error = true;
+            return e;
+        }
+        const float4 em = sample_tex(sc, m.emissive_tex, uv, true, bad);
+        emission = emission * V3{em.x, em.y, em.z};
+    }
     const bool front = side < 0;
     if (!front && !m.double_sided) return no_light();
-    out.radiance = V3{m.emission[0], m.emission[1], m.emission[2]};
+    out.radiance = emission;
     out.pdf = triangle_pdf(sc, lt, prim, from, out.target);
     return out;
 }

@@ -11,12 +11,11 @@
 //   Bounds, intersectTriangle, offsetRay ... src/geometry.cpp:120-209
 //   Camera::generate ........................ src/geometry.cpp:213-234
 //   Scene::accepts / finalize ............... src/geometry.cpp:279-397
-//   surfaceAt ............................... src/geometry.cpp:401-459 (factor-only materials)
+//   Scene::sample, interpolateUv ........... src/geometry.cpp:68-100, :238-275
+//   surfaceAt ............................... src/geometry.cpp:401-459
 //   Bvh::build / subdivide / intersect ...... src/geometry.cpp:463-639
 //   BSDF .................................... src/bsdf.cpp:6-237
 //   Environment, Lights, trace, render ...... src/render.cpp:12-456
-// Texture maps (src/geometry.cpp:238-275) are not restated: every scene the
-// oracle is asked to check is factor-only, and scene creation rejects maps.
 #include <algorithm>
 #include <array>
 #include <atomic>
@@ -146,6 +145,10 @@ struct Hit {
     double t = INF, u = 0, v = 0;
     uint32_t tri = 0;
 };
+struct Slot {
+    int texture = -1, texcoord = 0;
+    float offset[2] = {0, 0}, scale[2] = {1, 1}, rotation = 0;
+};
 struct Mat {
     V base{1, 1, 1};
     float alpha = 1, metallic = 1, roughness = 1;
@@ -153,7 +156,17 @@ struct Mat {
     float transmission = 0, ior = 1.5f, thickness = 0, attDist = INF;
     V attColor{1, 1, 1};
     bool twoSided = false, mask = false;
-    float cutoff = 0.5f;
+    float cutoff = 0.5f, normalScale = 1;
+    Slot baseTex, mrTex, normalTex, emissiveTex, transmissionTex;
+};
+struct Image {
+    int w = 0, h = 0;
+    std::vector<std::array<float, 4>> px;
+};
+enum Wrap { REPEAT = 0, CLAMP = 1, MIRROR = 2 };
+struct Texture {
+    int image = -1, wrapS = REPEAT, wrapT = REPEAT;
+    bool linear = true;
 };
 struct Tri {
     V p[3], n[3];
@@ -172,7 +185,63 @@ struct Scene {
     std::vector<Tri> tris;
     std::vector<Mat> mats;
     std::vector<Punctual> lights;
+    std::vector<Image> images;
+    std::vector<Texture> textures;
 };
+
+// ---------------------------------------------------------------- textures
+using F4 = std::array<float, 4>;
+inline float srgbLinear(float v) { return v <= 0.04045f ? v / 12.92f : std::pow((v + 0.055f) / 1.055f, 2.4f); }
+
+// UV reduction before texel indexing (reference src/geometry.cpp:77-84).
+double wrapCoord(double x, int wrap) {
+    if (wrap == CLAMP) return std::clamp(x, 0.0, 1.0);
+    if (wrap == REPEAT) return x - std::floor(x);
+    double r = std::fmod(x, 2.0);
+    if (r < 0) r += 2;
+    return r <= 1 ? r : 2 - r;
+}
+// Integer neighbour wrapping (reference src/geometry.cpp:87-95).
+int wrapIndex(int64_t i, int size, int wrap) {
+    const int64_t n = size;
+    if (wrap == CLAMP) return int(std::clamp(i, int64_t{0}, n - 1));
+    const int64_t period = wrap == MIRROR ? 2 * n : n;
+    i %= period;
+    if (i < 0) i += period;
+    if (wrap == MIRROR && i >= n) i = period - 1 - i;
+    return int(i);
+}
+F4 lerp4(F4 a, F4 b, float t) {
+    return {a[0] * (1 - t) + b[0] * t, a[1] * (1 - t) + b[1] * t, a[2] * (1 - t) + b[2] * t, a[3] * (1 - t) + b[3] * t};
+}
+// Scene::sample (reference src/geometry.cpp:238-275): scale, rotate, translate,
+// wrap, then nearest or bilinear with per-texel sRGB decode.
+F4 sampleTex(const Scene& sc, const Slot& slot, const float uv0[2], const float uv1[2], bool srgb) {
+    if (slot.texture == -1) return {1, 1, 1, 1};
+    const Texture& tex = sc.textures[size_t(slot.texture)];
+    const Image& im = sc.images[size_t(tex.image)];
+    const float* uv = slot.texcoord == 0 ? uv0 : uv1;
+    if (!std::isfinite(uv[0]) || !std::isfinite(uv[1]) || !std::isfinite(slot.offset[0]) ||
+        !std::isfinite(slot.offset[1]) || !std::isfinite(slot.scale[0]) || !std::isfinite(slot.scale[1]) ||
+        !std::isfinite(slot.rotation))
+        throw std::runtime_error("Non-finite texture coordinates or transform");
+    const double c = std::cos(double(slot.rotation)), s = std::sin(double(slot.rotation));
+    const double x = double(uv[0]) * slot.scale[0], y = double(uv[1]) * slot.scale[1];
+    const double u = wrapCoord(c * x - s * y + slot.offset[0], tex.wrapS);
+    const double v = wrapCoord(s * x + c * y + slot.offset[1], tex.wrapT);
+    auto texel = [&](int64_t ix, int64_t iy) {
+        const int tx = wrapIndex(ix, im.w, tex.wrapS), ty = wrapIndex(iy, im.h, tex.wrapT);
+        F4 val = im.px[size_t(ty) * size_t(im.w) + size_t(tx)];
+        if (srgb)
+            for (int k = 0; k < 3; ++k) val[k] = srgbLinear(val[k]);
+        return val;
+    };
+    if (!tex.linear) return texel(int64_t(std::floor(u * im.w)), int64_t(std::floor(v * im.h)));
+    const double px = u * im.w - 0.5, py = v * im.h - 0.5;
+    const int64_t ix = int64_t(std::floor(px)), iy = int64_t(std::floor(py));
+    const float fx = float(px - std::floor(px)), fy = float(py - std::floor(py));
+    return lerp4(lerp4(texel(ix, iy), texel(ix + 1, iy), fx), lerp4(texel(ix, iy + 1), texel(ix + 1, iy + 1), fx), fy);
+}
 
 bool validRay(const Ray& r) {
     return fin(r.o) && fin(r.lo) && fin(r.d) && nz(r.d) && !std::isnan(r.tmin) && !std::isnan(r.tmax) &&
@@ -209,14 +278,27 @@ bool hitTriangle(const Tri& t, const Ray& r, Hit& h) {
     return true;
 }
 
-bool accepted(const Scene& s, const Tri& t, const Ray& r) {
+// interpolateUv (reference src/geometry.cpp:68-74)
+void interpUv(const Tri& t, const Hit& h, int set, float out[2]) {
+    const double w = 1.0 - double(h.u) - h.v;
+    for (int k = 0; k < 2; ++k)
+        out[k] = float(w * t.uv[0][set][k] + double(h.u) * t.uv[1][set][k] + double(h.v) * t.uv[2][set][k]);
+}
+
+bool accepted(const Scene& s, const Tri& t, const Ray& r, const Hit& h) {
     const Mat& m = s.mats[t.mat];
     if (!m.twoSided && m.transmission <= 0) {
         V n = t.ng;
         if (!nz(n)) n = unitd(dcross(dd(t.p[1]) - dd(t.p[0]), dd(t.p[2]) - dd(t.p[0])));
         if (ddot(dd(n), dd(r.d)) >= 0) return false;
     }
-    if (m.mask) return m.alpha * 1.0f >= m.cutoff;
+    if (m.mask) {
+        float uv0[2], uv1[2];
+        interpUv(t, h, 0, uv0);
+        interpUv(t, h, 1, uv1);
+        const F4 c = sampleTex(s, m.baseTex, uv0, uv1, false);
+        return m.alpha * c[3] >= m.cutoff;
+    }
     return true;
 }
 
@@ -294,12 +376,14 @@ void finalize(Scene& s) {
         t.area = float(twice * 0.5);
         if (!(t.area > 0)) continue;
         t.ng = unitd(cr);
-        // UV-derived tangent directions (set 0: no normal map on factor-only materials)
+        // UV-derived tangent directions in the normal map's UV set (reference src/geometry.cpp:98-116)
+        const Mat& tm = s.mats[t.mat];
+        const int set = tm.normalTex.texture >= 0 ? tm.normalTex.texcoord : 0;
         D ut{0, 0, 0}, ub{0, 0, 0};
         bool usable = false;
         {
-            const double du1 = double(t.uv[1][0][0]) - t.uv[0][0][0], dv1 = double(t.uv[1][0][1]) - t.uv[0][0][1];
-            const double du2 = double(t.uv[2][0][0]) - t.uv[0][0][0], dv2 = double(t.uv[2][0][1]) - t.uv[0][0][1];
+            const double du1 = double(t.uv[1][set][0]) - t.uv[0][set][0], dv1 = double(t.uv[1][set][1]) - t.uv[0][set][1];
+            const double du2 = double(t.uv[2][set][0]) - t.uv[0][set][0], dv2 = double(t.uv[2][set][1]) - t.uv[0][set][1];
             const double det = du1 * dv2 - dv1 * du2;
             if (det != 0) {
                 const D e1 = dd(t.p[1]) - dd(t.p[0]), e2 = dd(t.p[2]) - dd(t.p[0]);
@@ -362,18 +446,42 @@ Surf surfaceAt(const Scene& sc, const Ray& r, const Hit& h) {
     auto tv = [&](int k) { return D{t.tan[k][0], t.tan[k][1], t.tan[k][2]}; };
     s.tan = ortho(tv(0) * w + tv(1) * h.u + tv(2) * h.v, s.n, Basis(s.n).x);
     const float hand = w * t.tan[0][3] + h.u * t.tan[1][3] + h.v * t.tan[2][3] < 0 ? -1.0f : 1.0f;
-    s.bit = cross(s.n, s.tan) * hand;
-    s.base = m.base * V{1, 1, 1};
-    s.metallic = std::clamp(m.metallic * 1.0f, 0.0f, 1.0f);
-    s.roughness = std::clamp(m.roughness * 1.0f, 0.0f, 1.0f);
-    s.emission = m.emission * V{1, 1, 1};
-    s.transmission = std::clamp(m.transmission * 1.0f, 0.0f, 1.0f);
+    float hand2 = hand;
+    s.bit = cross(s.n, s.tan) * hand2;
+    float uv0[2], uv1[2];
+    interpUv(t, h, 0, uv0);
+    interpUv(t, h, 1, uv1);
+    const F4 base = sampleTex(sc, m.baseTex, uv0, uv1, true);
+    s.base = m.base * V{base[0], base[1], base[2]};
+    const F4 mr = sampleTex(sc, m.mrTex, uv0, uv1, false);
+    s.metallic = std::clamp(m.metallic * mr[2], 0.0f, 1.0f);
+    s.roughness = std::clamp(m.roughness * mr[1], 0.0f, 1.0f);
+    const F4 em = sampleTex(sc, m.emissiveTex, uv0, uv1, true);
+    s.emission = m.emission * V{em[0], em[1], em[2]};
+    s.transmission = std::clamp(m.transmission * sampleTex(sc, m.transmissionTex, uv0, uv1, false)[0], 0.0f, 1.0f);
     s.ior = m.ior;
     s.thickness = m.thickness;
     s.attDist = m.attDist;
     s.attColor = m.attColor;
     s.mat = t.mat;
     s.twoSided = m.twoSided;
+    if (m.normalTex.texture >= 0) {  // normal mapping (reference src/geometry.cpp:436-457)
+        const Slot& sl = m.normalTex;
+        if (sl.scale[0] != 0 && sl.scale[1] != 0) {
+            const double c = std::cos(double(sl.rotation)), sn = std::sin(double(sl.rotation));
+            const D tt = dd(s.tan) * (c / sl.scale[0]) - dd(s.bit) * (sn / sl.scale[1]);
+            s.tan = ortho(tt, s.n, s.tan);
+            if ((sl.scale[0] < 0) != (sl.scale[1] < 0)) hand2 = -hand2;
+            s.bit = cross(s.n, s.tan) * hand2;
+        }
+        const F4 tx = sampleTex(sc, sl, uv0, uv1, false);
+        const D mapped = dd(s.tan) * ((2.0 * tx[0] - 1) * m.normalScale) + dd(s.bit) * ((2.0 * tx[1] - 1) * m.normalScale) +
+                         dd(s.n) * (2.0 * tx[2] - 1);
+        s.n = unitd(mapped, s.n);
+        if (dot(s.n, s.ng) < 0) s.n = -s.n;
+        s.tan = ortho(dd(s.tan), s.n, Basis(s.n).x);
+        s.bit = cross(s.n, s.tan) * hand2;
+    }
     return s;
 }
 
@@ -832,7 +940,7 @@ struct Bvh {
                     Hit c;
                     c.t = lim;
                     c.tri = k;
-                    if (!hitTriangle(s.tris[k], cl, c) || !accepted(s, s.tris[k], cl)) continue;
+                    if (!hitTriangle(s.tris[k], cl, c) || !accepted(s, s.tris[k], cl, c)) continue;
                     if (any) {
                         hit = c;
                         return true;
@@ -974,7 +1082,7 @@ struct Lights {
         out.dir = d / out.dist;
         const Hit h{out.dist, u1, u2, e.index};
         const Ray ray{from, out.dir, 0, INF, {}};
-        if (!accepted(s, t, ray)) return {};
+        if (!accepted(s, t, ray, h)) return {};
         const Surf sf = surfaceAt(s, ray, h);
         if (!sf.front && !sf.twoSided) return {};
         out.rad = sf.emission;
@@ -1281,10 +1389,24 @@ void* orc_scene_create(const crg_scene_desc* d) {
         Scene& s = box->s;
         for (uint32_t i = 0; i < d->material_count; ++i) {
             const crg_material& m = d->materials[i];
-            for (const crg_texture_slot* t : {&m.base_color_tex, &m.metallic_roughness_tex, &m.normal_tex,
-                                              &m.emissive_tex, &m.transmission_tex})
-                if (t->texture >= 0) throw std::runtime_error("the oracle restates factor-only materials");
             Mat x;
+            auto slotOf = [](const crg_texture_slot& t) {
+                Slot o;
+                o.texture = t.texture;
+                o.texcoord = t.texcoord;
+                o.offset[0] = t.offset[0];
+                o.offset[1] = t.offset[1];
+                o.scale[0] = t.scale[0];
+                o.scale[1] = t.scale[1];
+                o.rotation = t.rotation;
+                return o;
+            };
+            x.baseTex = slotOf(m.base_color_tex);
+            x.mrTex = slotOf(m.metallic_roughness_tex);
+            x.normalTex = slotOf(m.normal_tex);
+            x.emissiveTex = slotOf(m.emissive_tex);
+            x.transmissionTex = slotOf(m.transmission_tex);
+            x.normalScale = m.normal_scale;
             x.base = v3(m.base_color);
             x.alpha = m.base_color[3];
             x.metallic = m.metallic;
@@ -1313,6 +1435,32 @@ void* orc_scene_create(const crg_scene_desc* d) {
             p.outer = l.outer_cone;
             s.lights.push_back(p);
         }
+        for (uint32_t i = 0; i < d->image_count; ++i) {
+            Image im;
+            im.w = d->images[i].width;
+            im.h = d->images[i].height;
+            if (im.w <= 0 || im.h <= 0 || !d->images[i].rgba)
+                throw std::runtime_error("Image " + std::to_string(i) + " has invalid dimensions or pixel count");
+            im.px.resize(size_t(im.w) * size_t(im.h));
+            for (size_t q = 0; q < im.px.size(); ++q)
+                for (int k = 0; k < 4; ++k) im.px[q][k] = d->images[i].rgba[4 * q + k];
+            s.images.push_back(std::move(im));
+        }
+        for (uint32_t i = 0; i < d->texture_count; ++i) {
+            Texture t;
+            t.image = d->textures[i].image;
+            t.wrapS = d->textures[i].wrap_s;
+            t.wrapT = d->textures[i].wrap_t;
+            t.linear = d->textures[i].linear != 0;
+            if (t.image < 0 || size_t(t.image) >= s.images.size())
+                throw std::runtime_error("Texture " + std::to_string(i) + " has an invalid image index");
+            s.textures.push_back(t);
+        }
+        for (size_t i = 0; i < s.mats.size(); ++i)
+            for (const Slot* sl : {&s.mats[i].baseTex, &s.mats[i].mrTex, &s.mats[i].normalTex, &s.mats[i].emissiveTex,
+                                   &s.mats[i].transmissionTex})
+                if (sl->texture < -1 || (sl->texture >= 0 && size_t(sl->texture) >= s.textures.size()))
+                    throw std::runtime_error("Material " + std::to_string(i) + " has an invalid texture index");
         s.tris.resize(d->triangle_count);
         for (uint64_t t = 0; t < d->triangle_count; ++t) {
             Tri& tr = s.tris[t];

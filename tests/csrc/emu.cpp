@@ -29,12 +29,8 @@ thread_local std::string lastError;
 
 struct EmuScene {
     HostScene hs;
-    WideBvh bvh;
-    std::vector<PrimRec> prims;
-    std::vector<float4> png;
-    std::vector<PrimAux> aux;
-    std::vector<DevMaterial> mats;
-    std::vector<uint32_t> ranks, tri_to_prim;
+    SceneArrays arr;
+    std::vector<uint32_t> ranks;
     SceneLights lights;
     DevScene dev{};
 };
@@ -55,39 +51,15 @@ void* emu_scene_create(const crg_scene_desc* d) {
     try {
         auto s = std::make_unique<EmuScene>();
         s->hs = finalize_scene(*d);
-        s->bvh = build_wide_bvh(s->hs);
+        s->arr = scene_arrays(s->hs);
         const size_t n = s->hs.tris.size();
-        s->tri_to_prim.assign(n, 0);
-        for (size_t i = 0; i < n; ++i) s->tri_to_prim[s->bvh.order[i]] = uint32_t(i);
-        s->prims.resize(n);
-        s->png.resize(n);
-        s->aux.resize(n);
-        for (size_t i = 0; i < n; ++i) {
-            const HTriangle& t = s->hs.tris[s->bvh.order[i]];
-            const crg_material& m = s->hs.materials[t.material];
-            uint32_t flags = 0;
-            if (!m.double_sided && m.transmission <= 0) flags |= kPrimCull;
-            if (m.alpha_mode == 1 && !(m.base_color[3] * 1.0f >= m.alpha_cutoff)) flags |= kPrimNeverHit;
-            s->prims[i] = PrimRec{t.p[0].x, t.p[0].y, t.p[0].z, s->bvh.order[i], t.p[1].x, t.p[1].y, t.p[1].z, flags,
-                                  t.p[2].x, t.p[2].y, t.p[2].z, t.material};
-            s->png[i] = make_float4(t.ng.x, t.ng.y, t.ng.z, 0);
-            PrimAux& a = s->aux[i];
-            const HVec3* nv[3] = {&t.n[0], &t.n[1], &t.n[2]};
-            float* dst[3] = {a.n0, a.n1, a.n2};
-            for (int k = 0; k < 3; ++k) {
-                dst[k][0] = nv[k]->x;
-                dst[k][1] = nv[k]->y;
-                dst[k][2] = nv[k]->z;
-            }
-            a.ng[0] = t.ng.x; a.ng[1] = t.ng.y; a.ng[2] = t.ng.z;
-            a.ung[0] = t.ung.x; a.ung[1] = t.ung.y; a.ung[2] = t.ung.z;
-            a.area = t.area;
-        }
         s->lights = scene_lights(s->hs);
-        s->ranks = light_ranks(s->lights, s->tri_to_prim, n);
-        s->mats = device_materials(s->hs);
-        s->dev = DevScene{s->bvh.nodes.data(), s->prims.data(), s->png.data(), s->aux.data(), s->mats.data(),
-                          s->ranks.data(), s->lights.punctual.data(), uint32_t(s->bvh.nodes.size()), uint32_t(n),
+        s->ranks = light_ranks(s->lights, s->arr.tri_to_prim, n);
+        SceneArrays& a = s->arr;
+        s->dev = DevScene{a.bvh.nodes.data(), a.prims.data(), a.prim_ng.data(), a.aux.data(), a.materials.data(),
+                          s->ranks.data(), s->lights.punctual.data(), a.uvs.empty() ? nullptr : a.uvs.data(),
+                          a.tangents.empty() ? nullptr : a.tangents.data(), a.texels.data(), a.images.data(),
+                          a.textures.data(), uint32_t(a.bvh.nodes.size()), uint32_t(n),
                           uint32_t(s->lights.punctual.size()), 0};
         return s.release();
     } catch (const std::exception& e) {
@@ -98,14 +70,14 @@ void* emu_scene_create(const crg_scene_desc* d) {
 
 void emu_scene_destroy(void* s) { delete static_cast<EmuScene*>(s); }
 uint64_t emu_scene_triangles(void* s) { return static_cast<EmuScene*>(s)->hs.tris.size(); }
-uint64_t emu_scene_wide_nodes(void* s) { return static_cast<EmuScene*>(s)->bvh.nodes.size(); }
+uint64_t emu_scene_wide_nodes(void* s) { return static_cast<EmuScene*>(s)->arr.bvh.nodes.size(); }
 
 // Structural check of the 4-wide tree: every primitive in exactly one leaf,
 // every child box encloses its subtree, and leaves respect the encoding.
 int emu_scene_check(void* sp) {
     const EmuScene& s = *static_cast<EmuScene*>(sp);
-    const auto& nodes = s.bvh.nodes;
-    const size_t n = s.prims.size();
+    const auto& nodes = s.arr.bvh.nodes;
+    const size_t n = s.arr.prims.size();
     if (n == 0) return nodes.empty() ? 0 : 1;
     std::vector<int> seen(n, 0);
     struct Item {
@@ -120,7 +92,7 @@ int emu_scene_check(void* sp) {
             for (uint32_t i = leaf_first(it.ref); i < leaf_first(it.ref) + leaf_count(it.ref); ++i) {
                 if (i >= n) return 2;
                 ++seen[i];
-                const PrimRec& p = s.prims[i];
+                const PrimRec& p = s.arr.prims[i];
                 const float xs[3] = {p.ax, p.bx, p.cx}, ys[3] = {p.ay, p.by, p.cy}, zs[3] = {p.az, p.bz, p.cz};
                 for (int k = 0; k < 3; ++k)
                     if (xs[k] < it.lo[0] || xs[k] > it.hi[0] || ys[k] < it.lo[1] || ys[k] > it.hi[1] ||
@@ -224,6 +196,10 @@ int emu_trace(void* sp, const crg_ray* rays, uint64_t n, crg_hit* hits, int any,
             lastError = "BVH traversal exceeded its bounded stack";
             return 1;
         }
+        if (b.tex_error) {
+            lastError = "Non-finite texture coordinates or transform";
+            return 1;
+        }
         crg_hit& h = hits[i];
         h.hit = b.prim != kNoTriangle;
         h.triangle = h.hit ? b.id : kNoTriangle;
@@ -272,7 +248,7 @@ int emu_render(void* sp, void* ep, const crg_camera* camera, const crg_options* 
         DevJob J{};
         J.scene = &s.dev;
         J.env = &env.dev;
-        const LightTable lt = build_light_table(s.lights, s.tri_to_prim, env.he.integral);
+        const LightTable lt = build_light_table(s.lights, s.arr.tri_to_prim, env.he.integral);
         J.lights = DevLights{lt.entries.data(), uint32_t(lt.entries.size()), lt.env_probability};
         J.cam = prepare_camera(*camera, o->width, o->height);
         J.width = o->width;
@@ -310,8 +286,8 @@ int emu_render(void* sp, void* ep, const crg_camera* camera, const crg_options* 
                             c = traverse<false, true>(s.dev, qx, tc);
                         }
                         ++ext;
-                        if (c.overflow) {
-                            code = kErrStack;
+                        if (c.overflow || c.tex_error) {
+                            code = c.overflow ? kErrStack : kErrTexture;
                             break;
                         }
                         ShadowOut sh;
@@ -330,6 +306,10 @@ int emu_render(void* sp, void* ep, const crg_camera* camera, const crg_options* 
                                 b = traverse<true, true>(s.dev, sx, tc);
                             }
                             ++shd;
+                            if (b.tex_error) {
+                                code = kErrTexture;
+                                break;
+                            }
                             if (b.prim == kNoTriangle) st.L = st.L + sh.c;
                         }
                         if (a == kReplay) {

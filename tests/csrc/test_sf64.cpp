@@ -81,6 +81,21 @@ int main(int argc, char** argv) {
             if (!same(std::sqrt(v), sf::sqrt(x))) report("sqrt-edge", v, 0, std::sqrt(v), sf::sqrt(x));
         }
     }
+    // floor / fmod(x, 2) / to_i64 (texture coordinate wrapping).
+    for (long i = 0; i < n / 4; ++i) {
+        double x;
+        const uint64_t r = g();
+        switch (i % 3) {
+            case 0: x = std::ldexp(double(int64_t(r) >> 11), -int(g() % 80)); break;
+            case 1: x = (double(int(g() % 4001)) - 2000) / 8.0; break;
+            default: x = (double(r >> 11) * 0x1p-53 - 0.5) * std::ldexp(1.0, int(g() % 130) - 65); break;
+        }
+        const sf::f64 a{db(x)};
+        if (!same(std::floor(x), sf::floor_(a))) report("floor", x, 0, std::floor(x), sf::floor_(a));
+        if (!same(std::fmod(x, 2.0), sf::fmod2(a))) report("fmod2", x, 2, std::fmod(x, 2.0), sf::fmod2(a));
+        const double f = std::floor(x);
+        if (std::fabs(f) < 0x1p61 && sf::to_i64(sf::floor_(a)) != int64_t(f)) report("to_i64", x, 0, f, sf::floor_(a));
+    }
     std::printf("sf64: %ld random cases, %d failures\n", n, failures);
     return failures ? 1 : 0;
 }

@@ -80,6 +80,8 @@ def trace_sets():
     pr = abi.make_rays([(0.25, 0.25, 1), (0.25, 0.25, 1)], [(0, 0, -1), (0, 0, -1)],
                        origin_low=[(0, 0, 1e-9), (0, 0, 0)])
     sets.append(("precise", single.scene(), pr, True))
+    mask, _, _ = rs.textured_alpha_mask()
+    sets.append(("alpha_mask", mask, scenes.random_rays(mask, 3000, seed=4, spread=1.5), False))
     return sets
 
 
@@ -287,7 +289,38 @@ def render_cases():
     cases.append(("c2_small", w2.scene, w2.env, w2.camera, w2.options(width=96, height=54, samples=4), None))
     w3 = scenes.config3(triangles=20000)
     cases.append(("c3_small", w3.scene, w3.env, w3.camera, w3.options(width=96, height=54, samples=4), None))
+    for name, fn in (("tex_factors", rs.textured_factors), ("tex_normal", rs.textured_normal_map),
+                     ("tex_mask", rs.textured_alpha_mask)):
+        s, e, c = fn()
+        cases.append((name, s, e, c, abi.options(48, 40, 8, 6, seed=42), None))
     return cases
+
+
+def textured_surface_sets():
+    """(name, scene, rays) whose hits pin textured surfaceAt (all maps, normal mapping)."""
+    out = []
+    for name, fn in (("tex_factors", rs.textured_factors), ("tex_normal", rs.textured_normal_map),
+                     ("tex_mask", rs.textured_alpha_mask)):
+        s, _, _ = fn()
+        out.append((name, s, scenes.random_rays(s, 2000, seed=5, spread=1.5)))
+    return out
+
+
+def gen_textured_surface(ref):
+    out = {}
+    for name, scene, rays in textured_surface_sets():
+        ps = ref.scene(scene)
+        hits = ps.trace(rays)
+        idx = np.nonzero(hits["hit"])[0][:500]
+        surf = []
+        for i in idx:
+            ray = abi.Ray.from_buffer_copy(rays[i].tobytes())
+            hit = abi.Hit.from_buffer_copy(hits[i].tobytes())
+            surf.append(np.frombuffer(bytes(ps.surface_at(ray, hit)), np.uint8).copy())
+        out[f"{name}/index"] = idx
+        out[f"{name}/surface"] = np.array(surf)
+        print(f"  {name}: {len(idx)} textured surfaces")
+    np.savez_compressed(OUT / "textured_surface.npz", **out)
 
 
 def gen_render(ref):
@@ -311,7 +344,10 @@ def gen_render(ref):
 
 def main():
     ref = probe.Probe("ref")
-    for fn in (gen_rng_camera, gen_trace, gen_offset_surface, gen_bsdf, gen_env, gen_render):
+    only = set(sys.argv[1:])
+    for fn in (gen_rng_camera, gen_trace, gen_offset_surface, gen_bsdf, gen_env, gen_render, gen_textured_surface):
+        if only and fn.__name__ not in only:
+            continue
         print(fn.__name__)
         fn(ref)
 

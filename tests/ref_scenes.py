@@ -248,3 +248,138 @@ def bvh_ordered():
     e = f32(1e-6)
     b.tri((0, 0, e), (1, 0, e), (0, 1, e), 0)
     return b.scene()
+
+
+# ---------------------------------------------------------------- textured scenes
+def _slot(texture, texcoord=0, offset=(0.0, 0.0), scale=(1.0, 1.0), rotation=0.0):
+    return abi.TextureSlot(texture, texcoord, abi.F2(*offset), abi.F2(*scale), rotation)
+
+
+def _image(h, w, seed, kind):
+    """Deterministic test images (h, w, 4) float32 in [0, 1]."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    y, x = np.mgrid[0:h, 0:w].astype(np.float32)
+    img = np.empty((h, w, 4), np.float32)
+    if kind == "checker":
+        c = ((x // 2 + y // 2) % 2).astype(np.float32)
+        img[..., 0] = 0.15 + 0.8 * c
+        img[..., 1] = 0.9 - 0.7 * c
+        img[..., 2] = g.random((h, w), dtype=np.float32) * 0.5 + 0.25
+        img[..., 3] = c  # alpha 0/1 checker (bilinear gives intermediate values)
+    elif kind == "noise":
+        img[:] = g.random((h, w, 4), dtype=np.float32)
+    else:  # tangent-space bump normals, z > 0
+        nx = 0.35 * np.sin(x * 0.9) * np.cos(y * 0.7)
+        ny = 0.35 * np.cos(x * 0.5 + y * 0.3)
+        nz = np.sqrt(np.maximum(0.0, 1.0 - nx * nx - ny * ny))
+        img[..., 0] = nx * 0.5 + 0.5
+        img[..., 1] = ny * 0.5 + 0.5
+        img[..., 2] = nz * 0.5 + 0.5
+        img[..., 3] = 1.0
+    return img
+
+
+def textured_factors():
+    """Base colour (sRGB, bilinear, repeat, scaled/rotated/offset), metallic-roughness
+    (nearest, clamp, UV set 1), emissive (mirror) and transmission (nearest, repeat)
+    maps, reference src/geometry.cpp:238-275 and :401-435."""
+    images = [_image(8, 8, 1, "checker"), _image(5, 7, 2, "noise"), _image(6, 4, 3, "noise")]
+    textures = [abi.Texture(0, 0, 0, 1), abi.Texture(1, 1, 1, 0), abi.Texture(2, 2, 2, 1), abi.Texture(1, 0, 0, 0)]
+    mats = []
+    m = abi.material(base_color=(0.9, 0.85, 0.8), metallic=1.0, roughness=1.0)
+    m.base_color_tex = _slot(0, 0, (0.1, -0.2), (1.5, 2.0), 0.3)
+    m.metallic_roughness_tex = _slot(1, 1, (0.0, 0.0), (1.0, 1.0), 0.0)
+    mats.append(m)
+    m = abi.material(base_color=(0.2, 0.2, 0.2), metallic=0.0, roughness=0.6, emission=(2.0, 1.5, 1.0))
+    m.emissive_tex = _slot(2, 0, (0.25, 0.0), (-1.3, 2.2), -0.7)
+    mats.append(m)
+    m = abi.material(base_color=(1, 1, 1), metallic=0.0, roughness=0.1, transmission=1.0, double_sided=True)
+    m.transmission_tex = _slot(3, 0, (0.0, 0.5), (3.0, 3.0), 1.1)
+    mats.append(m)
+    pos, uvs, mid = [], [], []
+
+    def quad(a, b, c, d, uv_a, uv_b, uv_c, uv_d, mat, uv1_scale=1.0):
+        for tri, tuv in (((a, b, c), (uv_a, uv_b, uv_c)), ((a, c, d), (uv_a, uv_c, uv_d))):
+            pos.append(np.concatenate([_v(p) for p in tri]))
+            uvs.append(np.concatenate([[u[0], u[1], u[0] * uv1_scale + 0.3, u[1] * uv1_scale - 0.1] for u in tuv]))
+            mid.append(mat)
+
+    quad((-2, -1, -2), (2, -1, -2), (2, -1, 2), (-2, -1, 2), (-1.2, -0.5), (2.7, -0.5), (2.7, 3.1), (-1.2, 3.1), 0, 0.8)
+    quad((-2, -1, -2), (-2, 2, -2), (2, 2, -2), (2, -1, -2), (0, 0), (0, 1.7), (2.4, 1.7), (2.4, 0), 1)
+    quad((-0.8, -0.9, 0.4), (0.7, -0.9, 0.1), (0.7, 0.8, 0.1), (-0.8, 0.8, 0.4), (0, 0), (1, 0), (1, 1), (0, 1), 2)
+    # A spot light with a finite range exercises the cone and range falloff of
+    # punctual NEE (reference src/render.cpp:182-209).
+    spot = abi.light(abi.SPOT, position=(0.6, 1.7, 1.6), direction=tuple(normalize((-0.2, -1.0, -0.6))),
+                     color=(1.0, 0.95, 0.9), intensity=9.0, range=6.0, inner_cone=0.25, outer_cone=0.7)
+    scene = abi.SceneData(np.array(pos, np.float32), np.array(mid, np.uint32), mats,
+                          uvs=np.array(uvs, np.float32), images=images, textures=textures, lights=[spot])
+    env = abi.EnvData(constant=(0.3, 0.35, 0.4))
+    cam = test_camera(position=(0.2, 0.6, 3.6), forward=normalize((-0.05, -0.25, -1)), yfov=0.9,
+                      up=normalize((0, 1, -0.25)))
+    return scene, env, cam
+
+
+def textured_normal_map():
+    """Normal-mapped sphere (UV set 1 with a rotated, mirrored slot transform; half the
+    vertices without tangents so finalize derives them from UVs), reference
+    src/geometry.cpp:296-397 (tangents) and :436-457 (mapping)."""
+    from conftest import scenes
+    unit = scenes.icosphere(2)
+    pos = (unit * 0.9).reshape(-1, 9).astype(np.float32)
+    nrm = unit.reshape(-1, 9).astype(np.float32)
+    n = len(pos)
+    v = unit.reshape(-1, 3)
+    uu = (np.arctan2(v[:, 2], v[:, 0]) / (2 * np.pi) + 0.5).astype(np.float32)
+    vv = (np.arccos(np.clip(v[:, 1], -1, 1)) / np.pi).astype(np.float32)
+    uv = np.stack([uu, vv, uu * 2.0 + 0.1, vv * 1.5], 1).reshape(n, 12)
+    tan = np.zeros((n, 3, 4), np.float32)
+    t3 = np.cross(np.array([0.0, 1.0, 0.0]), v).reshape(n, 3, 3)
+    tan[:, :, :3] = t3
+    tan[:, :, 3] = np.where(np.arange(n)[:, None] % 3 == 0, -1.0, 1.0)
+    tan[::2] = 0.0  # every other triangle: no tangents (derived from UVs in finalize)
+    images = [_image(16, 16, 4, "normal"), _image(4, 4, 5, "noise")]
+    textures = [abi.Texture(0, 0, 2, 1), abi.Texture(1, 0, 0, 1)]
+    m = abi.material(base_color=(0.8, 0.6, 0.4), metallic=0.0, roughness=0.45, normal_scale=1.7)
+    m.normal_tex = _slot(0, 1, (0.2, 0.1), (-1.5, 2.0), 0.4)
+    m.base_color_tex = _slot(1, 0)
+    floor = abi.material(base_color=(0.5, 0.5, 0.5), metallic=0.0, roughness=1.0)
+    fpos = np.array([[-3, -1, -3, 3, -1, -3, 3, -1, 3], [-3, -1, -3, 3, -1, 3, -3, -1, 3]], np.float32)
+    scene = abi.SceneData(np.concatenate([pos, fpos]), np.concatenate([np.zeros(n), np.ones(2)]).astype(np.uint32),
+                          [m, floor], normals=np.concatenate([nrm, np.zeros((2, 9), np.float32)]),
+                          tangents=np.concatenate([tan.reshape(n, 12), np.zeros((2, 12), np.float32)]),
+                          uvs=np.concatenate([uv, np.zeros((2, 12), np.float32)]), images=images, textures=textures,
+                          lights=[abi.light(abi.POINT, position=(1.5, 2.0, 2.0), color=(1, 0.9, 0.8), intensity=12.0)])
+    env = abi.EnvData(constant=(0.15, 0.2, 0.3))
+    cam = test_camera(position=(0.0, 0.3, 3.2), forward=normalize((0, -0.1, -1)), yfov=0.75,
+                      up=normalize((0, 1, -0.1)))
+    return scene, env, cam
+
+
+def textured_alpha_mask():
+    """Alpha-masked quad (checker alpha, bilinear) in front of a lit wall: the mask applies
+    to camera, bounce and shadow rays alike (reference src/geometry.cpp:279-292)."""
+    images = [_image(8, 8, 6, "checker")]
+    textures = [abi.Texture(0, 0, 0, 1)]
+    leaf = abi.material(base_color=(0.3, 0.8, 0.3, 0.9), metallic=0.0, roughness=0.8, alpha_mode=1,
+                        alpha_cutoff=0.45, double_sided=True)
+    leaf.base_color_tex = _slot(0, 0, (0.05, 0.0), (2.0, 2.0), 0.0)
+    wall = abi.material(base_color=(0.7, 0.7, 0.7), metallic=0.0, roughness=1.0)
+    b = Builder()
+    b.materials = [leaf, wall]
+    pos, uvs, mid = [], [], []
+    for tri, tuv in ((((-1, -1, 0), (1, -1, 0), (1, 1, 0)), ((0, 0), (1, 0), (1, 1))),
+                     (((-1, -1, 0), (1, 1, 0), (-1, 1, 0)), ((0, 0), (1, 1), (0, 1)))):
+        pos.append(np.concatenate([_v(p) for p in tri]))
+        uvs.append(np.concatenate([[u[0], u[1], 0.0, 0.0] for u in tuv]))
+        mid.append(0)
+    for tri in (((-3, -3, -1.5), (3, -3, -1.5), (3, 3, -1.5)), ((-3, -3, -1.5), (3, 3, -1.5), (-3, 3, -1.5))):
+        pos.append(np.concatenate([_v(p) for p in tri]))
+        uvs.append(np.zeros(12, np.float32))
+        mid.append(1)
+    scene = abi.SceneData(np.array(pos, np.float32), np.array(mid, np.uint32), [leaf, wall],
+                          uvs=np.array(uvs, np.float32), images=images, textures=textures,
+                          lights=[abi.light(abi.POINT, position=(0.3, 0.2, 2.5), intensity=20.0)])
+    env = abi.EnvData(constant=(0.05, 0.05, 0.05))
+    cam = test_camera(position=(0.4, 0.3, 3.0), forward=normalize((-0.1, -0.1, -1)), yfov=0.9,
+                      up=normalize((0, 1, -0.1)))
+    return scene, env, cam
