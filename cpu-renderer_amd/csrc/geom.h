@@ -202,7 +202,7 @@ CR_HDI int fast_triangle(const RayQuery& q, V3 va, V3 vb, V3 vc, float& tlo, flo
 CR_HDI bool prim_accepts(const DevScene& s, uint32_t prim, uint32_t flags, V3 d) {
     if (flags & kPrimNeverHit) return false;
     if (flags & kPrimCull) {
-        const float4 n = s.prim_ng[prim];
+        const CR_GLOBAL PrimAuxNg& n = gref(reinterpret_cast<const PrimAuxNg*>(s.prim_ng + prim));
         if (dot_sign_exact(V3{n.x, n.y, n.z}, d) >= 0) return false;
     }
     return true;
@@ -210,7 +210,7 @@ CR_HDI bool prim_accepts(const DevScene& s, uint32_t prim, uint32_t flags, V3 d)
 
 // Slab test against the four children of a node.  Returns a 4-bit hit mask and
 // the (widened) entry distances.
-CR_HDI uint32_t node_hit(const RayQuery& q, const WNode& n, float limit, float near_out[4]) {
+CR_HDI uint32_t node_hit(const RayQuery& q, const CR_GLOBAL WNode& n, float limit, float near_out[4]) {
     uint32_t mask = 0;
     const bool px = !(q.inv[0] < 0), py = !(q.inv[1] < 0), pz = !(q.inv[2] < 0);
 #pragma unroll
@@ -272,10 +272,10 @@ struct TraceCount {
 // found, or an FP32-only traversal needs the exact rerun).  EXACT=false
 // compiles no binary64 emulation at all: undecided tests flag need_exact.
 template <bool ANY, bool EXACT>
-CR_HD inline bool leaf_prims(const DevScene& s, RayQuery& q, uint32_t first, uint32_t count, ClosestResult& best,
-                             TraceCount& tc) {
+CR_HD inline bool leaf_prims(const DevScene& s, const PrimRec* prims, RayQuery& q, uint32_t first, uint32_t count,
+                             ClosestResult& best, TraceCount& tc) {
     for (uint32_t i = first; i < first + count; ++i) {
-        const PrimRec p = s.prims[i];
+        const CR_GLOBAL PrimRec& p = gref(prims + i);
         if (p.id == q.ignore) continue;
         const V3 va{p.ax, p.ay, p.az}, vb{p.bx, p.by, p.bz}, vc{p.cx, p.cy, p.cz};
         ++tc.tris;
@@ -312,7 +312,7 @@ CR_HD inline bool leaf_prims(const DevScene& s, RayQuery& q, uint32_t first, uin
         } else {
         ++tc.fallbacks;
         if (!ANY && !best.exact) {
-            const PrimRec bp = s.prims[best.prim];
+            const CR_GLOBAL PrimRec& bp = gref(prims + best.prim);
             const ExactHit e = exact_triangle(q, V3{bp.ax, bp.ay, bp.az}, V3{bp.bx, bp.by, bp.bz},
                                               V3{bp.cx, bp.cy, bp.cz}, sf::inf(0));
             take_exact(best, best.prim, best.id, e);
@@ -335,6 +335,71 @@ CR_HD inline bool leaf_prims(const DevScene& s, RayQuery& q, uint32_t first, uin
         }
         if (best.prim == kNoTriangle || sf::lt(e.t, best.t) || (sf::eq(e.t, best.t) && p.id < best.id))
             take_exact(best, i, p.id, e);
+        }
+    }
+    return false;
+}
+
+// FP32-only leaf test with the primitive records of each group of four loaded
+// before any is tested, so their loads are in flight together.  Same decisions
+// as leaf_prims<ANY, false>.
+#ifndef CR_LEAF_PRELOAD
+#define CR_LEAF_PRELOAD 0
+#endif
+#ifndef CR_LEAF_GROUP
+#define CR_LEAF_GROUP 4
+#endif
+// Keep the scene's node and primitive array pointers of a query in registers
+// (1) or read them from the scene record at each visit (0).
+#ifndef CR_CACHE_PTRS
+#define CR_CACHE_PTRS 1
+#endif
+template <bool ANY>
+CR_HD inline bool leaf_fast(const DevScene& s, const PrimRec* prims, RayQuery& q, uint32_t first, uint32_t count,
+                            ClosestResult& best, TraceCount& tc) {
+    const uint32_t end = first + count;
+    for (uint32_t base = first; base < end; base += CR_LEAF_GROUP) {
+        const uint32_t m = end - base < CR_LEAF_GROUP ? end - base : CR_LEAF_GROUP;
+        PrimRec pr[CR_LEAF_GROUP];
+#pragma unroll
+        for (uint32_t j = 0; j < CR_LEAF_GROUP; ++j)
+            if (j < m) {
+                const CR_GLOBAL PrimRec& g = gref(prims + base + j);
+                pr[j].ax = g.ax, pr[j].ay = g.ay, pr[j].az = g.az, pr[j].id = g.id;
+                pr[j].bx = g.bx, pr[j].by = g.by, pr[j].bz = g.bz, pr[j].flags = g.flags;
+                pr[j].cx = g.cx, pr[j].cy = g.cy, pr[j].cz = g.cz, pr[j].material = g.material;
+            }
+#pragma unroll
+        for (uint32_t j = 0; j < CR_LEAF_GROUP; ++j) {
+            if (j >= m) break;
+            const PrimRec& p = pr[j];
+            if (p.id == q.ignore) continue;
+            ++tc.tris;
+            float clo = 0, chi = 0;
+            const int fr = fast_triangle(q, V3{p.ax, p.ay, p.az}, V3{p.bx, p.by, p.bz}, V3{p.cx, p.cy, p.cz}, clo, chi);
+            if (fr == kReject) continue;
+            if (!ANY && clo > best.thi) continue;  // certainly beyond the current limit (also when undecided)
+            if (fr == kAccept && !(p.flags & kPrimMaskTexture)) {
+                if (!prim_accepts(s, base + j, p.flags, q.d)) continue;
+                if (ANY) {
+                    best.prim = base + j;
+                    best.id = p.id;
+                    best.exact = false;
+                    best.tlo = clo;
+                    best.thi = chi;
+                    return true;
+                }
+                if (chi < best.tlo) {  // certainly nearer than the current best
+                    best.prim = base + j;
+                    best.id = p.id;
+                    best.exact = false;
+                    best.tlo = clo;
+                    best.thi = chi;
+                    continue;
+                }
+            }
+            best.need_exact = true;
+            if (!ANY) return true;
         }
     }
     return false;
@@ -363,7 +428,7 @@ struct LocalRing {
     }
 };
 
-template <bool ANY, bool EXACT = true, class Ring = LocalRing>
+template <bool ANY, bool EXACT = true, class Ring = LocalRing, bool PRELOAD = (CR_LEAF_PRELOAD != 0)>
 struct Traversal {
     ClosestResult best;
     StackEntry stack[kStackSize];  // overflow: entries older than the ring's
@@ -372,6 +437,9 @@ struct Traversal {
     int head;  // ring slot of the next push
     int nr;    // entries in the ring
     uint32_t ref;
+    uint32_t leaf;  // postponed leaf (kEmptyChild: none), see inner()/leaves()
+    const WNode* nodes;  // the scene's arrays, read once per query
+    const PrimRec* prims;
 
     CR_HDI void push(uint32_t r, float n) {
         if (nr == Ring::kSize) {  // ring full: its oldest entry (at head) moves to the overflow array
@@ -413,48 +481,106 @@ struct Traversal {
         head = 0;
         nr = 0;
         ref = 0;
+        leaf = kEmptyChild;
+        nodes = s.nodes;
+        prims = s.prims;
         return !q.valid || s.node_count == 0 || q.tmax < q.tmin;
+    }
+
+    CR_HDI bool test_leaf(const DevScene& s, RayQuery& q, uint32_t r, TraceCount& tc) {
+        const PrimRec* pr = CR_CACHE_PTRS ? prims : s.prims;
+        if constexpr (!EXACT && PRELOAD) return leaf_fast<ANY>(s, pr, q, leaf_first(r), leaf_count(r), best, tc);
+        else return leaf_prims<ANY, EXACT>(s, pr, q, leaf_first(r), leaf_count(r), best, tc);
+    }
+
+    // Visit of the inner node `ref`: pushes the farther hit children, moves to
+    // the nearest one (or to the next stack entry).  Returns false when the
+    // stack bound is exceeded (best.overflow set).
+    CR_HDI bool visit_node(const DevScene& s, RayQuery& q, TraceCount& tc) {
+        const CR_GLOBAL WNode& n = gref((CR_CACHE_PTRS ? nodes : s.nodes) + ref);
+        ++tc.nodes;
+        float nears[4];
+        uint32_t mask = node_hit(q, n, best.thi, nears);
+        int order[4];
+        int cnt = 0;
+        while (mask) {
+            const int c = __builtin_ctz(mask);
+            mask &= mask - 1;
+            int k = cnt++;
+            while (k > 0 && nears[order[k - 1]] < nears[c]) {
+                order[k] = order[k - 1];
+                --k;
+            }
+            order[k] = c;  // descending near: order[cnt-1] is the nearest
+        }
+        if (cnt > 0) {
+            if (sp + nr + cnt - 1 > kStackSize) {
+                best.overflow = true;
+                return false;
+            }
+            for (int k = 0; k < cnt - 1; ++k) push(n.child[order[k]], nears[order[k]]);
+            ref = n.child[order[cnt - 1]];
+        } else {
+            next_ref();
+        }
+        return true;
+    }
+
+    // Next stack entry that may still hold a nearer hit (kEmptyChild when none).
+    CR_HDI void next_ref() {
+        StackEntry e;
+        while (pop(e)) {
+            if (!ANY && e.near > best.thi) continue;
+            ref = e.ref;
+            return;
+        }
+        ref = kEmptyChild;
+    }
+
+    // Postponed-leaf traversal (Aila and Laine's speculative while-while
+    // scheme): inner() visits inner nodes only; the first leaf reached is
+    // parked in `leaf` and the walk goes on from the stack, a second one waits
+    // in `ref`.  leaves() tests the parked leaf and a waiting one.  A wave
+    // alternates the two phases so that its lanes load and test the same kind
+    // of record at the same time.  Both return true when the query is
+    // finished; the result does not depend on how the phases are interleaved.
+    CR_HDI bool can_descend() const { return ref != kEmptyChild && !ref_is_leaf(ref); }
+    CR_HDI bool has_leaf() const { return leaf != kEmptyChild || ref_is_leaf(ref); }
+
+    CR_HDI bool inner(const DevScene& s, RayQuery& q, TraceCount& tc) {
+        if (can_descend() && !visit_node(s, q, tc)) return true;
+        if (ref_is_leaf(ref) && leaf == kEmptyChild) {
+            leaf = ref;
+            next_ref();
+        }
+        return ref == kEmptyChild && leaf == kEmptyChild;
+    }
+
+    CR_HDI bool leaves(const DevScene& s, RayQuery& q, TraceCount& tc) {
+        if (leaf != kEmptyChild) {
+            if (test_leaf(s, q, leaf, tc)) return true;
+            leaf = kEmptyChild;
+        }
+        if (ref_is_leaf(ref)) {
+            if (test_leaf(s, q, ref, tc)) return true;
+            next_ref();
+            if (ref_is_leaf(ref)) {
+                leaf = ref;
+                next_ref();
+            }
+        }
+        return ref == kEmptyChild && leaf == kEmptyChild;
     }
 
     // One visit; returns true when the query is finished (result in best).
     CR_HDI bool step(const DevScene& s, RayQuery& q, TraceCount& tc) {
         if (!ref_is_leaf(ref)) {
-            const WNode& n = s.nodes[ref];
-            ++tc.nodes;
-            float nears[4];
-            uint32_t mask = node_hit(q, n, best.thi, nears);
-            int order[4];
-            int cnt = 0;
-            while (mask) {
-                const int c = __builtin_ctz(mask);
-                mask &= mask - 1;
-                int k = cnt++;
-                while (k > 0 && nears[order[k - 1]] < nears[c]) {
-                    order[k] = order[k - 1];
-                    --k;
-                }
-                order[k] = c;  // descending near: order[cnt-1] is the nearest
-            }
-            if (cnt > 0) {
-                if (sp + nr + cnt - 1 > kStackSize) {
-                    best.overflow = true;
-                    return true;
-                }
-                for (int k = 0; k < cnt - 1; ++k) push(n.child[order[k]], nears[order[k]]);
-                ref = n.child[order[cnt - 1]];
-                return false;
-            }
-        } else if (leaf_prims<ANY, EXACT>(s, q, leaf_first(ref), leaf_count(ref), best, tc)) {
-            return true;
+            if (!visit_node(s, q, tc)) return true;
+            return ref == kEmptyChild;
         }
-        // pop the next entry that may still hold a nearer hit
-        StackEntry e;
-        while (pop(e)) {
-            if (!ANY && e.near > best.thi) continue;
-            ref = e.ref;
-            return false;
-        }
-        return true;
+        if (test_leaf(s, q, ref, tc)) return true;
+        next_ref();
+        return ref == kEmptyChild;
     }
 };
 

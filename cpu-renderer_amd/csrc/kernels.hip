@@ -22,7 +22,7 @@
 // their stacks in scratch anyway; shading needs registers for the emulated
 // binary64 arithmetic.
 #ifndef CR_TRAVERSE_MIN_BLOCKS
-#define CR_TRAVERSE_MIN_BLOCKS 6
+#define CR_TRAVERSE_MIN_BLOCKS 5
 #endif
 #ifndef CR_SHADE_MIN_BLOCKS
 #define CR_SHADE_MIN_BLOCKS 2
@@ -240,8 +240,49 @@ __device__ inline void start_item(const PathSoA& P, const DevJob* jobs, uint32_t
 // the idle lanes claim the next queue entries together (one atomic per wave),
 // so a wave's time follows the mean query length rather than its longest
 // query.  Rays with an undecided triangle go to the exact queue.
-constexpr int kVisitsPerRefill = 8;
+#ifndef CR_VISITS
+#define CR_VISITS 16
+#endif
+constexpr int kVisitsPerRefill = CR_VISITS;
 constexpr int kRefillBelow = 48;  // refill when at most this many lanes are busy
+
+// kVisitsPerRefill rounds of traversal for the busy lanes of a wave.  With
+// CR_TRAV_PP each round is one inner-node visit per lane, followed by a leaf
+// round for the lanes holding a leaf when no lane is still looking for its
+// first leaf or at least CR_LEAF_MIN lanes hold one; otherwise each round is
+// one node or leaf visit per lane.
+#ifndef CR_TRAV_PP
+#define CR_TRAV_PP 1
+#endif
+#ifndef CR_LEAF_MIN
+#define CR_LEAF_MIN 24
+#endif
+template <class T>
+__device__ inline void run_visits(T& tr, const DevScene* sc, RayQuery& q, TraceCount& tc, bool& busy, bool& fin) {
+#if CR_TRAV_PP
+    for (int k = 0; k < kVisitsPerRefill; ++k) {
+        if (busy && tr.inner(*sc, q, tc)) {
+            busy = false;
+            fin = true;
+        }
+        if (!__ballot(busy)) break;
+        const bool searching = busy && tr.leaf == kEmptyChild && tr.can_descend();
+        const bool ready = busy && tr.has_leaf();
+        if (!__ballot(searching) || __popcll(__ballot(ready)) >= CR_LEAF_MIN) {
+            if (ready && tr.leaves(*sc, q, tc)) {
+                busy = false;
+                fin = true;
+            }
+        }
+    }
+#else
+    for (int k = 0; k < kVisitsPerRefill && busy; ++k)
+        if (tr.step(*sc, q, tc)) {
+            busy = false;
+            fin = true;
+        }
+#endif
+}
 
 // Newest traversal stack entries of each lane in LDS (lane-minor layout, so a
 // wave's accesses to one ring slot are conflict-free); older entries overflow
@@ -249,11 +290,15 @@ constexpr int kRefillBelow = 48;  // refill when at most this many lanes are bus
 #ifndef CR_LDS_RING
 #define CR_LDS_RING 12
 #endif
+// The ring pointers carry the LDS address space explicitly: through generic
+// pointers the accesses would compile to flat instructions.
+using lds_u32 = __attribute__((address_space(3))) uint32_t;
+using lds_f32 = __attribute__((address_space(3))) float;
 template <int N>
 struct LdsRing {
     static constexpr int kSize = N;
-    uint32_t* ref;  // this lane's column: ref[i * stride]
-    float* near;
+    lds_u32* ref;  // this lane's column: ref[i * stride]
+    lds_f32* near;
     uint32_t stride;
     __device__ void put(int i, uint32_t r, float n) {
         ref[i * stride] = r;
@@ -298,7 +343,7 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_extend(PathSoA 
     __shared__ uint32_t ring_ref[CR_LDS_RING * 256];
     __shared__ float ring_near[CR_LDS_RING * 256];
     Traversal<false, false, TraverseRing> tr;
-    tr.ring = TraverseRing{ring_ref + threadIdx.x, ring_near + threadIdx.x, blockDim.x};
+    tr.ring = TraverseRing{(lds_u32*)(ring_ref + threadIdx.x), (lds_f32*)(ring_near + threadIdx.x), blockDim.x};
     RayQuery q;
     const DevScene* sc = nullptr;
     uint32_t path = 0, job = 0;
@@ -324,11 +369,7 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_extend(PathSoA 
             continue;
         }
         bool fin = false;
-        for (int k = 0; k < kVisitsPerRefill && busy; ++k)
-            if (tr.step(*sc, q, tc)) {
-                busy = false;
-                fin = true;
-            }
+        run_visits(tr, sc, q, tc, busy, fin);
         const bool exact = fin && tr.best.need_exact && !tr.best.overflow;
         if (fin && !exact) P.ray[path].hit = encode_hit(*sc, tr.best);
         const uint32_t pos = wave_append(&ctl->xcount, exact);
@@ -481,7 +522,7 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_shadow(PathSoA 
     __shared__ uint32_t ring_ref[CR_LDS_RING * 256];
     __shared__ float ring_near[CR_LDS_RING * 256];
     Traversal<true, false, TraverseRing> tr;
-    tr.ring = TraverseRing{ring_ref + threadIdx.x, ring_near + threadIdx.x, blockDim.x};
+    tr.ring = TraverseRing{(lds_u32*)(ring_ref + threadIdx.x), (lds_f32*)(ring_near + threadIdx.x), blockDim.x};
     RayQuery q;
     const DevScene* sc = nullptr;
     uint32_t i = 0, job = 0;
@@ -506,11 +547,7 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_shadow(PathSoA 
             if (drained) break;
             continue;
         }
-        for (int k = 0; k < kVisitsPerRefill && busy; ++k)
-            if (tr.step(*sc, q, tc)) {
-                busy = false;
-                fin = true;
-            }
+        run_visits(tr, sc, q, tc, busy, fin);
         bool exact = false;
         if (fin && tr.best.prim == kNoTriangle && !tr.best.overflow) {
             if (tr.best.need_exact) exact = true;

@@ -25,6 +25,27 @@ inline float4 make_float4(float x, float y, float z, float w) { return float4{x,
 
 namespace cr {
 
+// Scene records are read through references in the global address space.
+// Scene pointers are themselves read from memory, so the compiler cannot prove
+// that they point to global memory and would emit flat loads, which also count
+// against the LDS/scalar-load counter and so stall LDS traffic behind them.
+// Members are read through the reference (a copy of the whole record would go
+// through a generic reference again).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define CR_GLOBAL __attribute__((address_space(1)))
+#else
+#define CR_GLOBAL
+#endif
+template <class T>
+CR_HDI const CR_GLOBAL T& gref(const T* p) {
+    return *(const CR_GLOBAL T*)p;
+}
+
+// prim_ng entries read as four plain floats (geometric normal, unused w).
+struct PrimAuxNg {
+    float x, y, z, w;
+};
+
 // Child reference: bit 31 clear -> inner node index; bit 31 set -> leaf with
 // count-1 in bits 27..30 (1..16 primitives) and the first primitive in bits 0..26.
 constexpr uint32_t kEmptyChild = 0x7fffffffu;

@@ -136,6 +136,8 @@ typedef struct crg_stats {
     uint64_t shadow_rays;      /* any-hit queries                              */
     int32_t workers;           /* 0 = device                                   */
     int32_t iterations;        /* wavefront iterations executed                */
+    /* Work counters: they depend on how queries share waves, so a request's
+     * values differ slightly between runs alone and in a batch. */
     uint64_t extend_nodes;     /* 4-wide nodes visited by closest-hit queries  */
     uint64_t extend_tris;      /* triangles tested by closest-hit queries      */
     uint64_t shadow_nodes;     /* 4-wide nodes visited by any-hit queries      */

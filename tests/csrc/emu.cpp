@@ -43,6 +43,23 @@ struct EmuEnv {
 
 }  // namespace
 
+// FP32 pass of the postponed-leaf traversal (inner()/leaves()) with the leaf
+// rounds placed by a pseudo-random schedule: any interleaving must give the
+// same result as step().  Odd schedules use the plain leaf test, even ones the
+// preloading one.
+template <bool ANY, bool PRELOAD>
+ClosestResult phased(const DevScene& s, RayQuery& q, TraceCount& tc, uint64_t& rng) {
+    Traversal<ANY, false, LocalRing, PRELOAD> tr;
+    if (tr.begin(s, q)) return tr.best;
+    for (;;) {
+        if (tr.inner(s, q, tc)) break;
+        rng = rng * 6364136223846793005ull + 1442695040888963407ull;
+        const bool go = !tr.can_descend() || ((rng >> 61) & 1);
+        if (go && tr.has_leaf() && tr.leaves(s, q, tc)) break;
+    }
+    return tr.best;
+}
+
 extern "C" {
 
 const char* emu_last_error(void) { return lastError.c_str(); }
@@ -174,7 +191,8 @@ void emu_env_radiance(void* e, const float* d, float* rgb) {
     rgb[0] = r.x; rgb[1] = r.y; rgb[2] = r.z;
 }
 
-int emu_trace(void* sp, const crg_ray* rays, uint64_t n, crg_hit* hits, int any, uint64_t* fallbacks) {
+int emu_trace(void* sp, const crg_ray* rays, uint64_t n, crg_hit* hits, int any, uint64_t* fallbacks,
+              uint64_t schedule) {
     const EmuScene& s = *static_cast<EmuScene*>(sp);
     uint64_t fbt = 0;
     for (uint64_t i = 0; i < n; ++i) {
@@ -184,7 +202,11 @@ int emu_trace(void* sp, const crg_ray* rays, uint64_t n, crg_hit* hits, int any,
                     r.tmin, r.tmax, V3{r.origin_low[0], r.origin_low[1], r.origin_low[2]}, r.ignore);
         TraceCount tc{0, 0, 0};
         // FP32-only pass first, exact rerun only when flagged (the kernels' order).
-        ClosestResult b = any ? traverse<true, false>(s.dev, q, tc) : traverse<false, false>(s.dev, q, tc);
+        uint64_t rng = schedule * 0x9e3779b97f4a7c15ull + i;
+        ClosestResult b;
+        if (!schedule) b = any ? traverse<true, false>(s.dev, q, tc) : traverse<false, false>(s.dev, q, tc);
+        else if (schedule & 1) b = any ? phased<true, false>(s.dev, q, tc, rng) : phased<false, false>(s.dev, q, tc, rng);
+        else b = any ? phased<true, true>(s.dev, q, tc, rng) : phased<false, true>(s.dev, q, tc, rng);
         const bool rerun = b.need_exact && (!any || b.prim == kNoTriangle);
         if (rerun) {
             setup_query(q, V3{r.origin[0], r.origin[1], r.origin[2]}, V3{r.direction[0], r.direction[1], r.direction[2]},

@@ -43,7 +43,7 @@ class Emu:
             "emu_env_sample": (None, [vp, C.POINTER(u64), fp, fp]),
             "emu_env_pdf": (C.c_float, [vp, fp]),
             "emu_env_radiance": (None, [vp, fp, fp]),
-            "emu_trace": (C.c_int, [vp, vp, u64, vp, C.c_int, C.POINTER(u64)]),
+            "emu_trace": (C.c_int, [vp, vp, u64, vp, C.c_int, C.POINTER(u64), u64]),
             "emu_camera_generate": (C.c_int, [C.POINTER(abi.Camera), C.c_float, C.c_float, C.c_int, C.c_int,
                                               C.POINTER(abi.Ray)]),
             "emu_render": (C.c_int, [vp, vp, C.POINTER(abi.Camera), C.POINTER(abi.Options), fp,
@@ -82,11 +82,14 @@ class EmuScene:
     def triangles(self):
         return self.e.lib.emu_scene_triangles(self.h)
 
-    def trace(self, rays, any_hit=False):
+    def trace(self, rays, any_hit=False, schedule=0):
+        """schedule 0: one node or leaf per step; k > 0: postponed-leaf phases in a
+        pseudo-random interleaving seeded by k (FP32 pass only)."""
         rays = np.ascontiguousarray(rays, abi.RAY_DTYPE)
         hits = np.zeros(len(rays), abi.HIT_DTYPE)
         fb = C.c_uint64()
-        if self.e.lib.emu_trace(self.h, rays.ctypes.data, len(rays), hits.ctypes.data, int(any_hit), C.byref(fb)):
+        if self.e.lib.emu_trace(self.h, rays.ctypes.data, len(rays), hits.ctypes.data, int(any_hit), C.byref(fb),
+                                schedule):
             raise RuntimeError(self.e.error())
         self.fallbacks = fb.value
         return hits

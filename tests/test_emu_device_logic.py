@@ -45,6 +45,12 @@ def test_emu_trace_bit_exact(emu, name):
     for k in ("t", "u", "v"):
         np.testing.assert_array_equal(h[k].view(np.uint64), g[f"{name}/{k}"].view(np.uint64))
     np.testing.assert_array_equal(es.trace(rays, True)["hit"], g[f"{name}/any"])
+    # Postponed-leaf traversal (plain and preloaded leaf tests): any
+    # interleaving of inner and leaf rounds gives the same closest and any hits.
+    for schedule in (1, 2):
+        hp = es.trace(rays, schedule=schedule)
+        np.testing.assert_array_equal(hp, h)
+        np.testing.assert_array_equal(es.trace(rays, True, schedule=schedule)["hit"], g[f"{name}/any"])
 
 
 @pytest.mark.parametrize("name", [e[0] for e in make_golden.test_envs()] + ["sky1024"])
