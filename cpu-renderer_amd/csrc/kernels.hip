@@ -42,17 +42,6 @@ __device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
     return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
 }
 
-__device__ inline unsigned long long wave_claim(unsigned long long* counter, bool pred) {
-    const unsigned long long mask = __ballot(pred);
-    if (!mask) return ~0ull;
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
-    base = __shfl(base, leader);
-    return base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
-}
-
 // Sum a per-lane counter over the wave and add it once to `dst`.
 __device__ inline void wave_add(unsigned long long* dst, uint32_t v) {
     unsigned long long s = v;
@@ -584,12 +573,24 @@ __global__ void __launch_bounds__(256) k_shadow_exact(PathSoA P, ShadowSoA S, co
 }
 
 // ------------------------------------------------------------------ retire + regenerate
-// Finished paths store their sample; each slot then claims the next work item
-// or, when none is left, goes onto the free stack.
+// Finished paths store their sample.  Retire-queue entry i takes work item
+// issued + i while items are left (and joins the next extend queue after the
+// paths shade continued), otherwise its slot goes onto the free stack.  All
+// positions follow from i and the counters as k_retire found them, so no
+// atomics are needed; k_reset then advances the counters.
+__device__ inline uint32_t retire_takes(const Control* ctl) {
+    const unsigned long long issued = ctl->issued, total = ctl->total;
+    const unsigned long long avail = issued < total ? total - issued : 0ull;
+    return avail < ctl->rcount ? uint32_t(avail) : ctl->rcount;
+}
+
 __global__ void __launch_bounds__(256) k_retire(PathSoA P, const DevJob* jobs, const uint32_t* retire,
                                                 uint32_t* next_queue, uint32_t* freeq, int parity, Control* ctl,
                                                 JobCounters* cnt) {
     const uint32_t n = ctl->rcount;
+    const uint32_t take = retire_takes(ctl);
+    const unsigned long long issued = ctl->issued;
+    const uint32_t q0 = ctl->qcount[parity ^ 1], f0 = ctl->fcount;
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t rounds = (n + stride - 1) / stride;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -612,21 +613,26 @@ __global__ void __launch_bounds__(256) k_retire(PathSoA P, const DevJob* jobs, c
             if (at < J.items) J.samples[at] = make_float4(l.x, l.y, l.z, 0);
         }
         wave_add_keyed(cnt->retired, ji, live ? 1u : 0u);
-        const unsigned long long item = wave_claim(&ctl->issued, live);
-        const bool queued = live && item < ctl->total;
-        if (queued) start_item(P, jobs, path, item, ctl);
-        const uint32_t qpos = wave_append(&ctl->qcount[parity ^ 1], queued);
-        if (queued) next_queue[qpos] = path;
-        const bool idle = live && !queued;
-        const uint32_t fpos = wave_append(&ctl->fcount, idle);
-        if (idle) freeq[fpos] = path;
+        if (live) {
+            if (i < take) {
+                start_item(P, jobs, path, issued + i, ctl);
+                next_queue[q0 + i] = path;
+            } else {
+                freeq[f0 + (i - take)] = path;
+            }
+        }
     }
 }
 
-// End of an iteration: clear the consumed counters and reserve the work items
-// that idle slots start in the following refill.  Claims past `total` made by
-// k_retire were not used (those slots went idle) and are given back here.
+// End of an iteration: apply k_retire's claims, clear the consumed counters
+// and reserve the work items that idle slots start in the following refill.
+// Idempotent at a step boundary (rcount and the refill range are then zero
+// or already accounted), so the host may run it again before a refill.
 __global__ void k_reset(int parity, Control* ctl) {
+    const uint32_t take = retire_takes(ctl);
+    ctl->qcount[parity ^ 1] += take;
+    ctl->fcount += ctl->rcount - take;
+    ctl->issued += take;
     ctl->qcount[parity] = 0;
     ctl->scount = 0;
     ctl->rcount = 0;
@@ -634,16 +640,16 @@ __global__ void k_reset(int parity, Control* ctl) {
     ctl->xscount = 0;
     ctl->ehead = 0;
     ctl->shead = 0;
-    unsigned long long issued = ctl->issued;
-    const unsigned long long total = ctl->total;
-    if (issued > total) issued = total;
-    const unsigned long long avail = total - issued;
-    const uint32_t take = avail < ctl->fcount ? uint32_t(avail) : ctl->fcount;
+    const unsigned long long issued = ctl->issued, total = ctl->total;
+    const unsigned long long avail = issued < total ? total - issued : 0ull;
+    const uint32_t refill = avail < ctl->fcount ? uint32_t(avail) : ctl->fcount;
     ctl->refill_base = issued;
-    ctl->refill_n = take;
-    ctl->fcount -= take;
+    ctl->refill_n = refill;
+    ctl->fcount -= refill;
     ctl->refill_from = ctl->fcount;
-    ctl->issued = issued + take;
+    ctl->refill_q = ctl->qcount[parity ^ 1];
+    ctl->qcount[parity ^ 1] += refill;
+    ctl->issued = issued + refill;
 }
 
 // Idle slots start the items reserved by k_reset; they join the next extend queue.
@@ -660,8 +666,7 @@ __global__ void __launch_bounds__(256) k_refill(PathSoA P, const DevJob* jobs, c
             path = freeq[ctl->refill_from + i];
             start_item(P, jobs, path, ctl->refill_base + i, ctl);
         }
-        const uint32_t qpos = wave_append(&ctl->qcount[parity ^ 1], live);
-        if (live) next_queue[qpos] = path;
+        if (live) next_queue[ctl->refill_q + i] = path;
     }
 }
 
@@ -692,7 +697,7 @@ __global__ void __launch_bounds__(256) k_pool_init(uint32_t* freeq, uint32_t poo
         ctl->qcount[0] = ctl->qcount[1] = 0;
         ctl->scount = ctl->rcount = ctl->xcount = ctl->xscount = 0;
         ctl->fcount = pool;
-        ctl->refill_n = ctl->refill_from = 0;
+        ctl->refill_n = ctl->refill_from = ctl->refill_q = 0;
         ctl->seq_lo = ctl->seq_hi = 0;
         ctl->err_count = 0;
         ctl->ehead = ctl->shead = 0;

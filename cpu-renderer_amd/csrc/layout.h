@@ -284,6 +284,7 @@ struct Control {
     uint32_t fcount;           // idle path slots on the free stack
     uint32_t refill_n;         // free slots the next refill starts
     uint32_t refill_from;      // free-stack index of the first of them
+    uint32_t refill_q;         // next-queue index of the first of them
     uint32_t seq_lo, seq_hi;   // live job sequence range
     uint32_t err_count;        // monotonic; records go to errors[n % kMaxErrors]
     uint32_t ehead, shead;     // next unclaimed entry of the extend / shadow queue
