@@ -349,6 +349,10 @@ CR_HD inline bool leaf_prims(const DevScene& s, const PrimRec* prims, RayQuery& 
 #ifndef CR_LEAF_GROUP
 #define CR_LEAF_GROUP 4
 #endif
+// Child order of a node visit by ranks (1) or by an insertion sort (0).
+#ifndef CR_RANK_PUSH
+#define CR_RANK_PUSH 1
+#endif
 // Keep the scene's node and primitive array pointers of a query in registers
 // (1) or read them from the scene record at each visit (0).
 #ifndef CR_CACHE_PTRS
@@ -501,6 +505,50 @@ struct Traversal {
         ++tc.nodes;
         float nears[4];
         uint32_t mask = node_hit(q, n, best.thi, nears);
+#if CR_RANK_PUSH
+        // Order by rank instead of an insertion sort (no dynamically indexed
+        // arrays): pos[c] = hit children farther than c, ties broken by index.
+        if (!mask) {
+            next_ref();
+            return true;
+        }
+        const int hits = __builtin_popcount(mask);
+        if (sp + nr + hits - 1 > kStackSize) {
+            best.overflow = true;
+            return false;
+        }
+        int pos[4];
+        uint32_t ch[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            ch[c] = n.child[c];
+            int p = 0;
+#pragma unroll
+            for (int o = 0; o < 4; ++o)
+                if (o != c && ((mask >> o) & 1u) && (nears[o] > nears[c] || (nears[o] == nears[c] && o > c))) ++p;
+            pos[c] = p;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (k < hits - 1) {
+                uint32_t r = 0;
+                float nk = 0;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    if (((mask >> c) & 1u) && pos[c] == k) {
+                        r = ch[c];
+                        nk = nears[c];
+                    }
+                push(r, nk);
+            }
+        }
+        uint32_t nearest = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (((mask >> c) & 1u) && pos[c] == hits - 1) nearest = ch[c];
+        ref = nearest;
+        return true;
+#else
         int order[4];
         int cnt = 0;
         while (mask) {
@@ -524,6 +572,7 @@ struct Traversal {
             next_ref();
         }
         return true;
+#endif
     }
 
     // Next stack entry that may still hold a nearer hit (kEmptyChild when none).
