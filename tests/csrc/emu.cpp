@@ -185,6 +185,24 @@ void emu_env_sample(void* e, uint64_t* rng, float* dir, float* pdf) {
     dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
     *rng = r.state;
 }
+// Slab test of one box (child 0 of a node whose other children are empty):
+// 1 when node_hit reports the box as hit within [tmin, tmax].
+int emu_box_hit(const float* o, const float* d, float tmin, float tmax, const float* lo, const float* hi) {
+    RayQuery q;
+    setup_query(q, V3{o[0], o[1], o[2]}, V3{d[0], d[1], d[2]}, tmin, tmax, V3{0, 0, 0}, kNoTriangle);
+    WNode n{};
+    for (int c = 0; c < 4; ++c) {
+        n.lox[c] = n.loy[c] = n.loz[c] = INFINITY;
+        n.hix[c] = n.hiy[c] = n.hiz[c] = -INFINITY;
+        n.child[c] = kEmptyChild;
+    }
+    n.lox[0] = lo[0], n.loy[0] = lo[1], n.loz[0] = lo[2];
+    n.hix[0] = hi[0], n.hiy[0] = hi[1], n.hiz[0] = hi[2];
+    n.child[0] = 1;
+    float nears[4];
+    return int(node_hit(q, n, tmax, nears) & 1u);
+}
+
 float emu_env_pdf(void* e, const float* d) { return env_pdf(static_cast<EmuEnv*>(e)->dev, V3{d[0], d[1], d[2]}); }
 void emu_env_radiance(void* e, const float* d, float* rgb) {
     const V3 r = env_radiance(static_cast<EmuEnv*>(e)->dev, V3{d[0], d[1], d[2]});

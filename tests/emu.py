@@ -44,6 +44,7 @@ class Emu:
             "emu_env_pdf": (C.c_float, [vp, fp]),
             "emu_env_radiance": (None, [vp, fp, fp]),
             "emu_trace": (C.c_int, [vp, vp, u64, vp, C.c_int, C.POINTER(u64), u64]),
+            "emu_box_hit": (C.c_int, [fp, fp, C.c_float, C.c_float, fp, fp]),
             "emu_camera_generate": (C.c_int, [C.POINTER(abi.Camera), C.c_float, C.c_float, C.c_int, C.c_int,
                                               C.POINTER(abi.Ray)]),
             "emu_render": (C.c_int, [vp, vp, C.POINTER(abi.Camera), C.POINTER(abi.Options), fp,

@@ -74,6 +74,71 @@ def test_emu_environment_bit_exact(emu, name):
     np.testing.assert_array_equal(np.array([pe.radiance(v) for v in q]), g[f"{name}/qrad"][:500])
 
 
+def _exact_slab(o, d, tmin, tmax, lo, hi):
+    """Slab test in exact rational arithmetic: True when the ray meets the box
+    for some t in [tmin, tmax]."""
+    from fractions import Fraction as F
+    tn = F(float(tmin))
+    tf = None if np.isinf(tmax) else F(float(tmax))
+    for a in range(3):
+        oa, da = F(float(o[a])), F(float(d[a]))
+        la, ha = F(float(lo[a])), F(float(hi[a]))
+        if da == 0:
+            if oa < la or oa > ha:
+                return False
+            continue
+        t1, t2 = (la - oa) / da, (ha - oa) / da
+        n_, f_ = min(t1, t2), max(t1, t2)
+        tn = max(tn, n_)
+        tf = f_ if tf is None else min(tf, f_)
+    return tf is None or tn <= tf
+
+
+def test_slab_test_is_conservative(emu):
+    """node_hit never culls a box the ray meets in exact arithmetic (the FP32
+    traversal may only over-approximate, reference src/geometry.cpp:565-586):
+    grazing boxes whose faces pass through points of the ray, at origin scales
+    from 1e-3 to 1e5, with tiny and zero direction components."""
+    import ctypes as C
+    rng = np.random.default_rng(11)
+    fp = C.POINTER(C.c_float)
+    f32 = lambda v: np.ascontiguousarray(v, np.float32)
+    missed = tight_misses = 0
+    for case in range(3000):
+        scale = [1e-3, 1.0, 1e3, 1e5][case % 4]
+        o = f32(rng.uniform(-1, 1, 3) * scale)
+        d = rng.normal(size=3)
+        if case % 7 == 0:
+            d[case % 3] *= 1e-6
+        if case % 11 == 0:
+            d[(case // 11) % 3] = 0.0
+        d = f32(d / np.linalg.norm(d))
+        t0 = rng.uniform(0.01, 50.0) * max(scale, 1.0)
+        p = o.astype(np.float64) + t0 * d.astype(np.float64)
+        ext = np.abs(rng.normal(size=3)) * t0 * 10.0 ** rng.uniform(-7, -1)
+        lo, hi = f32(p - ext), f32(p + ext)
+        # Push one face exactly onto the ray point (grazing contact), or just past it.
+        a = rng.integers(3)
+        nudge = rng.integers(-2, 3)
+        if rng.random() < 0.5:
+            lo[a] = np.nextafter(np.float32(p[a]), np.float32(np.inf) if nudge > 0 else np.float32(-np.inf)) \
+                if nudge else np.float32(p[a])
+        else:
+            hi[a] = np.nextafter(np.float32(p[a]), np.float32(-np.inf) if nudge > 0 else np.float32(np.inf)) \
+                if nudge else np.float32(p[a])
+        lo, hi = f32(np.minimum(lo, hi)), f32(np.maximum(lo, hi))
+        tmax = np.float32(np.inf) if case % 3 else np.float32(t0 * (1 + rng.uniform(-1e-6, 1e-6)))
+        got = emu.lib.emu_box_hit(o.ctypes.data_as(fp), d.ctypes.data_as(fp), 0.0, float(tmax),
+                                  lo.ctypes.data_as(fp), hi.ctypes.data_as(fp))
+        exact = _exact_slab(o, d, 0.0, float(tmax), lo, hi)
+        if exact and not got:
+            missed += 1
+        if not exact:
+            tight_misses += 1
+    assert missed == 0
+    assert tight_misses > 100  # the set really probes both sides of contact
+
+
 def test_emu_camera_bit_exact(emu):
     import ctypes as C
     g = load_golden("rng_camera.npz")
