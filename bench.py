@@ -74,9 +74,11 @@ def cpu_baseline(w, budget_s: float, threads: int) -> dict:
     import probe
     kind = "reference" if probe.available("ref") else "port"
     p = probe.Probe("ref" if kind == "reference" else "orc")
-    t0 = time.time()
-    ps, pe = p.scene(w.scene), p.env(w.env)
-    log(f"cpu baseline ({kind}): scene prepared in {time.time() - t0:.1f}s on {threads} threads")
+    t0 = time.perf_counter()
+    ps = p.scene(w.scene)
+    scene_s = time.perf_counter() - t0
+    pe = p.env(w.env)
+    log(f"cpu baseline ({kind}): scene finalize + BVH build in {scene_s:.2f}s")
     spp = 1
     best = None
     while True:
@@ -91,7 +93,8 @@ def cpu_baseline(w, budget_s: float, threads: int) -> dict:
     samples, secs, spp = best
     return {"value": round(samples / secs / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": kind,
             "sample": f"{w.width}x{w.height} full frame at {spp} spp (pixel/sample streams identical to the "
-                      f"GPU run's first {spp} samples), render() time excluding BVH build"}
+                      f"GPU run's first {spp} samples), render() time excluding BVH build",
+            "scene_seconds": round(scene_s, 3)}
 
 
 def _pct(xs, q):
@@ -376,9 +379,15 @@ def main():
             batched = None
 
     cpu = None
+    host_build = {"triangles": int(info.triangles), "seconds": round(info.build_seconds, 3),
+                  "what": "finalize + 4-wide SAH build + device-layout flattening on all host threads"}
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
         cpu = cpu_baseline(w, args.cpu_budget, threads)
+        ref_s = cpu.pop("scene_seconds")
+        host_build.update({"reference_seconds": ref_s,
+                           "reference_what": f"the CPU baseline's Scene::finalize + Bvh::build ({cpu['kind']})",
+                           "speedup": round(ref_s / max(info.build_seconds, 1e-9), 2)})
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
@@ -397,6 +406,7 @@ def main():
         "served": served,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        "host_build": host_build,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -11,6 +11,7 @@
 #include <limits>
 #include <numeric>
 #include <sstream>
+#include <thread>
 
 namespace cr {
 namespace {
@@ -59,6 +60,28 @@ bool uv_directions(const HTriangle& t, int set, DV& tangent, DV& bitangent) {
     tangent = (e1 * dv2 - e2 * dv1) * sign;
     bitangent = (e2 * du1 - e1 * du2) * sign;
     return nonzero3_(unit_of(tangent)) && nonzero3_(unit_of(bitangent));
+}
+
+// Runs f(begin, end, chunk) over [0, n) in up to hardware_concurrency
+// contiguous chunks of at least min_chunk items (one chunk: on this thread).
+template <class F>
+void parallel_chunks(size_t n, size_t min_chunk, F&& f) {
+    const size_t threads = std::max(1u, std::thread::hardware_concurrency());
+    const size_t chunks = std::max<size_t>(1, std::min(threads, n / std::max<size_t>(min_chunk, 1)));
+    if (chunks == 1) {
+        f(size_t(0), n, size_t(0));
+        return;
+    }
+    std::vector<std::thread> pool;
+    pool.reserve(chunks - 1);
+    for (size_t c = 1; c < chunks; ++c) pool.emplace_back([&, c] { f(c * n / chunks, (c + 1) * n / chunks, c); });
+    f(size_t(0), n / chunks, size_t(0));
+    for (auto& t : pool) t.join();
+}
+
+size_t chunk_count(size_t n, size_t min_chunk) {
+    const size_t threads = std::max(1u, std::thread::hardware_concurrency());
+    return std::max<size_t>(1, std::min(threads, n / std::max<size_t>(min_chunk, 1)));
 }
 
 bool fin(float v) { return std::isfinite(v); }
@@ -125,83 +148,126 @@ HostScene finalize_scene(const crg_scene_desc& d) {
             s.warnings.push_back(label + " has a singular normal-map UV transform; using the untransformed tangent frame");
     }
     const uint64_t n = d.triangle_count;
-    for (uint64_t i = 0; i < n; ++i) {
-        if (d.material_ids[i] >= d.material_count)
-            throw std::runtime_error("Triangle " + std::to_string(i) + " has an invalid material index");
-        bool ok = true;
-        for (int k = 0; k < 9; ++k) ok = ok && fin(d.positions[9 * i + k]);
-        if (d.normals)
-            for (int k = 0; k < 9; ++k) ok = ok && fin(d.normals[9 * i + k]);
-        if (d.tangents)
-            for (int k = 0; k < 12; ++k) ok = ok && fin(d.tangents[12 * i + k]);
-        if (d.uvs)
-            for (int k = 0; k < 12; ++k) ok = ok && fin(d.uvs[12 * i + k]);
-        if (!ok) throw std::runtime_error("Triangle " + std::to_string(i) + " contains non-finite vertex data");
-    }
-    s.tris.reserve(n);
-    size_t degenerate = 0, first_degenerate = 0, generated = 0, fallback_tangents = 0;
-    for (uint64_t i = 0; i < n; ++i) {
-        HTriangle t;
-        for (int k = 0; k < 3; ++k) {
-            t.p[k] = {d.positions[9 * i + 3 * k], d.positions[9 * i + 3 * k + 1], d.positions[9 * i + 3 * k + 2]};
-            t.n[k] = d.normals ? HVec3{d.normals[9 * i + 3 * k], d.normals[9 * i + 3 * k + 1], d.normals[9 * i + 3 * k + 2]}
-                               : HVec3{0, 0, 0};
-            for (int q = 0; q < 4; ++q) {
-                t.tan[k][q] = d.tangents ? d.tangents[12 * i + 4 * k + q] : (q == 3 ? 1.0f : 0.0f);
-                t.uv[k][q] = d.uvs ? d.uvs[12 * i + 4 * k + q] : 0.0f;
+    // Validation and per-triangle preparation run in contiguous chunks on all
+    // host threads; the first failing triangle by index decides the error, as
+    // in a sequential pass.
+    constexpr size_t kMinChunk = 16384;
+    const size_t nchunks = chunk_count(n, kMinChunk);
+    std::vector<uint64_t> bad(nchunks, UINT64_MAX);
+    std::vector<int> bad_kind(nchunks, 0);
+    parallel_chunks(n, kMinChunk, [&](size_t b, size_t e, size_t c) {
+        for (uint64_t i = b; i < e; ++i) {
+            if (d.material_ids[i] >= d.material_count) {
+                bad[c] = i, bad_kind[c] = 1;
+                return;
+            }
+            bool ok = true;
+            for (int k = 0; k < 9; ++k) ok = ok && fin(d.positions[9 * i + k]);
+            if (d.normals)
+                for (int k = 0; k < 9; ++k) ok = ok && fin(d.normals[9 * i + k]);
+            if (d.tangents)
+                for (int k = 0; k < 12; ++k) ok = ok && fin(d.tangents[12 * i + k]);
+            if (d.uvs)
+                for (int k = 0; k < 12; ++k) ok = ok && fin(d.uvs[12 * i + k]);
+            if (!ok) {
+                bad[c] = i, bad_kind[c] = 2;
+                return;
             }
         }
-        t.material = d.material_ids[i];
-        const DV normal = dcross(dv(t.p[1]) - dv(t.p[0]), dv(t.p[2]) - dv(t.p[0]));
-        const double twice = std::sqrt(ddot_(normal, normal));
-        if (twice * 0.5 > std::numeric_limits<float>::max())
-            throw std::runtime_error("Triangle " + std::to_string(i) + " area exceeds the supported finite range");
-        t.area = float(twice * 0.5);
-        if (!(t.area > 0)) {
-            if (degenerate == 0) first_degenerate = i;
-            ++degenerate;
-            continue;
-        }
-        t.ng = unit_of(normal);
-        // Tangent frames (reference src/geometry.cpp:365-388): vertex tangents
-        // orthogonalized against the repaired normal, else UV-derived in the
-        // normal map's UV set, else an orthogonal basis.
-        const crg_material& mat = s.materials[t.material];
-        const int set = mat.normal_tex.texture >= 0 ? mat.normal_tex.texcoord : 0;
-        DV ut{0, 0, 0}, ub{0, 0, 0};
-        const bool usable = uv_directions(t, set, ut, ub);
-        for (int k = 0; k < 3; ++k) {
-            HVec3 nn = unit_of(dv(t.n[k]));
-            float hand = t.tan[k][3] < 0 ? -1.0f : 1.0f;
-            if (!nonzero3(nn) || fdot(nn, t.ng) == 0) {
-                nn = t.ng;
-                ++generated;
-            } else if (fdot(nn, t.ng) < 0) {
-                nn = {-nn.x, -nn.y, -nn.z};
-                hand = -hand;
-            }
-            HVec3 tg = orthogonal_of(DV{t.tan[k][0], t.tan[k][1], t.tan[k][2]}, nn);
-            if (!nonzero3(tg)) {
-                if (usable) tg = orthogonal_of(ut, nn);
-                if (nonzero3(tg)) {
-                    const HVec3 c = hcross(nn, tg);
-                    hand = ddot_(dv(c), ub) < 0 ? -1.0f : 1.0f;
-                } else {
-                    const V3 fx = make_frame(V3{nn.x, nn.y, nn.z}).x;
-                    tg = {fx.x, fx.y, fx.z};
-                    hand = 1;
-                    if (mat.normal_tex.texture >= 0) ++fallback_tangents;
+    });
+    for (size_t c = 0; c < nchunks; ++c)
+        if (bad[c] != UINT64_MAX)
+            throw std::runtime_error("Triangle " + std::to_string(bad[c]) +
+                                     (bad_kind[c] == 1 ? " has an invalid material index"
+                                                       : " contains non-finite vertex data"));
+    struct Chunk {
+        std::vector<HTriangle> tris;
+        size_t degenerate = 0, first_degenerate = 0, generated = 0, fallback_tangents = 0;
+        uint64_t too_large = UINT64_MAX;
+    };
+    std::vector<Chunk> parts(nchunks);
+    parallel_chunks(n, kMinChunk, [&](size_t b, size_t e, size_t c) {
+        Chunk& out = parts[c];
+        out.tris.reserve(e - b);
+        for (uint64_t i = b; i < e; ++i) {
+            HTriangle t;
+            for (int k = 0; k < 3; ++k) {
+                t.p[k] = {d.positions[9 * i + 3 * k], d.positions[9 * i + 3 * k + 1], d.positions[9 * i + 3 * k + 2]};
+                t.n[k] = d.normals ? HVec3{d.normals[9 * i + 3 * k], d.normals[9 * i + 3 * k + 1],
+                                           d.normals[9 * i + 3 * k + 2]}
+                                   : HVec3{0, 0, 0};
+                for (int q = 0; q < 4; ++q) {
+                    t.tan[k][q] = d.tangents ? d.tangents[12 * i + 4 * k + q] : (q == 3 ? 1.0f : 0.0f);
+                    t.uv[k][q] = d.uvs ? d.uvs[12 * i + 4 * k + q] : 0.0f;
                 }
             }
-            t.n[k] = nn;
-            t.tan[k][0] = tg.x;
-            t.tan[k][1] = tg.y;
-            t.tan[k][2] = tg.z;
-            t.tan[k][3] = hand;
+            t.material = d.material_ids[i];
+            const DV normal = dcross(dv(t.p[1]) - dv(t.p[0]), dv(t.p[2]) - dv(t.p[0]));
+            const double twice = std::sqrt(ddot_(normal, normal));
+            if (twice * 0.5 > std::numeric_limits<float>::max()) {
+                out.too_large = i;
+                return;
+            }
+            t.area = float(twice * 0.5);
+            if (!(t.area > 0)) {
+                if (out.degenerate == 0) out.first_degenerate = i;
+                ++out.degenerate;
+                continue;
+            }
+            t.ng = unit_of(normal);
+            // Tangent frames (reference src/geometry.cpp:365-388): vertex tangents
+            // orthogonalized against the repaired normal, else UV-derived in the
+            // normal map's UV set, else an orthogonal basis.
+            const crg_material& mat = s.materials[t.material];
+            const int set = mat.normal_tex.texture >= 0 ? mat.normal_tex.texcoord : 0;
+            DV ut{0, 0, 0}, ub{0, 0, 0};
+            const bool usable = uv_directions(t, set, ut, ub);
+            for (int k = 0; k < 3; ++k) {
+                HVec3 nn = unit_of(dv(t.n[k]));
+                float hand = t.tan[k][3] < 0 ? -1.0f : 1.0f;
+                if (!nonzero3(nn) || fdot(nn, t.ng) == 0) {
+                    nn = t.ng;
+                    ++out.generated;
+                } else if (fdot(nn, t.ng) < 0) {
+                    nn = {-nn.x, -nn.y, -nn.z};
+                    hand = -hand;
+                }
+                HVec3 tg = orthogonal_of(DV{t.tan[k][0], t.tan[k][1], t.tan[k][2]}, nn);
+                if (!nonzero3(tg)) {
+                    if (usable) tg = orthogonal_of(ut, nn);
+                    if (nonzero3(tg)) {
+                        const HVec3 bt = hcross(nn, tg);
+                        hand = ddot_(dv(bt), ub) < 0 ? -1.0f : 1.0f;
+                    } else {
+                        const V3 fx = make_frame(V3{nn.x, nn.y, nn.z}).x;
+                        tg = {fx.x, fx.y, fx.z};
+                        hand = 1;
+                        if (mat.normal_tex.texture >= 0) ++out.fallback_tangents;
+                    }
+                }
+                t.n[k] = nn;
+                t.tan[k][0] = tg.x;
+                t.tan[k][1] = tg.y;
+                t.tan[k][2] = tg.z;
+                t.tan[k][3] = hand;
+            }
+            t.ung = unit_of(dv(t.ng));
+            out.tris.push_back(t);
         }
-        t.ung = unit_of(dv(t.ng));
-        s.tris.push_back(t);
+    });
+    size_t degenerate = 0, first_degenerate = 0, generated = 0, fallback_tangents = 0, total = 0;
+    for (const Chunk& c : parts) {
+        if (c.too_large != UINT64_MAX)
+            throw std::runtime_error("Triangle " + std::to_string(c.too_large) +
+                                     " area exceeds the supported finite range");
+        if (c.degenerate && degenerate == 0) first_degenerate = c.first_degenerate;
+        degenerate += c.degenerate;
+        generated += c.generated;
+        fallback_tangents += c.fallback_tangents;
+        total += c.tris.size();
     }
+    s.tris.reserve(total);
+    for (Chunk& c : parts) s.tris.insert(s.tris.end(), c.tris.begin(), c.tris.end());
     if (degenerate)
         s.warnings.push_back("Skipped " + std::to_string(degenerate) + " degenerate triangles (first input triangle " +
                              std::to_string(first_degenerate) + ")");
@@ -217,8 +283,12 @@ HostScene finalize_scene(const crg_scene_desc& d) {
 // ------------------------------------------------------------------ BVH
 namespace {
 
+// Build record: bounds, centroid and the finalized triangle index.  The
+// builder partitions these records themselves (not an index array), so every
+// scan over a range reads memory sequentially.
 struct BPrim {
     float lo[3], hi[3], c[3];
+    uint32_t id;
 };
 
 struct Box {
@@ -242,18 +312,35 @@ constexpr uint32_t kLeafSize = 4;
 constexpr int kMaxWideDepth = 40;
 
 struct Builder {
-    const std::vector<BPrim>& prims;
-    std::vector<uint32_t>& idx;
+    std::vector<BPrim>& prims;
     std::vector<WNode>& nodes;
     std::atomic<uint32_t> next{0};
     std::atomic<uint64_t> leaves{0};
 
+    // Ranges above this size are scanned by all host threads (the top levels
+    // of the tree, where the subtree tasks have not fanned out yet).
+    static constexpr uint32_t kParallelScan = 1u << 17;
+
     Box bounds(uint32_t b, uint32_t e, Box* centers) const {
+        auto scan = [&](uint32_t lo, uint32_t hi, Box& box, Box& cen) {
+            for (uint32_t i = lo; i < hi; ++i) {
+                const BPrim& p = prims[i];
+                box.grow(p.lo, p.hi);
+                cen.grow(p.c, p.c);
+            }
+        };
         Box box, cen;
-        for (uint32_t i = b; i < e; ++i) {
-            const BPrim& p = prims[idx[i]];
-            box.grow(p.lo, p.hi);
-            cen.grow(p.c, p.c);
+        if (e - b < kParallelScan) {
+            scan(b, e, box, cen);
+        } else {
+            const size_t nc = chunk_count(e - b, kParallelScan / 4);
+            std::vector<Box> bx(nc), cx(nc);
+            parallel_chunks(e - b, kParallelScan / 4,
+                            [&](size_t lo, size_t hi, size_t c) { scan(b + uint32_t(lo), b + uint32_t(hi), bx[c], cx[c]); });
+            for (size_t c = 0; c < nc; ++c) {
+                box.grow(bx[c]);
+                cen.grow(cx[c]);
+            }
         }
         if (centers) *centers = cen;
         return box;
@@ -269,19 +356,39 @@ struct Builder {
             Box box;
             uint32_t count = 0;
         };
-        std::array<std::array<Bin, kBins>, 3> bins{};
-        auto bin_at = [&](float c, int a) {
+        using Bins = std::array<std::array<Bin, kBins>, 3>;
+        Bins bins{};
+        double scale[3];
+        for (int a = 0; a < 3; ++a) {
             const double ext = double(cen.hi[a]) - cen.lo[a];
-            if (!(ext > 0)) return 0;
-            return std::clamp(int((double(c) - cen.lo[a]) / ext * kBins), 0, kBins - 1);
+            scale[a] = ext > 0 ? kBins / ext : 0.0;
+        }
+        auto bin_at = [&](float c, int a) {
+            return std::clamp(int((double(c) - cen.lo[a]) * scale[a]), 0, kBins - 1);
         };
-        for (uint32_t i = b; i < e; ++i) {
-            const BPrim& p = prims[idx[i]];
-            for (int a = 0; a < 3; ++a) {
-                Bin& bn = bins[a][bin_at(p.c[a], a)];
-                bn.box.grow(p.lo, p.hi);
-                ++bn.count;
+        auto fill = [&](uint32_t lo, uint32_t hi, Bins& out) {
+            for (uint32_t i = lo; i < hi; ++i) {
+                const BPrim& p = prims[i];
+                for (int a = 0; a < 3; ++a) {
+                    Bin& bn = out[a][bin_at(p.c[a], a)];
+                    bn.box.grow(p.lo, p.hi);
+                    ++bn.count;
+                }
             }
+        };
+        if (e - b < kParallelScan) {
+            fill(b, e, bins);
+        } else {  // per-thread bins, merged (min/max and counts: order-independent)
+            const size_t nc = chunk_count(e - b, kParallelScan / 4);
+            std::vector<Bins> part(nc);
+            parallel_chunks(e - b, kParallelScan / 4,
+                            [&](size_t lo, size_t hi, size_t c) { fill(b + uint32_t(lo), b + uint32_t(hi), part[c]); });
+            for (const Bins& pb : part)
+                for (int a = 0; a < 3; ++a)
+                    for (int k = 0; k < kBins; ++k) {
+                        if (pb[a][k].count) bins[a][k].box.grow(pb[a][k].box);
+                        bins[a][k].count += pb[a][k].count;
+                    }
         }
         int best_axis = -1, best_split = -1;
         double best = double(e - b) * parent;
@@ -313,9 +420,40 @@ struct Builder {
         }
         if (best_axis < 0) return 0;
         const int a = best_axis;
-        uint32_t* mid = std::partition(idx.data() + b, idx.data() + e,
-                                       [&](uint32_t i) { return bin_at(prims[i].c[a], a) <= best_split; });
-        const uint32_t m = uint32_t(mid - idx.data());
+        auto left = [&](const BPrim& p) { return bin_at(p.c[a], a) <= best_split; };
+        uint32_t m;
+        if (e - b < kParallelScan) {
+            m = uint32_t(std::partition(prims.data() + b, prims.data() + e, left) - prims.data());
+        } else {  // stable partition by chunks: counts, offsets, scatter
+            const size_t nc = chunk_count(e - b, kParallelScan / 4);
+            std::vector<uint32_t> nleft(nc, 0);
+            std::vector<BPrim> tmp(e - b);
+            parallel_chunks(e - b, kParallelScan / 4, [&](size_t lo, size_t hi, size_t c) {
+                uint32_t k = 0;
+                for (size_t i = lo; i < hi; ++i) k += left(prims[b + i]) ? 1u : 0u;
+                nleft[c] = k;
+            });
+            std::vector<uint32_t> loff(nc), roff(nc);
+            uint32_t total_left = 0;
+            for (size_t c = 0; c < nc; ++c) total_left += nleft[c];
+            uint32_t l = 0, r = total_left;
+            for (size_t c = 0; c < nc; ++c) {
+                loff[c] = l;
+                roff[c] = r;
+                l += nleft[c];
+                r += uint32_t((c + 1) * (e - b) / nc - c * (e - b) / nc) - nleft[c];
+            }
+            parallel_chunks(e - b, kParallelScan / 4, [&](size_t lo, size_t hi, size_t c) {
+                uint32_t li = loff[c], ri = roff[c];
+                for (size_t i = lo; i < hi; ++i) {
+                    const BPrim& v = prims[b + i];
+                    if (left(v)) tmp[li++] = v;
+                    else tmp[ri++] = v;
+                }
+            });
+            std::copy(tmp.begin(), tmp.end(), prims.begin() + b);
+            m = b + total_left;
+        }
         if (m == b || m == e) return 0;
         return m;
     }
@@ -333,8 +471,8 @@ struct Builder {
             }
         const uint32_t m = b + (e - b) / 2;
         if (ext > 0)
-            std::nth_element(idx.data() + b, idx.data() + m, idx.data() + e,
-                             [&](uint32_t x, uint32_t y) { return prims[x].c[a] < prims[y].c[a]; });
+            std::nth_element(prims.data() + b, prims.data() + m, prims.data() + e,
+                             [&](const BPrim& x, const BPrim& y) { return x.c[a] < y.c[a]; });
         return m;
     }
 
@@ -416,9 +554,11 @@ WideBvh build_wide_bvh(const HostScene& s) {
     if (n > size_t(std::numeric_limits<uint32_t>::max() / 2))
         throw std::length_error("Too many triangles for 32-bit BVH node indices");
     std::vector<BPrim> prims(n);
-    for (size_t i = 0; i < n; ++i) {
+    parallel_chunks(n, 65536, [&](size_t first, size_t last, size_t) {
+    for (size_t i = first; i < last; ++i) {
         const HTriangle& t = s.tris[i];
         BPrim& p = prims[i];
+        p.id = uint32_t(i);
         for (int a = 0; a < 3; ++a) {
             const float v0 = a == 0 ? t.p[0].x : (a == 1 ? t.p[0].y : t.p[0].z);
             const float v1 = a == 0 ? t.p[1].x : (a == 1 ? t.p[1].y : t.p[1].z);
@@ -428,11 +568,12 @@ WideBvh build_wide_bvh(const HostScene& s) {
             p.c[a] = float((double(v0) + double(v1) + double(v2)) * (1.0 / 3.0));
         }
     }
-    out.order.resize(n);
-    std::iota(out.order.begin(), out.order.end(), 0u);
+    });
     out.nodes.resize(2 * n + 2);  // every wide node owns >= 1 leaf or >= 2 children
-    Builder b{prims, out.order, out.nodes};
+    Builder b{prims, out.nodes};
     b.build(0, uint32_t(n), 0);
+    out.order.resize(n);
+    for (size_t i = 0; i < n; ++i) out.order[i] = prims[i].id;
     out.nodes.resize(b.next.load());
     out.binary_nodes = 2 * b.leaves.load() - 1;
     return out;
@@ -715,7 +856,9 @@ SceneArrays scene_arrays(const HostScene& hs) {
     a.bvh = build_wide_bvh(hs);
     const size_t n = hs.tris.size();
     a.tri_to_prim.assign(n, 0);
-    for (size_t i = 0; i < n; ++i) a.tri_to_prim[a.bvh.order[i]] = uint32_t(i);
+    parallel_chunks(n, 65536, [&](size_t b, size_t e, size_t) {
+        for (size_t i = b; i < e; ++i) a.tri_to_prim[a.bvh.order[i]] = uint32_t(i);
+    });
     a.materials = device_materials(hs);
     bool any_map = false, any_normal_map = false;
     for (const crg_material& m : hs.materials) {
@@ -729,7 +872,8 @@ SceneArrays scene_arrays(const HostScene& hs) {
     a.aux.resize(n);
     if (any_map) a.uvs.resize(3 * n);
     if (any_normal_map) a.tangents.resize(3 * n);
-    for (size_t i = 0; i < n; ++i) {
+    parallel_chunks(n, 65536, [&](size_t first, size_t last, size_t) {
+    for (size_t i = first; i < last; ++i) {
         const HTriangle& t = hs.tris[a.bvh.order[i]];
         const crg_material& m = hs.materials[t.material];
         uint32_t flags = 0;
@@ -757,6 +901,7 @@ SceneArrays scene_arrays(const HostScene& hs) {
         x.ung[0] = t.ung.x; x.ung[1] = t.ung.y; x.ung[2] = t.ung.z;
         x.area = t.area;
     }
+    });
     for (const HImage& im : hs.images) {
         a.images.push_back(DevImage{uint64_t(a.texels.size()), im.width, im.height});
         a.texels.insert(a.texels.end(), im.texels.begin(), im.texels.end());
