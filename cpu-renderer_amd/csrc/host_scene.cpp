@@ -13,6 +13,8 @@
 #include <sstream>
 #include <thread>
 
+#include <sched.h>
+
 namespace cr {
 namespace {
 
@@ -62,11 +64,22 @@ bool uv_directions(const HTriangle& t, int set, DV& tangent, DV& bitangent) {
     return nonzero3_(unit_of(tangent)) && nonzero3_(unit_of(bitangent));
 }
 
-// Runs f(begin, end, chunk) over [0, n) in up to hardware_concurrency
-// contiguous chunks of at least min_chunk items (one chunk: on this thread).
+// Host threads this process may run on (its CPU affinity, which can be far
+// below the machine's CPU count in a container).
+size_t host_threads() {
+    static const size_t n = [] {
+        cpu_set_t set;
+        if (sched_getaffinity(0, sizeof(set), &set) == 0) return size_t(std::max(1, CPU_COUNT(&set)));
+        return size_t(std::max(1u, std::thread::hardware_concurrency()));
+    }();
+    return n;
+}
+
+// Runs f(begin, end, chunk) over [0, n) in up to host_threads() contiguous
+// chunks of at least min_chunk items (one chunk: on this thread).
 template <class F>
 void parallel_chunks(size_t n, size_t min_chunk, F&& f) {
-    const size_t threads = std::max(1u, std::thread::hardware_concurrency());
+    const size_t threads = host_threads();
     const size_t chunks = std::max<size_t>(1, std::min(threads, n / std::max<size_t>(min_chunk, 1)));
     if (chunks == 1) {
         f(size_t(0), n, size_t(0));
@@ -80,7 +93,7 @@ void parallel_chunks(size_t n, size_t min_chunk, F&& f) {
 }
 
 size_t chunk_count(size_t n, size_t min_chunk) {
-    const size_t threads = std::max(1u, std::thread::hardware_concurrency());
+    const size_t threads = host_threads();
     return std::max<size_t>(1, std::min(threads, n / std::max<size_t>(min_chunk, 1)));
 }
 
