@@ -135,7 +135,8 @@ CR_HDI f64 sub_mags(f64 a, f64 b, uint32_t s) {
     return round_pack(s, ea - sh, d);
 }
 
-CR_HD CR_NOINLINE inline f64 add(f64 a, f64 b) {
+// General case of add (special values, subnormal operands or result).
+CR_HDI f64 add_general(f64 a, f64 b) {
     if (is_nan(a) || is_nan(b)) return qnan();
     const uint32_t sa = sign_of(a), sb = sign_of(b);
     if (is_inf(a)) {
@@ -151,9 +152,50 @@ CR_HD CR_NOINLINE inline f64 add(f64 a, f64 b) {
     return sa == sb ? add_mags(a, b, sa) : sub_mags(a, b, sa);
 }
 
+// Round-to-nearest-even of a significand with its leading one at bit 62 and
+// 10 guard bits (bit 0 sticky), packed with biased exponent e >= 1; a carry out
+// of the significand moves into the exponent (0x7ff: infinity).
+CR_HDI f64 pack_normal(uint64_t sign, int32_t e, uint64_t sig) {
+    const uint64_t rb = sig & 0x3ff;
+    uint64_t m = sig >> 10;
+    m += (rb > 0x200 || (rb == 0x200 && (m & 1))) ? 1 : 0;
+    return f64{sign | ((uint64_t(e - 1) << 52) + m)};
+}
+
+// Common case of both additions and subtractions: normal operands whose sum
+// is normal (no special value, no subnormal operand or result); everything else
+// goes to add_general.  Same rounding as the general path.
+CR_HD CR_NOINLINE inline f64 add(f64 a, f64 b) {
+    const uint32_t ea = uint32_t(a.b >> 52) & 0x7ffu, eb = uint32_t(b.b >> 52) & 0x7ffu;
+    if (ea - 1u < 0x7fdu && eb - 1u < 0x7fdu) {  // exponents in [1, 0x7fd]
+        const uint64_t kMag = 0x7fffffffffffffffull, kFrac = 0x000fffffffffffffull, kHid = 0x0010000000000000ull;
+        const bool swap = (b.b & kMag) > (a.b & kMag);
+        const uint64_t x = swap ? b.b : a.b, y = swap ? a.b : b.b;
+        const int32_t ex = int32_t((x >> 52) & 0x7ff), ey = int32_t((y >> 52) & 0x7ff);
+        const uint64_t sx = ((x & kFrac) | kHid) << 10;
+        const uint64_t sy = shift_right_jam(((y & kFrac) | kHid) << 10, ex - ey);
+        const uint64_t sign = x & ~kMag;
+        if (((a.b ^ b.b) >> 63) == 0) {
+            uint64_t s_ = sx + sy;
+            int32_t e = ex;
+            if (s_ >> 63) {
+                s_ = (s_ >> 1) | (s_ & 1);
+                ++e;
+            }
+            return pack_normal(sign, e, s_);
+        }
+        uint64_t d = sx - sy;
+        if (d == 0) return zero(0);
+        const int sh = clz64(d) - 1;
+        if (ex - sh >= 1) return pack_normal(sign, ex - sh, d << sh);
+    }
+    return add_general(a, b);
+}
+
 CR_HDI f64 sub(f64 a, f64 b) { return add(a, is_nan(b) ? b : neg(b)); }
 
-CR_HD CR_NOINLINE inline f64 mul(f64 a, f64 b) {
+// General case of mul (special values, subnormal operands or result).
+CR_HDI f64 mul_general(f64 a, f64 b) {
     if (is_nan(a) || is_nan(b)) return qnan();
     const uint32_t s = sign_of(a) ^ sign_of(b);
     if (is_inf(a) || is_inf(b)) {
@@ -175,6 +217,25 @@ CR_HD CR_NOINLINE inline f64 mul(f64 a, f64 b) {
         --e;
     }
     return round_pack(s, e, sig);
+}
+
+// Common case: normal operands with a normal product (exponent range checked
+// before the multiply); everything else goes to mul_general.
+CR_HD CR_NOINLINE inline f64 mul(f64 a, f64 b) {
+    const uint32_t ea = uint32_t(a.b >> 52) & 0x7ffu, eb = uint32_t(b.b >> 52) & 0x7ffu;
+    const int32_t e0 = int32_t(ea + eb) - 1023;  // biased exponent when the significand product is < 2
+    if (ea - 1u < 0x7feu && eb - 1u < 0x7feu && e0 >= 1 && e0 <= 0x7fd) {
+        const uint64_t kFrac = 0x000fffffffffffffull, kHid = 0x0010000000000000ull;
+        const uint64_t ma = ((a.b & kFrac) | kHid) << 10, mb = ((b.b & kFrac) | kHid) << 11;
+        uint64_t sig = mul_hi(ma, mb) | uint64_t(ma * mb != 0);  // leading one at bit 62 or 61
+        int32_t e = e0 + 1;
+        if (sig < 0x4000000000000000ull) {
+            sig <<= 1;
+            --e;
+        }
+        return pack_normal((a.b ^ b.b) & 0x8000000000000000ull, e, sig);
+    }
+    return mul_general(a, b);
 }
 
 // Integer -> float conversions used for quotient-digit estimates.
