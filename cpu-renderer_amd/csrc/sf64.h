@@ -249,22 +249,9 @@ CR_HDI float i64_to_f(int64_t x) { return float(x); }
 // true value is below 2^58 in magnitude, so wrapping arithmetic is exact),
 // and a final loop brings the remainder into [0, B).  Digit errors are <= 4,
 // <= 17 and <= 2 units, so the loop runs at most a few times.
-CR_HD inline f64 div(f64 a, f64 b) {
-    if (is_nan(a) || is_nan(b)) return qnan();
-    const uint32_t s = sign_of(a) ^ sign_of(b);
-    if (is_inf(a)) return is_inf(b) ? qnan() : inf(s);
-    if (is_inf(b)) return zero(s);
-    if (is_zero(b)) return is_zero(a) ? qnan() : inf(s);
-    if (is_zero(a)) return zero(s);
-    int32_t ea, eb;
-    uint64_t ma, mb;
-    unpack(a, ea, ma);
-    unpack(b, eb, mb);
-    int32_t e = ea - eb;
-    if (ma < mb) {
-        ma <<= 1;
-        --e;
-    }
+// Quotient significand for ma in [mb, 2*mb), mb in [2^52, 2^53): Q with its
+// leading one at bit 62 and the remainder's sticky bit in bit 0.
+CR_HDI uint64_t quotient_sig(uint64_t ma, uint64_t mb) {
     const float rb = 1.0f / u64_to_f(mb);  // ~2^-53, relative error <= 2^-23
     // digit 1 (weight 2^39): ~A*2^23/B
     const int64_t q1 = int64_t(u64_to_f(ma) * (rb * 8388608.0f));
@@ -284,7 +271,37 @@ CR_HD inline f64 div(f64 a, f64 b) {
         r -= int64_t(mb);
     }
     const uint64_t q = (uint64_t(q1) << 39) + (uint64_t(q2) << 16) + uint64_t(q3);
-    return round_pack(s, e, q | uint64_t(r != 0));
+    return q | uint64_t(r != 0);
+}
+
+CR_HD inline f64 div(f64 a, f64 b) {
+    const uint32_t xa = uint32_t(a.b >> 52) & 0x7ffu, xb = uint32_t(b.b >> 52) & 0x7ffu;
+    const uint32_t s = sign_of(a) ^ sign_of(b);
+    int32_t e;
+    uint64_t ma, mb;
+    if (xa - 1u < 0x7feu && xb - 1u < 0x7feu) {  // normal operands: no special case, no unpack shift
+        ma = frac_of(a) | 0x0010000000000000ull;
+        mb = frac_of(b) | 0x0010000000000000ull;
+        e = int32_t(xa) - int32_t(xb);
+    } else {
+        if (is_nan(a) || is_nan(b)) return qnan();
+        if (is_inf(a)) return is_inf(b) ? qnan() : inf(s);
+        if (is_inf(b)) return zero(s);
+        if (is_zero(b)) return is_zero(a) ? qnan() : inf(s);
+        if (is_zero(a)) return zero(s);
+        int32_t ea, eb;
+        unpack(a, ea, ma);
+        unpack(b, eb, mb);
+        e = ea - eb;
+    }
+    if (ma < mb) {
+        ma <<= 1;
+        --e;
+    }
+    const uint64_t q = quotient_sig(ma, mb);
+    const int32_t be = e + 1023;
+    if (be >= 1 && be <= 0x7fe) return pack_normal(uint64_t(s) << 63, be, q);
+    return round_pack(s, e, q);
 }
 
 // 128-bit helpers for the square root.
@@ -411,6 +428,14 @@ CR_HDI f64 from_float(float x) {
 
 // Correctly rounded narrowing double -> float (round-to-nearest-even).
 CR_HD CR_NOINLINE inline float to_float(f64 a) {
+    const uint32_t xa = uint32_t(a.b >> 52) & 0x7ffu;
+    if (xa - 897u < 254u) {  // binary32 exponent xa - 896 in [1, 254]: normal result
+        const uint64_t m = frac_of(a) | 0x0010000000000000ull;
+        const uint64_t rb = m & ((uint64_t(1) << 29) - 1), half = uint64_t(1) << 28;
+        uint32_t r = uint32_t(m >> 29);
+        r += (rb > half || (rb == half && (r & 1u))) ? 1u : 0u;
+        return bitsf((uint32_t(a.b >> 63) << 31) | ((uint32_t(xa - 897u) << 23) + r));
+    }
     const uint32_t s = sign_of(a);
     if (is_nan(a)) return bitsf(0x7fc00000u);
     if (is_inf(a)) return bitsf((s << 31) | 0x7f800000u);
