@@ -97,6 +97,37 @@ def cpu_baseline(w, budget_s: float, threads: int) -> dict:
             "scene_seconds": round(scene_s, 3)}
 
 
+def film_encode_form(ctx, film, with_reference: bool) -> dict:
+    """io::writePng + io::writeHdr of the rendered film (reference src/io/image.cpp:201-236):
+    the device encoders (PCIe transfers included), the same with the files written
+    (zlib deflate for the PNG on the host), and the reference's own writers on the
+    host (stb, single thread) with the outputs compared."""
+    import tempfile
+    t = time.perf_counter()
+    png8 = ctx.film_png8(film, 0.0)
+    hdr = ctx.film_hdr(film)
+    enc_s = time.perf_counter() - t
+    out = {"pixels": int(film.shape[0] * film.shape[1]), "encode_seconds": round(enc_s, 4)}
+    with tempfile.TemporaryDirectory() as d:
+        d = pathlib.Path(d)
+        t = time.perf_counter()
+        ctx.write_png(d / "a.png", film, 0.0)
+        ctx.write_hdr(d / "a.hdr", film)
+        out["files_seconds"] = round(time.perf_counter() - t, 4)
+        if with_reference:
+            sys.path.insert(0, str(ROOT / "oracle"))
+            import probe
+            if probe.available("ref"):
+                p = probe.Probe("ref")
+                t = time.perf_counter()
+                p.film_write(film, 0.0, d / "r.png", d / "r.hdr")
+                out["reference_seconds"] = round(time.perf_counter() - t, 4)
+                out["hdr_bytes_identical"] = (d / "r.hdr").read_bytes() == hdr
+                import numpy as np
+                out["png_pixels_identical"] = bool(np.array_equal(p.film_png8(film, 0.0), png8))
+    return out
+
+
 def _pct(xs, q):
     import numpy as np
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q))
@@ -378,6 +409,9 @@ def main():
         if "batched" not in forms:
             batched = None
 
+    film_enc = film_encode_form(ctx, res.pixels, not args.no_cpu)
+    log(f"film encode: {film_enc}")
+
     cpu = None
     host_build = {"triangles": int(info.triangles), "seconds": round(info.build_seconds, 3),
                   "what": "finalize + 4-wide SAH build + device-layout flattening on all host threads"}
@@ -407,6 +441,7 @@ def main():
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
         "host_build": host_build,
+        "film_encode": film_enc,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -24,7 +24,7 @@ COMMON = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-ffp-contract=
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
 
 UNITS = [("kernels.hip", []), ("capi.cpp", ["--offload-host-only"]), ("engine.cpp", ["--offload-host-only"]),
-         ("host_scene.cpp", ["--offload-host-only"])]
+         ("host_scene.cpp", ["--offload-host-only"]), ("film_host.cpp", ["--offload-host-only"])]
 
 
 def _run(cmd):
@@ -47,7 +47,7 @@ def build(verbose: bool = False) -> pathlib.Path:
             if verbose and out:
                 print(out)
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", str(tmp), *[str(j[0]) for j in jobs], "-lpthread"])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", str(tmp), *[str(j[0]) for j in jobs], "-lpthread", "-lz"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -63,6 +63,11 @@ def load_library(path: Optional[pathlib.Path] = None) -> C.CDLL:
         "crg_autotune": (i32, [vp, C.POINTER(abi.Request), C.POINTER(abi.Tuning), C.POINTER(C.c_double)]),
         "crg_set_profiling": (i32, [vp, i32]),
         "crg_kernel_times": (i32, [vp, C.POINTER(abi.KernelTime), i32, C.POINTER(i32)]),
+        "crg_film_encode": (i32, [vp, C.POINTER(C.c_float), i32, i32, C.c_float, C.POINTER(C.c_uint8),
+                                  C.POINTER(C.c_uint8), u64, C.POINTER(u64)]),
+        "crg_film_hdr_bound": (u64, [i32, i32]),
+        "crg_write_png": (i32, [vp, C.c_char_p, C.POINTER(C.c_float), i32, i32, C.c_float]),
+        "crg_write_hdr": (i32, [vp, C.c_char_p, C.POINTER(C.c_float), i32, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -230,6 +235,39 @@ class Context:
         n = C.c_int()
         self._check(self.lib.crg_kernel_times(self.handle, arr, 16, C.byref(n)))
         return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(min(n.value, 16))}
+
+    # --- film encode (io::writePng / io::writeHdr, reference src/io/image.cpp:201-236)
+    def film_png8(self, film: np.ndarray, exposure: float = 0.0) -> np.ndarray:
+        """Display bytes (H, W, 3) that writePng stores for a linear film."""
+        film = np.ascontiguousarray(film, np.float32)
+        h, w = film.shape[:2]
+        out = np.zeros((h, w, 3), np.uint8)
+        self._check(self.lib.crg_film_encode(self.handle, film.ctypes.data_as(C.POINTER(C.c_float)), w, h, exposure,
+                                             out.ctypes.data_as(C.POINTER(C.c_uint8)), None, 0, None))
+        return out
+
+    def film_hdr(self, film: np.ndarray) -> bytes:
+        """The complete Radiance .hdr file writeHdr writes for a linear film."""
+        film = np.ascontiguousarray(film, np.float32)
+        h, w = film.shape[:2]
+        cap = max(int(self.lib.crg_film_hdr_bound(w, h)), 1)
+        out = np.zeros(cap, np.uint8)
+        size = C.c_uint64()
+        self._check(self.lib.crg_film_encode(self.handle, film.ctypes.data_as(C.POINTER(C.c_float)), w, h, 0.0, None,
+                                             out.ctypes.data_as(C.POINTER(C.c_uint8)), cap, C.byref(size)))
+        return bytes(out[:size.value])
+
+    def write_png(self, path, film: np.ndarray, exposure: float = 0.0) -> None:
+        film = np.ascontiguousarray(film, np.float32)
+        h, w = film.shape[:2]
+        self._check(self.lib.crg_write_png(self.handle, str(path).encode(), film.ctypes.data_as(C.POINTER(C.c_float)),
+                                           w, h, exposure))
+
+    def write_hdr(self, path, film: np.ndarray) -> None:
+        film = np.ascontiguousarray(film, np.float32)
+        h, w = film.shape[:2]
+        self._check(self.lib.crg_write_hdr(self.handle, str(path).encode(), film.ctypes.data_as(C.POINTER(C.c_float)),
+                                           w, h))
 
     def trace(self, scene_id: int, rays: np.ndarray, any_hit: bool = False) -> np.ndarray:
         rays = np.ascontiguousarray(rays, abi.RAY_DTYPE)

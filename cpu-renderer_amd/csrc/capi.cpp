@@ -9,7 +9,10 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -20,6 +23,8 @@
 #include "../../include/crgpu.h"
 #include "device_buf.h"
 #include "engine.h"
+#include "film.h"
+#include "film_host.h"
 #include "host_scene.h"
 #include "kernels.h"
 
@@ -35,6 +40,10 @@ struct crg_ctx {
     std::map<uint64_t, std::shared_ptr<EnvRes>> envs;
     std::unique_ptr<Engine> engine;
     DevBuf ctl, rays, hits;
+    // film encode
+    DevBuf film, ldr, planes, streams, lens, thresholds, first_bad;
+    float cached_exposure = std::numeric_limits<float>::quiet_NaN();
+    float cached_thresholds[255] = {};
 };
 
 namespace {
@@ -309,6 +318,141 @@ int crg_trace_rays(crg_ctx* ctx, uint64_t scene_id, const crg_ray* rays, uint64_
             if (hits[i].hit == 2u) throw std::logic_error("BVH traversal exceeded its bounded stack");
             if (hits[i].hit == 3u) throw std::runtime_error("Non-finite texture coordinates or transform");
         }
+    });
+}
+
+uint64_t crg_film_hdr_bound(int32_t width, int32_t height) {
+    if (width <= 0 || height <= 0) return 0;
+    const uint64_t w = uint64_t(width), h = uint64_t(height);
+    return cr::hdr_header(width, height).size() + h * (4 + 4 * std::max<uint64_t>(cr::rle_bound(uint32_t(w)), w));
+}
+
+namespace {
+const char* kPixelsMessage = ": pixels must contain finite, nonnegative linear radiance";
+
+void film_encode(crg_ctx* ctx, const float* film_rgb, int32_t width, int32_t height, float exposure, uint8_t* rgb8,
+                 std::vector<uint8_t>* hdr) {
+    if (!film_rgb) throw std::invalid_argument("null film");
+    if (rgb8) cr::check_dimensions(width, height, "PNG output");
+    if (hdr) cr::check_dimensions(width, height, "HDR output");
+    if (!rgb8 && !hdr) return;
+    const uint32_t w = uint32_t(width), h = uint32_t(height);
+    const uint64_t npix = uint64_t(w) * h;
+    const bool rle = w >= 8 && w < 32768;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    hipStream_t st = ctx->stream;
+    const bool exposure_ok = std::isfinite(exposure) && exposure >= -100 && exposure <= 100;
+    if (rgb8 && exposure_ok && !(exposure == ctx->cached_exposure)) {
+        cr::display_thresholds(std::exp2(double(exposure)), ctx->cached_thresholds);
+        ctx->cached_exposure = exposure;
+    }
+    ctx->film.alloc(npix * 12);
+    ctx->first_bad.alloc(8);
+    ctx->thresholds.alloc(sizeof(ctx->cached_thresholds));
+    if (rgb8) ctx->ldr.alloc(npix * 3);
+    if (hdr) ctx->planes.alloc(npix * 4);
+    check(hipMemcpyAsync(ctx->film.p, film_rgb, npix * 12, hipMemcpyHostToDevice, st), "film upload");
+    check(hipMemsetAsync(ctx->first_bad.p, 0xff, 8, st), "memset");
+    if (rgb8)
+        check(hipMemcpyAsync(ctx->thresholds.p, ctx->cached_thresholds, sizeof(ctx->cached_thresholds),
+                             hipMemcpyHostToDevice, st),
+              "thresholds");
+    check(cr::launch_film_encode(ctx->film.as<float>(), npix, w, ctx->thresholds.as<float>(),
+                                 rgb8 ? ctx->ldr.as<uint8_t>() : nullptr, hdr ? ctx->planes.as<uint8_t>() : nullptr,
+                                 ctx->first_bad.as<uint32_t>(), st),
+          "film encode launch");
+    const uint32_t bound = cr::rle_bound(w);
+    if (hdr && rle) {
+        ctx->streams.alloc(uint64_t(h) * 4 * bound);
+        ctx->lens.alloc(uint64_t(h) * 4 * sizeof(uint32_t));
+        check(cr::launch_film_rle(ctx->planes.as<uint8_t>(), w, h, ctx->streams.as<uint8_t>(), ctx->lens.as<uint32_t>(),
+                                  st),
+              "film rle launch");
+    }
+    uint32_t bad[2];
+    check(hipMemcpyAsync(bad, ctx->first_bad.p, 8, hipMemcpyDeviceToHost, st), "copy");
+    check(hipStreamSynchronize(st), "film encode");
+    if (rgb8) {
+        if (bad[0] != ~0u) throw std::runtime_error(std::string("PNG output") + kPixelsMessage);
+        cr::check_exposure(exposure);
+        check(hipMemcpyAsync(rgb8, ctx->ldr.p, npix * 3, hipMemcpyDeviceToHost, st), "copy");
+    }
+    if (hdr) {
+        if (std::min(bad[0], bad[1]) != ~0u) {
+            if (bad[0] <= bad[1]) throw std::runtime_error(std::string("HDR output") + kPixelsMessage);
+            throw std::runtime_error("HDR output: radiance exceeds the Radiance RGBE exponent range");
+        }
+        const std::string head = cr::hdr_header(width, height);
+        hdr->assign(head.begin(), head.end());
+        if (rle) {
+            std::vector<uint32_t> lens(size_t(h) * 4);
+            std::vector<uint8_t> streams(size_t(h) * 4 * bound);
+            check(hipMemcpyAsync(lens.data(), ctx->lens.p, lens.size() * 4, hipMemcpyDeviceToHost, st), "copy");
+            check(hipMemcpyAsync(streams.data(), ctx->streams.p, streams.size(), hipMemcpyDeviceToHost, st), "copy");
+            check(hipStreamSynchronize(st), "film copy");
+            size_t total = hdr->size();
+            for (uint32_t v : lens) total += v;
+            hdr->reserve(total + size_t(h) * 4);
+            for (uint32_t y = 0; y < h; ++y) {
+                const uint8_t sh[4] = {2, 2, uint8_t((w >> 8) & 0xff), uint8_t(w & 0xff)};
+                hdr->insert(hdr->end(), sh, sh + 4);
+                for (uint32_t c = 0; c < 4; ++c) {
+                    const size_t j = size_t(y) * 4 + c;
+                    if (lens[j] > bound) throw std::logic_error("run-length code exceeds its bound");
+                    const uint8_t* p = streams.data() + j * bound;
+                    hdr->insert(hdr->end(), p, p + lens[j]);
+                }
+            }
+        } else {  // flat RGBE quadruples
+            std::vector<uint8_t> planes(npix * 4);
+            check(hipMemcpyAsync(planes.data(), ctx->planes.p, planes.size(), hipMemcpyDeviceToHost, st), "copy");
+            check(hipStreamSynchronize(st), "film copy");
+            const size_t off = hdr->size();
+            hdr->resize(off + npix * 4);
+            for (uint32_t y = 0; y < h; ++y)
+                for (uint32_t x = 0; x < w; ++x)
+                    for (uint32_t c = 0; c < 4; ++c)
+                        (*hdr)[off + (size_t(y) * w + x) * 4 + c] = planes[size_t(y) * 4 * w + size_t(c) * w + x];
+        }
+    }
+    check(hipStreamSynchronize(st), "film copy");
+}
+}  // namespace
+
+int crg_film_encode(crg_ctx* ctx, const float* film_rgb, int32_t width, int32_t height, float exposure, uint8_t* rgb8,
+                    uint8_t* hdr, uint64_t hdr_capacity, uint64_t* hdr_size) {
+    return guarded(ctx, [&] {
+        std::vector<uint8_t> file;
+        film_encode(ctx, film_rgb, width, height, exposure, rgb8, hdr ? &file : nullptr);
+        if (hdr) {
+            if (hdr_size) *hdr_size = file.size();
+            if (file.size() > hdr_capacity) throw std::invalid_argument("HDR output buffer too small");
+            std::memcpy(hdr, file.data(), file.size());
+        }
+    });
+}
+
+int crg_write_png(crg_ctx* ctx, const char* path, const float* film_rgb, int32_t width, int32_t height,
+                  float exposure) {
+    return guarded(ctx, [&] {
+        if (!path) throw std::invalid_argument("null path");
+        cr::check_dimensions(width, height, "PNG output");
+        std::vector<uint8_t> rgb8(size_t(width) * size_t(height) * 3);
+        film_encode(ctx, film_rgb, width, height, exposure, rgb8.data(), nullptr);
+        if (!cr::write_png_rgb8(path, width, height, rgb8.data()))
+            throw std::runtime_error(std::string("Failed writing image '") + path + "'");
+    });
+}
+
+int crg_write_hdr(crg_ctx* ctx, const char* path, const float* film_rgb, int32_t width, int32_t height) {
+    return guarded(ctx, [&] {
+        if (!path) throw std::invalid_argument("null path");
+        std::vector<uint8_t> file;
+        film_encode(ctx, film_rgb, width, height, 0.0f, nullptr, &file);
+        std::FILE* f = std::fopen(path, "wb");
+        const bool ok = f && std::fwrite(file.data(), 1, file.size(), f) == file.size();
+        if (!f || std::fclose(f) != 0 || !ok) throw std::runtime_error(std::string("Failed writing image '") + path + "'");
     });
 }
 

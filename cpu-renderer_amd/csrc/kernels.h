@@ -60,5 +60,16 @@ hipError_t launch_pool_init(uint32_t* freeq, uint32_t pool, Control* ctl, hipStr
 hipError_t launch_admit(Control* ctl, JobCounters* cnt, uint64_t total, uint32_t seq_lo, uint32_t seq_hi, uint32_t slot,
                         hipStream_t st);
 hipError_t launch_accumulate(const DevJob* jobs, uint32_t job, uint64_t npix, float* film, hipStream_t st);
+// Film encode: display bytes (ldr, 3 per pixel; skipped when null) and RGBE in
+// per-row planes (rgbe: row-major rows of 4 planes of `width` bytes; skipped
+// when null); first_bad[0] / [1] (preset to ~0u) receive the lowest pixel index
+// that is non-finite or negative / whose largest component reaches 2^127.
+// thresholds: 255 floats on the device.
+hipError_t launch_film_encode(const float* film, uint64_t npix, uint32_t width, const float* thresholds, uint8_t* ldr,
+                              uint8_t* rgbe, uint32_t* first_bad, hipStream_t st);
+// Run-length codes every (row, plane) of `rgbe` into streams[(row*4+plane)*rle_bound(width)]
+// and its byte count into lens[row*4+plane].
+hipError_t launch_film_rle(const uint8_t* rgbe, uint32_t width, uint32_t height, uint8_t* streams, uint32_t* lens,
+                           hipStream_t st);
 
 }  // namespace cr

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
+#include "film.h"
 #include "path.h"
 
 // Minimum workgroups of 256 lanes per compute unit the compiler must fit
@@ -778,6 +779,61 @@ hipError_t launch_pool_init(uint32_t* freeq, uint32_t pool, Control* ctl, hipStr
 hipError_t launch_admit(Control* ctl, JobCounters* cnt, uint64_t total, uint32_t seq_lo, uint32_t seq_hi, uint32_t slot,
                         hipStream_t st) {
     hipLaunchKernelGGL(k_admit, dim3(1), dim3(1), 0, st, ctl, cnt, (unsigned long long)total, seq_lo, seq_hi, slot);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ film encode
+// One lane per pixel: validity (lowest failing pixel per condition), three
+// display bytes by threshold search in LDS, and the RGBE bytes into the row's
+// four planes.
+__global__ void __launch_bounds__(256) k_film_encode(const float* film, uint64_t npix, uint32_t width,
+                                                     const float* thresholds, uint8_t* ldr, uint8_t* rgbe,
+                                                     uint32_t* first_bad) {
+    __shared__ float t[kDisplayLevels - 1];
+    if (ldr)
+        for (uint32_t k = threadIdx.x; k < kDisplayLevels - 1; k += blockDim.x) t[k] = thresholds[k];
+    __syncthreads();
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    const float r = film[3 * i], g = film[3 * i + 1], b = film[3 * i + 2];
+    if (!finite1(r) || !finite1(g) || !finite1(b) || r < 0 || g < 0 || b < 0) atomicMin(&first_bad[0], uint32_t(i));
+    else if (fmaxf(r, fmaxf(g, b)) >= 1.7014118346046923e38f) atomicMin(&first_bad[1], uint32_t(i));  // 2^127
+    if (ldr) {
+        ldr[3 * i] = uint8_t(display_byte(r, t));
+        ldr[3 * i + 1] = uint8_t(display_byte(g, t));
+        ldr[3 * i + 2] = uint8_t(display_byte(b, t));
+    }
+    if (rgbe) {
+        uint8_t q[4];
+        rgbe_of(r, g, b, q);
+        const uint64_t row = i / width, x = i - row * width;
+        uint8_t* base = rgbe + row * 4ull * width + x;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) base[uint64_t(k) * width] = q[k];
+    }
+}
+
+// One lane per (row, plane): the plane's run-length code, written to its own
+// fixed-size slot.
+__global__ void __launch_bounds__(256) k_film_rle(const uint8_t* rgbe, uint32_t width, uint32_t height,
+                                                  uint8_t* streams, uint32_t* lens) {
+    const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= uint64_t(height) * 4u) return;
+    const uint8_t* in = rgbe + j * width;  // plane j % 4 of row j / 4
+    lens[j] = rle_component(in, 1u, width, streams + j * rle_bound(width));
+}
+
+hipError_t launch_film_encode(const float* film, uint64_t npix, uint32_t width, const float* thresholds, uint8_t* ldr,
+                              uint8_t* rgbe, uint32_t* first_bad, hipStream_t st) {
+    hipLaunchKernelGGL(k_film_encode, dim3(blocks_for(npix, 256)), dim3(256), 0, st, film, npix, width, thresholds, ldr,
+                       rgbe, first_bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_film_rle(const uint8_t* rgbe, uint32_t width, uint32_t height, uint8_t* streams, uint32_t* lens,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(k_film_rle, dim3(blocks_for(uint64_t(height) * 4u, 256)), dim3(256), 0, st, rgbe, width, height,
+                       streams, lens);
     return hipGetLastError();
 }
 

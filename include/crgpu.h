@@ -274,6 +274,31 @@ int crg_set_tuning(crg_ctx* ctx, const crg_tuning* tuning);
  * fastest and returns it (with its measured rate in *samples_per_second). */
 int crg_autotune(crg_ctx* ctx, const crg_request* sample, crg_tuning* out, double* samples_per_second);
 
+/* ---- film encode -------------------------------------------------------- */
+/*
+ * The reference's output writers (reference src/io/image.cpp:201-236, called by
+ * src/main.cpp:247-248): io::writePng (exposure in stops, ACES-like filmic
+ * curve, sRGB, 8 bits) and io::writeHdr (unexposed linear radiance as Radiance
+ * RGBE with run-length coded scanlines).  The film (width*height RGB floats, as
+ * crg_render writes it) is checked and encoded on the device.
+ *
+ * rgb8 (width*height*3 bytes) receives the display bytes writePng stores; hdr
+ * (capacity hdr_capacity, at least crg_film_hdr_bound) receives the complete
+ * .hdr file writeHdr writes, its length in *hdr_size.  Either may be NULL.
+ * Checks and messages follow the writers: dimensions, then pixels (finite,
+ * non-negative; for HDR below 2^127), then (PNG) the exposure range.
+ */
+int crg_film_encode(crg_ctx* ctx, const float* film_rgb, int32_t width, int32_t height, float exposure,
+                    uint8_t* rgb8, uint8_t* hdr, uint64_t hdr_capacity, uint64_t* hdr_size);
+/* Upper bound on the .hdr file size for a width x height film. */
+uint64_t crg_film_hdr_bound(int32_t width, int32_t height);
+/* io::writePng / io::writeHdr to a file (reference src/io/image.cpp:201-236).
+ * The PNG holds the same pixel bytes as the reference's (its deflate stream
+ * is zlib's, not stb's); the .hdr file is byte-identical. */
+int crg_write_png(crg_ctx* ctx, const char* path, const float* film_rgb, int32_t width, int32_t height,
+                  float exposure);
+int crg_write_hdr(crg_ctx* ctx, const char* path, const float* film_rgb, int32_t width, int32_t height);
+
 /* ---- measurement ------------------------------------------------------- */
 /* Per-kernel device time, measured with HIP events recorded on the context's
  * stream around every launch while profiling is enabled.  No counterpart in

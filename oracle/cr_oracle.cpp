@@ -16,6 +16,9 @@
 //   Bvh::build / subdivide / intersect ...... src/geometry.cpp:463-639
 //   BSDF .................................... src/bsdf.cpp:6-237
 //   Environment, Lights, trace, render ...... src/render.cpp:12-456
+//   writePng / writeHdr (film encode) ...... src/io/image.cpp:101-124, :201-236, with the
+//                                            RGBE and run-length rules of the pinned
+//                                            third_party/stb_image_write.h:635-790
 #include <algorithm>
 #include <array>
 #include <atomic>
@@ -1370,6 +1373,96 @@ int guard(F&& f) {
 
 using namespace orc;
 
+namespace orc {
+// ---------------------------------------------------------------- film encode
+void checkOutput(int w, int h, const float* film, const std::string& label, bool hdr) {
+    if (w <= 0 || h <= 0 || w > 16384 || h > 16384 || size_t(w) > (16ull * 1024 * 1024) / size_t(h))
+        throw std::runtime_error(label + ": image dimensions must be positive, at most 16384 per axis and 16777216 pixels");
+    const size_t n = size_t(w) * size_t(h);
+    for (size_t i = 0; i < n; ++i) {
+        const float* p = film + 3 * i;
+        if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2]) || p[0] < 0 || p[1] < 0 || p[2] < 0)
+            throw std::runtime_error(label + ": pixels must contain finite, nonnegative linear radiance");
+        if (hdr && std::max({p[0], p[1], p[2]}) >= std::ldexp(1.0f, 127))
+            throw std::runtime_error(label + ": radiance exceeds the Radiance RGBE exponent range");
+    }
+}
+// Filmic (ACES-like rational), sRGB, 8-bit (reference src/io/image.cpp:117-124, src/math.h:62).
+float toSrgb(float v) { return v <= 0.0031308f ? 12.92f * v : 1.055f * std::pow(v, 1 / 2.4f) - 0.055f; }
+uint8_t displayByte(float value, double scale) {
+    const double x = double(value) * scale;
+    const double f = std::clamp((x * (2.51 * x + 0.03)) / (x * (2.43 * x + 0.59) + 0.14), 0.0, 1.0);
+    const float s = toSrgb(float(f));
+    return uint8_t(std::lround(std::clamp(s, 0.0f, 1.0f) * 255.0f));
+}
+// Shared-exponent RGBE of one pixel.
+void toRgbe(const float* c, uint8_t out[4]) {
+    const float m = c[0] > (c[1] > c[2] ? c[1] : c[2]) ? c[0] : (c[1] > c[2] ? c[1] : c[2]);
+    if (m < 1e-32f) {
+        out[0] = out[1] = out[2] = out[3] = 0;
+        return;
+    }
+    int e;
+    const float norm = float(std::frexp(m, &e)) * 256.0f / m;
+    for (int k = 0; k < 3; ++k) out[k] = uint8_t(c[k] * norm);
+    out[3] = uint8_t(e + 128);
+}
+// New-style run-length encoding of one component of a scanline: literal
+// blocks of at most 128 bytes up to the next run of three equal bytes, runs in
+// blocks of at most 127.
+void rleComponent(const uint8_t* c, int w, std::vector<uint8_t>& out) {
+    int x = 0;
+    while (x < w) {
+        int r = x;
+        while (r + 2 < w && !(c[r] == c[r + 1] && c[r] == c[r + 2])) ++r;
+        const bool run = r + 2 < w;
+        if (!run) r = w;
+        while (x < r) {
+            const int len = std::min(r - x, 128);
+            out.push_back(uint8_t(len));
+            out.insert(out.end(), c + x, c + x + len);
+            x += len;
+        }
+        if (run) {
+            while (r < w && c[r] == c[x]) ++r;
+            while (x < r) {
+                const int len = std::min(r - x, 127);
+                out.push_back(uint8_t(128 + len));
+                out.push_back(c[x]);
+                x += len;
+            }
+        }
+    }
+}
+std::vector<uint8_t> hdrFile(int w, int h, const float* film) {
+    std::vector<uint8_t> out;
+    const std::string head = "#?RADIANCE\n# Written by stb_image_write.h\nFORMAT=32-bit_rle_rgbe\n"
+                             "EXPOSURE=          1.0000000000000\n\n-Y " + std::to_string(h) + " +X " + std::to_string(w) + "\n";
+    out.insert(out.end(), head.begin(), head.end());
+    std::vector<uint8_t> comp(size_t(w) * 4);
+    for (int y = 0; y < h; ++y) {
+        const float* row = film + size_t(y) * size_t(w) * 3;
+        if (w < 8 || w >= 32768) {
+            for (int x = 0; x < w; ++x) {
+                uint8_t q[4];
+                toRgbe(row + 3 * x, q);
+                out.insert(out.end(), q, q + 4);
+            }
+            continue;
+        }
+        for (int x = 0; x < w; ++x) {
+            uint8_t q[4];
+            toRgbe(row + 3 * x, q);
+            for (int k = 0; k < 4; ++k) comp[size_t(k) * w + x] = q[k];
+        }
+        const uint8_t sh[4] = {2, 2, uint8_t((w >> 8) & 0xff), uint8_t(w & 0xff)};
+        out.insert(out.end(), sh, sh + 4);
+        for (int k = 0; k < 4; ++k) rleComponent(comp.data() + size_t(k) * w, w, out);
+    }
+    return out;
+}
+}  // namespace orc
+
 extern "C" {
 
 const char* orc_last_error(void) { return lastError.c_str(); }
@@ -1550,6 +1643,26 @@ int orc_render(void* scene, void* env, const crg_camera* c, const crg_options* o
             st->shadow_rays = r.shadow;
             st->workers = r.workers;
         }
+    });
+}
+
+int orc_film_png8(const float* film, int width, int height, float exposure, uint8_t* rgb8) {
+    return guard([&] {
+        orc::checkOutput(width, height, film, "PNG output", false);
+        if (!std::isfinite(exposure) || exposure < -100 || exposure > 100)
+            throw std::runtime_error("PNG exposure must be finite and between -100 and 100 stops");
+        const double scale = std::exp2(double(exposure));
+        const size_t n = size_t(width) * size_t(height) * 3;
+        for (size_t i = 0; i < n; ++i) rgb8[i] = orc::displayByte(film[i], scale);
+    });
+}
+int orc_film_hdr(const float* film, int width, int height, uint8_t* out, uint64_t capacity, uint64_t* size) {
+    return guard([&] {
+        orc::checkOutput(width, height, film, "HDR output", true);
+        const std::vector<uint8_t> b = orc::hdrFile(width, height, film);
+        *size = b.size();
+        if (b.size() > capacity) throw std::runtime_error("HDR output buffer too small");
+        std::memcpy(out, b.data(), b.size());
     });
 }
 }

@@ -56,10 +56,17 @@ typedef struct probe_bsdf_sample {
     float P##_env_pdf(void* env, const float* direction);                                             \
     void P##_env_radiance(void* env, const float* direction, float* rgb);                             \
     int P##_render(void* scene, void* env, const crg_camera* camera, const crg_options* options,      \
-                   float* film, crg_stats* stats);
+                   float* film, crg_stats* stats);                                                     \
+    int P##_film_png8(const float* film, int width, int height, float exposure, uint8_t* rgb8);       \
+    int P##_film_hdr(const float* film, int width, int height, uint8_t* out, uint64_t capacity,       \
+                     uint64_t* size);
 
 PROBE_DECLARE(ref)
 PROBE_DECLARE(orc)
+
+/* Reference only: io::writePng and io::writeHdr to the given paths (timing). */
+int ref_film_write(const float* film, int width, int height, float exposure, const char* png_path,
+                   const char* hdr_path);
 
 #ifdef __cplusplus
 }

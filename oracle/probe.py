@@ -84,7 +84,11 @@ class Probe:
             "env_pdf": (f, [vp, fp]),
             "env_radiance": (None, [vp, fp, fp]),
             "render": (C.c_int, [vp, vp, C.POINTER(abi.Camera), C.POINTER(abi.Options), fp, C.POINTER(abi.Stats)]),
+            "film_png8": (C.c_int, [fp, C.c_int, C.c_int, f, C.POINTER(C.c_uint8)]),
+            "film_hdr": (C.c_int, [fp, C.c_int, C.c_int, C.POINTER(C.c_uint8), u64, C.POINTER(u64)]),
         }
+        if kind == "ref":
+            sig["film_write"] = (C.c_int, [fp, C.c_int, C.c_int, f, C.c_char_p, C.c_char_p])
         for name, (res, args) in sig.items():
             fn = getattr(self.lib, p + name)
             fn.restype = res
@@ -125,6 +129,37 @@ class Probe:
         out = (C.c_float * 4)()
         self._bsdf_eval(C.byref(surface), (C.c_float * 3)(*wo), (C.c_float * 3)(*wi), out)
         return np.array(out[:3], np.float32), np.float32(out[3])
+
+    # --- film encode (writePng / writeHdr)
+    def film_png8(self, film, exposure: float):
+        """Display bytes (H, W, 3) the PNG writer stores, or raises with its message."""
+        film = np.ascontiguousarray(film, np.float32)
+        h, w = film.shape[:2]
+        out = np.zeros((h, w, 3), np.uint8)
+        if self._film_png8(film.ctypes.data_as(C.POINTER(C.c_float)), w, h, exposure,
+                           out.ctypes.data_as(C.POINTER(C.c_uint8))):
+            raise RuntimeError(self.error())
+        return out
+
+    def film_write(self, film, exposure: float, png_path, hdr_path) -> None:
+        """Reference only: writePng and writeHdr to files."""
+        film = np.ascontiguousarray(film, np.float32)
+        h, w = film.shape[:2]
+        if self._film_write(film.ctypes.data_as(C.POINTER(C.c_float)), w, h, exposure, str(png_path).encode(),
+                            str(hdr_path).encode()):
+            raise RuntimeError(self.error())
+
+    def film_hdr(self, film) -> bytes:
+        """The complete Radiance .hdr file the HDR writer produces."""
+        film = np.ascontiguousarray(film, np.float32)
+        h, w = film.shape[:2]
+        cap = 256 + h * (4 + 4 * (w + w // 128 + 2))
+        out = np.zeros(cap, np.uint8)
+        size = C.c_uint64()
+        if self._film_hdr(film.ctypes.data_as(C.POINTER(C.c_float)), w, h, out.ctypes.data_as(C.POINTER(C.c_uint8)),
+                          cap, C.byref(size)):
+            raise RuntimeError(self.error())
+        return bytes(out[:size.value])
 
     # --- scene / environment objects
     def scene(self, data) -> "ProbeScene":

@@ -10,7 +10,13 @@
 #include "probe.h"
 #include "render.h"
 
+#include <atomic>
+#include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <filesystem>
+#include <unistd.h>
+#include <vector>
 #include <exception>
 #include <memory>
 #include <string>
@@ -276,6 +282,66 @@ int ref_render(void* scene, void* env, const crg_camera* camera, const crg_optio
             stats->boundary_retries = r.boundaryRetries;
             stats->workers = r.workers;
         }
+    });
+}
+
+
+// Film encode through the reference's own writers: writePng / writeHdr to a
+// temporary file; the PNG is decoded again with the reference's decoder.
+static std::filesystem::path film_temp(const char* ext) {
+    static std::atomic<unsigned> counter{0};
+    return std::filesystem::temp_directory_path() /
+           ("crref_" + std::to_string(::getpid()) + "_" + std::to_string(counter++) + ext);
+}
+static std::vector<cr::Vec3> film_vec(const float* film, int width, int height) {
+    std::vector<cr::Vec3> v(size_t(std::max(width, 0)) * size_t(std::max(height, 0)));
+    for (size_t i = 0; i < v.size(); ++i) v[i] = {film[3 * i], film[3 * i + 1], film[3 * i + 2]};
+    return v;
+}
+static std::vector<unsigned char> read_all(const std::filesystem::path& p) {
+    std::FILE* f = std::fopen(p.c_str(), "rb");
+    std::vector<unsigned char> b;
+    if (!f) return b;
+    unsigned char buf[65536];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) b.insert(b.end(), buf, buf + n);
+    std::fclose(f);
+    return b;
+}
+
+int ref_film_png8(const float* film, int width, int height, float exposure, uint8_t* rgb8) {
+    return guarded([&] {
+        const auto path = film_temp(".png");
+        struct Rm { std::filesystem::path p; ~Rm() { std::error_code e; std::filesystem::remove(p, e); } } rm{path};
+        cr::io::writePng(path, width, height, film_vec(film, width, height), exposure);
+        const auto bytes = read_all(path);
+        const cr::Image im = cr::io::decodeImage(bytes.data(), bytes.size(), "png", false);
+        for (size_t i = 0; i < im.pixels.size(); ++i) {
+            rgb8[3 * i] = uint8_t(std::lround(im.pixels[i].x * 255.0f));
+            rgb8[3 * i + 1] = uint8_t(std::lround(im.pixels[i].y * 255.0f));
+            rgb8[3 * i + 2] = uint8_t(std::lround(im.pixels[i].z * 255.0f));
+        }
+    });
+}
+
+int ref_film_hdr(const float* film, int width, int height, uint8_t* out, uint64_t capacity, uint64_t* size) {
+    return guarded([&] {
+        const auto path = film_temp(".hdr");
+        struct Rm { std::filesystem::path p; ~Rm() { std::error_code e; std::filesystem::remove(p, e); } } rm{path};
+        cr::io::writeHdr(path, width, height, film_vec(film, width, height));
+        const auto bytes = read_all(path);
+        *size = bytes.size();
+        if (bytes.size() > capacity) throw std::runtime_error("HDR output buffer too small");
+        std::memcpy(out, bytes.data(), bytes.size());
+    });
+}
+
+int ref_film_write(const float* film, int width, int height, float exposure, const char* png_path,
+                   const char* hdr_path) {
+    return guarded([&] {
+        const auto v = film_vec(film, width, height);
+        cr::io::writePng(png_path, width, height, v, exposure);
+        cr::io::writeHdr(hdr_path, width, height, v);
     });
 }
 }

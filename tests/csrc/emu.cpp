@@ -19,6 +19,8 @@
 #include <vector>
 
 #include "../../cpu-renderer_amd/csrc/host_scene.h"
+#include "../../cpu-renderer_amd/csrc/film.h"
+#include "../../cpu-renderer_amd/csrc/film_host.h"
 #include "../../cpu-renderer_amd/csrc/path.h"
 
 using namespace cr;
@@ -201,6 +203,67 @@ int emu_box_hit(const float* o, const float* d, float tmin, float tmax, const fl
     n.child[0] = 1;
     float nears[4];
     return int(node_hit(q, n, tmax, nears) & 1u);
+}
+
+// Film encoders on the CPU with the device's shared code (film.h) and the host
+// half (film_host.cpp): display bytes by threshold search, RGBE and the
+// scanline run-length code, the same checks and messages.
+int emu_film_png8(const float* film, int width, int height, float exposure, uint8_t* rgb8) {
+    try {
+        check_dimensions(width, height, "PNG output");
+        const size_t n = size_t(width) * size_t(height);
+        for (size_t i = 0; i < 3 * n; ++i)
+            if (!std::isfinite(film[i]) || film[i] < 0)
+                throw std::runtime_error("PNG output: pixels must contain finite, nonnegative linear radiance");
+        check_exposure(exposure);
+        float t[255];
+        display_thresholds(std::exp2(double(exposure)), t);
+        for (size_t i = 0; i < 3 * n; ++i) rgb8[i] = uint8_t(display_byte(film[i], t));
+        return 0;
+    } catch (const std::exception& e) {
+        lastError = e.what();
+        return 1;
+    }
+}
+
+int emu_film_hdr(const float* film, int width, int height, uint8_t* out, uint64_t capacity, uint64_t* size) {
+    try {
+        check_dimensions(width, height, "HDR output");
+        const uint32_t w = uint32_t(width), h = uint32_t(height);
+        for (size_t i = 0; i < size_t(w) * h; ++i) {
+            const float* p = film + 3 * i;
+            if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2]) || p[0] < 0 || p[1] < 0 || p[2] < 0)
+                throw std::runtime_error("HDR output: pixels must contain finite, nonnegative linear radiance");
+            if (std::max({p[0], p[1], p[2]}) >= std::ldexp(1.0f, 127))
+                throw std::runtime_error("HDR output: radiance exceeds the Radiance RGBE exponent range");
+        }
+        const std::string head = hdr_header(width, height);
+        std::vector<uint8_t> file(head.begin(), head.end());
+        std::vector<uint8_t> planes(size_t(w) * 4), code(rle_bound(w));
+        for (uint32_t y = 0; y < h; ++y) {
+            for (uint32_t x = 0; x < w; ++x) {
+                uint8_t q[4];
+                const float* p = film + 3 * (size_t(y) * w + x);
+                rgbe_of(p[0], p[1], p[2], q);
+                if (w < 8 || w >= 32768) file.insert(file.end(), q, q + 4);
+                for (int c = 0; c < 4; ++c) planes[size_t(c) * w + x] = q[c];
+            }
+            if (w < 8 || w >= 32768) continue;
+            const uint8_t sh[4] = {2, 2, uint8_t((w >> 8) & 0xff), uint8_t(w & 0xff)};
+            file.insert(file.end(), sh, sh + 4);
+            for (int c = 0; c < 4; ++c) {
+                const uint32_t k = rle_component(planes.data() + size_t(c) * w, 1u, w, code.data());
+                file.insert(file.end(), code.begin(), code.begin() + k);
+            }
+        }
+        *size = file.size();
+        if (file.size() > capacity) throw std::runtime_error("HDR output buffer too small");
+        std::memcpy(out, file.data(), file.size());
+        return 0;
+    } catch (const std::exception& e) {
+        lastError = e.what();
+        return 1;
+    }
 }
 
 float emu_env_pdf(void* e, const float* d) { return env_pdf(static_cast<EmuEnv*>(e)->dev, V3{d[0], d[1], d[2]}); }

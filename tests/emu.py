@@ -10,7 +10,8 @@ import numpy as np
 
 from conftest import ROOT, abi
 
-SRC = [ROOT / "tests" / "csrc" / "emu.cpp", ROOT / "cpu-renderer_amd" / "csrc" / "host_scene.cpp"]
+SRC = [ROOT / "tests" / "csrc" / "emu.cpp", ROOT / "cpu-renderer_amd" / "csrc" / "host_scene.cpp",
+       ROOT / "cpu-renderer_amd" / "csrc" / "film_host.cpp"]
 DEPS = SRC + sorted((ROOT / "cpu-renderer_amd" / "csrc").glob("*.h")) + [ROOT / "include" / "crgpu.h"]
 OUT = ROOT / "tests" / "csrc" / "_build" / "libemu.so"
 
@@ -21,7 +22,7 @@ def build() -> pathlib.Path:
     OUT.parent.mkdir(exist_ok=True)
     tmp = OUT.with_suffix(".so.tmp")
     subprocess.run(["clang++", "-std=c++20", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
-                    "-pthread", "-o", str(tmp), *map(str, SRC)], check=True)
+                    "-pthread", "-o", str(tmp), *map(str, SRC), "-lz"], check=True)
     tmp.replace(OUT)
     return OUT
 
@@ -45,6 +46,8 @@ class Emu:
             "emu_env_radiance": (None, [vp, fp, fp]),
             "emu_trace": (C.c_int, [vp, vp, u64, vp, C.c_int, C.POINTER(u64), u64]),
             "emu_box_hit": (C.c_int, [fp, fp, C.c_float, C.c_float, fp, fp]),
+            "emu_film_png8": (C.c_int, [fp, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_uint8)]),
+            "emu_film_hdr": (C.c_int, [fp, C.c_int, C.c_int, C.POINTER(C.c_uint8), u64, C.POINTER(u64)]),
             "emu_camera_generate": (C.c_int, [C.POINTER(abi.Camera), C.c_float, C.c_float, C.c_int, C.c_int,
                                               C.POINTER(abi.Ray)]),
             "emu_render": (C.c_int, [vp, vp, C.POINTER(abi.Camera), C.POINTER(abi.Options), fp,
@@ -56,6 +59,26 @@ class Emu:
 
     def error(self):
         return self.lib.emu_last_error().decode()
+
+    def film_png8(self, film, exposure):
+        film = np.ascontiguousarray(film, np.float32)
+        h, w = film.shape[:2]
+        out = np.zeros((h, w, 3), np.uint8)
+        if self.lib.emu_film_png8(film.ctypes.data_as(C.POINTER(C.c_float)), w, h, exposure,
+                                  out.ctypes.data_as(C.POINTER(C.c_uint8))):
+            raise RuntimeError(self.error())
+        return out
+
+    def film_hdr(self, film) -> bytes:
+        film = np.ascontiguousarray(film, np.float32)
+        h, w = film.shape[:2]
+        cap = 256 + h * (4 + 4 * (w + w // 128 + 2))
+        out = np.zeros(cap, np.uint8)
+        size = C.c_uint64()
+        if self.lib.emu_film_hdr(film.ctypes.data_as(C.POINTER(C.c_float)), w, h,
+                                 out.ctypes.data_as(C.POINTER(C.c_uint8)), cap, C.byref(size)):
+            raise RuntimeError(self.error())
+        return bytes(out[:size.value])
 
     def scene(self, data):
         return EmuScene(self, data)

@@ -342,10 +342,84 @@ def gen_render(ref):
     np.savez_compressed(OUT / "render.npz", **out)
 
 
+def film_cases():
+    """(name, film, exposure) for the film encoders (reference src/io/image.cpp:201-236):
+    log-spaced radiance across every display byte at three exposures, RGBE
+    mantissa/exponent boundaries, long runs and short literals for the scanline
+    run-length coder, the flat-scanline widths (< 8), and a rendered film."""
+    g = np.random.default_rng(21)
+    out = []
+    vals = np.concatenate([[0.0, 1e-38, 1e-33, 1e-32, 1.0000001e-32], np.geomspace(1e-6, 64.0, 3000),
+                           [2.0 ** k for k in range(-20, 20)], [np.nextafter(np.float32(2.0 ** k), np.float32(0))
+                                                                for k in range(-20, 20)], [1e30, 1.7e38]])
+    vals = vals.astype(np.float32)
+    n = len(vals)
+    w = 64
+    h = -(-n * 3 // (w * 3)) + 1
+    flat = np.resize(vals, w * h * 3)
+    g.shuffle(flat)
+    ramp = flat.reshape(h, w, 3).astype(np.float32)
+    for ex in (0.0, -3.3, 4.75):
+        out.append((f"ramp_e{ex:+.2f}", ramp, ex))
+    runs = np.zeros((7, 300, 3), np.float32)
+    runs[:, :140] = 0.25
+    runs[:, 140:141] = 1.0
+    runs[:, 141:290] = g.random((7, 149, 3)).astype(np.float32) * 3
+    runs[:, 290:] = 7.5
+    runs[3] = 0.0
+    out.append(("runs_w300", runs, 0.0))
+    for w_ in (1, 5, 7, 8, 9):
+        out.append((f"narrow_w{w_}", (g.random((3, w_, 3)) ** 2 * 5).astype(np.float32), 1.0))
+    r = load_golden_render("c1_64")
+    out.append(("render_c1_64", r, 0.0))
+    return out
+
+
+def film_error_cases():
+    """(name, film, exposure) the writers reject, with the reference's messages."""
+    good = np.full((2, 3, 3), 0.5, np.float32)
+    neg = good.copy()
+    neg[1, 2, 1] = -1e-9
+    nan = good.copy()
+    nan[0, 1, 0] = np.nan
+    huge = good.copy()
+    huge[1, 0, 2] = np.float32(2.0 ** 127)
+    return [("negative", neg, 0.0), ("nan", nan, 0.0), ("rgbe_range", huge, 0.0), ("exposure", good, 100.5),
+            ("exposure_nan", good, float("nan"))]
+
+
+def load_golden_render(name):
+    g = np.load(OUT / "render.npz")
+    return g[f"{name}/film"]
+
+
+def gen_film_encode(ref):
+    out = {}
+    for name, film, ex in film_cases():
+        out[f"{name}/film"] = film
+        out[f"{name}/exposure"] = np.array(ex, np.float32)
+        out[f"{name}/png8"] = ref.film_png8(film, ex)
+        out[f"{name}/hdr"] = np.frombuffer(ref.film_hdr(film), np.uint8)
+        print(f"  {name}: {film.shape[1]}x{film.shape[0]} hdr {out[f'{name}/hdr'].size} bytes")
+    for name, film, ex in film_error_cases():
+        for kind, fn in (("png", lambda: ref.film_png8(film, ex)), ("hdr", lambda: ref.film_hdr(film))):
+            try:
+                fn()
+                msg = ""
+            except RuntimeError as e:
+                msg = str(e)
+            out[f"err_{name}/{kind}"] = np.array(msg)
+        out[f"err_{name}/film"] = film
+        out[f"err_{name}/exposure"] = np.array(ex, np.float32)
+        print(f"  err_{name}: png '{out[f'err_{name}/png']}' hdr '{out[f'err_{name}/hdr']}'")
+    np.savez_compressed(OUT / "film_encode.npz", **out)
+
+
 def main():
     ref = probe.Probe("ref")
     only = set(sys.argv[1:])
-    for fn in (gen_rng_camera, gen_trace, gen_offset_surface, gen_bsdf, gen_env, gen_render, gen_textured_surface):
+    for fn in (gen_rng_camera, gen_trace, gen_offset_surface, gen_bsdf, gen_env, gen_render, gen_textured_surface,
+               gen_film_encode):
         if only and fn.__name__ not in only:
             continue
         print(fn.__name__)

@@ -176,3 +176,51 @@ def test_emu_render(emu, name):
         assert ok.all()
     assert stats.boundary_retries == int(g[f"{name}/retries"])
 
+
+
+def _film_names():
+    g = load_golden("film_encode.npz")
+    return sorted({k.split("/")[0] for k in g.files})
+
+
+@pytest.mark.parametrize("name", _film_names())
+def test_emu_film_encode(emu, name):
+    """The device's film encoders (film.h) with the host thresholds
+    (film_host.cpp), run on the CPU: writePng display bytes and the whole
+    writeHdr file bit for bit against the reference's writers, and the same
+    rejections (reference src/io/image.cpp:101-124, :201-236)."""
+    g = load_golden("film_encode.npz")
+    film, ex = g[f"{name}/film"], float(g[f"{name}/exposure"])
+    if name.startswith("err_"):
+        for kind, fn in (("png", lambda: emu.film_png8(film, ex)), ("hdr", lambda: emu.film_hdr(film))):
+            want = str(g[f"{name}/{kind}"])
+            if want:
+                with pytest.raises(RuntimeError) as e:
+                    fn()
+                assert str(e.value) == want
+            else:
+                fn()
+        return
+    np.testing.assert_array_equal(emu.film_png8(film, ex), g[f"{name}/png8"])
+    assert emu.film_hdr(film) == g[f"{name}/hdr"].tobytes()
+
+
+def test_display_thresholds_match_the_curve(emu, oracle):
+    """Threshold search equals the display curve evaluated per value, on values
+    at and one float step either side of every threshold and on a dense sweep,
+    at several exposures (the curve's monotonicity is what makes the table
+    exact)."""
+    g = np.random.default_rng(3)
+    for ex in (-7.0, -1.25, 0.0, 0.5, 3.0, 9.0):
+        v = np.concatenate([np.geomspace(1e-9, 1e4, 20000), g.random(5000) * 3]).astype(np.float32)
+        film = np.resize(v, (len(v) // 300 + 1) * 300).reshape(-1, 100, 3)
+        np.testing.assert_array_equal(emu.film_png8(film, ex), oracle.film_png8(film, ex))
+        # values straddling each byte boundary of the reference curve
+        ref = oracle.film_png8(film, ex).ravel()
+        edges = np.nonzero(np.diff(ref[np.argsort(film.ravel(), kind="stable")]))[0]
+        s = np.sort(film.ravel())
+        probe_v = np.concatenate([s[edges], s[edges + 1]])
+        near = np.concatenate([probe_v, np.nextafter(probe_v, np.float32(np.inf)),
+                               np.nextafter(probe_v, np.float32(0))]).astype(np.float32)
+        near = np.resize(near, (len(near) // 300 + 1) * 300).reshape(-1, 100, 3)
+        np.testing.assert_array_equal(emu.film_png8(near, ex), oracle.film_png8(near, ex))

@@ -115,3 +115,29 @@ def test_textured_surfaces(oracle):
             ray = abi.Ray.from_buffer_copy(rays[i].tobytes())
             hit = abi.Hit.from_buffer_copy(hits[i].tobytes())
             assert bytes(ps.surface_at(ray, hit)) == surf.tobytes(), f"{name}: surfaceAt mismatch at ray {i}"
+
+
+def _film_names():
+    g = load_golden("film_encode.npz")
+    return sorted({k.split("/")[0] for k in g.files})
+
+
+@pytest.mark.parametrize("name", _film_names())
+def test_film_encode(oracle, name):
+    """writePng display bytes and the complete writeHdr file (reference
+    src/io/image.cpp:101-124, :201-236), bit for bit; rejected inputs with the
+    reference's messages."""
+    g = load_golden("film_encode.npz")
+    film, ex = g[f"{name}/film"], float(g[f"{name}/exposure"])
+    if name.startswith("err_"):
+        for kind, fn in (("png", lambda: oracle.film_png8(film, ex)), ("hdr", lambda: oracle.film_hdr(film))):
+            want = str(g[f"{name}/{kind}"])
+            if want:
+                with pytest.raises(RuntimeError) as e:
+                    fn()
+                assert str(e.value) == want
+            else:
+                fn()
+        return
+    np.testing.assert_array_equal(oracle.film_png8(film, ex), g[f"{name}/png8"])
+    assert oracle.film_hdr(film) == g[f"{name}/hdr"].tobytes()
