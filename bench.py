@@ -39,6 +39,7 @@ import time
 
 ROOT = pathlib.Path(__file__).resolve().parent
 PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
 METRIC = "Mrays/s & Msamples/s at relMSE≤1e-3; batched/served vs plain; p50/p99 latency"
 
 
@@ -410,6 +411,12 @@ def main():
                 "kernel": "k_extend", "avg_launch_ms": round(ext_ms / max(1, ext_launches), 4),
                 "launches": ext_launches, "bytes_per_ray": round(bytes_per_ray, 1),
                 "nodes_per_ray": round(n_node, 2), "tris_per_ray": round(n_tri, 2)}
+    # The same kernel against the FP32 vector roofline: 12 FLOP per child slab test of a
+    # 4-wide node and 40 per FP32 triangle test (the survey's per-query accounting).
+    flop_per_ray = 12 * 4 * n_node + 40 * n_tri
+    tflops = flop_per_ray * pr.extend_rays / (ext_ms * 1e-3) / 1e12
+    valu = {"achieved": round(tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tflops / PEAK_FP32_TFLOPS, 5), "flop_per_ray": round(flop_per_ray, 1), "kernel": "k_extend"}
     kernels = {k: {"launches": v[0], "ms": round(v[1], 2)} for k, v in kt.items()}
     log(f"kernels: {kernels}")
     ctx.release_scene(sid)
@@ -453,6 +460,7 @@ def main():
         "exact_fallbacks_per_mray": round(pr.exact_fallbacks / max(1, pr.extend_rays + pr.shadow_rays) * 1e6, 2),
         "exact_rays_per_mray": round(pr.exact_rays / max(1, pr.extend_rays + pr.shadow_rays) * 1e6, 2),
         "roofline": roofline,
+        "valu_roofline": valu,
         "kernels_ms": kernels,
         "quality": qual,
         "batched": batched,
