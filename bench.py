@@ -129,6 +129,25 @@ def film_encode_form(ctx, film, with_reference: bool) -> dict:
     return out
 
 
+def scene_cache_form(ctx, w) -> dict:
+    """Making the workload's scene resident from a scene cache file (device-format arrays,
+    no finalize or BVH build) against building it: the served form's time to residency."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        path = pathlib.Path(d) / "scene.crgscene"
+        sid, info = ctx.upload_scene(w.scene)
+        t = time.perf_counter()
+        ctx.save_scene(sid, path)
+        save_s = time.perf_counter() - t
+        ctx.release_scene(sid)
+        t = time.perf_counter()
+        lid, linfo = ctx.load_scene(path)
+        load_s = time.perf_counter() - t
+        ctx.release_scene(lid)
+        return {"bytes": path.stat().st_size, "save_seconds": round(save_s, 3), "load_seconds": round(load_s, 3),
+                "build_and_upload_seconds": round(info.build_seconds + info.upload_seconds, 3)}
+
+
 def _pct(xs, q):
     import numpy as np
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q))
@@ -437,6 +456,8 @@ def main():
 
     film_enc = film_encode_form(ctx, res.pixels, not args.no_cpu)
     log(f"film encode: {film_enc}")
+    cache = scene_cache_form(ctx, w)
+    log(f"scene cache: {cache}")
 
     cpu = None
     host_build = {"triangles": int(info.triangles), "seconds": round(info.build_seconds, 3),
@@ -469,6 +490,7 @@ def main():
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
         "host_build": host_build,
         "film_encode": film_enc,
+        "scene_cache": cache,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -63,6 +63,8 @@ def load_library(path: Optional[pathlib.Path] = None) -> C.CDLL:
         "crg_autotune": (i32, [vp, C.POINTER(abi.Request), C.POINTER(abi.Tuning), C.POINTER(C.c_double)]),
         "crg_set_profiling": (i32, [vp, i32]),
         "crg_kernel_times": (i32, [vp, C.POINTER(abi.KernelTime), i32, C.POINTER(i32)]),
+        "crg_scene_save": (i32, [vp, u64, C.c_char_p]),
+        "crg_scene_load": (i32, [vp, C.c_char_p, C.POINTER(u64), C.POINTER(abi.SceneInfo)]),
         "crg_film_encode": (i32, [vp, C.POINTER(C.c_float), i32, i32, C.c_float, C.POINTER(C.c_uint8),
                                   C.POINTER(C.c_uint8), u64, C.POINTER(u64)]),
         "crg_film_hdr_bound": (u64, [i32, i32]),
@@ -235,6 +237,16 @@ class Context:
         n = C.c_int()
         self._check(self.lib.crg_kernel_times(self.handle, arr, 16, C.byref(n)))
         return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(min(n.value, 16))}
+
+    # --- scene cache file (device-format arrays of a resident scene)
+    def save_scene(self, scene_id: int, path) -> None:
+        self._check(self.lib.crg_scene_save(self.handle, scene_id, str(path).encode()))
+
+    def load_scene(self, path):
+        sid = C.c_uint64()
+        info = abi.SceneInfo()
+        self._check(self.lib.crg_scene_load(self.handle, str(path).encode(), C.byref(sid), C.byref(info)))
+        return sid.value, info
 
     # --- film encode (io::writePng / io::writeHdr, reference src/io/image.cpp:201-236)
     def film_png8(self, film: np.ndarray, exposure: float = 0.0) -> np.ndarray:

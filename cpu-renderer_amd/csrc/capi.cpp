@@ -7,7 +7,13 @@
 // the reference's wording.
 #include <hip/hip_runtime.h>
 
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <atomic>
+#include <cerrno>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -27,6 +33,7 @@
 #include "film_host.h"
 #include "host_scene.h"
 #include "kernels.h"
+#include "parallel.h"
 
 using namespace cr;
 
@@ -119,6 +126,241 @@ int status_of_wait(Engine& eng, uint64_t ticket, double timeout, crg_stats* stat
     return rc;
 }
 
+// One array of device-format records in host memory (a vector, or a section
+// of a scene cache file, which need not be aligned for the record type).
+struct Section {
+    const void* p = nullptr;
+    size_t count = 0;
+    size_t bytes = 0;
+};
+template <class T>
+Section section(const std::vector<T>& v) {
+    return {v.data(), v.size(), v.size() * sizeof(T)};
+}
+
+// The device-format arrays of a scene, wherever they are held on the host.
+struct SceneSections {
+    uint64_t binary_nodes = 0;
+    Section nodes, prims, prim_ng, aux, materials, uvs, tangents, texels, images, textures;
+};
+
+SceneSections sections_of(const SceneArrays& a) {
+    return {a.bvh.binary_nodes, section(a.bvh.nodes), section(a.prims),  section(a.prim_ng),
+            section(a.aux),     section(a.materials), section(a.uvs),    section(a.tangents),
+            section(a.texels),  section(a.images),    section(a.textures)};
+}
+
+void upload_section(DevBuf& b, const Section& s, hipStream_t st) {
+    b.alloc(s.bytes);
+    if (s.bytes) check(hipMemcpyAsync(b.p, s.p, s.bytes, hipMemcpyHostToDevice, st), "upload");
+}
+
+// Uploads the device-format arrays of a scene; the caller holds ctx->mu.
+// prim_to_tri / tri_to_prim map between leaf order and input order.
+std::shared_ptr<SceneRes> make_resident(crg_ctx* ctx, const SceneSections& a, std::vector<uint32_t> prim_to_tri,
+                                        std::vector<uint32_t> tri_to_prim, SceneLights lights, uint64_t n) {
+    auto res = std::make_shared<SceneRes>();
+    res->triangles = n;
+    res->binary_nodes = a.binary_nodes;
+    res->prim_to_tri = std::move(prim_to_tri);
+    res->tri_to_prim = std::move(tri_to_prim);
+    res->lights = std::move(lights);
+    res->texel_count = a.texels.count;
+    res->image_count = a.images.count;
+    res->texture_count = a.textures.count;
+    res->material_count = a.materials.count;
+    const std::vector<uint32_t> ranks = light_ranks(res->lights, res->tri_to_prim, n);
+    hipStream_t st = ctx->stream;
+    upload_section(res->nodes, a.nodes, st);
+    upload_section(res->prims, a.prims, st);
+    upload_section(res->prim_ng, a.prim_ng, st);
+    upload_section(res->aux, a.aux, st);
+    upload_section(res->materials, a.materials, st);
+    upload(res->light_rank, ranks, st);
+    upload(res->punctual, res->lights.punctual, st);
+    upload_section(res->uvs, a.uvs, st);
+    upload_section(res->tangents, a.tangents, st);
+    upload_section(res->texels, a.texels, st);
+    upload_section(res->images, a.images, st);
+    upload_section(res->textures, a.textures, st);
+    res->host = DevScene{res->nodes.as<WNode>(),
+                         res->prims.as<PrimRec>(),
+                         res->prim_ng.as<float4>(),
+                         res->aux.as<PrimAux>(),
+                         res->materials.as<DevMaterial>(),
+                         res->light_rank.as<uint32_t>(),
+                         res->punctual.as<DevPunctual>(),
+                         a.uvs.count ? res->uvs.as<float4>() : nullptr,
+                         a.tangents.count ? res->tangents.as<float4>() : nullptr,
+                         res->texels.as<float4>(),
+                         res->images.as<DevImage>(),
+                         res->textures.as<DevTexture>(),
+                         uint32_t(a.nodes.count),
+                         uint32_t(n),
+                         uint32_t(res->lights.punctual.size()),
+                         0};
+    res->desc.alloc(sizeof(DevScene));
+    check(hipMemcpyAsync(res->desc.p, &res->host, sizeof(DevScene), hipMemcpyHostToDevice, st), "upload");
+    check(hipStreamSynchronize(st), "scene upload");
+    res->bytes = res->nodes.bytes + res->prims.bytes + res->prim_ng.bytes + res->aux.bytes + res->materials.bytes +
+                 res->light_rank.bytes + res->punctual.bytes + res->uvs.bytes + res->tangents.bytes +
+                 res->texels.bytes + res->images.bytes + res->textures.bytes + res->desc.bytes;
+    return res;
+}
+
+crg_scene_info scene_info(const SceneRes& r) {
+    crg_scene_info info{};
+    info.triangles = r.triangles;
+    info.bvh_nodes = r.binary_nodes;
+    info.wide_nodes = r.host.node_count;
+    info.device_bytes = r.bytes;
+    return info;
+}
+
+// ---- scene cache file: the device-format arrays of a resident scene, so a
+// server can make a scene resident again without finalize and BVH build.
+constexpr char kCacheMagic[8] = {'C', 'R', 'G', 'S', 'C', 'N', '1', 0};
+
+struct CacheWriter {
+    std::vector<uint8_t> out;
+    void raw(const void* p, size_t n) {
+        const uint8_t* b = static_cast<const uint8_t*>(p);
+        out.insert(out.end(), b, b + n);
+    }
+    template <class T>
+    void pod(const T& v) { raw(&v, sizeof(T)); }
+    template <class T>
+    void vec(const std::vector<T>& v) {
+        pod(uint64_t(v.size()));
+        raw(v.data(), v.size() * sizeof(T));
+    }
+};
+
+// Reads the sections of a cache file held in memory; `section` returns a view
+// into the file's bytes, `vec` a copy.
+struct CacheReader {
+    const uint8_t* in;
+    size_t size;
+    size_t pos;
+    std::string label;
+    void raw(void* p, size_t n) {
+        if (n > size - pos) throw std::runtime_error(label + ": truncated");
+        std::memcpy(p, in + pos, n);
+        pos += n;
+    }
+    template <class T>
+    T pod() {
+        T v;
+        raw(&v, sizeof(T));
+        return v;
+    }
+    template <class T>
+    Section section() {
+        const uint64_t n = pod<uint64_t>();
+        if (n > (size - pos) / sizeof(T)) throw std::runtime_error(label + ": truncated");
+        const Section s{in + pos, size_t(n), size_t(n) * sizeof(T)};
+        pos += s.bytes;
+        return s;
+    }
+    template <class T>
+    std::vector<T> vec() {
+        const Section s = section<T>();
+        std::vector<T> v(s.count);
+        if (s.bytes) std::memcpy(v.data(), s.p, s.bytes);
+        return v;
+    }
+};
+
+// Checksum of a cache file's body: 1 MiB blocks hashed in parallel, each as
+// four interleaved lanes of an FNV-1a-style step over 64-bit words (xor,
+// multiply by the FNV prime, rotate: a bijection of the lane for each word, so
+// any change to one word changes the block's hash), tail bytes in lane 0; then
+// FNV-1a over the byte count and the block hashes.
+constexpr uint64_t kFnvBasis = 0xcbf29ce484222325ull, kFnvPrime = 0x100000001b3ull;
+constexpr size_t kHashBlock = size_t(1) << 20;
+
+uint64_t fnv_step(uint64_t h, uint64_t w) {
+    h = (h ^ w) * kFnvPrime;
+    return (h << 23) | (h >> 41);
+}
+
+uint64_t block_hash(const uint8_t* p, size_t n) {
+    uint64_t h[4] = {kFnvBasis, kFnvBasis + 1, kFnvBasis + 2, kFnvBasis + 3};
+    size_t i = 0;
+    for (; i + 32 <= n; i += 32) {
+        uint64_t w[4];
+        std::memcpy(w, p + i, 32);
+        for (int k = 0; k < 4; ++k) h[k] = fnv_step(h[k], w[k]);
+    }
+    for (; i < n; ++i) h[0] = fnv_step(h[0], p[i]);
+    uint64_t r = kFnvBasis;
+    for (int k = 0; k < 4; ++k) r = fnv_step(r, h[k]);
+    return r;
+}
+
+uint64_t cache_checksum(const uint8_t* p, size_t n) {
+    const size_t blocks = (n + kHashBlock - 1) / kHashBlock;
+    std::vector<uint64_t> bh(blocks);
+    parallel_chunks(blocks, 8, [&](size_t b, size_t e, size_t) {
+        for (size_t i = b; i < e; ++i) bh[i] = block_hash(p + i * kHashBlock, std::min(kHashBlock, n - i * kHashBlock));
+    });
+    uint64_t r = fnv_step(kFnvBasis, uint64_t(n));
+    for (uint64_t h : bh) r = fnv_step(r, h);
+    return r;
+}
+
+// A whole file in host memory, read by parallel positioned reads.
+struct FileBytes {
+    std::unique_ptr<uint8_t[]> p;
+    size_t n = 0;
+};
+
+FileBytes read_file(const char* path, const std::string& label) {
+    const int fd = ::open(path, O_RDONLY | O_CLOEXEC);
+    if (fd < 0) throw std::runtime_error(label + ": cannot open");
+    struct stat st;
+    if (::fstat(fd, &st) != 0 || !S_ISREG(st.st_mode)) {
+        ::close(fd);
+        throw std::runtime_error(label + ": cannot open");
+    }
+    FileBytes f;
+    f.n = size_t(st.st_size);
+    f.p.reset(new uint8_t[std::max<size_t>(f.n, 1)]);
+    std::atomic<bool> ok{true};
+    parallel_chunks(f.n, size_t(16) << 20, [&](size_t b, size_t e, size_t) {
+        while (b < e) {
+            const ssize_t k = ::pread(fd, f.p.get() + b, std::min<size_t>(e - b, size_t(1) << 30), off_t(b));
+            if (k < 0 && errno == EINTR) continue;
+            if (k <= 0) {
+                ok = false;
+                return;
+            }
+            b += size_t(k);
+        }
+    });
+    ::close(fd);
+    if (!ok) throw std::runtime_error(label + ": truncated");
+    return f;
+}
+
+const uint32_t kRecordSizes[8] = {sizeof(WNode), sizeof(PrimRec), sizeof(PrimAux), sizeof(DevMaterial),
+                                  sizeof(DevPunctual), sizeof(DevImage), sizeof(DevTexture), sizeof(float4)};
+
+template <class T>
+std::vector<T> download(const DevBuf& b, uint64_t count, hipStream_t st) {
+    std::vector<T> v(count);
+    if (count) check(hipMemcpyAsync(v.data(), b.p, count * sizeof(T), hipMemcpyDeviceToHost, st), "download");
+    return v;
+}
+
+// Registers a resident scene; the caller holds ctx->mu.
+uint64_t add_scene(crg_ctx* ctx, std::shared_ptr<SceneRes> res) {
+    ctx->engine->add_resident(int64_t(res->bytes));
+    const uint64_t id = ctx->next_id++;
+    ctx->scenes[id] = std::move(res);
+    return id;
+}
+
 }  // namespace
 
 extern "C" {
@@ -173,63 +415,133 @@ int crg_scene_upload(crg_ctx* ctx, const crg_scene_desc* desc, uint64_t* scene_i
         const auto t0 = std::chrono::steady_clock::now();
         HostScene hs = finalize_scene(*desc);
         SceneArrays arr = scene_arrays(hs);
-        auto res = std::make_shared<SceneRes>();
-        const size_t n = hs.tris.size();
-        res->triangles = n;
-        res->prim_to_tri = arr.bvh.order;
-        res->tri_to_prim = arr.tri_to_prim;
-        res->lights = scene_lights(hs);
-        const std::vector<uint32_t> ranks = light_ranks(res->lights, res->tri_to_prim, n);
+        SceneLights lights = scene_lights(hs);
         const auto t1 = std::chrono::steady_clock::now();
         std::lock_guard<std::mutex> lock(ctx->mu);
-        hipStream_t st = ctx->stream;
-        upload(res->nodes, arr.bvh.nodes, st);
-        upload(res->prims, arr.prims, st);
-        upload(res->prim_ng, arr.prim_ng, st);
-        upload(res->aux, arr.aux, st);
-        upload(res->materials, arr.materials, st);
-        upload(res->light_rank, ranks, st);
-        upload(res->punctual, res->lights.punctual, st);
-        upload(res->uvs, arr.uvs, st);
-        upload(res->tangents, arr.tangents, st);
-        upload(res->texels, arr.texels, st);
-        upload(res->images, arr.images, st);
-        upload(res->textures, arr.textures, st);
-        res->host = DevScene{res->nodes.as<WNode>(),
-                             res->prims.as<PrimRec>(),
-                             res->prim_ng.as<float4>(),
-                             res->aux.as<PrimAux>(),
-                             res->materials.as<DevMaterial>(),
-                             res->light_rank.as<uint32_t>(),
-                             res->punctual.as<DevPunctual>(),
-                             arr.uvs.empty() ? nullptr : res->uvs.as<float4>(),
-                             arr.tangents.empty() ? nullptr : res->tangents.as<float4>(),
-                             res->texels.as<float4>(),
-                             res->images.as<DevImage>(),
-                             res->textures.as<DevTexture>(),
-                             uint32_t(arr.bvh.nodes.size()),
-                             uint32_t(n),
-                             uint32_t(res->lights.punctual.size()),
-                             0};
-        res->desc.alloc(sizeof(DevScene));
-        check(hipMemcpyAsync(res->desc.p, &res->host, sizeof(DevScene), hipMemcpyHostToDevice, st), "upload");
-        check(hipStreamSynchronize(st), "scene upload");
+        auto res = make_resident(ctx, sections_of(arr), std::move(arr.bvh.order), std::move(arr.tri_to_prim),
+                                 std::move(lights), hs.tris.size());
         const auto t2 = std::chrono::steady_clock::now();
-        res->bytes = res->nodes.bytes + res->prims.bytes + res->prim_ng.bytes + res->aux.bytes + res->materials.bytes +
-                     res->light_rank.bytes + res->punctual.bytes + res->uvs.bytes + res->tangents.bytes +
-                     res->texels.bytes + res->images.bytes + res->textures.bytes + res->desc.bytes;
         if (info) {
-            info->triangles = n;
-            info->bvh_nodes = arr.bvh.binary_nodes;
-            info->wide_nodes = arr.bvh.nodes.size();
-            info->device_bytes = res->bytes;
+            *info = scene_info(*res);
             info->build_seconds = std::chrono::duration<double>(t1 - t0).count();
             info->upload_seconds = std::chrono::duration<double>(t2 - t1).count();
         }
-        ctx->engine->add_resident(int64_t(res->bytes));
-        const uint64_t id = ctx->next_id++;
-        ctx->scenes[id] = std::move(res);
-        *scene_id = id;
+        *scene_id = add_scene(ctx, std::move(res));
+    });
+}
+
+int crg_scene_save(crg_ctx* ctx, uint64_t scene_id, const char* path) {
+    return guarded(ctx, [&] {
+        if (!path) throw std::invalid_argument("null path");
+        std::shared_ptr<SceneRes> r;
+        CacheWriter w;
+        {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            auto it = ctx->scenes.find(scene_id);
+            if (it == ctx->scenes.end()) throw std::out_of_range("unknown scene handle");
+            r = it->second;
+            check(hipSetDevice(ctx->device), "hipSetDevice");
+            hipStream_t st = ctx->stream;
+            const uint64_t n = r->triangles;
+            w.raw(kCacheMagic, 8);
+            w.raw(kRecordSizes, sizeof(kRecordSizes));
+            w.pod(n);
+            w.pod(r->binary_nodes);
+            auto nodes = download<WNode>(r->nodes, r->host.node_count, st);
+            auto prims = download<PrimRec>(r->prims, n, st);
+            auto ng = download<float4>(r->prim_ng, n, st);
+            auto aux = download<PrimAux>(r->aux, n, st);
+            auto mats = download<DevMaterial>(r->materials, r->material_count, st);
+            auto uvs = download<float4>(r->uvs, r->host.uvs ? 3 * n : 0, st);
+            auto tans = download<float4>(r->tangents, r->host.tangents ? 3 * n : 0, st);
+            auto texels = download<float4>(r->texels, r->texel_count, st);
+            auto images = download<DevImage>(r->images, r->image_count, st);
+            auto textures = download<DevTexture>(r->textures, r->texture_count, st);
+            check(hipStreamSynchronize(st), "scene download");
+            w.out.reserve(4096 + (nodes.size() * sizeof(WNode) + n * (sizeof(PrimRec) + sizeof(PrimAux) + 24) +
+                                              (uvs.size() + tans.size() + ng.size() + texels.size()) * sizeof(float4) +
+                                              mats.size() * sizeof(DevMaterial) + images.size() * sizeof(DevImage) +
+                                              textures.size() * sizeof(DevTexture)));
+            w.vec(nodes);
+            w.vec(prims);
+            w.vec(ng);
+            w.vec(aux);
+            w.vec(mats);
+            w.vec(uvs);
+            w.vec(tans);
+            w.vec(texels);
+            w.vec(images);
+            w.vec(textures);
+        }
+        w.vec(r->prim_to_tri);
+        w.vec(r->tri_to_prim);
+        w.pod(r->lights.radius2);
+        w.vec(r->lights.punctual_weight);
+        w.vec(r->lights.emissive);
+        w.vec(r->lights.emissive_weight);
+        w.vec(r->lights.punctual);
+        w.pod(cache_checksum(w.out.data(), w.out.size()));
+        std::FILE* f = std::fopen(path, "wb");
+        const bool ok = f && std::fwrite(w.out.data(), 1, w.out.size(), f) == w.out.size();
+        if (!f || std::fclose(f) != 0 || !ok) throw std::runtime_error(std::string("Failed writing scene cache '") + path + "'");
+    });
+}
+
+int crg_scene_load(crg_ctx* ctx, const char* path, uint64_t* scene_id, crg_scene_info* info) {
+    return guarded(ctx, [&] {
+        if (!path || !scene_id) throw std::invalid_argument("null path or scene id");
+        const std::string label = std::string("Scene cache '") + path + "'";
+        const auto t0 = std::chrono::steady_clock::now();
+        const FileBytes file = read_file(path, label);
+        const uint8_t* bytes = file.p.get();
+        if (file.n < 8 + sizeof(kRecordSizes) + 24 || std::memcmp(bytes, kCacheMagic, 8) != 0)
+            throw std::runtime_error(label + ": not a scene cache of this library");
+        uint64_t stored;
+        std::memcpy(&stored, bytes + file.n - 8, 8);
+        if (stored != cache_checksum(bytes, file.n - 8)) throw std::runtime_error(label + ": checksum mismatch");
+        CacheReader rd{bytes, file.n - 8, 8, label};
+        uint32_t sizes[8];
+        rd.raw(sizes, sizeof(sizes));
+        if (std::memcmp(sizes, kRecordSizes, sizeof(sizes)) != 0)
+            throw std::runtime_error(label + ": written by a build with other record layouts");
+        const uint64_t n = rd.pod<uint64_t>();
+        SceneSections sec;
+        sec.binary_nodes = rd.pod<uint64_t>();
+        sec.nodes = rd.section<WNode>();
+        sec.prims = rd.section<PrimRec>();
+        sec.prim_ng = rd.section<float4>();
+        sec.aux = rd.section<PrimAux>();
+        sec.materials = rd.section<DevMaterial>();
+        sec.uvs = rd.section<float4>();
+        sec.tangents = rd.section<float4>();
+        sec.texels = rd.section<float4>();
+        sec.images = rd.section<DevImage>();
+        sec.textures = rd.section<DevTexture>();
+        std::vector<uint32_t> order = rd.vec<uint32_t>();
+        std::vector<uint32_t> tri_to_prim = rd.vec<uint32_t>();
+        SceneLights lights;
+        lights.radius2 = rd.pod<double>();
+        lights.punctual_weight = rd.vec<double>();
+        lights.emissive = rd.vec<uint32_t>();
+        lights.emissive_weight = rd.vec<double>();
+        lights.punctual = rd.vec<DevPunctual>();
+        if (rd.pos != rd.size || sec.prims.count != n || sec.prim_ng.count != n || sec.aux.count != n ||
+            order.size() != n || tri_to_prim.size() != n || (sec.uvs.count && sec.uvs.count != 3 * n) ||
+            (sec.tangents.count && sec.tangents.count != 3 * n) || (n && !sec.nodes.count) ||
+            lights.emissive.size() != lights.emissive_weight.size() ||
+            lights.punctual.size() != lights.punctual_weight.size())
+            throw std::runtime_error(label + ": inconsistent section sizes");
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        const auto t1 = std::chrono::steady_clock::now();
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        auto res = make_resident(ctx, sec, std::move(order), std::move(tri_to_prim), std::move(lights), n);
+        const auto t2 = std::chrono::steady_clock::now();
+        if (info) {
+            *info = scene_info(*res);
+            info->build_seconds = std::chrono::duration<double>(t1 - t0).count();
+            info->upload_seconds = std::chrono::duration<double>(t2 - t1).count();
+        }
+        *scene_id = add_scene(ctx, std::move(res));
     });
 }
 

@@ -40,7 +40,8 @@ struct SceneRes {
     SceneLights lights;
     std::vector<uint32_t> tri_to_prim;
     std::vector<uint32_t> prim_to_tri;
-    uint64_t triangles = 0, bytes = 0;
+    uint64_t triangles = 0, bytes = 0, binary_nodes = 0;
+    uint64_t texel_count = 0, image_count = 0, texture_count = 0, material_count = 0;  // device array lengths
 };
 
 struct EnvRes {

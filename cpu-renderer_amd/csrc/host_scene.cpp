@@ -13,7 +13,7 @@
 #include <sstream>
 #include <thread>
 
-#include <sched.h>
+#include "parallel.h"
 
 namespace cr {
 namespace {
@@ -62,39 +62,6 @@ bool uv_directions(const HTriangle& t, int set, DV& tangent, DV& bitangent) {
     tangent = (e1 * dv2 - e2 * dv1) * sign;
     bitangent = (e2 * du1 - e1 * du2) * sign;
     return nonzero3_(unit_of(tangent)) && nonzero3_(unit_of(bitangent));
-}
-
-// Host threads this process may run on (its CPU affinity, which can be far
-// below the machine's CPU count in a container).
-size_t host_threads() {
-    static const size_t n = [] {
-        cpu_set_t set;
-        if (sched_getaffinity(0, sizeof(set), &set) == 0) return size_t(std::max(1, CPU_COUNT(&set)));
-        return size_t(std::max(1u, std::thread::hardware_concurrency()));
-    }();
-    return n;
-}
-
-// Runs f(begin, end, chunk) over [0, n) in up to host_threads() contiguous
-// chunks of at least min_chunk items (one chunk: on this thread).
-template <class F>
-void parallel_chunks(size_t n, size_t min_chunk, F&& f) {
-    const size_t threads = host_threads();
-    const size_t chunks = std::max<size_t>(1, std::min(threads, n / std::max<size_t>(min_chunk, 1)));
-    if (chunks == 1) {
-        f(size_t(0), n, size_t(0));
-        return;
-    }
-    std::vector<std::thread> pool;
-    pool.reserve(chunks - 1);
-    for (size_t c = 1; c < chunks; ++c) pool.emplace_back([&, c] { f(c * n / chunks, (c + 1) * n / chunks, c); });
-    f(size_t(0), n / chunks, size_t(0));
-    for (auto& t : pool) t.join();
-}
-
-size_t chunk_count(size_t n, size_t min_chunk) {
-    const size_t threads = host_threads();
-    return std::max<size_t>(1, std::min(threads, n / std::max<size_t>(min_chunk, 1)));
 }
 
 bool fin(float v) { return std::isfinite(v); }

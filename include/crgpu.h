@@ -191,6 +191,18 @@ const char* crg_version(void);
 /* Scene::finalize + Bvh::build on the host, then upload (reference
  * src/geometry.cpp:296-397, :463-550).  Returns a resident handle. */
 int crg_scene_upload(crg_ctx* ctx, const crg_scene_desc* desc, uint64_t* scene_id, crg_scene_info* info);
+/*
+ * Scene cache file.  No counterpart in the reference (its scenes are rebuilt
+ * from glTF on every run); for servers that keep many scenes: crg_scene_save
+ * writes the device-format arrays of a resident scene (wide BVH, primitive and
+ * shading records, materials, textures, light data) with a checksum;
+ * crg_scene_load makes such a file resident again without Scene::finalize or
+ * the BVH build (info->build_seconds then counts the file read).  Renders of a
+ * loaded scene are bit-identical to renders of the scene it was saved from.
+ * A file is only valid for builds with the same record layouts.
+ */
+int crg_scene_save(crg_ctx* ctx, uint64_t scene_id, const char* path);
+int crg_scene_load(crg_ctx* ctx, const char* path, uint64_t* scene_id, crg_scene_info* info);
 int crg_scene_release(crg_ctx* ctx, uint64_t scene_id);
 /* Environment construction (reference src/render.cpp:12-42). */
 int crg_env_upload(crg_ctx* ctx, const crg_env_desc* desc, uint64_t* env_id, uint64_t* device_bytes);
