@@ -353,6 +353,24 @@ std::vector<T> download(const DevBuf& b, uint64_t count, hipStream_t st) {
     return v;
 }
 
+// The sampling-angle table of `device`: built on the host once per process and
+// uploaded once per device, then kept until the process ends (128 MiB), shared
+// by every context on that device.
+void ensure_turn_table(int device) {
+    static std::mutex mu;
+    static std::map<int, void*> tables;
+    std::lock_guard<std::mutex> lock(mu);
+    if (tables.count(device)) return;
+    const size_t bytes = size_t(kTurnEntries) * sizeof(TurnCS);
+    std::vector<TurnCS> host(kTurnEntries);
+    fill_turn_table(host.data());
+    void* p = nullptr;
+    check(hipMalloc(&p, bytes), "turn table");
+    check(hipMemcpy(p, host.data(), bytes, hipMemcpyHostToDevice), "turn table upload");
+    check(set_turn_table(static_cast<const TurnCS*>(p)), "turn table symbol");
+    tables[device] = p;
+}
+
 // Registers a resident scene; the caller holds ctx->mu.
 uint64_t add_scene(crg_ctx* ctx, std::shared_ptr<SceneRes> res) {
     ctx->engine->add_resident(int64_t(res->bytes));
@@ -380,6 +398,7 @@ int crg_context_create(int device, crg_ctx** out) {
     ctx->device = device;
     try {
         check(hipSetDevice(device), "hipSetDevice");
+        ensure_turn_table(device);
         check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
         ctx->engine = std::make_unique<Engine>(device, unsigned(prop.multiProcessorCount));
     } catch (const std::exception&) {

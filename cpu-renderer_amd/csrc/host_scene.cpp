@@ -64,6 +64,23 @@ bool uv_directions(const HTriangle& t, int set, DV& tangent, DV& bitangent) {
     return nonzero3_(unit_of(tangent)) && nonzero3_(unit_of(bitangent));
 }
 
+}  // namespace
+
+void fill_turn_table(TurnCS* out) {
+    parallel_chunks(kTurnEntries, 1 << 16, [&](size_t b, size_t e, size_t) {
+        for (size_t k = b; k < e; ++k) {
+            const float u = float(uint32_t(k)) * (1.0f / 16777216.0f);
+            const float phi = 2 * kHostPi * u;
+            out[k] = TurnCS{std::cos(phi), std::sin(phi)};
+        }
+    });
+}
+
+namespace {
+
+// sRGB to linear of one channel, the reference's expression (reference src/math.h:61).
+float srgb_to_linear(float v) { return v <= 0.04045f ? v / 12.92f : std::pow((v + 0.055f) / 1.055f, 2.4f); }
+
 bool fin(float v) { return std::isfinite(v); }
 bool fin3(const float* v) { return fin(v[0]) && fin(v[1]) && fin(v[2]); }
 bool nonzero3(HVec3 v) { return v.x != 0 || v.y != 0 || v.z != 0; }
@@ -882,9 +899,33 @@ SceneArrays scene_arrays(const HostScene& hs) {
         x.area = t.area;
     }
     });
-    for (const HImage& im : hs.images) {
-        a.images.push_back(DevImage{uint64_t(a.texels.size()), im.width, im.height});
+    // Images sampled through a colour slot (base colour, emission) also get an
+    // sRGB-decoded copy, decoded per texel with this host's powf exactly as the
+    // reference decodes each fetched texel (reference src/math.h:61,
+    // src/geometry.cpp:260-270), so the device never evaluates pow.
+    std::vector<char> srgb_use(hs.images.size(), 0);
+    for (const crg_material& m : hs.materials)
+        for (const crg_texture_slot* slot : {&m.base_color_tex, &m.emissive_tex})
+            if (slot->texture >= 0 && size_t(slot->texture) < hs.textures.size()) {
+                const int32_t img = hs.textures[size_t(slot->texture)].image;
+                if (img >= 0 && size_t(img) < srgb_use.size()) srgb_use[size_t(img)] = 1;
+            }
+    for (size_t i = 0; i < hs.images.size(); ++i) {
+        const HImage& im = hs.images[i];
+        DevImage d{uint64_t(a.texels.size()), uint64_t(a.texels.size()), im.width, im.height};
         a.texels.insert(a.texels.end(), im.texels.begin(), im.texels.end());
+        if (srgb_use[i]) {
+            d.srgb_offset = a.texels.size();
+            a.texels.resize(a.texels.size() + im.texels.size());
+            float4* out = a.texels.data() + d.srgb_offset;
+            parallel_chunks(im.texels.size(), 65536, [&](size_t b, size_t e, size_t) {
+                for (size_t k = b; k < e; ++k) {
+                    const float4 v = im.texels[k];
+                    out[k] = make_float4(srgb_to_linear(v.x), srgb_to_linear(v.y), srgb_to_linear(v.z), v.w);
+                }
+            });
+        }
+        a.images.push_back(d);
     }
     for (const crg_texture& t : hs.textures) a.textures.push_back(DevTexture{t.image, t.wrap_s, t.wrap_t, t.linear});
     return a;

@@ -124,4 +124,9 @@ struct SceneArrays {
 };
 SceneArrays scene_arrays(const HostScene& hs);
 
+// (cos, sin)(2*pi*k/2^24) for every 24-bit draw k, with this host's cosf/sinf
+// on the reference's float expression (reference src/bsdf.cpp, src/render.cpp
+// sampling sites); out holds kTurnEntries entries.
+void fill_turn_table(TurnCS* out);
+
 }  // namespace cr

@@ -71,5 +71,8 @@ hipError_t launch_film_encode(const float* film, uint64_t npix, uint32_t width, 
 // and its byte count into lens[row*4+plane].
 hipError_t launch_film_rle(const uint8_t* rgbe, uint32_t width, uint32_t height, uint8_t* streams, uint32_t* lens,
                            hipStream_t st);
+// Points the kernels of the current device at its sampling-angle table
+// (kTurnEntries entries in device memory, filled by fill_turn_table).
+hipError_t set_turn_table(const TurnCS* table);
 
 }  // namespace cr

@@ -837,6 +837,10 @@ hipError_t launch_film_rle(const uint8_t* rgbe, uint32_t width, uint32_t height,
     return hipGetLastError();
 }
 
+hipError_t set_turn_table(const TurnCS* table) {
+    return hipMemcpyToSymbol(HIP_SYMBOL(g_turn_table), &table, sizeof(table));
+}
+
 hipError_t launch_accumulate(const DevJob* jobs, uint32_t job, uint64_t npix, float* film, hipStream_t st) {
     hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(npix, 256)), dim3(256), 0, st, jobs, job, film);
     return hipGetLastError();

@@ -41,6 +41,13 @@ CR_HDI const CR_GLOBAL T& gref(const T* p) {
     return *(const CR_GLOBAL T*)p;
 }
 
+// (cos, sin) of the sampling angle phi = 2*pi*u for one 24-bit draw u = k/2^24;
+// the table holds all kTurnEntries draws, evaluated once on the host.
+struct alignas(8) TurnCS {
+    float c, s;
+};
+constexpr uint32_t kTurnEntries = 1u << 24;
+
 // prim_ng entries read as four plain floats (geometric normal, unused w).
 struct PrimAuxNg {
     float x, y, z, w;
@@ -110,7 +117,8 @@ struct DevSlot {
 };
 
 struct DevImage {
-    uint64_t offset;  // first texel in the scene's texel array
+    uint64_t offset;       // first texel in the scene's texel array
+    uint64_t srgb_offset;  // first texel of the sRGB-decoded copy (images sampled by colour slots)
     int32_t width, height;
 };
 

@@ -173,6 +173,23 @@ CR_OUTLINE Surf surface_at(const DevScene& sc, uint32_t prim, V3 ray_dir, sf::f6
     return s;
 }
 
+// ---------------------------------------------------------------- sampling angles
+// cos and sin of phi = 2*pi*u for the 24-bit draw k (u = k/2^24) come from a
+// table the host fills with its own cosf/sinf on the reference's float
+// expression (2 * pi * u): the device's cosf/sinf round differently on about
+// one input in five, and a direction off by one ulp can change which triangle
+// a ray hits.
+#if defined(__HIPCC__)
+static __device__ const TurnCS* g_turn_table;
+#else
+inline const TurnCS* g_turn_table = nullptr;
+#endif
+CR_HDI void turn_cos_sin(uint32_t k, float& c, float& s) {
+    const TurnCS t = g_turn_table[k];
+    c = t.c;
+    s = t.s;
+}
+
 // ---------------------------------------------------------------- BSDF
 CR_HDI float fresnel_dielectric(float cosine, float eta_i, float eta_t) {
     if (eta_i == eta_t) return 0;
@@ -217,14 +234,16 @@ CR_HDI float smith_g1(float alpha, V3 v) {
     return 2 / (1 + sqrtf(1 + alpha * alpha * (v.x * v.x + v.y * v.y) / cos2));
 }
 
-CR_HDI V3 visible_normal(V3 wo, float alpha, float u, float v) {
+CR_HDI V3 visible_normal(V3 wo, float alpha, float u, uint32_t kv) {
     const V3 vh = normalize(V3{alpha * wo.x, alpha * wo.y, wo.z});
     const float lensq = vh.x * vh.x + vh.y * vh.y;
     const V3 t1 = lensq > 0 ? V3{-vh.y, vh.x, 0} / sqrtf(lensq) : V3{1, 0, 0};
     const V3 t2 = cross(vh, t1);
-    const float r = sqrtf(u), phi = 2 * kPi * v;
-    const float x = r * cosf(phi);
-    float y = r * sinf(phi);
+    const float r = sqrtf(u);
+    float cp, sp;
+    turn_cos_sin(kv, cp, sp);
+    const float x = r * cp;
+    float y = r * sp;
     const float s = 0.5f * (1 + vh.z);
     y = (1 - s) * sqrtf(fmax_(0.0f, 1 - x * x)) + s * y;
     const V3 nh = t1 * x + t2 * y + vh * sqrtf(fmax_(0.0f, 1 - x * x - y * y));
@@ -355,8 +374,10 @@ CR_OUTLINE BsdfSample sample_bsdf(const Surf& s, V3 wo, Rng& rng) {
     V3 wi;
     const float lobe = rng.uniform();
     if (lobe < c.diffuse_p) {
-        const float u = rng.uniform(), phi = 2 * kPi * rng.uniform();
-        wi = V3{sqrtf(u) * cosf(phi), sqrtf(u) * sinf(phi), sqrtf(1 - u)};
+        const float u = rng.uniform();
+        float cp, sp;
+        turn_cos_sin(rng.bits24(), cp, sp);
+        wi = V3{sqrtf(u) * cp, sqrtf(u) * sp, sqrtf(1 - u)};
     } else if (lobe < c.diffuse_p + c.delta_t_p) {
         out.direction = -wo;
         out.delta = true;
@@ -365,8 +386,9 @@ CR_OUTLINE BsdfSample sample_bsdf(const Surf& s, V3 wo, Rng& rng) {
         out.weight = s.base * ((1 - s.metallic) * s.transmission / out.pdf);
         return correct_side(s, wo, out.direction, true) ? out : empty_sample();
     } else {
-        const float u = rng.uniform(), v = rng.uniform();
-        const V3 h = c.smooth ? V3{0, 0, 1} : visible_normal(c.wo, c.alpha, u, v);
+        const float u = rng.uniform();
+        const uint32_t kv = rng.bits24();
+        const V3 h = c.smooth ? V3{0, 0, 1} : visible_normal(c.wo, c.alpha, u, kv);
         const float oh = dot(c.wo, h), f = c.fresnel(oh), pr = c.reflection_probability(f, oh);
         const bool refl = rng.uniform() < pr;
         if (refl) wi = reflect(-c.wo, h);
@@ -455,9 +477,12 @@ CR_HDI uint32_t env_invert(const DevEnv& e, uint32_t k) {
 
 CR_OUTLINE V3 env_sample(const DevEnv& e, Rng& rng, float& density) {
     if (!e.is_image) {
-        const float y = 1 - 2 * rng.uniform(), phi = 2 * kPi * rng.uniform(), r = sqrtf(fmax_(0.0f, 1 - y * y));
+        const float y = 1 - 2 * rng.uniform();
+        float cp, sp;
+        turn_cos_sin(rng.bits24(), cp, sp);
+        const float r = sqrtf(fmax_(0.0f, 1 - y * y));
         density = 1 / (4 * kPi);
-        return V3{r * cosf(phi), y, r * sinf(phi)};
+        return V3{r * cp, y, r * sp};
     }
     const uint32_t index = env_invert(e, rng.bits24());
     const int x = int(index % uint32_t(e.width)), y = int(index / uint32_t(e.width));

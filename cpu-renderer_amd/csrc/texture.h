@@ -8,7 +8,6 @@
 
 namespace cr {
 
-CR_HDI float srgb_linear(float v) { return v <= 0.04045f ? v / 12.92f : powf((v + 0.055f) / 1.055f, 2.4f); }
 
 // wrappedCoordinate (reference src/geometry.cpp:77-84).
 CR_HDI sf::f64 wrap_coord(sf::f64 x, int wrap) {
@@ -69,15 +68,12 @@ CR_OUTLINE float4 sample_tex(const DevScene& sc, const DevSlot& slot, const UvPa
         wrap_coord(sf::add(sf::sub(sf::mul(c, x), sf::mul(sn, y)), sf::from_float(slot.offset[0])), tex.wrap_s);
     const sf::f64 v =
         wrap_coord(sf::add(sf::add(sf::mul(sn, x), sf::mul(c, y)), sf::from_float(slot.offset[1])), tex.wrap_t);
+    // Colour slots read the image's sRGB-decoded copy, decoded per texel on
+    // the host with its libm (the reference decodes each fetched texel).
+    const uint64_t base = srgb ? im.srgb_offset : im.offset;
     auto texel = [&](int64_t ix, int64_t iy) {
         const int tx = wrap_index(ix, im.width, tex.wrap_s), ty = wrap_index(iy, im.height, tex.wrap_t);
-        float4 val = sc.texels[im.offset + uint64_t(ty) * uint64_t(im.width) + uint64_t(tx)];
-        if (srgb) {
-            val.x = srgb_linear(val.x);
-            val.y = srgb_linear(val.y);
-            val.z = srgb_linear(val.z);
-        }
-        return val;
+        return sc.texels[base + uint64_t(ty) * uint64_t(im.width) + uint64_t(tx)];
     };
     const sf::f64 w = sf::from_i32(im.width), h = sf::from_i32(im.height);
     if (!tex.linear)

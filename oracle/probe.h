@@ -68,6 +68,17 @@ PROBE_DECLARE(orc)
 int ref_film_write(const float* film, int width, int height, float exposure, const char* png_path,
                    const char* hdr_path);
 
+/* Reference only: io::loadGltf (reference src/io/gltf.cpp:743-879) flattened into a
+ * crgpu scene descriptor by include/crgpu_cr.hpp's describe(), so real assets can be
+ * written out as fixtures; render_loaded renders the loaded cr::Scene itself. */
+void* ref_gltf_load(const char* path);
+void ref_gltf_destroy(void* gltf);
+const crg_scene_desc* ref_gltf_desc(void* gltf);
+int ref_gltf_camera_count(void* gltf);
+int ref_gltf_camera(void* gltf, int index, crg_camera* out);
+int ref_gltf_render_loaded(void* gltf, void* env, const crg_camera* camera, const crg_options* options,
+                           float* film, crg_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -89,6 +89,13 @@ class Probe:
         }
         if kind == "ref":
             sig["film_write"] = (C.c_int, [fp, C.c_int, C.c_int, f, C.c_char_p, C.c_char_p])
+            sig["gltf_load"] = (vp, [C.c_char_p])
+            sig["gltf_destroy"] = (None, [vp])
+            sig["gltf_desc"] = (C.POINTER(abi.SceneDesc), [vp])
+            sig["gltf_camera_count"] = (C.c_int, [vp])
+            sig["gltf_camera"] = (C.c_int, [vp, C.c_int, C.POINTER(abi.Camera)])
+            sig["gltf_render_loaded"] = (C.c_int, [vp, vp, C.POINTER(abi.Camera), C.POINTER(abi.Options), fp,
+                                                   C.POINTER(abi.Stats)])
         for name, (res, args) in sig.items():
             fn = getattr(self.lib, p + name)
             fn.restype = res
@@ -168,6 +175,9 @@ class Probe:
     def env(self, data) -> "ProbeEnv":
         return ProbeEnv(self, data)
 
+    def gltf(self, path) -> "ProbeGltf":
+        return ProbeGltf(self, path)
+
 
 class ProbeScene:
     def __init__(self, probe: Probe, data):
@@ -244,3 +254,64 @@ class ProbeEnv:
         out = (C.c_float * 3)()
         self.p._env_radiance(self.h, (C.c_float * 3)(*direction), out)
         return np.array(out[:], np.float32)
+
+
+def _copy_array(ptr, count, dtype, shape):
+    if not ptr or count == 0:
+        return None
+    return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True).reshape(shape)
+
+
+class ProbeGltf:
+    """A glTF/GLB file loaded by the reference's io::loadGltf (reference build only)."""
+
+    def __init__(self, probe: Probe, path):
+        if probe.kind != "ref":
+            raise ValueError("glTF import is a reference-build probe")
+        self.p = probe
+        self.h = probe._gltf_load(str(path).encode())
+        if not self.h:
+            raise RuntimeError(probe.error())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.p._gltf_destroy(self.h)
+            self.h = None
+
+    def scene_data(self):
+        """The loaded scene as SceneData (arrays copied out of the descriptor)."""
+        d = self.p._gltf_desc(self.h).contents
+        n = int(d.triangle_count)
+        images = []
+        for i in range(d.image_count):
+            im = d.images[i]
+            images.append(_copy_array(im.rgba, 4 * im.width * im.height, np.float32, (im.height, im.width, 4)))
+        return abi.SceneData(
+            positions=_copy_array(d.positions, 9 * n, np.float32, (n, 9)),
+            material_ids=_copy_array(d.material_ids, n, np.uint32, (n,)),
+            materials=[abi.Material.from_buffer_copy(bytes(d.materials[i])) for i in range(d.material_count)],
+            normals=_copy_array(d.normals, 9 * n, np.float32, (n, 9)),
+            tangents=_copy_array(d.tangents, 12 * n, np.float32, (n, 12)),
+            uvs=_copy_array(d.uvs, 12 * n, np.float32, (n, 12)),
+            lights=[abi.Light.from_buffer_copy(bytes(d.lights[i])) for i in range(d.light_count)],
+            images=images,
+            textures=[abi.Texture.from_buffer_copy(bytes(d.textures[i])) for i in range(d.texture_count)])
+
+    def cameras(self):
+        out = []
+        for i in range(self.p._gltf_camera_count(self.h)):
+            c = abi.Camera()
+            if self.p._gltf_camera(self.h, i, C.byref(c)):
+                raise RuntimeError(self.p.error())
+            out.append(c)
+        return out
+
+    def render_loaded(self, env: "ProbeEnv", camera, options):
+        """cr::render of the loaded cr::Scene itself (finalize + build on a copy)."""
+        film = np.zeros((options.height, options.width, 3), np.float32)
+        stats = abi.Stats()
+        rc = self.p._gltf_render_loaded(self.h, env.h, C.byref(camera), C.byref(options),
+                                        film.ctypes.data_as(C.POINTER(C.c_float)), C.byref(stats))
+        if rc:
+            raise RuntimeError(self.p.error())
+        return film, stats

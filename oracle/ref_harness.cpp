@@ -9,6 +9,7 @@
 // (3) time the reference CPU renderer as bench.py's cpu_baseline.
 #include "probe.h"
 #include "render.h"
+#include "../include/crgpu_cr.hpp"
 
 #include <atomic>
 #include <cmath>
@@ -266,8 +267,8 @@ void ref_env_radiance(void* env, const float* direction, float* rgb) {
     put3(rgb, static_cast<cr::Environment*>(env)->radiance(v3(direction)));
 }
 
-int ref_render(void* scene, void* env, const crg_camera* camera, const crg_options* o, float* film, crg_stats* stats) {
-    const SceneBox& b = *static_cast<SceneBox*>(scene);
+static int render_box(const SceneBox& b, void* env, const crg_camera* camera, const crg_options* o, float* film,
+                      crg_stats* stats) {
     return guarded([&] {
         cr::RenderOptions opt;
         opt.width = o->width; opt.height = o->height; opt.samples = o->samples; opt.bounces = o->bounces;
@@ -283,6 +284,51 @@ int ref_render(void* scene, void* env, const crg_camera* camera, const crg_optio
             stats->workers = r.workers;
         }
     });
+}
+
+int ref_render(void* scene, void* env, const crg_camera* camera, const crg_options* o, float* film, crg_stats* stats) {
+    return render_box(*static_cast<SceneBox*>(scene), env, camera, o, film, stats);
+}
+
+// glTF import through the reference's loader; the descriptor is the adapter's
+// flattening of the loaded (not yet finalized) scene.
+namespace {
+struct GltfBox {
+    cr::Scene scene;
+    cr::gpu::SceneStorage storage;
+    crg_scene_desc desc{};
+};
+}  // namespace
+
+void* ref_gltf_load(const char* path) {
+    auto box = std::make_unique<GltfBox>();
+    const int rc = guarded([&] {
+        box->scene = cr::io::loadGltf(path);
+        box->desc = cr::gpu::describe(box->scene, box->storage);
+    });
+    return rc ? nullptr : box.release();
+}
+
+void ref_gltf_destroy(void* g) { delete static_cast<GltfBox*>(g); }
+const crg_scene_desc* ref_gltf_desc(void* g) { return &static_cast<GltfBox*>(g)->desc; }
+int ref_gltf_camera_count(void* g) { return int(static_cast<GltfBox*>(g)->scene.cameras.size()); }
+
+int ref_gltf_camera(void* g, int index, crg_camera* out) {
+    const auto& cams = static_cast<GltfBox*>(g)->scene.cameras;
+    if (index < 0 || size_t(index) >= cams.size()) return CRG_EINVAL;
+    *out = cr::gpu::camera_of(cams[size_t(index)]);
+    return CRG_OK;
+}
+
+int ref_gltf_render_loaded(void* g, void* env, const crg_camera* camera, const crg_options* o, float* film,
+                           crg_stats* stats) {
+    SceneBox b;
+    const int rc = guarded([&] {
+        b.scene = static_cast<GltfBox*>(g)->scene;
+        b.scene.finalize();
+        b.bvh.build(b.scene);
+    });
+    return rc ? rc : render_box(b, env, camera, o, film, stats);
 }
 
 

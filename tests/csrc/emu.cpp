@@ -29,6 +29,14 @@ namespace {
 
 thread_local std::string lastError;
 
+// The sampling-angle table, filled by the host code the library uses.
+const bool kTurnTableReady = [] {
+    static std::vector<TurnCS> table(kTurnEntries);
+    fill_turn_table(table.data());
+    g_turn_table = table.data();
+    return true;
+}();
+
 struct EmuScene {
     HostScene hs;
     SceneArrays arr;

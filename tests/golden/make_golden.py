@@ -293,7 +293,72 @@ def render_cases():
                      ("tex_mask", rs.textured_alpha_mask)):
         s, e, c = fn()
         cases.append((name, s, e, c, abi.options(48, 40, 8, 6, seed=42), None))
+    if (OUT / rs.GLTF_FIXTURE).exists():
+        s, e, c = rs.gltf_export_contract()
+        cases.append(("gltf_contract", s, e, c, abi.options(96, 54, 16, 8, seed=42), None))
+        cases.append(("gltf_contract_nonee", s, e, c, abi.options(64, 36, 16, 8, seed=7, nee=False), None))
+    if (OUT / rs.HARBOR_FIXTURE).exists():
+        s, e, c = rs.gltf_harbor()
+        cases.append(("gltf_harbor", s, e, c, abi.options(160, 90, 4, 8, seed=42), None))
     return cases
+
+
+GLTF_ASSETS = ("/root/reference/scenes/fixtures/export_contract.glb",
+               "/root/reference/scenes/fixtures/export_contract.gltf")
+HARBOR_ASSET = "/root/reference/scenes/harbor/harbor.glb"
+
+
+def _scene_arrays(sd):
+    out = {"positions": sd.positions, "material_ids": sd.material_ids}
+    for k in ("normals", "tangents", "uvs"):
+        out[k] = getattr(sd, k)
+    for k in ("materials", "lights", "textures"):
+        out[k] = np.array([np.frombuffer(bytes(x), np.uint8) for x in getattr(sd, k)])
+    return out
+
+
+def _write_gltf_fixture(g, path):
+    sd = g.scene_data()
+    out = _scene_arrays(sd)
+    for i, im in enumerate(sd.images):
+        k = np.round(im * 255).astype(np.uint8)
+        assert np.array_equal(k.astype(np.float32) / np.float32(255), im), "texels are not k/255"
+        out[f"image{i}"] = k
+    out["image_count"] = np.array(len(sd.images))
+    out["camera"] = np.frombuffer(bytes(g.cameras()[0]), np.uint8)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    np.savez_compressed(path, **out)
+    return sd
+
+
+def _check_fixture_render(ref, g, fixture, opt):
+    scene, env, cam = fixture()
+    pe = ref.env(env)
+    direct, _ = g.render_loaded(pe, cam, opt)
+    via_desc, _ = ref.scene(scene).render(pe, cam, opt)
+    assert np.array_equal(direct, via_desc), "fixture render differs from the loaded scene's render"
+    return scene
+
+
+def gen_gltf(ref):
+    """The export-contract asset loaded by the reference's own glTF loader (GLB and
+    glTF+bin forms must load identically, reference tests/test_io.cpp:395-455), written
+    as a scene fixture; the loaded cr::Scene rendered directly must equal the render
+    of the fixture's descriptor (the adapter's flattening loses nothing)."""
+    loaded = [ref.gltf(p) for p in GLTF_ASSETS]
+    arrays = [_scene_arrays(g.scene_data()) for g in loaded]
+    for k in arrays[0]:
+        assert np.array_equal(arrays[0][k], arrays[1][k]), f"GLB and glTF differ in {k}"
+    sd = _write_gltf_fixture(loaded[0], OUT / rs.GLTF_FIXTURE)
+    scene = _check_fixture_render(ref, loaded[0], rs.gltf_export_contract, abi.options(48, 27, 4, 8, seed=3))
+    print(f"  export_contract: {scene.triangle_count} triangles, {len(sd.materials)} materials, "
+          f"{len(sd.images)} images, {len(sd.lights)} lights; direct render == fixture render")
+    if pathlib.Path(HARBOR_ASSET).exists():
+        g = ref.gltf(HARBOR_ASSET)
+        sd = _write_gltf_fixture(g, OUT / rs.HARBOR_FIXTURE)
+        scene = _check_fixture_render(ref, g, rs.gltf_harbor, abi.options(64, 36, 2, 8, seed=3))
+        print(f"  harbor: {scene.triangle_count} triangles, {len(sd.materials)} materials, "
+              f"{len(sd.images)} images, {len(sd.lights)} lights; direct render == fixture render")
 
 
 def textured_surface_sets():
@@ -418,8 +483,8 @@ def gen_film_encode(ref):
 def main():
     ref = probe.Probe("ref")
     only = set(sys.argv[1:])
-    for fn in (gen_rng_camera, gen_trace, gen_offset_surface, gen_bsdf, gen_env, gen_render, gen_textured_surface,
-               gen_film_encode):
+    for fn in (gen_gltf, gen_rng_camera, gen_trace, gen_offset_surface, gen_bsdf, gen_env, gen_render,
+               gen_textured_surface, gen_film_encode):
         if only and fn.__name__ not in only:
             continue
         print(fn.__name__)

@@ -383,3 +383,39 @@ def textured_alpha_mask():
     cam = test_camera(position=(0.4, 0.3, 3.0), forward=normalize((-0.1, -0.1, -1)), yfov=0.9,
                       up=normalize((0, 1, -0.1)))
     return scene, env, cam
+
+
+GLTF_FIXTURE = "gltf_export_contract.npz"
+HARBOR_FIXTURE = "large/harbor.npz"
+
+
+def _gltf_fixture(name):
+    import pathlib
+    z = np.load(pathlib.Path(__file__).resolve().parent / "golden" / name)
+    materials = [abi.Material.from_buffer_copy(z["materials"][i].tobytes()) for i in range(len(z["materials"]))]
+    lights = [abi.Light.from_buffer_copy(z["lights"][i].tobytes()) for i in range(len(z["lights"]))]
+    textures = [abi.Texture.from_buffer_copy(z["textures"][i].tobytes()) for i in range(len(z["textures"]))]
+    images = [z[f"image{i}"].astype(np.float32) / np.float32(255) for i in range(int(z["image_count"]))]
+    scene = abi.SceneData(z["positions"], z["material_ids"], materials, normals=z["normals"],
+                          tangents=z["tangents"], uvs=z["uvs"], lights=lights, images=images, textures=textures)
+    cam = abi.Camera.from_buffer_copy(z["camera"].tobytes())
+    env = abi.EnvData(constant=(0.32, 0.36, 0.42))
+    return scene, env, cam
+
+
+def gltf_export_contract():
+    """The reference's glTF export-contract asset (reference scenes/fixtures/export_contract.glb:
+    1602 triangles, 7 materials with base-colour, emissive, metallic-roughness and normal maps, an
+    alpha-masked double-sided material, a transmissive volume, a point and a directional light),
+    as the reference's io::loadGltf loads it (before Scene::finalize), with its camera.
+    The arrays are a fixture written by tests/golden/make_golden.py from the reference build;
+    texels are stored as the 8-bit codes k with value float32(k) / 255 (exact for this asset)."""
+    return _gltf_fixture(GLTF_FIXTURE)
+
+
+def gltf_harbor():
+    """The reference's harbor asset (reference scenes/harbor/harbor.glb: 358,440 triangles,
+    53 materials, 14 textures of up to 1024x1024, a sun and a lamp) as io::loadGltf loads it,
+    with its camera.  The fixture (28 MB) is not committed: tests/golden/make_golden.py writes
+    it where the reference tree exists, and the cases that use it are left out elsewhere."""
+    return _gltf_fixture(HARBOR_FIXTURE)

@@ -15,7 +15,7 @@ def _cases():
     import ref_scenes as rs
     out = [("config1", scenes.config1(width=48, height=40, samples=4))]
     for name, fn in (("tex_factors", rs.textured_factors), ("tex_normal", rs.textured_normal_map),
-                     ("tex_mask", rs.textured_alpha_mask)):
+                     ("tex_mask", rs.textured_alpha_mask), ("gltf_contract", rs.gltf_export_contract)):
         sc, env, cam = fn()
         out.append((name, (sc, env, cam)))
     return out
