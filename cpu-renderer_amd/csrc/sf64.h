@@ -517,4 +517,32 @@ CR_HDI int64_t to_i64(f64 a) {
 }
 
 
+// e^x of a binary32 argument, rounded once to binary32.  Binary64 reduction
+// x = n*ln2 + r with ln2 split into a 32-bit head (n*head and x - n*head are
+// exact for |n| < 2^8) and a binary64 tail, Taylor series of e^r to r^13 for
+// |r| <= ln2/2 (truncation below 2^-56), scaling by 2^n, and one rounding to
+// binary32.  The binary64 value is within about 2^-51 relative of e^x, so the
+// result is e^x correctly rounded except where e^x lies that close to a
+// binary32 rounding boundary.
+CR_HD CR_NOINLINE inline float expf_cr(float xf) {
+    if (!(xf == xf)) return xf;
+    if (xf < -104.0f) return 0.0f;       // e^x < 2^-150 rounds to +0
+    if (xf > 89.0f) return bitsf(0x7f800000u);
+    const f64 x = from_float(xf);
+    const f64 t = mul(x, f64{0x3ff71547652b82feull});                       // x / ln2
+    const int32_t n = int32_t(to_i64(floor_(add(t, f64{0x3fe0000000000000ull}))));
+    const f64 fn = from_i32(n);
+    const f64 r = sub(sub(x, mul(fn, f64{0x3fe62e42fee00000ull})), mul(fn, f64{0x3dea39ef35793c76ull}));
+    // 1/k! for k = 13 .. 2, then 1, 1.
+    constexpr uint64_t kInvFact[12] = {0x3de6124613a86d09ull, 0x3e21eed8eff8d898ull, 0x3e5ae64567f544e4ull,
+                                       0x3e927e4fb7789f5cull, 0x3ec71de3a556c734ull, 0x3efa01a01a01a01aull,
+                                       0x3f2a01a01a01a01aull, 0x3f56c16c16c16c17ull, 0x3f81111111111111ull,
+                                       0x3fa5555555555555ull, 0x3fc5555555555555ull, 0x3fe0000000000000ull};
+    f64 p{kInvFact[0]};
+    for (int k = 1; k < 12; ++k) p = add(mul(p, r), f64{kInvFact[k]});
+    p = add(mul(p, r), one());
+    p = add(mul(p, r), one());
+    return to_float(mul(p, f64{uint64_t(1023 + n) << 52}));
+}
+
 }  // namespace sf

@@ -195,6 +195,11 @@ void emu_env_sample(void* e, uint64_t* rng, float* dir, float* pdf) {
     dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
     *rng = r.state;
 }
+// The device's expf (binary64-emulated, correctly rounded) over n arguments.
+void emu_expf(const float* x, float* out, uint64_t n) {
+    for (uint64_t i = 0; i < n; ++i) out[i] = sf::expf_cr(x[i]);
+}
+
 // Slab test of one box (child 0 of a node whose other children are empty):
 // 1 when node_hit reports the box as hit within [tmin, tmax].
 int emu_box_hit(const float* o, const float* d, float tmin, float tmax, const float* lo, const float* hi) {

@@ -15,6 +15,7 @@ offsets and accumulation) from device-libm differences:
 """
 from __future__ import annotations
 
+import ctypes as C
 import sys
 
 import numpy as np
@@ -224,3 +225,25 @@ def test_display_thresholds_match_the_curve(emu, oracle):
                                np.nextafter(probe_v, np.float32(0))]).astype(np.float32)
         near = np.resize(near, (len(near) // 300 + 1) * 300).reshape(-1, 100, 3)
         np.testing.assert_array_equal(emu.film_png8(near, ex), oracle.film_png8(near, ex))
+
+
+def test_expf_is_correctly_rounded(emu):
+    """The device's expf (Beer-Lambert transmittance) equals e^x correctly rounded to binary32
+    on a sweep of every 1009th binary32 argument over [-104, 89] (the underflow-to-zero and
+    overflow thresholds bracket it), on the subnormal-result range densely, and on special values."""
+    fp = C.POINTER(C.c_float)
+    neg = np.arange(0x80000000, 0xC2D00001, 1009, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    pos = np.arange(0, 0x42B20001, 1009, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    sub = np.linspace(-103.98, -87.3, 200001, dtype=np.float32)
+    special = np.array([0.0, -0.0, np.inf, -np.inf, -104.0, -103.97, 88.72, 88.73, 89.0, 1e-30, -1e-30],
+                       np.float32)
+    x = np.concatenate([neg, pos, sub, special])
+    out = np.empty_like(x)
+    emu.lib.emu_expf(x.ctypes.data_as(fp), out.ctypes.data_as(fp), len(x))
+    with np.errstate(over="ignore", under="ignore"):
+        want = np.exp(x.astype(np.float64)).astype(np.float32)
+    bad = np.nonzero(out.view(np.uint32) != want.view(np.uint32))[0]
+    assert len(bad) == 0, [(float(x[i]), float(out[i]), float(want[i])) for i in bad[:5]]
+    nan = np.array([np.nan], np.float32)
+    emu.lib.emu_expf(nan.ctypes.data_as(fp), out.ctypes.data_as(fp), 1)
+    assert np.isnan(out[0])
