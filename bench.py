@@ -148,6 +148,66 @@ def scene_cache_form(ctx, w) -> dict:
                 "build_and_upload_seconds": round(info.build_seconds + info.upload_seconds, 3)}
 
 
+HARBOR = ROOT / "tests" / "golden" / "large" / "harbor.npz"
+
+
+def real_asset_form(ctx, with_cpu: bool, steps: int = 3) -> dict:
+    """The reference's harbor glTF asset (358,440 triangles, 53 materials, 14 textures) as the
+    reference's own loader loads it (a fixture written where the reference tree exists; this
+    form is skipped elsewhere), rendered at the reference's default options (960x540, 32 spp,
+    8 bounces) under the synthetic sky: time to residency (finalize + build + upload, and from a
+    scene cache file) and render throughput, with the reference CPU renderer beside it."""
+    if not HARBOR.exists():
+        return None
+    import tempfile
+    from cr_amd import abi, scenes
+    scene, cam = scenes.scene_fixture(HARBOR)
+    env = abi.EnvData(rgba=scenes.synthetic_sky())
+    opt = abi.options(960, 540, 32, 8, seed=42)
+    t = time.perf_counter()
+    sid, info = ctx.upload_scene(scene)
+    resident_s = time.perf_counter() - t
+    eid, _ = ctx.upload_env(env)
+    with tempfile.TemporaryDirectory() as d:
+        path = pathlib.Path(d) / "harbor.crgscene"
+        ctx.save_scene(sid, path)
+        t = time.perf_counter()
+        lid, _ = ctx.load_scene(path)
+        load_s = time.perf_counter() - t
+        ctx.release_scene(lid)
+    ctx.render(sid, eid, cam, opt)
+    t = time.perf_counter()
+    rays = 0
+    for _ in range(steps):
+        r = ctx.render(sid, eid, cam, opt)
+        rays += r.extend_rays + r.shadow_rays
+    secs = (time.perf_counter() - t) / steps
+    samples = opt.width * opt.height * opt.samples
+    out = {"triangles": int(info.triangles), "images": len(scene.images), "device_mib": round(info.device_bytes / 2**20, 1),
+           "resident_seconds": round(resident_s, 3), "host_build_seconds": round(info.build_seconds, 3),
+           "cache_load_seconds": round(load_s, 3), "render": "960x540, 32 spp, 8 bounces, synthetic sky",
+           "seconds_per_render": round(secs, 4), "msamples_per_s": round(samples / secs / 1e6, 2),
+           "mrays_per_s": round(rays / steps / secs / 1e6, 2)}
+    ctx.release_scene(sid)
+    ctx.release_env(eid)
+    if with_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import probe
+        if probe.available("ref"):
+            p = probe.Probe("ref")
+            t = time.perf_counter()
+            ps = p.scene(scene)
+            ref_scene_s = time.perf_counter() - t
+            small = abi.options(960, 540, 1, 8, seed=42, threads=len(os.sched_getaffinity(0)))
+            _, st = ps.render(p.env(env), cam, small)
+            out["reference"] = {"finalize_and_build_seconds": round(ref_scene_s, 3),
+                                "msamples_per_s": round(960 * 540 / st.seconds / 1e6, 4),
+                                "sample": "960x540 at 1 spp (the same first sample of every pixel)",
+                                "cores": small.threads}
+            out["gpu_vs_cpu"] = round(out["msamples_per_s"] / out["reference"]["msamples_per_s"], 1)
+    return out
+
+
 def _pct(xs, q):
     import numpy as np
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q))
@@ -458,6 +518,8 @@ def main():
     log(f"film encode: {film_enc}")
     cache = scene_cache_form(ctx, w)
     log(f"scene cache: {cache}")
+    asset = real_asset_form(ctx, not args.no_cpu)
+    log(f"real asset: {asset}")
 
     cpu = None
     host_build = {"triangles": int(info.triangles), "seconds": round(info.build_seconds, 3),
@@ -491,6 +553,7 @@ def main():
         "host_build": host_build,
         "film_encode": film_enc,
         "scene_cache": cache,
+        "real_asset_harbor": asset,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

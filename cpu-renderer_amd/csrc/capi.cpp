@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cerrno>
+#include <cstddef>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -170,6 +171,20 @@ std::shared_ptr<SceneRes> make_resident(crg_ctx* ctx, const SceneSections& a, st
     res->texture_count = a.textures.count;
     res->material_count = a.materials.count;
     const std::vector<uint32_t> ranks = light_ranks(res->lights, res->tri_to_prim, n);
+    // Alpha-mask primitives select the traversal variant with FP32 mask tests.
+    std::atomic<bool> masks{false};
+    parallel_chunks(a.prims.count, size_t(1) << 18, [&](size_t b, size_t e, size_t) {
+        const uint8_t* base = static_cast<const uint8_t*>(a.prims.p) + offsetof(PrimRec, flags);
+        for (size_t i = b; i < e; ++i) {
+            uint32_t f;
+            std::memcpy(&f, base + i * sizeof(PrimRec), sizeof(f));
+            if (f & kPrimMaskTexture) {
+                masks = true;
+                return;
+            }
+        }
+    });
+    res->masks = masks;
     hipStream_t st = ctx->stream;
     upload_section(res->nodes, a.nodes, st);
     upload_section(res->prims, a.prims, st);

@@ -17,7 +17,7 @@ constexpr int kStallSteps = 96;
 
 uint64_t pool_bytes_for(uint32_t paths) {
     // ray + state records, 4 float4 + 2 u32 of shadow records, 6 u32 queues.
-    return uint64_t(paths) * (sizeof(PathRay) + sizeof(PathRec) + 4 * 16 + 2 * 4 + 6 * 4);
+    return uint64_t(paths) * (sizeof(PathRay) + sizeof(PathRec) + 4 * 16 + 2 * 4 + 6 * 4 + 2 * sizeof(Pending));
 }
 
 double seconds_between(Clock::time_point a, Clock::time_point b) {
@@ -274,16 +274,18 @@ void Engine::configure() {
     rec_.alloc(size_t(n) * sizeof(PathRec));
     for (DevBuf* b : {&sh_o_, &sh_d_, &sh_lo_, &sh_c_}) b->alloc(f4);
     for (DevBuf* b : {&sh_path_, &sh_job_, &q0_, &q1_, &retire_, &xq_, &xsq_, &freeq_}) b->alloc(u32);
+    for (DevBuf* b : {&xpend_, &xspend_}) b->alloc(size_t(n) * sizeof(Pending));
     P_ = PathSoA{ray_.as<PathRay>(), rec_.as<PathRec>()};
     S_ = ShadowSoA{sh_path_.as<uint32_t>(), sh_job_.as<uint32_t>(), sh_o_.as<float4>(), sh_d_.as<float4>(),
                    sh_lo_.as<float4>(), sh_c_.as<float4>()};
     Q_ = Queues{{q0_.as<uint32_t>(), q1_.as<uint32_t>()}, retire_.as<uint32_t>(), xq_.as<uint32_t>(),
-                xsq_.as<uint32_t>(), freeq_.as<uint32_t>()};
+                xsq_.as<uint32_t>(), freeq_.as<uint32_t>(), xpend_.as<Pending>(), xspend_.as<Pending>()};
     ctl_.alloc(sizeof(Control));
     cnt_.alloc(sizeof(JobCounters));
     jobs_.alloc(sizeof(DevJob) * kJobSlots);
     check(hipMemsetAsync(cnt_.p, 0, sizeof(JobCounters), st_), "hipMemsetAsync");
     check(hipMemsetAsync(ctl_.p, 0, sizeof(Control), st_), "hipMemsetAsync");
+    masks_ = false;
     check(launch_pool_init(Q_.freeq, n, ctl_.as<Control>(), st_), "pool init");
     next_seq_ = seq_lo_ = 0;
     total_items_ = 0;
@@ -370,6 +372,7 @@ void Engine::admit_locked() {
         any = true;
     }
     if (any) {
+        update_masks_locked();
         // Idle slots start the new items right away (same reset + refill the
         // graph ends with; both are idempotent at a step boundary).
         const LaunchShape shape{cus_ * tune_.groups_per_cu, tune_.workgroup};
@@ -378,6 +381,14 @@ void Engine::admit_locked() {
                                st_),
                   "refill");
     }
+}
+
+void Engine::update_masks_locked() {
+    bool m = false;
+    for (auto& jp : running_) m = m || jp->req.scene->masks;
+    if (m == masks_) return;
+    check(launch_set_masks(ctl_.as<Control>(), m ? 1u : 0u, st_), "traversal variant");
+    masks_ = m;
 }
 
 void Engine::launch_step(Readback& rb) {
@@ -433,6 +444,7 @@ void Engine::process_locked(const Readback& rb) {
         still.push_back(jp);
     }
     running_.swap(still);
+    update_masks_locked();
     // Oldest job whose items are not all issued yet (lookups never go below it).
     uint32_t lo = next_seq_;
     for (auto& jp : running_)

@@ -42,6 +42,7 @@ struct SceneRes {
     std::vector<uint32_t> prim_to_tri;
     uint64_t triangles = 0, bytes = 0, binary_nodes = 0;
     uint64_t texel_count = 0, image_count = 0, texture_count = 0, material_count = 0;  // device array lengths
+    bool masks = false;  // some primitive has an alpha-mask texture
 };
 
 struct EnvRes {
@@ -116,6 +117,7 @@ private:
     void run();
     void configure();
     void admit_locked();
+    void update_masks_locked();  // traversal variant for the live jobs' scenes
     void launch_step(Readback& rb);
     void process_locked(const Readback& rb);
     void complete_locked(const std::shared_ptr<Job>& j, const JobCounters& c);
@@ -138,13 +140,14 @@ private:
     crg_tuning tune_{};
     bool dirty_ = true;
     bool configured_ = false;
+    bool masks_ = false;  // ctl->masks as last set
     hipGraphExec_t graph_ = nullptr;
     Profiler prof_;
 
     // device state
     DevBuf ray_, rec_;
     DevBuf sh_path_, sh_job_, sh_o_, sh_d_, sh_lo_, sh_c_;
-    DevBuf q0_, q1_, retire_, xq_, xsq_, freeq_;
+    DevBuf q0_, q1_, retire_, xq_, xsq_, freeq_, xpend_, xspend_;
     DevBuf ctl_, cnt_, jobs_;
     PathSoA P_{};
     ShadowSoA S_{};

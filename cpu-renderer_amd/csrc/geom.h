@@ -197,6 +197,125 @@ CR_HDI int fast_triangle(const RayQuery& q, V3 va, V3 vb, V3 vc, float& tlo, flo
     return kAccept;
 }
 
+// Alpha-mask decision of a certain FP32 hit without binary64: kAccept or
+// kReject when the MASK test (reference src/geometry.cpp:279-292) has that
+// outcome at every point the binary64 barycentrics could lie at, kUnsure
+// otherwise.  The barycentrics come from the FP32 edge functions (bounds as in
+// fast_triangle), their error is carried through the UV interpolation, the slot
+// transform and the wrap into texel space; every texel the bilinear (or
+// nearest) footprint can reach is read, and the test must hold over the range
+// of alpha values those texels allow (narrowed for bilinear filtering by the
+// blend's Lipschitz bound) with a relative margin of 2^-16, which covers the
+// rounding of the filter's float blend.  Every bound is doubled.
+CR_HDI int mask_fp32(const DevScene& sc, const RayQuery& q, uint32_t prim, uint32_t material, V3 va, V3 vb, V3 vc) {
+    const DevMaterial& m = sc.materials[material];
+    const DevSlot& slot = m.base_tex;
+    if (slot.texture < 0) return kUnsure;
+    const V3 a = va - q.o, b = vb - q.o, c = vc - q.o;
+    const float az = at(a, q.kz), bz = at(b, q.kz), cz = at(c, q.kz);
+    const float ax = at(a, q.kx) - q.sx * az, ay = at(a, q.ky) - q.sy * az;
+    const float bx = at(b, q.kx) - q.sx * bz, by = at(b, q.ky) - q.sy * bz;
+    const float cx = at(c, q.kx) - q.sx * cz, cy = at(c, q.ky) - q.sy * cz;
+    const float w0 = cx * by - cy * bx;
+    const float w1 = ax * cy - ay * cx;
+    const float w2 = bx * ay - by * ax;
+    const float k1 = 1.02f * kU;
+    const float ea = k1 * ((fabsf(at(a, q.kx)) + fabsf(at(a, q.ky))) + 3.0f * fabsf(az) + (fabsf(ax) + fabsf(ay))) + q.ovec;
+    const float eb = k1 * ((fabsf(at(b, q.kx)) + fabsf(at(b, q.ky))) + 3.0f * fabsf(bz) + (fabsf(bx) + fabsf(by))) + q.ovec;
+    const float ec = k1 * ((fabsf(at(c, q.kx)) + fabsf(at(c, q.ky))) + 3.0f * fabsf(cz) + (fabsf(cx) + fabsf(cy))) + q.ovec;
+    const float ma = fabsf(ax) + fabsf(ay), mb = fabsf(bx) + fabsf(by), mc = fabsf(cx) + fabsf(cy);
+    const float k2 = 1.01f * kU;
+    const float e0 = (mc * eb + mb * ec + 2.0f * eb * ec + k2 * (fabsf(cx * by) + fabsf(cy * bx) + fabsf(w0))) * 1.02f;
+    const float e1 = (ma * ec + mc * ea + 2.0f * ec * ea + k2 * (fabsf(ax * cy) + fabsf(ay * cx) + fabsf(w1))) * 1.02f;
+    const float e2 = (mb * ea + ma * eb + 2.0f * ea * eb + k2 * (fabsf(bx * ay) + fabsf(by * ax) + fabsf(w2))) * 1.02f;
+    const float det = (w0 + w1) + w2;
+    const float edet = (e0 + e1) + e2 + 2.02f * kU * (fabsf(w0) + fabsf(w1) + fabsf(w2));
+    const float adet = fabsf(det);
+    if (!(adet > 4.0f * edet)) return kUnsure;
+    const float den = adet - edet;
+    // Barycentrics of b (u), c (v) and a (1 - u - v) with their error bounds.
+    const float bu = w1 / det, bv = w2 / det, bw = w0 / det;
+    const float eu = 2.0f * ((e1 + fabsf(bu) * edet) / den + 2.0f * kU * fabsf(bu));
+    const float ev = 2.0f * ((e2 + fabsf(bv) * edet) / den + 2.0f * kU * fabsf(bv));
+    const float ew = 2.0f * ((e0 + fabsf(bw) * edet) / den + 2.0f * kU * fabsf(bw));
+    // interpolateUv of the slot's set, rounded to float by the reference.
+    const int off = slot.texcoord == 0 ? 0 : 2;
+    const float* ua = &sc.uvs[3 * prim].x + off;
+    const float* ub = &sc.uvs[3 * prim + 1].x + off;
+    const float* uc = &sc.uvs[3 * prim + 2].x + off;
+    float uv[2], euv[2];
+    for (int d = 0; d < 2; ++d) {
+        const float pa = bw * ua[d], pb = bu * ub[d], pc = bv * uc[d];
+        uv[d] = (pa + pb) + pc;
+        euv[d] = 2.0f * (ew * fabsf(ua[d]) + eu * fabsf(ub[d]) + ev * fabsf(uc[d]) +
+                         4.0f * kU * (fabsf(pa) + fabsf(pb) + fabsf(pc)) + kU * fabsf(uv[d]));
+    }
+    // Slot transform (scale, rotation, offset) and the wrap, in float.
+    const float cr = sf::to_float(sf::f64{slot.cos_bits}), sr = sf::to_float(sf::f64{slot.sin_bits});
+    const float x = uv[0] * slot.scale[0], y = uv[1] * slot.scale[1];
+    const float ex = euv[0] * fabsf(slot.scale[0]) + kU * fabsf(x), ey = euv[1] * fabsf(slot.scale[1]) + kU * fabsf(y);
+    float tc[2], et[2];
+    tc[0] = (cr * x - sr * y) + slot.offset[0];
+    tc[1] = (sr * x + cr * y) + slot.offset[1];
+    et[0] = 2.0f * (fabsf(cr) * ex + fabsf(sr) * ey + 4.0f * kU * (fabsf(cr * x) + fabsf(sr * y) + fabsf(slot.offset[0])) +
+                    2.0f * kU * (fabsf(x) + fabsf(y)));
+    et[1] = 2.0f * (fabsf(sr) * ex + fabsf(cr) * ey + 4.0f * kU * (fabsf(sr * x) + fabsf(cr * y) + fabsf(slot.offset[1])) +
+                    2.0f * kU * (fabsf(x) + fabsf(y)));
+    const DevTexture tex = sc.textures[slot.texture];
+    const DevImage im = sc.images[tex.image];
+    const int wrap[2] = {tex.wrap_s, tex.wrap_t};
+    const int size[2] = {im.width, im.height};
+    int64_t lo[2], hi[2];
+    float pos[2], err[2];
+    for (int d = 0; d < 2; ++d) {
+        if (!finite1(tc[d]) || !finite1(et[d]) || fabsf(tc[d]) > 1048576.0f) return kUnsure;
+        float t = tc[d];
+        if (wrap[d] == 1) t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+        else if (wrap[d] == 0) t = t - floorf(t);
+        else {
+            float r = fmodf(t, 2.0f);
+            if (r < 0.0f) r += 2.0f;
+            t = r <= 1.0f ? r : 2.0f - r;
+        }
+        const float X = t * float(size[d]) - (tex.linear ? 0.5f : 0.0f);
+        const float eX = 2.0f * ((et[d] + 4.0f * kU) * float(size[d]) + 2.0f * kU * fabsf(X)) + 1e-4f;
+        if (!(eX < 0.75f)) return kUnsure;
+        lo[d] = int64_t(floorf(X - eX));
+        hi[d] = int64_t(floorf(X + eX)) + (tex.linear ? 1 : 0);
+        pos[d] = X;
+        err[d] = eX;
+    }
+    auto alpha_at = [&](int64_t ix, int64_t iy) {
+        const int tx = wrap_index(ix, im.width, tex.wrap_s), ty = wrap_index(iy, im.height, tex.wrap_t);
+        return sc.texels[im.offset + uint64_t(ty) * uint64_t(im.width) + uint64_t(tx)].w;
+    };
+    float amin = 3.4e38f, amax = -3.4e38f;
+    for (int64_t iy = lo[1]; iy <= hi[1]; ++iy)
+        for (int64_t ix = lo[0]; ix <= hi[0]; ++ix) {
+            const float al = alpha_at(ix, iy);
+            amin = fminf(amin, al);
+            amax = fmaxf(amax, al);
+        }
+    // Bilinear: the blend at the FP32 position, off from the blend at the
+    // binary64 position by at most the footprint's alpha range times the
+    // position error per axis (the blend is Lipschitz on the wrapped lattice).
+    if (tex.linear) {
+        const float fx0 = floorf(pos[0]), fy0 = floorf(pos[1]);
+        const int64_t x0 = int64_t(fx0), y0 = int64_t(fy0);
+        const float fx = pos[0] - fx0, fy = pos[1] - fy0;
+        const float top = alpha_at(x0, y0) * (1 - fx) + alpha_at(x0 + 1, y0) * fx;
+        const float bot = alpha_at(x0, y0 + 1) * (1 - fx) + alpha_at(x0 + 1, y0 + 1) * fx;
+        const float est = top * (1 - fy) + bot * fy;
+        const float e = (amax - amin) * (err[0] + err[1]) * 1.01f + 8.0f * kU * fmaxf(fabsf(amin), fabsf(amax));
+        amin = fmaxf(amin, est - e);
+        amax = fminf(amax, est + e);
+    }
+    const float f = m.base[3];
+    if (f * amin * (1.0f - 0x1p-16f) >= m.alpha_cutoff && f * amin >= 0.0f) return kAccept;
+    if (f * amax * (1.0f + 0x1p-16f) < m.alpha_cutoff && f * amax >= 0.0f) return kReject;
+    return kUnsure;
+}
+
 // accepts(): single-sided opaque culling on the binary64 sign of dot(Ng, d)
 // (reference src/geometry.cpp:279-292).
 CR_HDI bool prim_accepts(const DevScene& s, uint32_t prim, uint32_t flags, V3 d) {
@@ -244,9 +363,12 @@ struct ClosestResult {
     bool exact;       // t,u,v hold the binary64 values
     bool overflow;    // traversal stack exhausted (reported as an error)
     bool tex_error;   // alpha-mask texture coordinates not finite (reported as an error)
-    bool need_exact;  // FP32-only traversal met an undecided test: rerun with EXACT
+    bool need_exact;  // FP32-only traversal left candidates (see exact_work)
     sf::f64 t, u, v;
     float tlo, thi;  // FP32 enclosure of the binary64 t (or the query limit)
+    uint32_t ncand;  // candidates met by the FP32-only traversal (kCand + 1: too many)
+    uint32_t cand[kCand];
+    float cand_lo[kCand];
 };
 
 // Replace the current best by an exactly evaluated candidate when it wins the
@@ -263,6 +385,94 @@ CR_HDI void take_exact(ClosestResult& best, uint32_t prim, uint32_t id, const Ex
     best.thi = next_up(tf);
 }
 
+// FP32-only traversal: a primitive it cannot decide alone becomes a candidate
+// for the exact pass; the traversal goes on.
+CR_HDI void add_cand(ClosestResult& b, uint32_t prim, float lo) {
+#pragma unroll
+    for (uint32_t k = 0; k < kCand; ++k)
+        if (k == b.ncand) {
+            b.cand[k] = prim;
+            b.cand_lo[k] = lo;
+        }
+    if (b.ncand <= kCand) ++b.ncand;
+    b.need_exact = true;
+}
+
+// FP32-only decision for one tested primitive (fr != kReject, [clo, chi] the
+// FP32 enclosure of its t when fast_triangle got that far, else 0).  Certain
+// hits update the best, whose upper bound is the traversal's limit; a certain
+// hit whose order against the best is undecided keeps the smaller upper bound
+// as the best and makes the other a candidate; undecided tests and alpha-mask
+// primitives become candidates.  Returns true when an any-hit query found a
+// certain blocker.
+// MASK=false leaves every alpha-mask primitive to the exact pass (same
+// results; the traversal loop then carries no texture code, for scenes
+// without alpha masks).
+template <bool ANY, bool MASK = true>
+CR_HDI bool fp32_take(const DevScene& s, const RayQuery& q, ClosestResult& best, uint32_t i, uint32_t id,
+                      uint32_t flags, int fr, float clo, float chi) {
+    if (!ANY && clo > best.thi) return false;           // certainly beyond the limit (also when undecided)
+    if (!prim_accepts(s, i, flags, q.d)) return false;  // culling does not depend on t
+    bool certain = fr == kAccept && !(flags & kPrimMaskTexture);
+    if (MASK && fr == kAccept && !certain) {  // alpha mask: decided in FP32 where the texels allow
+        const CR_GLOBAL PrimRec& p = gref(s.prims + i);
+        const int mr = mask_fp32(s, q, i, p.material, V3{p.ax, p.ay, p.az}, V3{p.bx, p.by, p.bz}, V3{p.cx, p.cy, p.cz});
+        if (mr == kReject) return false;
+        certain = mr == kAccept;
+    }
+    if (certain) {
+        if (ANY || best.prim == kNoTriangle || chi < best.tlo) {  // certainly nearer than the best
+            best.prim = i;
+            best.id = id;
+            best.exact = false;
+            best.tlo = clo;
+            best.thi = chi;
+            return ANY;
+        }
+        if (chi < best.thi) {
+            add_cand(best, best.prim, best.tlo);
+            best.prim = i;
+            best.id = id;
+            best.tlo = clo;
+            best.thi = chi;
+        } else {
+            add_cand(best, i, clo);
+        }
+        return false;
+    }
+    add_cand(best, i, clo);
+    return false;
+}
+
+// What an FP32-only result still needs: 0 none, 1 exact resolution of its
+// candidates (those that may lie within the best's bound), 2 the exact
+// traversal (more candidates than were kept).
+template <bool ANY>
+CR_HDI int exact_work(const ClosestResult& b) {
+    if (b.overflow || b.tex_error || b.ncand == 0) return 0;
+    if (ANY && b.prim != kNoTriangle) return 0;  // a certain blocker
+    if (b.ncand > kCand) return 2;
+    bool relevant = false;
+#pragma unroll
+    for (uint32_t k = 0; k < kCand; ++k)
+        if (k < b.ncand && (ANY || !(b.cand_lo[k] > b.thi))) relevant = true;
+    return relevant ? 1 : 0;
+}
+
+CR_HDI Pending pending_of(const ClosestResult& b, int work) {
+    Pending p;
+    p.best = b.prim;
+    p.n = b.ncand > kCand ? kCand : b.ncand;
+    p.thi = b.thi;
+    p.full = work == 2 ? 1u : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < kCand; ++k) {
+        p.cand[k] = b.cand[k];
+        p.lo[k] = b.cand_lo[k];
+    }
+    return p;
+}
+
 // Per-query work counters (wide nodes visited, triangles tested, exact fallbacks).
 struct TraceCount {
     uint32_t nodes, tris, fallbacks;
@@ -271,7 +481,7 @@ struct TraceCount {
 // Tests a leaf's primitives.  Returns true when traversal can stop (any-hit
 // found, or an FP32-only traversal needs the exact rerun).  EXACT=false
 // compiles no binary64 emulation at all: undecided tests flag need_exact.
-template <bool ANY, bool EXACT>
+template <bool ANY, bool EXACT, bool MASK = true>
 CR_HD inline bool leaf_prims(const DevScene& s, const PrimRec* prims, RayQuery& q, uint32_t first, uint32_t count,
                              ClosestResult& best, TraceCount& tc) {
     for (uint32_t i = first; i < first + count; ++i) {
@@ -282,6 +492,10 @@ CR_HD inline bool leaf_prims(const DevScene& s, const PrimRec* prims, RayQuery& 
         float clo = 0, chi = 0;
         const int fr = fast_triangle(q, va, vb, vc, clo, chi);
         if (fr == kReject) continue;
+        if constexpr (!EXACT) {
+            if (fp32_take<ANY, MASK>(s, q, best, i, p.id, p.flags, fr, clo, chi)) return true;
+            continue;
+        } else {
         if (fr == kAccept && !(p.flags & kPrimMaskTexture)) {
             if (!ANY && clo > best.thi) continue;  // certainly beyond the current limit
             if (!prim_accepts(s, i, p.flags, q.d)) continue;
@@ -305,11 +519,6 @@ CR_HD inline bool leaf_prims(const DevScene& s, const PrimRec* prims, RayQuery& 
         if (fr == kAccept && !ANY && clo > best.thi) continue;
         // Undecided in FP32, or an alpha mask that needs the binary64
         // barycentrics: decide with the binary64 emulation.
-        if constexpr (!EXACT) {
-            best.need_exact = true;
-            if (!ANY) return true;  // any-hit keeps looking for a certain blocker
-            continue;
-        } else {
         ++tc.fallbacks;
         if (!ANY && !best.exact) {
             const CR_GLOBAL PrimRec& bp = gref(prims + best.prim);
@@ -358,7 +567,7 @@ CR_HD inline bool leaf_prims(const DevScene& s, const PrimRec* prims, RayQuery& 
 #ifndef CR_CACHE_PTRS
 #define CR_CACHE_PTRS 1
 #endif
-template <bool ANY>
+template <bool ANY, bool MASK = true>
 CR_HD inline bool leaf_fast(const DevScene& s, const PrimRec* prims, RayQuery& q, uint32_t first, uint32_t count,
                             ClosestResult& best, TraceCount& tc) {
     const uint32_t end = first + count;
@@ -382,28 +591,7 @@ CR_HD inline bool leaf_fast(const DevScene& s, const PrimRec* prims, RayQuery& q
             float clo = 0, chi = 0;
             const int fr = fast_triangle(q, V3{p.ax, p.ay, p.az}, V3{p.bx, p.by, p.bz}, V3{p.cx, p.cy, p.cz}, clo, chi);
             if (fr == kReject) continue;
-            if (!ANY && clo > best.thi) continue;  // certainly beyond the current limit (also when undecided)
-            if (fr == kAccept && !(p.flags & kPrimMaskTexture)) {
-                if (!prim_accepts(s, base + j, p.flags, q.d)) continue;
-                if (ANY) {
-                    best.prim = base + j;
-                    best.id = p.id;
-                    best.exact = false;
-                    best.tlo = clo;
-                    best.thi = chi;
-                    return true;
-                }
-                if (chi < best.tlo) {  // certainly nearer than the current best
-                    best.prim = base + j;
-                    best.id = p.id;
-                    best.exact = false;
-                    best.tlo = clo;
-                    best.thi = chi;
-                    continue;
-                }
-            }
-            best.need_exact = true;
-            if (!ANY) return true;
+            if (fp32_take<ANY, MASK>(s, q, best, base + j, p.id, p.flags, fr, clo, chi)) return true;
         }
     }
     return false;
@@ -432,7 +620,7 @@ struct LocalRing {
     }
 };
 
-template <bool ANY, bool EXACT = true, class Ring = LocalRing, bool PRELOAD = (CR_LEAF_PRELOAD != 0)>
+template <bool ANY, bool EXACT = true, class Ring = LocalRing, bool PRELOAD = (CR_LEAF_PRELOAD != 0), bool MASK = true>
 struct Traversal {
     ClosestResult best;
     StackEntry stack[kStackSize];  // overflow: entries older than the ring's
@@ -478,6 +666,7 @@ struct Traversal {
         best.overflow = false;
         best.tex_error = false;
         best.need_exact = false;
+        best.ncand = 0;
         best.t = sf::from_float(q.tmax);
         best.u = best.v = sf::zero(0);
         best.tlo = best.thi = q.tmax;
@@ -493,8 +682,8 @@ struct Traversal {
 
     CR_HDI bool test_leaf(const DevScene& s, RayQuery& q, uint32_t r, TraceCount& tc) {
         const PrimRec* pr = CR_CACHE_PTRS ? prims : s.prims;
-        if constexpr (!EXACT && PRELOAD) return leaf_fast<ANY>(s, pr, q, leaf_first(r), leaf_count(r), best, tc);
-        else return leaf_prims<ANY, EXACT>(s, pr, q, leaf_first(r), leaf_count(r), best, tc);
+        if constexpr (!EXACT && PRELOAD) return leaf_fast<ANY, MASK>(s, pr, q, leaf_first(r), leaf_count(r), best, tc);
+        else return leaf_prims<ANY, EXACT, MASK>(s, pr, q, leaf_first(r), leaf_count(r), best, tc);
     }
 
     // Visit of the inner node `ref`: pushes the farther hit children, moves to
@@ -646,6 +835,66 @@ CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, TraceCount& 
 CR_HDI ExactHit exact_of(const DevScene& s, RayQuery& q, uint32_t prim) {
     const PrimRec p = s.prims[prim];
     return exact_triangle(q, V3{p.ax, p.ay, p.az}, V3{p.bx, p.by, p.bz}, V3{p.cx, p.cy, p.cz}, sf::inf(0));
+}
+
+// Exact pass over the candidates an FP32-only traversal left (exact_work 1):
+// the best so far and every candidate that may lie within its bound are
+// evaluated in binary64 and ordered by (t, id) as the reference orders hits
+// (reference src/geometry.cpp:609); any-hit stops at the first candidate that
+// hits.  Every primitive the FP32 pass did not keep was decided there (certain
+// miss, or certainly farther than a certain hit), so this equals the exact
+// traversal's result.
+template <bool ANY>
+CR_HD inline ClosestResult resolve_candidates(const DevScene& s, RayQuery& q, const Pending& pd, TraceCount& tc) {
+    ClosestResult best;
+    best.prim = kNoTriangle;
+    best.id = 0;
+    best.exact = true;
+    best.overflow = false;
+    best.tex_error = false;
+    best.need_exact = false;
+    best.ncand = 0;
+    best.t = sf::from_float(q.tmax);
+    best.u = best.v = sf::zero(0);
+    best.tlo = best.thi = q.tmax;
+    if (!ANY && pd.best != kNoTriangle) {
+        ++tc.fallbacks;
+        take_exact(best, pd.best, s.prims[pd.best].id, exact_of(s, q, pd.best));
+    }
+    for (uint32_t k = 0; k < pd.n; ++k) {
+        if (!ANY && pd.lo[k] > pd.thi) continue;
+        const uint32_t i = pd.cand[k];
+        const CR_GLOBAL PrimRec& p = gref(s.prims + i);
+        ++tc.fallbacks;
+        const ExactHit e = exact_triangle(q, V3{p.ax, p.ay, p.az}, V3{p.bx, p.by, p.bz}, V3{p.cx, p.cy, p.cz},
+                                          ANY ? sf::inf(0) : best.t);
+        if (!e.hit) continue;
+        if (!prim_accepts(s, i, p.flags, q.d)) continue;
+        if (p.flags & kPrimMaskTexture) {
+            bool bad = false;
+            const bool keep = mask_accepts(s, i, p.material, e.u, e.v, bad);
+            if (bad) {
+                best.tex_error = true;
+                return best;
+            }
+            if (!keep) continue;
+        }
+        if (ANY) {
+            take_exact(best, i, p.id, e);
+            return best;
+        }
+        if (best.prim == kNoTriangle || sf::lt(e.t, best.t) || (sf::eq(e.t, best.t) && p.id < best.id))
+            take_exact(best, i, p.id, e);
+    }
+    return best;
+}
+
+// The exact pass the kernels run after an FP32-only traversal that left work
+// (exact_work != 0): candidate resolution, or the exact traversal.
+template <bool ANY>
+CR_HD inline ClosestResult exact_pass(const DevScene& s, RayQuery& q, const Pending& pd, TraceCount& tc) {
+    if (pd.full) return traverse<ANY, true>(s, q, tc);
+    return resolve_candidates<ANY>(s, q, pd, tc);
 }
 
 }  // namespace cr

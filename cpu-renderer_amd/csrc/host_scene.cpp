@@ -94,22 +94,27 @@ HostScene finalize_scene(const crg_scene_desc& d) {
     HostScene s;
     s.materials.assign(d.materials, d.materials + d.material_count);
     s.lights.assign(d.lights, d.lights + d.light_count);
+    // Images in order: dimensions, then finiteness (the first failing image is
+    // reported), copied into texel records in the same parallel pass.
     for (uint32_t i = 0; i < d.image_count; ++i) {
         const crg_image& im = d.images[i];
         if (im.width <= 0 || im.height <= 0 || !im.rgba)
             throw std::runtime_error("Image " + std::to_string(i) + " has invalid dimensions or pixel count");
-        const size_t n = size_t(im.width) * size_t(im.height) * 4;
-        for (size_t k = 0; k < n; ++k)
-            if (!fin(im.rgba[k])) throw std::runtime_error("Image " + std::to_string(i) + " contains non-finite pixels");
-    }
-    for (uint32_t i = 0; i < d.image_count; ++i) {
-        const crg_image& im = d.images[i];
         HImage h;
         h.width = im.width;
         h.height = im.height;
         h.texels.resize(size_t(im.width) * size_t(im.height));
-        for (size_t k = 0; k < h.texels.size(); ++k)
-            h.texels[k] = make_float4(im.rgba[4 * k], im.rgba[4 * k + 1], im.rgba[4 * k + 2], im.rgba[4 * k + 3]);
+        std::atomic<bool> finite{true};
+        parallel_chunks(h.texels.size(), 65536, [&](size_t b, size_t e, size_t) {
+            bool ok = true;
+            for (size_t k = b; k < e; ++k) {
+                const float* px = im.rgba + 4 * k;
+                ok &= fin(px[0]) & fin(px[1]) & fin(px[2]) & fin(px[3]);
+                h.texels[k] = make_float4(px[0], px[1], px[2], px[3]);
+            }
+            if (!ok) finite = false;
+        });
+        if (!finite) throw std::runtime_error("Image " + std::to_string(i) + " contains non-finite pixels");
         s.images.push_back(std::move(h));
     }
     for (uint32_t i = 0; i < d.texture_count; ++i) {
@@ -910,21 +915,38 @@ SceneArrays scene_arrays(const HostScene& hs) {
                 const int32_t img = hs.textures[size_t(slot->texture)].image;
                 if (img >= 0 && size_t(img) < srgb_use.size()) srgb_use[size_t(img)] = 1;
             }
+    size_t total = 0;
+    for (size_t i = 0; i < hs.images.size(); ++i) total += hs.images[i].texels.size() * (srgb_use[i] ? 2 : 1);
+    a.texels.resize(total);
+    // 8-bit texels (value k/255) take their decoded value from a table of the
+    // 256 decodes; any other value is decoded directly.
+    float srgb8[256];
+    for (int k = 0; k < 256; ++k) srgb8[k] = srgb_to_linear(float(k) / 255.0f);
+    const auto decode = [&](float v) {
+        const float r = std::nearbyint(v * 255.0f);
+        if (r >= 0.0f && r <= 255.0f && float(int(r)) / 255.0f == v && !std::signbit(v)) return srgb8[int(r)];
+        return srgb_to_linear(v);
+    };
+    size_t at = 0;
     for (size_t i = 0; i < hs.images.size(); ++i) {
         const HImage& im = hs.images[i];
-        DevImage d{uint64_t(a.texels.size()), uint64_t(a.texels.size()), im.width, im.height};
-        a.texels.insert(a.texels.end(), im.texels.begin(), im.texels.end());
+        DevImage d{at, at, im.width, im.height};
+        float4* plain = a.texels.data() + at;
+        at += im.texels.size();
+        float4* dec = nullptr;
         if (srgb_use[i]) {
-            d.srgb_offset = a.texels.size();
-            a.texels.resize(a.texels.size() + im.texels.size());
-            float4* out = a.texels.data() + d.srgb_offset;
-            parallel_chunks(im.texels.size(), 65536, [&](size_t b, size_t e, size_t) {
+            d.srgb_offset = at;
+            dec = a.texels.data() + at;
+            at += im.texels.size();
+        }
+        parallel_chunks(im.texels.size(), 65536, [&](size_t b, size_t e, size_t) {
+            std::memcpy(plain + b, im.texels.data() + b, (e - b) * sizeof(float4));
+            if (dec)
                 for (size_t k = b; k < e; ++k) {
                     const float4 v = im.texels[k];
-                    out[k] = make_float4(srgb_to_linear(v.x), srgb_to_linear(v.y), srgb_to_linear(v.z), v.w);
+                    dec[k] = make_float4(decode(v.x), decode(v.y), decode(v.z), v.w);
                 }
-            });
-        }
+        });
         a.images.push_back(d);
     }
     for (const crg_texture& t : hs.textures) a.textures.push_back(DevTexture{t.image, t.wrap_s, t.wrap_t, t.linear});

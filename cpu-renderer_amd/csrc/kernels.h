@@ -42,9 +42,11 @@ constexpr const char* kStageNames[kStageCount] = {"k_extend",       "k_extend_ex
 struct Queues {
     uint32_t* q[2];   // extend queues (current / next iteration)
     uint32_t* retire; // finished paths
-    uint32_t* xq;     // extend rays needing the exact traversal
-    uint32_t* xsq;    // shadow records needing the exact traversal
+    uint32_t* xq;     // extend rays needing the exact pass
+    uint32_t* xsq;    // shadow records needing the exact pass
     uint32_t* freeq;  // stack of idle path slots
+    Pending* xpend;   // per xq entry: what the FP32 pass left undecided
+    Pending* xspend;  // per xsq entry
 };
 
 // Launch shape of the wavefront kernels (persistent-style grids that loop
@@ -60,6 +62,8 @@ hipError_t launch_pool_init(uint32_t* freeq, uint32_t pool, Control* ctl, hipStr
 hipError_t launch_admit(Control* ctl, JobCounters* cnt, uint64_t total, uint32_t seq_lo, uint32_t seq_hi, uint32_t slot,
                         hipStream_t st);
 hipError_t launch_accumulate(const DevJob* jobs, uint32_t job, uint64_t npix, float* film, hipStream_t st);
+// Selects the traversal kernels' variant: with FP32 alpha-mask tests (masks != 0) or without.
+hipError_t launch_set_masks(Control* ctl, uint32_t masks, hipStream_t st);
 // Film encode: display bytes (ldr, 3 per pixel; skipped when null) and RGBE in
 // per-row planes (rgbe: row-major rows of 4 planes of `width` bytes; skipped
 // when null); first_bad[0] / [1] (preset to ~0u) receive the lowest pixel index

@@ -327,12 +327,17 @@ __device__ inline uint32_t encode_hit(const DevScene& s, const ClosestResult& b)
     return b.prim | (((s.prims[b.prim].flags >> kPrimClassShift) & 3u) << 30);
 }
 
+// Both traversal kernels come in two variants, with and without the FP32
+// alpha-mask tests (MASK); the one that does not match ctl->masks returns at
+// once, so an iteration runs the mask variant only while a live job needs it.
+template <bool MASK>
 __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_extend(PathSoA P, const DevJob* jobs, const uint32_t* queue, uint32_t* xq,
-                                                int parity, Control* ctl, JobCounters* cnt) {
+                                                Pending* xpend, int parity, Control* ctl, JobCounters* cnt) {
+    if ((ctl->masks != 0) != MASK) return;
     const uint32_t n = ctl->qcount[parity];
     __shared__ uint32_t ring_ref[CR_LDS_RING * 256];
     __shared__ float ring_near[CR_LDS_RING * 256];
-    Traversal<false, false, TraverseRing> tr;
+    Traversal<false, false, TraverseRing, (CR_LEAF_PRELOAD != 0), MASK> tr;
     tr.ring = TraverseRing{(lds_u32*)(ring_ref + threadIdx.x), (lds_f32*)(ring_near + threadIdx.x), blockDim.x};
     RayQuery q;
     const DevScene* sc = nullptr;
@@ -360,10 +365,13 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_extend(PathSoA 
         }
         bool fin = false;
         run_visits(tr, sc, q, tc, busy, fin);
-        const bool exact = fin && tr.best.need_exact && !tr.best.overflow;
-        if (fin && !exact) P.ray[path].hit = encode_hit(*sc, tr.best);
-        const uint32_t pos = wave_append(&ctl->xcount, exact);
-        if (exact) xq[pos] = path;
+        const int work = fin ? exact_work<false>(tr.best) : 0;
+        if (fin && !work) P.ray[path].hit = encode_hit(*sc, tr.best);
+        const uint32_t pos = wave_append(&ctl->xcount, work != 0);
+        if (work) {
+            xq[pos] = path;
+            xpend[pos] = pending_of(tr.best, work);
+        }
         wave_add_keyed(cnt->extend_nodes, job, fin ? tc.nodes : 0u);
         wave_add_keyed(cnt->extend_tris, job, fin ? tc.tris : 0u);
     }
@@ -379,9 +387,9 @@ __device__ inline uint32_t spread_first() {
 }
 __device__ inline uint32_t spread_stride() { return spread_waves() * 64; }
 
-// Closest hit with binary64-exact resolution of undecided tests.
-__global__ void __launch_bounds__(256) k_extend_exact(PathSoA P, const DevJob* jobs, const uint32_t* xq, Control* ctl,
-                                                      JobCounters* cnt) {
+// Closest hit with binary64-exact resolution of what the FP32 pass left.
+__global__ void __launch_bounds__(256) k_extend_exact(PathSoA P, const DevJob* jobs, const uint32_t* xq,
+                                                      const Pending* xpend, Control* ctl, JobCounters* cnt) {
     const uint32_t n = ctl->xcount;
     for (uint32_t i = spread_first(); i < n; i += spread_stride()) {
         const uint32_t path = xq[i];
@@ -391,7 +399,7 @@ __global__ void __launch_bounds__(256) k_extend_exact(PathSoA P, const DevJob* j
         RayQuery q;
         setup_query(q, xyz(r.o), xyz(r.d), r.o.w, r.d.w, xyz(r.lo), r.prev);
         TraceCount tc{0, 0, 0};
-        const ClosestResult best = traverse<false, true>(s, q, tc);
+        const ClosestResult best = exact_pass<false>(s, q, xpend[i], tc);
         P.ray[path].hit = encode_hit(s, best);
         atomicAdd(&cnt->fallbacks[job], (unsigned long long)tc.fallbacks);
         atomicAdd(&cnt->exact_rays[job], 1ull);
@@ -506,12 +514,14 @@ __device__ inline void shadow_query(RayQuery& q, const ShadowSoA& S, uint32_t i)
 
 // Persistent FP32-only visibility (same scheme as k_extend); undecided
 // segments without a certain blocker go to the exact queue.
+template <bool MASK>
 __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_shadow(PathSoA P, ShadowSoA S, const DevJob* jobs, uint32_t* xsq,
-                                                Control* ctl, JobCounters* cnt) {
+                                                Pending* xspend, Control* ctl, JobCounters* cnt) {
+    if ((ctl->masks != 0) != MASK) return;
     const uint32_t n = ctl->scount;
     __shared__ uint32_t ring_ref[CR_LDS_RING * 256];
     __shared__ float ring_near[CR_LDS_RING * 256];
-    Traversal<true, false, TraverseRing> tr;
+    Traversal<true, false, TraverseRing, (CR_LEAF_PRELOAD != 0), MASK> tr;
     tr.ring = TraverseRing{(lds_u32*)(ring_ref + threadIdx.x), (lds_f32*)(ring_near + threadIdx.x), blockDim.x};
     RayQuery q;
     const DevScene* sc = nullptr;
@@ -538,20 +548,23 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_shadow(PathSoA 
             continue;
         }
         run_visits(tr, sc, q, tc, busy, fin);
-        bool exact = false;
+        int work = 0;
         if (fin && tr.best.prim == kNoTriangle && !tr.best.overflow) {
-            if (tr.best.need_exact) exact = true;
-            else add_contribution(P, S, i);
+            work = exact_work<true>(tr.best);
+            if (!work) add_contribution(P, S, i);
         }
-        const uint32_t pos = wave_append(&ctl->xscount, exact);
-        if (exact) xsq[pos] = i;
+        const uint32_t pos = wave_append(&ctl->xscount, work != 0);
+        if (work) {
+            xsq[pos] = i;
+            xspend[pos] = pending_of(tr.best, work);
+        }
         wave_add_keyed(cnt->shadow_nodes, job, fin ? tc.nodes : 0u);
         wave_add_keyed(cnt->shadow_tris, job, fin ? tc.tris : 0u);
     }
 }
 
 __global__ void __launch_bounds__(256) k_shadow_exact(PathSoA P, ShadowSoA S, const DevJob* jobs, const uint32_t* xsq,
-                                                      Control* ctl, JobCounters* cnt) {
+                                                      const Pending* xspend, Control* ctl, JobCounters* cnt) {
     const uint32_t n = ctl->xscount;
     for (uint32_t j = spread_first(); j < n; j += spread_stride()) {
         const uint32_t i = xsq[j];
@@ -560,7 +573,7 @@ __global__ void __launch_bounds__(256) k_shadow_exact(PathSoA P, ShadowSoA S, co
         RayQuery q;
         shadow_query(q, S, i);
         TraceCount tc{0, 0, 0};
-        const ClosestResult b = traverse<true, true>(s, q, tc);
+        const ClosestResult b = exact_pass<true>(s, q, xspend[j], tc);
         if (b.tex_error) {
             const uint32_t path = S.path[i];
             const uint64_t key = uint64_t(P.rec[path].pixel) * uint64_t(jobs[job].spp) + P.rec[path].sample;
@@ -743,20 +756,22 @@ hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const
     const dim3 g(shape.grid), b(shape.block);
     switch (stage) {
         case kStageExtend:
-            hipLaunchKernelGGL(k_extend, g, b, 0, st, P, jobs, Q.q[parity], Q.xq, parity, ctl, cnt);
+            hipLaunchKernelGGL(k_extend<false>, g, b, 0, st, P, jobs, Q.q[parity], Q.xq, Q.xpend, parity, ctl, cnt);
+            hipLaunchKernelGGL(k_extend<true>, g, b, 0, st, P, jobs, Q.q[parity], Q.xq, Q.xpend, parity, ctl, cnt);
             break;
         case kStageExtendExact:
-            hipLaunchKernelGGL(k_extend_exact, g, b, 0, st, P, jobs, Q.xq, ctl, cnt);
+            hipLaunchKernelGGL(k_extend_exact, g, b, 0, st, P, jobs, Q.xq, Q.xpend, ctl, cnt);
             break;
         case kStageShade:
             hipLaunchKernelGGL(k_shade, g, b, 0, st, P, S, jobs, Q.q[parity], Q.q[parity ^ 1], Q.retire, parity, ctl,
                                cnt);
             break;
         case kStageShadow:
-            hipLaunchKernelGGL(k_shadow, g, b, 0, st, P, S, jobs, Q.xsq, ctl, cnt);
+            hipLaunchKernelGGL(k_shadow<false>, g, b, 0, st, P, S, jobs, Q.xsq, Q.xspend, ctl, cnt);
+            hipLaunchKernelGGL(k_shadow<true>, g, b, 0, st, P, S, jobs, Q.xsq, Q.xspend, ctl, cnt);
             break;
         case kStageShadowExact:
-            hipLaunchKernelGGL(k_shadow_exact, g, b, 0, st, P, S, jobs, Q.xsq, ctl, cnt);
+            hipLaunchKernelGGL(k_shadow_exact, g, b, 0, st, P, S, jobs, Q.xsq, Q.xspend, ctl, cnt);
             break;
         case kStageRetire:
             hipLaunchKernelGGL(k_retire, g, b, 0, st, P, jobs, Q.retire, Q.q[parity ^ 1], Q.freeq, parity, ctl, cnt);
@@ -773,6 +788,13 @@ hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const
 
 hipError_t launch_pool_init(uint32_t* freeq, uint32_t pool, Control* ctl, hipStream_t st) {
     hipLaunchKernelGGL(k_pool_init, dim3(blocks_for(pool, 256)), dim3(256), 0, st, freeq, pool, ctl);
+    return hipGetLastError();
+}
+
+__global__ void k_set_masks(Control* ctl, uint32_t masks) { ctl->masks = masks; }
+
+hipError_t launch_set_masks(Control* ctl, uint32_t masks, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_masks, dim3(1), dim3(1), 0, st, ctl, masks);
     return hipGetLastError();
 }
 

@@ -41,6 +41,24 @@ CR_HDI const CR_GLOBAL T& gref(const T* p) {
     return *(const CR_GLOBAL T*)p;
 }
 
+// A query the FP32-only traversal could not decide alone: its certain nearest
+// hit so far and up to kCand candidate primitives (undecided tests, alpha-mask
+// primitives, certain hits whose order against the best was undecided), for
+// the exact pass to resolve in binary64 without traversing again; `full` set
+// when more than kCand candidates were met (exact traversal instead).
+#ifndef CR_CAND
+#define CR_CAND 4
+#endif
+constexpr uint32_t kCand = CR_CAND;
+struct alignas(16) Pending {
+    uint32_t best;  // leaf-order primitive, kNoTriangle when none
+    uint32_t n;     // candidates stored
+    float thi;      // upper bound of the best's t (the query's tmax when none)
+    uint32_t full;
+    uint32_t cand[kCand];
+    float lo[kCand];  // lower bound of each candidate's t (0 when unknown)
+};
+
 // (cos, sin) of the sampling angle phi = 2*pi*u for one 24-bit draw u = k/2^24;
 // the table holds all kTurnEntries draws, evaluated once on the host.
 struct alignas(8) TurnCS {
@@ -297,6 +315,7 @@ struct Control {
     uint32_t err_count;        // monotonic; records go to errors[n % kMaxErrors]
     uint32_t ehead, shead;     // next unclaimed entry of the extend / shadow queue
     uint32_t pool;             // path slots (capacity of every queue)
+    uint32_t masks;            // a live job's scene has alpha-mask primitives: traversal variant with FP32 mask tests
     unsigned long long issued;
     unsigned long long total;
     unsigned long long refill_base;  // first item of the refill range

@@ -459,3 +459,16 @@ def random_rays(scene: abi.SceneData, count: int, seed: int, spread: float = 1.5
     tmax = np.where(i % 3 == 0, 1.0 + 29.0 * rng.random(count), np.inf)
     rays["tmax"] = tmax.astype(np.float32)
     return rays
+
+
+def scene_fixture(path) -> Tuple[abi.SceneData, abi.Camera]:
+    """A scene stored as a fixture file: the arrays of a crg_scene_desc (materials, lights and
+    textures as their C records' bytes, texels as 8-bit codes k with value float32(k) / 255) and
+    one camera, as written from a glTF asset loaded by the reference's own loader."""
+    z = np.load(path)
+    rec = lambda cls, key: [cls.from_buffer_copy(z[key][i].tobytes()) for i in range(len(z[key]))]
+    images = [z[f"image{i}"].astype(np.float32) / np.float32(255) for i in range(int(z["image_count"]))]
+    scene = abi.SceneData(z["positions"], z["material_ids"], rec(abi.Material, "materials"), normals=z["normals"],
+                          tangents=z["tangents"], uvs=z["uvs"], lights=rec(abi.Light, "lights"), images=images,
+                          textures=rec(abi.Texture, "textures"))
+    return scene, abi.Camera.from_buffer_copy(z["camera"].tobytes())

@@ -57,9 +57,9 @@ struct EmuEnv {
 // rounds placed by a pseudo-random schedule: any interleaving must give the
 // same result as step().  Odd schedules use the plain leaf test, even ones the
 // preloading one.
-template <bool ANY, bool PRELOAD>
+template <bool ANY, bool PRELOAD, bool MASK = true>
 ClosestResult phased(const DevScene& s, RayQuery& q, TraceCount& tc, uint64_t& rng) {
-    Traversal<ANY, false, LocalRing, PRELOAD> tr;
+    Traversal<ANY, false, LocalRing, PRELOAD, MASK> tr;
     if (tr.begin(s, q)) return tr.best;
     for (;;) {
         if (tr.inner(s, q, tc)) break;
@@ -195,6 +195,39 @@ void emu_env_sample(void* e, uint64_t* rng, float* dir, float* pdf) {
     dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
     *rng = r.state;
 }
+// The FP32 alpha-mask decision (mask_fp32) against the binary64 one, for each
+// ray and each alpha-mask primitive the ray certainly hits in FP32: counts[0]
+// FP32 accepts, [1] FP32 rejects, [2] undecided, [3] FP32 decisions that
+// contradict the binary64 test, [4] certain FP32 hits the binary64 test misses.
+int emu_mask_check(void* sp, const crg_ray* rays, uint64_t n, uint64_t* counts) {
+    const EmuScene& s = *static_cast<EmuScene*>(sp);
+    for (uint64_t k = 0; k < n; ++k) {
+        const crg_ray& r = rays[k];
+        RayQuery q;
+        setup_query(q, V3{r.origin[0], r.origin[1], r.origin[2]}, V3{r.direction[0], r.direction[1], r.direction[2]},
+                    r.tmin, r.tmax, V3{r.origin_low[0], r.origin_low[1], r.origin_low[2]}, r.ignore);
+        if (!q.valid) continue;
+        for (uint32_t i = 0; i < s.dev.prim_count; ++i) {
+            const PrimRec& p = s.dev.prims[i];
+            if (!(p.flags & kPrimMaskTexture)) continue;
+            const V3 va{p.ax, p.ay, p.az}, vb{p.bx, p.by, p.bz}, vc{p.cx, p.cy, p.cz};
+            float lo = 0, hi = 0;
+            if (fast_triangle(q, va, vb, vc, lo, hi) != kAccept) continue;
+            const int mr = mask_fp32(s.dev, q, i, p.material, va, vb, vc);
+            const ExactHit e = exact_triangle(q, va, vb, vc, sf::inf(0));
+            if (!e.hit) {
+                ++counts[4];
+                continue;
+            }
+            bool bad = false;
+            const bool keep = mask_accepts(s.dev, i, p.material, e.u, e.v, bad);
+            ++counts[mr == kAccept ? 0 : mr == kReject ? 1 : 2];
+            if (bad || (mr == kAccept && !keep) || (mr == kReject && keep)) ++counts[3];
+        }
+    }
+    return 0;
+}
+
 // The device's expf (binary64-emulated, correctly rounded) over n arguments.
 void emu_expf(const float* x, float* out, uint64_t n) {
     for (uint64_t i = 0; i < n; ++i) out[i] = sf::expf_cr(x[i]);
@@ -299,13 +332,17 @@ int emu_trace(void* sp, const crg_ray* rays, uint64_t n, crg_hit* hits, int any,
         uint64_t rng = schedule * 0x9e3779b97f4a7c15ull + i;
         ClosestResult b;
         if (!schedule) b = any ? traverse<true, false>(s.dev, q, tc) : traverse<false, false>(s.dev, q, tc);
+        else if (schedule & 4)  // the traversal variant without FP32 alpha-mask tests
+            b = any ? phased<true, false, false>(s.dev, q, tc, rng) : phased<false, false, false>(s.dev, q, tc, rng);
         else if (schedule & 1) b = any ? phased<true, false>(s.dev, q, tc, rng) : phased<false, false>(s.dev, q, tc, rng);
         else b = any ? phased<true, true>(s.dev, q, tc, rng) : phased<false, true>(s.dev, q, tc, rng);
-        const bool rerun = b.need_exact && (!any || b.prim == kNoTriangle);
+        const int work = any ? exact_work<true>(b) : exact_work<false>(b);
+        const bool rerun = work != 0;
         if (rerun) {
+            const Pending pd = pending_of(b, work);
             setup_query(q, V3{r.origin[0], r.origin[1], r.origin[2]}, V3{r.direction[0], r.direction[1], r.direction[2]},
                         r.tmin, r.tmax, V3{r.origin_low[0], r.origin_low[1], r.origin_low[2]}, r.ignore);
-            b = any ? traverse<true, true>(s.dev, q, tc) : traverse<false, true>(s.dev, q, tc);
+            b = any ? exact_pass<true>(s.dev, q, pd, tc) : exact_pass<false>(s.dev, q, pd, tc);
         }
         fbt += rerun ? 1 : 0;
         if (b.overflow) {
@@ -396,10 +433,10 @@ int emu_render(void* sp, void* ep, const crg_camera* camera, const crg_options* 
                         TraceCount tc{0, 0, 0};
                         // Same two-phase order as k_extend / k_extend_exact.
                         ClosestResult c = traverse<false, false>(s.dev, q, tc);
-                        if (c.need_exact && !c.overflow) {
+                        if (const int work = exact_work<false>(c)) {
                             RayQuery qx;
                             setup_query(qx, st.o, st.d, st.tmin, st.tmax, st.lo, st.prev);
-                            c = traverse<false, true>(s.dev, qx, tc);
+                            c = exact_pass<false>(s.dev, qx, pending_of(c, work), tc);
                         }
                         ++ext;
                         if (c.overflow || c.tex_error) {
@@ -416,10 +453,10 @@ int emu_render(void* sp, void* ep, const crg_camera* camera, const crg_options* 
                             RayQuery sq;
                             setup_query(sq, sh.o, sh.d, 0, sh.tmax, sh.lo, sh.ignore);
                             ClosestResult b = traverse<true, false>(s.dev, sq, tc);
-                            if (b.prim == kNoTriangle && b.need_exact) {
+                            if (const int work = b.overflow ? 0 : exact_work<true>(b)) {
                                 RayQuery sx;
                                 setup_query(sx, sh.o, sh.d, 0, sh.tmax, sh.lo, sh.ignore);
-                                b = traverse<true, true>(s.dev, sx, tc);
+                                b = exact_pass<true>(s.dev, sx, pending_of(b, work), tc);
                             }
                             ++shd;
                             if (b.tex_error) {

@@ -47,6 +47,7 @@ class Emu:
             "emu_trace": (C.c_int, [vp, vp, u64, vp, C.c_int, C.POINTER(u64), u64]),
             "emu_box_hit": (C.c_int, [fp, fp, C.c_float, C.c_float, fp, fp]),
             "emu_expf": (None, [fp, fp, u64]),
+            "emu_mask_check": (C.c_int, [vp, vp, u64, C.POINTER(u64)]),
             "emu_film_png8": (C.c_int, [fp, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_uint8)]),
             "emu_film_hdr": (C.c_int, [fp, C.c_int, C.c_int, C.POINTER(C.c_uint8), u64, C.POINTER(u64)]),
             "emu_camera_generate": (C.c_int, [C.POINTER(abi.Camera), C.c_float, C.c_float, C.c_int, C.c_int,
@@ -109,7 +110,8 @@ class EmuScene:
 
     def trace(self, rays, any_hit=False, schedule=0):
         """schedule 0: one node or leaf per step; k > 0: postponed-leaf phases in a
-        pseudo-random interleaving seeded by k (FP32 pass only)."""
+        pseudo-random interleaving seeded by k (FP32 pass only); bit 2 set: the traversal
+        variant without FP32 alpha-mask tests."""
         rays = np.ascontiguousarray(rays, abi.RAY_DTYPE)
         hits = np.zeros(len(rays), abi.HIT_DTYPE)
         fb = C.c_uint64()

@@ -385,22 +385,45 @@ def textured_alpha_mask():
     return scene, env, cam
 
 
+def alpha_mask_stress():
+    """Five alpha-masked quads over noise alpha textures (cutoff 0.5, base alpha 0.9): on a
+    32x32 image, bilinear + repeat under a rotated, scaled, offset transform; nearest + clamp;
+    bilinear + mirror on UV set 1; nearest + repeat with a negative scale; on a 512x512 image
+    tiled 37x29 times, where the barycentric error spans a hundredth of a texel.  Bilinear
+    noise puts many hits near the cutoff, so the FP32 mask decision is exercised where it
+    must defer."""
+    images = [_image(32, 32, 11, "noise"), _image(512, 512, 12, "noise")]
+    textures = [abi.Texture(0, 0, 0, 1), abi.Texture(0, 1, 1, 0), abi.Texture(0, 2, 2, 1), abi.Texture(0, 0, 0, 0),
+                abi.Texture(1, 0, 2, 1)]
+    slots = [_slot(0, 0, (0.13, -0.4), (3.0, 2.5), 0.7), _slot(1, 0, (-0.2, 0.1), (1.4, 1.4), 0.0),
+             _slot(2, 1, (0.05, 0.3), (2.2, -1.7), -1.1), _slot(3, 0, (0.0, 0.0), (-4.0, 3.0), 0.25),
+             _slot(4, 0, (0.31, 0.77), (37.0, 29.0), 0.3)]
+    mats = []
+    for sl in slots:
+        m = abi.material(base_color=(0.6, 0.6, 0.6, 0.9), metallic=0.0, roughness=0.7, alpha_mode=1,
+                         alpha_cutoff=0.5, double_sided=True)
+        m.base_color_tex = sl
+        mats.append(m)
+    pos, uvs, mid = [], [], []
+    for q, z in enumerate((0.0, -0.7, -1.4, -2.1, -2.8)):
+        a, b, c, d = (-1, -1, z), (1, -1, z), (1, 1, z), (-1, 1, z)
+        for tri, tuv in (((a, b, c), ((0, 0), (1, 0), (1, 1))), ((a, c, d), ((0, 0), (1, 1), (0, 1)))):
+            pos.append(np.concatenate([_v(p) for p in tri]))
+            uvs.append(np.concatenate([[u[0], u[1], 0.3 + 1.7 * u[1], -0.2 + 1.3 * u[0]] for u in tuv]))
+            mid.append(q)
+    return abi.SceneData(np.array(pos, np.float32), np.array(mid, np.uint32), mats, uvs=np.array(uvs, np.float32),
+                         images=images, textures=textures)
+
+
 GLTF_FIXTURE = "gltf_export_contract.npz"
 HARBOR_FIXTURE = "large/harbor.npz"
 
 
 def _gltf_fixture(name):
     import pathlib
-    z = np.load(pathlib.Path(__file__).resolve().parent / "golden" / name)
-    materials = [abi.Material.from_buffer_copy(z["materials"][i].tobytes()) for i in range(len(z["materials"]))]
-    lights = [abi.Light.from_buffer_copy(z["lights"][i].tobytes()) for i in range(len(z["lights"]))]
-    textures = [abi.Texture.from_buffer_copy(z["textures"][i].tobytes()) for i in range(len(z["textures"]))]
-    images = [z[f"image{i}"].astype(np.float32) / np.float32(255) for i in range(int(z["image_count"]))]
-    scene = abi.SceneData(z["positions"], z["material_ids"], materials, normals=z["normals"],
-                          tangents=z["tangents"], uvs=z["uvs"], lights=lights, images=images, textures=textures)
-    cam = abi.Camera.from_buffer_copy(z["camera"].tobytes())
-    env = abi.EnvData(constant=(0.32, 0.36, 0.42))
-    return scene, env, cam
+    from conftest import scenes
+    scene, cam = scenes.scene_fixture(pathlib.Path(__file__).resolve().parent / "golden" / name)
+    return scene, abi.EnvData(constant=(0.32, 0.36, 0.42)), cam
 
 
 def gltf_export_contract():

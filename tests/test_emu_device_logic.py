@@ -21,7 +21,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, abi, load_golden
+from conftest import GOLDEN, abi, load_golden, scenes
 
 sys.path.insert(0, str(GOLDEN))
 import make_golden  # noqa: E402
@@ -48,7 +48,7 @@ def test_emu_trace_bit_exact(emu, name):
     np.testing.assert_array_equal(es.trace(rays, True)["hit"], g[f"{name}/any"])
     # Postponed-leaf traversal (plain and preloaded leaf tests): any
     # interleaving of inner and leaf rounds gives the same closest and any hits.
-    for schedule in (1, 2):
+    for schedule in (1, 2, 5):
         hp = es.trace(rays, schedule=schedule)
         np.testing.assert_array_equal(hp, h)
         np.testing.assert_array_equal(es.trace(rays, True, schedule=schedule)["hit"], g[f"{name}/any"])
@@ -247,3 +247,43 @@ def test_expf_is_correctly_rounded(emu):
     nan = np.array([np.nan], np.float32)
     emu.lib.emu_expf(nan.ctypes.data_as(fp), out.ctypes.data_as(fp), 1)
     assert np.isnan(out[0])
+
+
+def _camera_rays(emu, cam, width, height):
+    rays = np.zeros(width * height, abi.RAY_DTYPE)
+    r = abi.Ray()
+    for y in range(height):
+        for x in range(width):
+            emu.lib.emu_camera_generate(C.byref(cam), x + 0.37, y + 0.61, width, height, C.byref(r))
+            rays[y * width + x] = np.frombuffer(bytes(r), abi.RAY_DTYPE)[0]
+    return rays
+
+
+@pytest.mark.parametrize("case", ["stress", "tex_mask_random", "tex_mask_camera", "gltf_contract"])
+def test_fp32_mask_decision_is_conservative(emu, case):
+    """The traversal decides alpha masks in FP32 where the footprint's texels allow it
+    (mask_fp32); the decision must never contradict the binary64 MASK test at the exact
+    barycentrics (reference src/geometry.cpp:279-292), and must decide most hits of
+    ordinary rays (random rays include many grazing ones, whose barycentrics are loose)."""
+    import ref_scenes as rs
+    if case == "stress":
+        scene = rs.alpha_mask_stress()
+        rays, need = scenes.random_rays(scene, 60000, seed=21, spread=1.2), 0.7
+    elif case.startswith("tex_mask"):
+        scene, _, cam = rs.textured_alpha_mask()
+        if case.endswith("camera"):
+            rays, need = _camera_rays(emu, cam, 200, 160), 0.95
+        else:
+            rays, need = scenes.random_rays(scene, 60000, seed=21, spread=1.2), 0.3
+    else:
+        scene = rs.gltf_export_contract()[0]
+        rays, need = scenes.random_rays(scene, 20000, seed=21, spread=1.2), 0.8
+    sc = emu.scene(scene)
+    counts = (C.c_uint64 * 5)()
+    rays = np.ascontiguousarray(rays, abi.RAY_DTYPE)
+    assert emu.lib.emu_mask_check(sc.h, rays.ctypes.data, len(rays), counts) == 0
+    acc, rej, unsure, wrong, missed = counts[:]
+    print(f"{case}: accept {acc} reject {rej} undecided {unsure} contradictions {wrong} missed {missed}")
+    assert wrong == 0 and missed == 0
+    decided = acc + rej
+    assert decided > 0 and decided >= need * (decided + unsure)
