@@ -411,6 +411,9 @@ __global__ void __launch_bounds__(256) k_extend_exact(PathSoA P, const DevJob* j
 // first counting-sorted by shading class in LDS, so each wave shades one kind
 // of surface (or only misses) instead of paying for every branch.
 constexpr uint32_t kShadeChunk = 1024;
+#ifndef CR_SHADE_OCTSORT
+#define CR_SHADE_OCTSORT 1
+#endif
 
 __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, ShadowSoA S, const DevJob* jobs,
                                                                   const uint32_t* queue, uint32_t* next_queue,
@@ -418,6 +421,9 @@ __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, S
                                                                   JobCounters* cnt) {
     __shared__ uint32_t stage[kShadeChunk], sorted[kShadeChunk];
     __shared__ uint32_t counts[kShadeClasses], cursor[kShadeClasses];
+#if CR_SHADE_OCTSORT
+    __shared__ uint32_t ocount[8], ocursor[8], obase;
+#endif
     const uint32_t n = ctl->qcount[parity];
     const uint32_t per = kShadeChunk / blockDim.x;  // entries per thread and chunk
     for (uint32_t base = blockIdx.x * kShadeChunk; base < n; base += gridDim.x * kShadeChunk) {
@@ -437,11 +443,14 @@ __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, S
             for (uint32_t j = 0; j < c; ++j) off += counts[j];
             sorted[off + atomicAdd(&cursor[c], 1u)] = v & kHitPrimMask;
         }
+#if CR_SHADE_OCTSORT
+        if (threadIdx.x < 8) ocount[threadIdx.x] = 0;
+#endif
         __syncthreads();
         for (uint32_t r = 0; r < per; ++r) {
             const uint32_t k = r * blockDim.x + threadIdx.x;
             const bool live = k < m;
-            uint32_t path = 0, action = kRetire, ji = 0;
+            uint32_t path = 0, action = kRetire, ji = 0, oct = 0;
             bool replayed = false;
             ShadowOut sh;
             sh.emit = false;
@@ -474,6 +483,7 @@ __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, S
                     action = kContinue;
                 }
                 store_state(P, path, st);
+                oct = (st.d.x < 0 ? 1u : 0u) | (st.d.y < 0 ? 2u : 0u) | (st.d.z < 0 ? 4u : 0u);
             }
             wave_add_keyed(cnt->extend_rays, ji, live ? 1u : 0u);
             wave_add_keyed(cnt->shadow_rays, ji, live && sh.emit ? 1u : 0u);
@@ -487,11 +497,38 @@ __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, S
                 S.lo[spos] = make_float4(sh.lo.x, sh.lo.y, sh.lo.z, __uint_as_float(sh.ignore));
                 S.c[spos] = make_float4(sh.c.x, sh.c.y, sh.c.z, 0);
             }
+#if CR_SHADE_OCTSORT
+            if (live) {
+                const bool cont = action == kContinue;
+                stage[k] = cont ? (path | (oct << 29)) : 0xffffffffu;
+                if (cont) atomicAdd(&ocount[oct], 1u);
+            }
+#else
             const uint32_t qpos = wave_append(&ctl->qcount[parity ^ 1], live && action == kContinue);
             if (live && action == kContinue) next_queue[qpos] = path;
+#endif
             const uint32_t rpos = wave_append(&ctl->rcount, live && action == kRetire);
             if (live && action == kRetire) retire[rpos] = path;
         }
+#if CR_SHADE_OCTSORT
+        // The chunk's continuing paths go to the next extend queue grouped by
+        // direction octant, so the traversal's waves hold rays of one octant.
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (int o = 0; o < 8; ++o) {
+                ocursor[o] = t;
+                t += ocount[o];
+            }
+            obase = t ? atomicAdd(&ctl->qcount[parity ^ 1], t) : 0u;
+        }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
+            const uint32_t v = stage[k];
+            if (v == 0xffffffffu) continue;
+            next_queue[obase + atomicAdd(&ocursor[v >> 29], 1u)] = v & 0x1fffffffu;
+        }
+#endif
         __syncthreads();  // `sorted` is rewritten by the next chunk
     }
 }
