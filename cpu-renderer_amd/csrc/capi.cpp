@@ -170,7 +170,7 @@ std::shared_ptr<SceneRes> make_resident(crg_ctx* ctx, const SceneSections& a, st
     res->image_count = a.images.count;
     res->texture_count = a.textures.count;
     res->material_count = a.materials.count;
-    const std::vector<uint32_t> ranks = light_ranks(res->lights, res->tri_to_prim, n);
+    const std::vector<uint32_t> ranks = light_ranks(res->lights, res->prim_to_tri, n);
     // Alpha-mask primitives select the traversal variant with FP32 mask tests.
     std::atomic<bool> masks{false};
     parallel_chunks(a.prims.count, size_t(1) << 18, [&](size_t b, size_t e, size_t) {
@@ -211,7 +211,7 @@ std::shared_ptr<SceneRes> make_resident(crg_ctx* ctx, const SceneSections& a, st
                          res->images.as<DevImage>(),
                          res->textures.as<DevTexture>(),
                          uint32_t(a.nodes.count),
-                         uint32_t(n),
+                         uint32_t(a.prims.count),
                          uint32_t(res->lights.punctual.size()),
                          0};
     res->desc.alloc(sizeof(DevScene));
@@ -482,17 +482,18 @@ int crg_scene_save(crg_ctx* ctx, uint64_t scene_id, const char* path) {
             w.pod(n);
             w.pod(r->binary_nodes);
             auto nodes = download<WNode>(r->nodes, r->host.node_count, st);
-            auto prims = download<PrimRec>(r->prims, n, st);
-            auto ng = download<float4>(r->prim_ng, n, st);
-            auto aux = download<PrimAux>(r->aux, n, st);
+            const uint64_t np = r->host.prim_count;
+            auto prims = download<PrimRec>(r->prims, np, st);
+            auto ng = download<float4>(r->prim_ng, np, st);
+            auto aux = download<PrimAux>(r->aux, np, st);
             auto mats = download<DevMaterial>(r->materials, r->material_count, st);
-            auto uvs = download<float4>(r->uvs, r->host.uvs ? 3 * n : 0, st);
-            auto tans = download<float4>(r->tangents, r->host.tangents ? 3 * n : 0, st);
+            auto uvs = download<float4>(r->uvs, r->host.uvs ? 3 * np : 0, st);
+            auto tans = download<float4>(r->tangents, r->host.tangents ? 3 * np : 0, st);
             auto texels = download<float4>(r->texels, r->texel_count, st);
             auto images = download<DevImage>(r->images, r->image_count, st);
             auto textures = download<DevTexture>(r->textures, r->texture_count, st);
             check(hipStreamSynchronize(st), "scene download");
-            w.out.reserve(4096 + (nodes.size() * sizeof(WNode) + n * (sizeof(PrimRec) + sizeof(PrimAux) + 24) +
+            w.out.reserve(4096 + (nodes.size() * sizeof(WNode) + np * (sizeof(PrimRec) + sizeof(PrimAux) + 24) +
                                               (uvs.size() + tans.size() + ng.size() + texels.size()) * sizeof(float4) +
                                               mats.size() * sizeof(DevMaterial) + images.size() * sizeof(DevImage) +
                                               textures.size() * sizeof(DevTexture)));
@@ -559,9 +560,10 @@ int crg_scene_load(crg_ctx* ctx, const char* path, uint64_t* scene_id, crg_scene
         lights.emissive = rd.vec<uint32_t>();
         lights.emissive_weight = rd.vec<double>();
         lights.punctual = rd.vec<DevPunctual>();
-        if (rd.pos != rd.size || sec.prims.count != n || sec.prim_ng.count != n || sec.aux.count != n ||
-            order.size() != n || tri_to_prim.size() != n || (sec.uvs.count && sec.uvs.count != 3 * n) ||
-            (sec.tangents.count && sec.tangents.count != 3 * n) || (n && !sec.nodes.count) ||
+        const uint64_t np = sec.prims.count;  // leaf-order primitives (split triangles repeat)
+        if (rd.pos != rd.size || np < n || sec.prim_ng.count != np || sec.aux.count != np || order.size() != np ||
+            tri_to_prim.size() != n || (sec.uvs.count && sec.uvs.count != 3 * np) ||
+            (sec.tangents.count && sec.tangents.count != 3 * np) || (n && !sec.nodes.count) ||
             lights.emissive.size() != lights.emissive_weight.size() ||
             lights.punctual.size() != lights.punctual_weight.size())
             throw std::runtime_error(label + ": inconsistent section sizes");

@@ -412,6 +412,7 @@ template <bool ANY, bool MASK = true>
 CR_HDI bool fp32_take(const DevScene& s, const RayQuery& q, ClosestResult& best, uint32_t i, uint32_t id,
                       uint32_t flags, int fr, float clo, float chi) {
     if (!ANY && clo > best.thi) return false;           // certainly beyond the limit (also when undecided)
+    if (!ANY && id == best.id && best.prim != kNoTriangle) return false;  // another reference of the best's triangle
     if (!prim_accepts(s, i, flags, q.d)) return false;  // culling does not depend on t
     bool certain = fr == kAccept && !(flags & kPrimMaskTexture);
     if (MASK && fr == kAccept && !certain) {  // alpha mask: decided in FP32 where the texels allow

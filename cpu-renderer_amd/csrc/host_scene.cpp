@@ -549,6 +549,112 @@ struct Builder {
 
 }  // namespace
 
+// Early split clipping: a triangle whose box is far larger than the typical
+// one (ground planes, walls spanning a scene) inflates every node above it.
+// Its reference is split at the middle of the longest axis of its box, the
+// triangle clipped to each half (in binary64), and each half's clipped box
+// kept as a reference of its own, until a box is small enough (or 2^6
+// references).  Boxes are widened by 2^-20 of their extent and rounded
+// outward, so together they cover the triangle; the traversal tests the whole
+// triangle in every reference it reaches (leaf-order primitives repeat the
+// triangle; hit ordering is by triangle id, so repeats change no result).
+namespace {
+
+#ifndef CR_SPLIT_FACTOR
+#define CR_SPLIT_FACTOR 4.0
+#endif
+constexpr double kSplitFactor = CR_SPLIT_FACTOR;  // split boxes above this multiple of the mean box area
+#ifndef CR_SPLIT_DEPTH
+#define CR_SPLIT_DEPTH 6
+#endif
+constexpr int kSplitDepth = CR_SPLIT_DEPTH;
+
+using DPoint = std::array<double, 3>;
+
+// The part of a convex polygon with coordinate `axis` <= c (keep_below) or >= c.
+std::vector<DPoint> clip_half(const std::vector<DPoint>& poly, int axis, double c, bool keep_below) {
+    std::vector<DPoint> out;
+    const size_t k = poly.size();
+    for (size_t i = 0; i < k; ++i) {
+        const DPoint& a = poly[i];
+        const DPoint& b = poly[(i + 1) % k];
+        const double da = keep_below ? c - a[axis] : a[axis] - c, db = keep_below ? c - b[axis] : b[axis] - c;
+        if (da >= 0) out.push_back(a);
+        if ((da >= 0) != (db >= 0)) {
+            const double t = da / (da - db);
+            DPoint x;
+            for (int j = 0; j < 3; ++j) x[j] = a[j] + (b[j] - a[j]) * t;
+            x[axis] = c;
+            out.push_back(x);
+        }
+    }
+    return out;
+}
+
+void split_ref(const std::vector<DPoint>& poly, const Box& box, int depth, double limit, uint32_t id,
+               std::vector<BPrim>& out) {
+    if (depth >= kSplitDepth || box.area() <= limit) {
+        BPrim p;
+        for (int a = 0; a < 3; ++a) {
+            p.lo[a] = box.lo[a];
+            p.hi[a] = box.hi[a];
+            p.c[a] = float(0.5 * (double(box.lo[a]) + double(box.hi[a])));
+        }
+        p.id = id;
+        out.push_back(p);
+        return;
+    }
+    int axis = 0;
+    for (int a = 1; a < 3; ++a)
+        if (double(box.hi[a]) - box.lo[a] > double(box.hi[axis]) - box.lo[axis]) axis = a;
+    const double mid = 0.5 * (double(box.lo[axis]) + double(box.hi[axis]));
+    for (int side = 0; side < 2; ++side) {
+        const std::vector<DPoint> part = clip_half(poly, axis, mid, side == 0);
+        if (part.size() < 3) continue;
+        DPoint lo{INFINITY, INFINITY, INFINITY}, hi{-INFINITY, -INFINITY, -INFINITY};
+        for (const DPoint& q : part)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], q[a]);
+                hi[a] = std::max(hi[a], q[a]);
+            }
+        Box sub;
+        for (int a = 0; a < 3; ++a) {
+            const double pad = (hi[a] - lo[a]) * 0x1p-20 + (std::fabs(lo[a]) + std::fabs(hi[a])) * 0x1p-40;
+            sub.lo[a] = std::max(std::nextafter(float(lo[a] - pad), -INFINITY), box.lo[a]);
+            sub.hi[a] = std::min(std::nextafter(float(hi[a] + pad), INFINITY), box.hi[a]);
+        }
+        split_ref(part, sub, depth + 1, limit, id, out);
+    }
+}
+
+void split_large(const HostScene& s, std::vector<BPrim>& prims) {
+    const size_t n = prims.size();
+    double total = 0;
+    for (const BPrim& p : prims) {
+        Box b;
+        b.grow(p.lo, p.hi);
+        total += b.area();
+    }
+    const double limit = kSplitFactor * total / double(n);
+    std::vector<BPrim> extra;
+    for (size_t i = 0; i < n; ++i) {
+        Box b;
+        b.grow(prims[i].lo, prims[i].hi);
+        if (!(b.area() > limit) || extra.size() > n) continue;
+        const HTriangle& t = s.tris[prims[i].id];
+        std::vector<DPoint> poly(3);
+        for (int k = 0; k < 3; ++k) poly[k] = {t.p[k].x, t.p[k].y, t.p[k].z};
+        std::vector<BPrim> refs;
+        split_ref(poly, b, 0, limit, prims[i].id, refs);
+        if (refs.empty()) continue;
+        prims[i] = refs[0];
+        extra.insert(extra.end(), refs.begin() + 1, refs.end());
+    }
+    prims.insert(prims.end(), extra.begin(), extra.end());
+}
+
+}  // namespace
+
 WideBvh build_wide_bvh(const HostScene& s) {
     WideBvh out;
     const size_t n = s.tris.size();
@@ -571,11 +677,13 @@ WideBvh build_wide_bvh(const HostScene& s) {
         }
     }
     });
-    out.nodes.resize(2 * n + 2);  // every wide node owns >= 1 leaf or >= 2 children
+    split_large(s, prims);
+    const size_t m = prims.size();
+    out.nodes.resize(2 * m + 2);  // every wide node owns >= 1 leaf or >= 2 children
     Builder b{prims, out.nodes};
-    b.build(0, uint32_t(n), 0);
-    out.order.resize(n);
-    for (size_t i = 0; i < n; ++i) out.order[i] = prims[i].id;
+    b.build(0, uint32_t(m), 0);
+    out.order.resize(m);
+    for (size_t i = 0; i < m; ++i) out.order[i] = prims[i].id;
     out.nodes.resize(b.next.load());
     out.binary_nodes = 2 * b.leaves.load() - 1;
     return out;
@@ -692,12 +800,14 @@ SceneLights scene_lights(const HostScene& s) {
     return sl;
 }
 
-std::vector<uint32_t> light_ranks(const SceneLights& sl, const std::vector<uint32_t>& tri_to_prim, size_t prims) {
-    std::vector<uint32_t> rank(prims, kNoTriangle);
+std::vector<uint32_t> light_ranks(const SceneLights& sl, const std::vector<uint32_t>& prim_to_tri, size_t triangles) {
+    std::vector<uint32_t> tri_rank(triangles, kNoTriangle);
     uint32_t base = 0;
     for (double w : sl.punctual_weight)
         if (w > 0) ++base;
-    for (size_t k = 0; k < sl.emissive.size(); ++k) rank[tri_to_prim[sl.emissive[k]]] = base + uint32_t(k);
+    for (size_t k = 0; k < sl.emissive.size(); ++k) tri_rank[sl.emissive[k]] = base + uint32_t(k);
+    std::vector<uint32_t> rank(prim_to_tri.size());
+    for (size_t p = 0; p < prim_to_tri.size(); ++p) rank[p] = tri_rank[prim_to_tri[p]];
     return rank;
 }
 
@@ -856,11 +966,9 @@ std::vector<DevMaterial> device_materials(const HostScene& s) {
 SceneArrays scene_arrays(const HostScene& hs) {
     SceneArrays a;
     a.bvh = build_wide_bvh(hs);
-    const size_t n = hs.tris.size();
-    a.tri_to_prim.assign(n, 0);
-    parallel_chunks(n, 65536, [&](size_t b, size_t e, size_t) {
-        for (size_t i = b; i < e; ++i) a.tri_to_prim[a.bvh.order[i]] = uint32_t(i);
-    });
+    const size_t n = a.bvh.order.size();  // leaf-order primitives: triangles and their split references
+    a.tri_to_prim.assign(hs.tris.size(), kNoTriangle);
+    for (size_t i = n; i-- > 0;) a.tri_to_prim[a.bvh.order[i]] = uint32_t(i);  // each triangle's first reference
     a.materials = device_materials(hs);
     bool any_map = false, any_normal_map = false;
     for (const crg_material& m : hs.materials) {

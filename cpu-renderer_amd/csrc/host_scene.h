@@ -48,7 +48,7 @@ HostScene finalize_scene(const crg_scene_desc& desc);
 
 struct WideBvh {
     std::vector<WNode> nodes;
-    std::vector<uint32_t> order;  // prim i -> finalized triangle index
+    std::vector<uint32_t> order;  // prim i -> finalized triangle index (a split triangle has several prims)
     uint64_t binary_nodes = 0;    // equivalent binary node count (2*leaves-1)
 };
 
@@ -92,7 +92,8 @@ HostEnv build_env(const crg_env_desc& d);
 SceneLights scene_lights(const HostScene& s);
 LightTable build_light_table(const SceneLights& sl, const std::vector<uint32_t>& tri_to_prim, double env_integral);
 // Per-prim index of its entry in any table built from `sl` (kNoTriangle if not emissive).
-std::vector<uint32_t> light_ranks(const SceneLights& sl, const std::vector<uint32_t>& tri_to_prim, size_t prims);
+// Light rank of every leaf-order primitive (kNoTriangle: not emissive).
+std::vector<uint32_t> light_ranks(const SceneLights& sl, const std::vector<uint32_t>& prim_to_tri, size_t triangles);
 
 // Camera basis and raster constants (reference src/geometry.cpp:213-227).
 DevCamera prepare_camera(const crg_camera& c, int width, int height);

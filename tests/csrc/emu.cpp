@@ -81,12 +81,12 @@ void* emu_scene_create(const crg_scene_desc* d) {
         s->arr = scene_arrays(s->hs);
         const size_t n = s->hs.tris.size();
         s->lights = scene_lights(s->hs);
-        s->ranks = light_ranks(s->lights, s->arr.tri_to_prim, n);
+        s->ranks = light_ranks(s->lights, s->arr.bvh.order, n);
         SceneArrays& a = s->arr;
         s->dev = DevScene{a.bvh.nodes.data(), a.prims.data(), a.prim_ng.data(), a.aux.data(), a.materials.data(),
                           s->ranks.data(), s->lights.punctual.data(), a.uvs.empty() ? nullptr : a.uvs.data(),
                           a.tangents.empty() ? nullptr : a.tangents.data(), a.texels.data(), a.images.data(),
-                          a.textures.data(), uint32_t(a.bvh.nodes.size()), uint32_t(n),
+                          a.textures.data(), uint32_t(a.bvh.nodes.size()), uint32_t(a.prims.size()),
                           uint32_t(s->lights.punctual.size()), 0};
         return s.release();
     } catch (const std::exception& e) {
@@ -107,6 +107,10 @@ int emu_scene_check(void* sp) {
     const size_t n = s.arr.prims.size();
     if (n == 0) return nodes.empty() ? 0 : 1;
     std::vector<int> seen(n, 0);
+    std::vector<uint32_t> refs(s.hs.tris.size(), 0);  // leaf-order primitives per triangle
+    for (uint32_t t : s.arr.bvh.order) ++refs[t];
+    for (uint32_t c : refs)
+        if (c == 0) return 7;  // every triangle is referenced
     struct Item {
         uint32_t ref;
         float lo[3], hi[3];
@@ -121,10 +125,19 @@ int emu_scene_check(void* sp) {
                 ++seen[i];
                 const PrimRec& p = s.arr.prims[i];
                 const float xs[3] = {p.ax, p.bx, p.cx}, ys[3] = {p.ay, p.by, p.cy}, zs[3] = {p.az, p.bz, p.cz};
-                for (int k = 0; k < 3; ++k)
-                    if (xs[k] < it.lo[0] || xs[k] > it.hi[0] || ys[k] < it.lo[1] || ys[k] > it.hi[1] ||
-                        zs[k] < it.lo[2] || zs[k] > it.hi[2])
-                        return 3;
+                if (refs[p.id] == 1) {  // an unsplit triangle lies inside its leaf box
+                    for (int k = 0; k < 3; ++k)
+                        if (xs[k] < it.lo[0] || xs[k] > it.hi[0] || ys[k] < it.lo[1] || ys[k] > it.hi[1] ||
+                            zs[k] < it.lo[2] || zs[k] > it.hi[2])
+                            return 3;
+                } else {  // a split reference's box meets the triangle's box
+                    const float lo[3] = {std::min({xs[0], xs[1], xs[2]}), std::min({ys[0], ys[1], ys[2]}),
+                                         std::min({zs[0], zs[1], zs[2]})};
+                    const float hi[3] = {std::max({xs[0], xs[1], xs[2]}), std::max({ys[0], ys[1], ys[2]}),
+                                         std::max({zs[0], zs[1], zs[2]})};
+                    for (int a = 0; a < 3; ++a)
+                        if (hi[a] < it.lo[a] || lo[a] > it.hi[a]) return 3;
+                }
             }
             continue;
         }
