@@ -287,3 +287,46 @@ def test_fp32_mask_decision_is_conservative(emu, case):
     assert wrong == 0 and missed == 0
     decided = acc + rej
     assert decided > 0 and decided >= need * (decided + unsure)
+
+
+def test_split_references_cover_their_triangles(emu, oracle):
+    """The host build splits triangles whose box is far larger than the mean into several
+    references with clipped boxes.  Rays aimed at points spread over those triangles (and
+    grazing their edges) must hit exactly what the oracle's unsplit BVH reports: the union of
+    the reference boxes covers each triangle, and repeated references change no result."""
+    rng = np.random.Generator(np.random.PCG64(404))
+    b = None
+    import ref_scenes as rs
+    b = rs.Builder()
+    b.materials = [abi.material(base_color=(0.7, 0.7, 0.7), metallic=0.0, roughness=0.9)]
+    # two huge slanted triangles over a field of small ones
+    b.tri((-50, -1, -50), (50, -3, -50), (0, 4, 60), 0)
+    b.tri((-40, 8, -40), (45, 2, 30), (-45, 5, 50), 0)
+    for _ in range(3000):
+        c = rng.uniform(-40, 40, 3) * np.array([1, 0.05, 1])
+        e = rng.normal(0, 0.6, (3, 3))
+        b.tri(*(c + e[k] for k in range(3)), 0)
+    scene = b.scene()
+    p = scene.positions.reshape(-1, 3, 3).astype(np.float64)
+    rays = np.zeros(8000, abi.RAY_DTYPE)
+    for i in range(len(rays)):
+        t = p[i % 2]
+        w = rng.dirichlet((1, 1, 1)) if i % 4 else np.array([0.5, 0.5 - 1e-7, 1e-7])  # interior / near an edge
+        target = w @ t
+        o = target + rng.normal(0, 1, 3) * 30 + np.array([0, 40, 0])
+        d = target - o
+        rays[i]["origin"] = o
+        rays[i]["direction"] = d / np.linalg.norm(d)
+        rays[i]["tmin"] = 0.0
+        rays[i]["tmax"] = np.inf
+        rays[i]["ignore"] = 0xFFFFFFFF
+    es = emu.scene(scene)
+    assert es.check() == 0
+    got = es.trace(rays)
+    want = oracle.scene(scene).trace(rays)
+    for k in ("hit", "triangle"):
+        np.testing.assert_array_equal(got[k], want[k])
+    for k in ("t", "u", "v"):
+        np.testing.assert_array_equal(got[k].view(np.uint64), want[k].view(np.uint64))
+    assert np.isin([0, 1], got["triangle"]).all()
+    np.testing.assert_array_equal(es.trace(rays, True)["hit"], oracle.scene(scene).trace(rays, True)["hit"])
