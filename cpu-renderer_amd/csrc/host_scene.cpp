@@ -310,7 +310,14 @@ struct Box {
 };
 
 constexpr int kBins = 16;
-constexpr uint32_t kLeafSize = 4;
+#ifndef CR_LEAF_SIZE
+#define CR_LEAF_SIZE 4
+#endif
+#ifndef CR_SAH_TRAVERSAL
+#define CR_SAH_TRAVERSAL 1.0
+#endif
+constexpr uint32_t kLeafSize = CR_LEAF_SIZE;
+constexpr double kSahTraversal = CR_SAH_TRAVERSAL;  // cost of a node step relative to a triangle test
 constexpr int kMaxWideDepth = 40;
 
 struct Builder {
@@ -412,7 +419,7 @@ struct Builder {
                 if (bins[a][k].count) l.grow(bins[a][k].box);
                 cnt += bins[a][k].count;
                 if (!cnt || !rc[k + 1]) continue;
-                const double cost = parent + cnt * l.area() + rc[k + 1] * rb[k + 1].area();
+                const double cost = kSahTraversal * parent + cnt * l.area() + rc[k + 1] * rb[k + 1].area();
                 if (cost < best) {
                     best = cost;
                     best_axis = a;
