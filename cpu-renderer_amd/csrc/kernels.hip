@@ -410,7 +410,10 @@ __global__ void __launch_bounds__(256) k_extend_exact(PathSoA P, const DevJob* j
 // Shade in chunks of kShadeChunk queue entries per workgroup: the chunk is
 // first counting-sorted by shading class in LDS, so each wave shades one kind
 // of surface (or only misses) instead of paying for every branch.
-constexpr uint32_t kShadeChunk = 1024;
+#ifndef CR_SHADE_CHUNK
+#define CR_SHADE_CHUNK 1024
+#endif
+constexpr uint32_t kShadeChunk = CR_SHADE_CHUNK;
 #ifndef CR_SHADE_OCTSORT
 #define CR_SHADE_OCTSORT 1
 #endif

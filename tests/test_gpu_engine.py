@@ -185,3 +185,26 @@ def test_autotune_returns_a_valid_configuration(gpu, resident, alone):
         _same(alone[0], gpu.render(sid, eid, w.camera, w.options()), w.name)
     finally:
         gpu.set_tuning(default)
+
+
+def test_alpha_mask_scene_in_a_batch_equals_alone(gpu, resident, alone):
+    """A scene with alpha masks switches the traversal kernels to their variant with FP32
+    mask tests while its job is live; requests of mask-free scenes batched with it, and the
+    mask scene itself, render exactly as they do alone (the export-contract glTF asset has an
+    alpha-masked material)."""
+    import ref_scenes as rs
+    sc, env, cam = rs.gltf_export_contract()
+    sid, _ = gpu.upload_scene(sc)
+    eid, _ = gpu.upload_env(env)
+    try:
+        opt = abi.options(64, 36, 4, 6, seed=9)
+        mask_alone = gpu.render(sid, eid, cam, opt)
+        reqs = [(s, e, w.camera, w.options()) for w, s, e in resident]
+        reqs.insert(2, (sid, eid, cam, opt))
+        res = gpu.render_batch(reqs)
+        _same(mask_alone, res[2], "mask scene in a batch")
+        for k, r in enumerate(res[:2] + res[3:]):
+            _same(alone[k], r, f"request {k} batched with a mask scene")
+    finally:
+        gpu.release_scene(sid)
+        gpu.release_env(eid)
