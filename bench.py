@@ -491,7 +491,7 @@ def main():
                 "launches": ext_launches, "bytes_per_ray": round(bytes_per_ray, 1),
                 "nodes_per_ray": round(n_node, 2), "tris_per_ray": round(n_tri, 2)}
     # The same kernel against the FP32 vector roofline: 12 FLOP per child slab test of a
-    # 4-wide node and 40 per FP32 triangle test (the survey's per-query accounting).
+    # 4-wide node and 40 per FP32 triangle test (6 FMA per child slab, the watertight test's operations).
     flop_per_ray = 12 * 4 * n_node + 40 * n_tri
     tflops = flop_per_ray * pr.extend_rays / (ext_ms * 1e-3) / 1e12
     valu = {"achieved": round(tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
