@@ -418,15 +418,64 @@ CR_OUTLINE BsdfSample sample_bsdf(const Surf& s, V3 wo, Rng& rng) {
     return out;
 }
 
+// e^x correctly rounded to binary32, as sf::expf_cr, in float-float arithmetic:
+// x = n*ln2 + r with ln2 split into two 16-bit heads (exact products for
+// |n| < 2^8) and a float tail, r carried as an unevaluated float pair; the
+// Taylor series of e^r to r^10 in float-float Horner form (truncation and
+// rounding below 2^-40 relative for |r| <= ln2/2).  The pair is rounded once;
+// when its value lies within 2^-38 (relative) of a binary32 rounding boundary,
+// or the result would leave the normal range, the binary64 emulation decides.
+struct FF {
+    float hi, lo;
+};
+CR_HDI FF ff_two_sum(float a, float b) {
+    const float s = a + b, bb = s - a;
+    return FF{s, (a - (s - bb)) + (b - bb)};
+}
+CR_HDI FF ff_fast_two_sum(float a, float b) {  // |a| >= |b|
+    const float s = a + b;
+    return FF{s, b - (s - a)};
+}
+CR_HDI FF ff_mul(FF a, FF b) {
+    const float p = a.hi * b.hi;
+    const float e = fmaf(a.hi, b.hi, -p) + (a.hi * b.lo + a.lo * b.hi);
+    return ff_fast_two_sum(p, e);
+}
+CR_HDI FF ff_add(FF a, FF b) {
+    const FF s = ff_two_sum(a.hi, b.hi);
+    return ff_fast_two_sum(s.hi, s.lo + (a.lo + b.lo));
+}
+
+CR_HDI float expf_ff(float x) {
+    if (!(x >= -86.0f && x <= 88.0f)) return sf::expf_cr(x);  // NaN, underflow, overflow edges
+    const float n = rintf(x * 1.44269502163f);
+    const float a = x - n * 0x1.62e4p-1f;  // exact (Sterbenz)
+    FF r = ff_two_sum(a, -(n * 0x1.7f7ep-20f));
+    r = ff_two_sum(r.hi, r.lo - n * -0x1.c610cap-37f);
+    constexpr float kHi[11] = {0x1p+0f,        0x1p+0f,        0x1p-1f,         0x1.555556p-3f,
+                               0x1.555556p-5f, 0x1.111112p-7f, 0x1.6c16c2p-10f, 0x1.a01a02p-13f,
+                               0x1.a01a02p-16f, 0x1.71de3ap-19f, 0x1.27e4fcp-22f};
+    constexpr float kLo[11] = {0.0f,           0.0f,           0.0f,            -0x1.555556p-28f,
+                               -0x1.555556p-30f, -0x1.dddddep-32f, -0x1.27d27ep-35f, -0x1.7f97fap-39f,
+                               -0x1.7f97fap-42f, 0x1.55b1ccp-45f, -0x1.10ec14p-47f};
+    FF p{kHi[10], kLo[10]};
+    for (int k = 9; k >= 0; --k) p = ff_add(ff_mul(p, r), FF{kHi[k], kLo[k]});
+    const float f = p.hi + p.lo;
+    const float d = fabsf((p.hi - f) + p.lo);              // distance of the pair from f
+    const float half_ulp = 0.5f * fminf(nextafterf(f, 4.0f) - f, f - nextafterf(f, 0.0f));
+    if (half_ulp - d < 0x1p-38f * f) return sf::expf_cr(x);
+    return ldexpf(f, int(n));
+}
+
 // Beer-Lambert transmittance over `distance` (reference src/bsdf.cpp:71-74).
 CR_HDI V3 beer_lambert(V3 sigma, float distance) {
     if (distance == kInf) return V3{sigma.x > 0 ? 0.0f : 1.0f, sigma.y > 0 ? 0.0f : 1.0f, sigma.z > 0 ? 0.0f : 1.0f};
     const V3 a = sigma * (-fmax_(0.0f, distance));
     // expf correctly rounded: agrees with the host libm's expf on all but about
     // 1 in 10^5 arguments (the device's expf differs on about 1 in 16).
-    const float ex = sf::expf_cr(a.x);
+    const float ex = expf_ff(a.x);
     if (a.y == a.x && a.z == a.x) return V3{ex, ex, ex};
-    return V3{ex, sf::expf_cr(a.y), sf::expf_cr(a.z)};
+    return V3{ex, expf_ff(a.y), expf_ff(a.z)};
 }
 
 // ---------------------------------------------------------------- environment

@@ -241,9 +241,9 @@ int emu_mask_check(void* sp, const crg_ray* rays, uint64_t n, uint64_t* counts) 
     return 0;
 }
 
-// The device's expf (binary64-emulated, correctly rounded) over n arguments.
+// The device's expf (float-float with a binary64-emulated fallback, correctly rounded) over n arguments.
 void emu_expf(const float* x, float* out, uint64_t n) {
-    for (uint64_t i = 0; i < n; ++i) out[i] = sf::expf_cr(x[i]);
+    for (uint64_t i = 0; i < n; ++i) out[i] = expf_ff(x[i]);
 }
 
 // Slab test of one box (child 0 of a node whose other children are empty):
