@@ -221,6 +221,22 @@ CR_HDI float pow5(float x) {
     return sf::to_float(sf::mul(sf::mul(d2, d2), d));
 }
 
+// The same in float-float: x^2 exact (fma), x^4 and x^5 to about 2^-45
+// relative, one rounding; within 2^-36 (relative) of a binary32 rounding
+// boundary, and outside the normal range, the binary64 chain decides.
+CR_HDI float pow5_ff(float x) {
+    const float ax = fabsf(x);
+    if (!(ax >= 0x1p-12f && ax <= 0x1p+24f)) return pow5(x);  // low parts would leave the normal range; NaN
+    const float h2 = x * x, l2 = fmaf(x, x, -h2);
+    const float h4 = h2 * h2, l4 = fmaf(h2, h2, -h4) + 2.0f * h2 * l2;
+    const float h5 = h4 * x, l5 = fmaf(h4, x, -h5) + l4 * x;
+    const float f = h5 + l5, af = fabsf(f);
+    const float d = fabsf((h5 - f) + l5);
+    const float half_ulp = 0.5f * fminf(nextafterf(af, INFINITY) - af, af - nextafterf(af, 0.0f));
+    if (half_ulp - d < 0x1p-36f * af) return pow5(x);
+    return f;
+}
+
 CR_HDI float ggx_d(float alpha, V3 h) {
     if (h.z <= 0) return 0;
     const float a2 = alpha * alpha;
@@ -262,7 +278,7 @@ struct Bsdf {
         return thin ? 2 * f / (1 + f) : f;
     }
     CR_HDI V3 reflection(float f, float cosine) const {
-        const float schlick = pow5(1 - clampf(fabsf(cosine), 0.0f, 1.0f));
+        const float schlick = pow5_ff(1 - clampf(fabsf(cosine), 0.0f, 1.0f));
         return v3((1 - s->metallic) * f) + (s->base + (v3(1) - s->base) * schlick) * s->metallic;
     }
     CR_HDI V3 transmission(float f) const { return s->base * ((1 - s->metallic) * s->transmission * (1 - f)); }

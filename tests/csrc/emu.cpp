@@ -241,6 +241,14 @@ int emu_mask_check(void* sp, const crg_ray* rays, uint64_t n, uint64_t* counts) 
     return 0;
 }
 
+// The device's x^5 for the Schlick term: float-float (pow5_ff) and the binary64 chain (pow5).
+void emu_pow5(const float* x, float* ff, float* chain, uint64_t n) {
+    for (uint64_t i = 0; i < n; ++i) {
+        ff[i] = pow5_ff(x[i]);
+        chain[i] = pow5(x[i]);
+    }
+}
+
 // The device's expf (float-float with a binary64-emulated fallback, correctly rounded) over n arguments.
 void emu_expf(const float* x, float* out, uint64_t n) {
     for (uint64_t i = 0; i < n; ++i) out[i] = expf_ff(x[i]);

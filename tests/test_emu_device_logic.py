@@ -330,3 +330,22 @@ def test_split_references_cover_their_triangles(emu, oracle):
         np.testing.assert_array_equal(got[k].view(np.uint64), want[k].view(np.uint64))
     assert np.isin([0, 1], got["triangle"]).all()
     np.testing.assert_array_equal(es.trace(rays, True)["hit"], oracle.scene(scene).trace(rays, True)["hit"])
+
+
+def test_pow5_float_float_equals_the_binary64_chain(emu):
+    """The Schlick term's x^5 in float-float (with the binary64 chain for hard cases) equals the
+    correctly rounded binary64 chain on every 37th binary32 in [2^-30, 1] (the term's domain),
+    on negative arguments and on special values."""
+    fp = C.POINTER(C.c_float)
+    u = np.arange(np.float32(2.0 ** -30).view(np.uint32), np.float32(1.0).view(np.uint32) + 1, 37, dtype=np.uint64)
+    x = u.astype(np.uint32).view(np.float32)
+    x = np.concatenate([x, -x[::97], np.array([0.0, -0.0, 1.0, 2.0 ** -12, 2.0 ** -24, 2.0 ** 24, 3.0e10], np.float32)])
+    ff, chain = np.empty_like(x), np.empty_like(x)
+    emu.lib.emu_pow5(x.ctypes.data_as(fp), ff.ctypes.data_as(fp), chain.ctypes.data_as(fp), len(x))
+    bad = np.nonzero(ff.view(np.uint32) != chain.view(np.uint32))[0]
+    assert len(bad) == 0, [(float(x[i]), float(ff[i]), float(chain[i])) for i in bad[:5]]
+    # spot check against exact rational arithmetic
+    from fractions import Fraction
+    for v in x[:: max(1, len(x) // 200)]:
+        want = np.float32(float(Fraction(float(v)) ** 5))  # float() of the exact value, then to binary32
+        assert abs(float(chain[x == v][0]) - float(want)) <= abs(float(want)) * 2 ** -23
