@@ -93,7 +93,7 @@ CR_HDI uint32_t bounce(const DevJob& J, PathState& st, uint32_t prim, ShadowOut&
 
     if (prim == kNoTriangle) {
         if (st.medium >= 0) T = T * beer_lambert(st.sigma, kInf);
-        const float w = last_delta || !nee ? 1 : power_heuristic(st.last_pdf, environment_pdf(J.lights, env, ray_d));
+        const float w = last_delta || !nee ? 1 : power_heuristic_ff(st.last_pdf, environment_pdf(J.lights, env, ray_d));
         st.L = L + T * env_radiance(env, ray_d) * w;
         return kRetire;
     }
@@ -112,7 +112,7 @@ CR_HDI uint32_t bounce(const DevJob& J, PathState& st, uint32_t prim, ShadowOut&
     if ((s.front || s.double_sided) && max_component(s.emission) > 0) {
         const float w = last_delta || !nee
                             ? 1
-                            : power_heuristic(st.last_pdf, triangle_pdf(sc, J.lights, prim, st.last_pos, s.position));
+                            : power_heuristic_ff(st.last_pdf, triangle_pdf(sc, J.lights, prim, st.last_pos, s.position));
         L = L + T * s.emission * w;
     }
     if (int(depth) == J.bounces) {
@@ -166,7 +166,7 @@ CR_HDI uint32_t bounce(const DevJob& J, PathState& st, uint32_t prim, ShadowOut&
                     stmax = length(delta) * (1 - 2e-6f);
                     sd = normalize(delta);
                 }
-                const float w = light.delta ? 1 : power_heuristic(light.pdf, bsdf.pdf);
+                const float w = light.delta ? 1 : power_heuristic_ff(light.pdf, bsdf.pdf);
                 V3 shadow_sigma = st.sigma;
                 bool shadow_in_medium = st.medium >= 0;
                 if (s.thickness > 0 && dot(wo, s.ng) * dot(light.direction, s.ng) < 0) {

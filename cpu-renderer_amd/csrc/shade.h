@@ -483,6 +483,32 @@ CR_HDI float expf_ff(float x) {
     return ldexpf(f, int(n));
 }
 
+// powerHeuristic (reference src/math.h:124-128) without the binary64 division:
+// a^2 and b^2 exact as float pairs, their sum and the quotient to about 2^-45
+// relative in float-float, one rounding.  The reference's binary64 quotient is
+// within 2^-51 of the exact one, so both round alike unless the value lies
+// within 2^-38 (relative) of a binary32 rounding boundary; there, and for
+// arguments whose squares would leave the normal range, the binary64 chain
+// (power_heuristic) decides.
+CR_HDI float power_heuristic_ff(float a, float b) {
+    if (a <= 0) return 0;
+    const float ab = fabsf(b);
+    if (!(a >= 0x1p-40f && a <= 0x1p+40f && (ab == 0.0f || (ab >= 0x1p-40f && ab <= 0x1p+40f))))
+        return power_heuristic(a, b);
+    const float ah = a * a, al = fmaf(a, a, -ah);
+    const float bh = b * b, bl = fmaf(b, b, -bh);
+    const FF sum = ff_add(FF{ah, al}, FF{bh, bl});
+    const float q0 = ah / sum.hi;
+    const float p = q0 * sum.hi, pe = fmaf(q0, sum.hi, -p);
+    const float r = (((ah - p) - pe) + al) - q0 * sum.lo;
+    const float q1 = r / sum.hi;
+    const float f = q0 + q1, af = fabsf(f);
+    const float d = fabsf((q0 - f) + q1);
+    const float half_ulp = 0.5f * fminf(nextafterf(af, INFINITY) - af, af - nextafterf(af, 0.0f));
+    if (!(af >= 0x1p-100f) || !(half_ulp - d >= 0x1p-38f * af)) return power_heuristic(a, b);
+    return f;
+}
+
 // Beer-Lambert transmittance over `distance` (reference src/bsdf.cpp:71-74).
 CR_HDI V3 beer_lambert(V3 sigma, float distance) {
     if (distance == kInf) return V3{sigma.x > 0 ? 0.0f : 1.0f, sigma.y > 0 ? 0.0f : 1.0f, sigma.z > 0 ? 0.0f : 1.0f};

@@ -241,6 +241,14 @@ int emu_mask_check(void* sp, const crg_ray* rays, uint64_t n, uint64_t* counts) 
     return 0;
 }
 
+// The device's MIS weight: float-float (power_heuristic_ff) and the binary64 chain.
+void emu_mis(const float* a, const float* b, float* ff, float* chain, uint64_t n) {
+    for (uint64_t i = 0; i < n; ++i) {
+        ff[i] = power_heuristic_ff(a[i], b[i]);
+        chain[i] = power_heuristic(a[i], b[i]);
+    }
+}
+
 // The device's x^5 for the Schlick term: float-float (pow5_ff) and the binary64 chain (pow5).
 void emu_pow5(const float* x, float* ff, float* chain, uint64_t n) {
     for (uint64_t i = 0; i < n; ++i) {

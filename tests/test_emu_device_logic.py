@@ -349,3 +349,30 @@ def test_pow5_float_float_equals_the_binary64_chain(emu):
     for v in x[:: max(1, len(x) // 200)]:
         want = np.float32(float(Fraction(float(v)) ** 5))  # float() of the exact value, then to binary32
         assert abs(float(chain[x == v][0]) - float(want)) <= abs(float(want)) * 2 ** -23
+
+
+def test_power_heuristic_float_float_equals_the_binary64_chain(emu):
+    """The MIS weight a^2 / (a^2 + b^2) in float-float (with the binary64 chain for hard cases)
+    equals the reference's binary64 expression rounded to float (reference src/math.h:124-128)
+    on 3 M pairs: log-uniform pdfs over 2^-50..2^50, equal and nearly equal pairs, b = 0,
+    negative and non-positive a."""
+    fp = C.POINTER(C.c_float)
+    g = np.random.Generator(np.random.PCG64(77))
+    n = 1_000_000
+    a = np.exp2(g.uniform(-50, 50, n)).astype(np.float32)
+    b = np.exp2(g.uniform(-50, 50, n)).astype(np.float32)
+    a2 = np.exp2(g.uniform(-20, 20, n)).astype(np.float32)
+    b2 = (a2 * g.uniform(0.5, 2.0, n).astype(np.float32)).astype(np.float32)
+    a3 = g.uniform(0, 4, n).astype(np.float32)
+    b3 = np.nextafter(a3, np.float32(np.inf)) * (g.integers(0, 2, n) * 2 - 1).astype(np.float32)
+    special = np.array([[1, 0], [0, 1], [-1, 2], [3, -3], [2.0 ** -45, 1], [1, 2.0 ** 45], [5, 5]], np.float32)
+    A = np.concatenate([a, a2, a3, special[:, 0]])
+    B = np.concatenate([b, b2, b3, special[:, 1]])
+    ff, chain = np.empty_like(A), np.empty_like(A)
+    emu.lib.emu_mis(A.ctypes.data_as(fp), B.ctypes.data_as(fp), ff.ctypes.data_as(fp), chain.ctypes.data_as(fp),
+                    len(A))
+    bad = np.nonzero(ff.view(np.uint32) != chain.view(np.uint32))[0]
+    assert len(bad) == 0, [(float(A[i]), float(B[i]), float(ff[i]), float(chain[i])) for i in bad[:5]]
+    want = (A.astype(np.float64) ** 2 / (A.astype(np.float64) ** 2 + B.astype(np.float64) ** 2)).astype(np.float32)
+    ok = A > 0
+    np.testing.assert_array_equal(chain[ok], want[ok])
