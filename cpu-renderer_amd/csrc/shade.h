@@ -26,7 +26,7 @@ CR_OUTLINE bool camera_ray(const DevCamera& c, float px, float py, V3& o, V3& d,
         sf::mul(sf::sub(one, sf::div(sf::mul(two, sf::from_float(py)), sf::from_i32(c.height))), c.half_height);
     const V3 f{c.f[0], c.f[1], c.f[2]}, r{c.r[0], c.r[1], c.r[2]}, u{c.u[0], c.u[1], c.u[2]};
     const D3 dir = dadd(dadd(widen(f), dscale(widen(r), x)), dscale(widen(u), y));
-    d = dunit(dir, V3{0, 0, 0});
+    d = dunit_ff(dir, V3{0, 0, 0});
     const sf::f64 proj = ddot(widen(d), widen(f));
     const sf::f64 near_t = sf::div(sf::from_float(c.near_clip), proj);
     if (!sf::gt(proj, sf::zero(0)) || !sf::is_finite(near_t) || sf::gt(near_t, sf::from_float(kFltMax))) return false;
@@ -44,7 +44,7 @@ CR_OUTLINE uint32_t offset_ray(V3 p, V3 ng, V3 ung, V3 direction, float max_dist
     if (!finite(p) || !finite(ng) || !finite(direction) || !nonzero(ng) || !nonzero(direction) ||
         max_distance != max_distance || max_distance < 0)
         return kErrInvalidSecondary;
-    dir = dunit(widen(direction), V3{0, 0, 0});
+    dir = dunit_ff(widen(direction), V3{0, 0, 0});
     V3 n = ung;
     if (dot_sign_exact(n, dir) < 0) n = -n;
     float scale = fabsf(p.x);
@@ -100,7 +100,7 @@ CR_HDI V3 orthogonalize(const D3& v, V3 n, V3 fallback) {
     const D3 proj = dsub(v, dscale(dn, sf::div(ddot(v, dn), nn)));
     const sf::f64 tol{0x3ea0000000000000ull};  // 4 * FLT_EPSILON = 2^-21
     if (sf::le(ddot(proj, proj), sf::mul(sf::mul(ddot(v, v), tol), tol))) return fallback;
-    return dunit(proj, fallback);
+    return dunit_ff(proj, fallback);
 }
 
 // Material maps and normal mapping of surfaceAt (reference src/geometry.cpp:420-457).
@@ -142,7 +142,7 @@ CR_OUTLINE void apply_maps(const DevScene& sc, const DevMaterial& m, uint32_t pr
         const sf::f64 ky = sf::mul(sf::sub(sf::mul(two, sf::from_float(tx.y)), one), ns);
         const sf::f64 kz = sf::sub(sf::mul(two, sf::from_float(tx.z)), one);
         const D3 mapped = dadd(dadd(dscale(widen(tangent), kx), dscale(widen(bitangent), ky)), dscale(widen(s.normal), kz));
-        s.normal = dunit(mapped, s.normal);
+        s.normal = dunit_ff(mapped, s.normal);
         if (dot(s.normal, s.ng) < 0) s.normal = -s.normal;
     }
     s.texture_error = bad;
@@ -164,7 +164,7 @@ CR_OUTLINE Surf surface_at(const DevScene& sc, uint32_t prim, V3 ray_dir, sf::f6
     const D3 N = dadd(dadd(dscale(widen(V3{ax.n0[0], ax.n0[1], ax.n0[2]}), w),
                            dscale(widen(V3{ax.n1[0], ax.n1[1], ax.n1[2]}), hu)),
                       dscale(widen(V3{ax.n2[0], ax.n2[1], ax.n2[2]}), hv));
-    s.normal = dunit(N, s.ng);
+    s.normal = dunit_ff(N, s.ng);
     if (dot(s.normal, s.ng) < 0) s.normal = -s.normal;
     s.material = p.material;
     const DevMaterial& m = sc.materials[p.material];
@@ -441,27 +441,6 @@ CR_OUTLINE BsdfSample sample_bsdf(const Surf& s, V3 wo, Rng& rng) {
 // rounding below 2^-40 relative for |r| <= ln2/2).  The pair is rounded once;
 // when its value lies within 2^-38 (relative) of a binary32 rounding boundary,
 // or the result would leave the normal range, the binary64 emulation decides.
-struct FF {
-    float hi, lo;
-};
-CR_HDI FF ff_two_sum(float a, float b) {
-    const float s = a + b, bb = s - a;
-    return FF{s, (a - (s - bb)) + (b - bb)};
-}
-CR_HDI FF ff_fast_two_sum(float a, float b) {  // |a| >= |b|
-    const float s = a + b;
-    return FF{s, b - (s - a)};
-}
-CR_HDI FF ff_mul(FF a, FF b) {
-    const float p = a.hi * b.hi;
-    const float e = fmaf(a.hi, b.hi, -p) + (a.hi * b.lo + a.lo * b.hi);
-    return ff_fast_two_sum(p, e);
-}
-CR_HDI FF ff_add(FF a, FF b) {
-    const FF s = ff_two_sum(a.hi, b.hi);
-    return ff_fast_two_sum(s.hi, s.lo + (a.lo + b.lo));
-}
-
 CR_HDI float expf_ff(float x) {
     if (!(x >= -86.0f && x <= 88.0f)) return sf::expf_cr(x);  // NaN, underflow, overflow edges
     const float n = rintf(x * 1.44269502163f);

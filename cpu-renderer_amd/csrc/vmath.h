@@ -136,11 +136,82 @@ CR_HDI V3 narrow(const D3& v) { return {sf::to_float(v.x), sf::to_float(v.y), sf
 
 // unit(): double-precision normalization with a fallback for zero/invalid input
 // (reference src/geometry.cpp:31-37).
+// ---- float-float: an unevaluated pair hi + lo of floats (about 48 bits) ----
+struct FF {
+    float hi, lo;
+};
+CR_HDI FF ff_two_sum(float a, float b) {
+    const float s = a + b, bb = s - a;
+    return FF{s, (a - (s - bb)) + (b - bb)};
+}
+CR_HDI FF ff_fast_two_sum(float a, float b) {  // |a| >= |b|
+    const float s = a + b;
+    return FF{s, b - (s - a)};
+}
+CR_HDI FF ff_mul(FF a, FF b) {
+    const float p = a.hi * b.hi;
+    const float e = fmaf(a.hi, b.hi, -p) + (a.hi * b.lo + a.lo * b.hi);
+    return ff_fast_two_sum(p, e);
+}
+CR_HDI FF ff_add(FF a, FF b) {
+    const FF s = ff_two_sum(a.hi, b.hi);
+    return ff_fast_two_sum(s.hi, s.lo + (a.lo + b.lo));
+}
+
 CR_HDI V3 dunit(const D3& v, V3 fallback) {
     const sf::f64 sq = ddot(v, v);
     if (!sf::gt(sq, sf::zero(0)) || !sf::is_finite(sq)) return fallback;
     const sf::f64 inv = sf::div(sf::one(), sf::sqrt(sq));
     return {sf::to_float(sf::mul(v.x, inv)), sf::to_float(sf::mul(v.y, inv)), sf::to_float(sf::mul(v.z, inv))};
+}
+
+// A binary64 value as a float pair holding its leading 48 significand bits
+// (relative error below 2^-47); false outside 2^-60..2^60 (and for inf/NaN).
+CR_HDI bool ff_of(sf::f64 a, FF& out) {
+    const float sgn = sf::sign_of(a) ? -1.0f : 1.0f;
+    if (sf::is_zero(a)) {
+        out = FF{sgn * 0.0f, 0.0f};
+        return true;
+    }
+    const int32_t e = sf::exp_of(a) - 1023;
+    if (e < -60 || e > 60) return false;
+    const uint64_t m = sf::frac_of(a) | (uint64_t(1) << 52);
+    out.hi = sgn * ldexpf(float(uint32_t(m >> 29)), e - 23);
+    out.lo = sgn * ldexpf(float(uint32_t((m >> 5) & 0xffffffu)), e - 47);
+    return true;
+}
+
+// dunit without the binary64 emulation on the common path: the vector's
+// components as float pairs, its length and the three quotients to about
+// 2^-44 relative in float-float, each rounded once.  The reference's binary64
+// chain is within 2^-50 of the exact quotients, so both round alike unless a
+// quotient lies within 2^-38 (relative) of a binary32 rounding boundary;
+// there, and for components outside 2^-60..2^60, dunit decides.
+CR_HDI V3 dunit_ff(const D3& v, V3 fallback) {
+    FF c[3];
+    if (!ff_of(v.x, c[0]) || !ff_of(v.y, c[1]) || !ff_of(v.z, c[2])) return dunit(v, fallback);
+    const FF sq = ff_add(ff_add(ff_mul(c[0], c[0]), ff_mul(c[1], c[1])), ff_mul(c[2], c[2]));
+    if (!(sq.hi > 0x1p-110f)) return dunit(v, fallback);
+    const float s0 = sqrtf(sq.hi);
+    const float res = fmaf(-s0, s0, sq.hi) + sq.lo;
+    const FF nrm = ff_fast_two_sum(s0, res / (2.0f * s0));
+    float out[3];
+    for (int k = 0; k < 3; ++k) {
+        if (c[k].hi == 0.0f) {  // a signed zero stays itself, as in the binary64 product
+            out[k] = c[k].hi;
+            continue;
+        }
+        const float q0 = c[k].hi / nrm.hi;
+        const float p = q0 * nrm.hi, pe = fmaf(q0, nrm.hi, -p);
+        const float r = (((c[k].hi - p) - pe) + c[k].lo) - q0 * nrm.lo;
+        const float q1 = r / nrm.hi;
+        const float f = q0 + q1, af = fabsf(f);
+        const float d = fabsf((q0 - f) + q1);
+        const float half_ulp = 0.5f * fminf(nextafterf(af, INFINITY) - af, af - nextafterf(af, 0.0f));
+        if (!(af >= 0x1p-100f) || !(half_ulp - d >= 0x1p-38f * af)) return dunit(v, fallback);
+        out[k] = f;
+    }
+    return V3{out[0], out[1], out[2]};
 }
 
 // Sign of the binary64 dot product of two float vectors, exactly as the

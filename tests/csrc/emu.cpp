@@ -241,6 +241,17 @@ int emu_mask_check(void* sp, const crg_ray* rays, uint64_t n, uint64_t* counts) 
     return 0;
 }
 
+// The device's binary64 normalization: float-float (dunit_ff) and the emulation (dunit), on
+// vectors given as binary64 bit patterns (3 per vector).
+void emu_dunit(const uint64_t* v, float* ff, float* chain, uint64_t n) {
+    for (uint64_t i = 0; i < n; ++i) {
+        const D3 d{sf::f64{v[3 * i]}, sf::f64{v[3 * i + 1]}, sf::f64{v[3 * i + 2]}};
+        const V3 a = dunit_ff(d, V3{7, 7, 7}), b = dunit(d, V3{7, 7, 7});
+        ff[3 * i] = a.x, ff[3 * i + 1] = a.y, ff[3 * i + 2] = a.z;
+        chain[3 * i] = b.x, chain[3 * i + 1] = b.y, chain[3 * i + 2] = b.z;
+    }
+}
+
 // The device's MIS weight: float-float (power_heuristic_ff) and the binary64 chain.
 void emu_mis(const float* a, const float* b, float* ff, float* chain, uint64_t n) {
     for (uint64_t i = 0; i < n; ++i) {
