@@ -37,6 +37,28 @@ CR_OUTLINE bool camera_ray(const DevCamera& c, float px, float py, V3& o, V3& d,
 }
 
 // ---------------------------------------------------------------- offsetRay
+// One coordinate of offsetRay's shifted point, float(p + distance * n) with
+// distance = 2^-21 * scale: the binary64 sum of the reference (the product is
+// exact in binary64) rounded to float.  In float-float: scale * n as an exact
+// pair scaled by 2^-21, its sum with p to about 2^-47 relative, one rounding;
+// within 2^-38 (relative) of a binary32 rounding boundary, or where the pair's
+// low part could leave the normal range, the binary64 emulation decides.  A
+// zero n adds a zero of n's sign, as the binary64 sum does.
+CR_HDI float offset_coord(float p, float scale, float n, sf::f64 distance) {
+    if (n == 0.0f) return p + n;
+    const float ph = scale * n, pl = fmaf(scale, n, -ph);
+    const float aph = fabsf(ph);
+    if (aph >= 0x1p-50f && aph <= 0x1p+100f && fabsf(p) <= 0x1p+100f) {
+        const FF sum = ff_two_sum(p, ph * 0x1p-21f);
+        const float lo = sum.lo + pl * 0x1p-21f;
+        const float f = sum.hi + lo, af = fabsf(f);
+        const float d = fabsf((sum.hi - f) + lo);
+        const float half_ulp = 0.5f * fminf(nextafterf(af, INFINITY) - af, af - nextafterf(af, 0.0f));
+        if (af >= 0x1p-100f && half_ulp - d >= 0x1p-38f * af) return f;
+    }
+    return sf::to_float(sf::add(sf::from_float(p), sf::mul(distance, sf::from_float(n))));
+}
+
 // `ung` is unit(geometricNormal) (binary64 normalization done once per triangle
 // on the host); `ng` is the stored geometric normal used for the validity test.
 CR_OUTLINE uint32_t offset_ray(V3 p, V3 ng, V3 ung, V3 direction, float max_distance, V3& origin, V3& dir,
@@ -53,7 +75,7 @@ CR_OUTLINE uint32_t offset_ray(V3 p, V3 ng, V3 ung, V3 direction, float max_dist
     const sf::f64 distance = sf::mul(sf::from_float(4 * kFltEps), sf::from_float(scale));
     for (int a = 0; a < 3; ++a) {
         const float na = at(n, a);
-        float v = sf::to_float(sf::add(sf::from_float(at(p, a)), sf::mul(distance, sf::from_float(na))));
+        float v = offset_coord(at(p, a), scale, na, distance);
         if (na != 0) v = na > 0 ? next_up(v) : next_down(v);
         set(origin, a, v);
     }

@@ -252,6 +252,16 @@ void emu_dunit(const uint64_t* v, float* ff, float* chain, uint64_t n) {
     }
 }
 
+// One coordinate of offsetRay's shifted point: float-float (offset_coord) and the binary64
+// emulation, for coordinate p, scale and normal component n.
+void emu_offset(const float* p, const float* scale, const float* n, float* ff, float* chain, uint64_t count) {
+    for (uint64_t i = 0; i < count; ++i) {
+        const sf::f64 distance = sf::mul(sf::from_float(4 * kFltEps), sf::from_float(scale[i]));
+        ff[i] = offset_coord(p[i], scale[i], n[i], distance);
+        chain[i] = sf::to_float(sf::add(sf::from_float(p[i]), sf::mul(distance, sf::from_float(n[i]))));
+    }
+}
+
 // The device's MIS weight: float-float (power_heuristic_ff) and the binary64 chain.
 void emu_mis(const float* a, const float* b, float* ff, float* chain, uint64_t n) {
     for (uint64_t i = 0; i < n; ++i) {

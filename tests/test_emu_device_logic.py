@@ -399,3 +399,64 @@ def test_dunit_float_float_equals_the_binary64_emulation(emu):
                       len(v))
     bad = np.nonzero(np.any(ff.view(np.uint32) != chain.view(np.uint32), axis=1))[0]
     assert len(bad) == 0, [(v[i].tolist(), ff[i].tolist(), chain[i].tolist()) for i in bad[:3]]
+
+
+def test_offset_coordinate_float_float_equals_the_binary64_sum(emu):
+    """offsetRay's shifted coordinate float(p + 2^-21 * scale * n) in float-float with the
+    binary64 emulation for hard cases equals both the emulated binary64 sum and numpy's
+    binary64 evaluation of the reference expression (src/geometry.cpp:193-198) on 1.8 M
+    cases: random points and normals, small coordinates of far-out points, offsets placed on and a few ulps around binary32
+    rounding midpoints (where rounding twice through binary64 matters), tiny and zero
+    (signed) normal components, and a zero scale."""
+    fp = C.POINTER(C.c_float)
+    g = np.random.Generator(np.random.PCG64(123))
+    m = 300_000
+    f32 = np.float32
+    # random coordinates; scale at least |p|
+    p1 = (g.normal(size=m) * np.exp2(g.uniform(-20, 20, m))).astype(f32)
+    s1 = (np.abs(p1) * np.exp2(g.uniform(0, 6, m))).astype(f32)
+    n1 = g.uniform(-1, 1, m).astype(f32)
+    # offsets near (k + 1/2) ulp(p): n chosen so 2^-21 * scale * n sits on a midpoint, then nudged
+    p2 = (g.uniform(1, 2, m) * np.exp2(g.integers(-10, 10, m))).astype(f32)
+    s2 = np.abs(p2).astype(f32)
+    ulp = (np.nextafter(np.abs(p2), f32(np.inf)) - np.abs(p2)).astype(np.float64)
+    k = g.integers(0, 8, m) + 0.5
+    n2 = (k * ulp / (2.0 ** -21 * s2.astype(np.float64))).astype(f32)
+    nudge = g.integers(-3, 4, m)
+    for step in (1, -1):
+        sel = np.sign(nudge) == step
+        for _ in range(3):
+            n2[sel & (np.abs(nudge) > _)] = np.nextafter(n2[sel & (np.abs(nudge) > _)], f32(step * np.inf))
+    n2 *= np.where(g.integers(0, 2, m) == 1, f32(-1), f32(1))
+    n2 = np.clip(n2, f32(-1), f32(1))
+    # exact products at the midpoint, signs mixed
+    p3 = -p2
+    # tiny and zero components
+    p4 = (g.normal(size=m) * 100).astype(f32)
+    s4 = (np.abs(p4) + f32(1)).astype(f32)
+    n4 = np.where(g.integers(0, 3, m) == 0, f32(0.0), (g.normal(size=m) * 1e-12).astype(f32))
+    n4 = np.where(g.integers(0, 4, m) == 0, f32(-0.0), n4).astype(f32)
+    # small coordinates of points far out on another axis (scale up to 2^24 |p|), with offsets
+    # near the rounding midpoints of p or (random n) many times p
+    p5 = (g.uniform(1, 2, m) * np.exp2(g.integers(-12, 4, m))).astype(f32)
+    s5 = (np.abs(p5) * np.exp2(g.uniform(4, 24, m))).astype(f32)
+    ulp5 = (np.nextafter(p5, f32(np.inf)) - p5).astype(np.float64)
+    n5 = ((g.integers(0, 4, m) + 0.5) * ulp5 / (2.0 ** -21 * s5.astype(np.float64))).astype(f32)
+    n5 = (n5 * np.where(g.integers(0, 2, m) == 1, f32(-1), f32(1))).astype(f32)
+    n5 = np.where(g.integers(0, 2, m) == 1, n5, g.uniform(-1, 1, m).astype(f32))
+    special_p = np.array([0.0, -0.0, -0.0, 0.0, 1.0, -1.0, 3e-39, 1e30], f32)
+    special_s = np.array([0.0, 0.0, 0.0, 5.0, 1.0, 1.0, 3e-39, 1e30], f32)
+    special_n = np.array([0.0, 0.0, -0.0, 1.0, -0.0, 0.5, 1.0, -1.0], f32)
+    P = np.concatenate([p1, p2, p3, p4, p5, special_p])
+    S = np.concatenate([s1, s2, s2, s4, s5, special_s])
+    N = np.concatenate([n1, n2, n2, n4, n5, special_n])
+    ff, chain = np.empty_like(P), np.empty_like(P)
+    emu.lib.emu_offset(P.ctypes.data_as(fp), S.ctypes.data_as(fp), N.ctypes.data_as(fp), ff.ctypes.data_as(fp),
+                       chain.ctypes.data_as(fp), len(P))
+    bad = np.nonzero(ff.view(np.uint32) != chain.view(np.uint32))[0]
+    assert len(bad) == 0, [(float(P[i]), float(S[i]), float(N[i]), float(ff[i]), float(chain[i])) for i in bad[:5]]
+    want = (P.astype(np.float64) + (2.0 ** -21 * S.astype(np.float64)) * N.astype(np.float64)).astype(f32)
+    bad = np.nonzero(chain.view(np.uint32) != want.view(np.uint32))[0]
+    assert len(bad) == 0, [(float(P[i]), float(S[i]), float(N[i]), float(chain[i]), float(want[i])) for i in bad[:5]]
+    # the midpoint cases do reach the rounding question
+    assert np.count_nonzero(p2 + (2.0 ** -21 * s2) * n2 != p2) > m // 2
