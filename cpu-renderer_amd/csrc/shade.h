@@ -183,10 +183,9 @@ CR_OUTLINE Surf surface_at(const DevScene& sc, uint32_t prim, V3 ray_dir, sf::f6
                         sf::to_float(sf::sub(P.z, sf::from_float(s.position.z)))};
     s.ng = V3{ax.ng[0], ax.ng[1], ax.ng[2]};
     s.front = dot_sign_exact(ray_dir, s.ng) < 0;
-    const D3 N = dadd(dadd(dscale(widen(V3{ax.n0[0], ax.n0[1], ax.n0[2]}), w),
-                           dscale(widen(V3{ax.n1[0], ax.n1[1], ax.n1[2]}), hu)),
-                      dscale(widen(V3{ax.n2[0], ax.n2[1], ax.n2[2]}), hv));
-    s.normal = dunit_ff(N, s.ng);
+    const V3 n0{ax.n0[0], ax.n0[1], ax.n0[2]}, n1{ax.n1[0], ax.n1[1], ax.n1[2]}, n2{ax.n2[0], ax.n2[1], ax.n2[2]};
+    if (!interp_unit_ff(n0, n1, n2, w, hu, hv, s.normal))
+        s.normal = dunit_ff(dadd(dadd(dscale(widen(n0), w), dscale(widen(n1), hu)), dscale(widen(n2), hv)), s.ng);
     if (dot(s.normal, s.ng) < 0) s.normal = -s.normal;
     s.material = p.material;
     const DevMaterial& m = sc.materials[p.material];

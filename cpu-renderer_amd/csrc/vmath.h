@@ -187,17 +187,23 @@ CR_HDI bool ff_of(sf::f64 a, FF& out) {
 // chain is within 2^-50 of the exact quotients, so both round alike unless a
 // quotient lies within 2^-38 (relative) of a binary32 rounding boundary;
 // there, and for components outside 2^-60..2^60, dunit decides.
-CR_HDI V3 dunit_ff(const D3& v, V3 fallback) {
-    FF c[3];
-    if (!ff_of(v.x, c[0]) || !ff_of(v.y, c[1]) || !ff_of(v.z, c[2])) return dunit(v, fallback);
+// The core of the float-float normalizations below: the quotients
+// c_k / |c| rounded once to float, where err[k] bounds the absolute error of
+// component k against the binary64 vector the reference normalizes (0: the
+// pair holds that component to 2^-47).  A unit vector moves by at most
+// 2 |delta| / |c| when its vector moves by delta, so each quotient must lie
+// farther than 4 sum(err) / |c| plus 2^-38 (relative) from a binary32
+// rounding boundary.  False where the binary64 chain must decide.
+CR_HDI bool unit_ff(const FF c[3], const float err[3], float out[3]) {
     const FF sq = ff_add(ff_add(ff_mul(c[0], c[0]), ff_mul(c[1], c[1])), ff_mul(c[2], c[2]));
-    if (!(sq.hi > 0x1p-110f)) return dunit(v, fallback);
+    if (!(sq.hi > 0x1p-110f)) return false;
     const float s0 = sqrtf(sq.hi);
     const float res = fmaf(-s0, s0, sq.hi) + sq.lo;
     const FF nrm = ff_fast_two_sum(s0, res / (2.0f * s0));
-    float out[3];
+    const float slack = 4.0f * ((err[0] + err[1]) + err[2]) / nrm.hi;
     for (int k = 0; k < 3; ++k) {
         if (c[k].hi == 0.0f) {  // a signed zero stays itself, as in the binary64 product
+            if (err[k] != 0.0f) return false;
             out[k] = c[k].hi;
             continue;
         }
@@ -208,10 +214,48 @@ CR_HDI V3 dunit_ff(const D3& v, V3 fallback) {
         const float f = q0 + q1, af = fabsf(f);
         const float d = fabsf((q0 - f) + q1);
         const float half_ulp = 0.5f * fminf(nextafterf(af, INFINITY) - af, af - nextafterf(af, 0.0f));
-        if (!(af >= 0x1p-100f) || !(half_ulp - d >= 0x1p-38f * af)) return dunit(v, fallback);
+        if (!(af >= 0x1p-100f) || !(half_ulp - d >= 0x1p-38f * af + slack)) return false;
         out[k] = f;
     }
+    return true;
+}
+
+CR_HDI V3 dunit_ff(const D3& v, V3 fallback) {
+    FF c[3];
+    if (!ff_of(v.x, c[0]) || !ff_of(v.y, c[1]) || !ff_of(v.z, c[2])) return dunit(v, fallback);
+    const float exact[3] = {0.0f, 0.0f, 0.0f};
+    float out[3];
+    if (!unit_ff(c, exact, out)) return dunit(v, fallback);
     return V3{out[0], out[1], out[2]};
+}
+
+// unit(n0 * w + n1 * hu + n2 * hv) for float vectors and binary64 weights, the
+// reference evaluating the sum in binary64 ((n0 w + n1 hu) + n2 hv, each step
+// rounded): the sum in float-float, whose distance from the reference's is
+// below 2^-44 of the terms' magnitudes (plus 2^-120 for products near the
+// subnormal range), then unit_ff.  A component whose three
+// inputs are zero is the signed zero of the binary64 sum.  False where the
+// caller's binary64 chain must decide.
+CR_HDI bool interp_unit_ff(V3 n0, V3 n1, V3 n2, sf::f64 w, sf::f64 hu, sf::f64 hv, V3& out) {
+    FF fw, fu, fv;
+    if (!ff_of(w, fw) || !ff_of(hu, fu) || !ff_of(hv, fv)) return false;
+    FF c[3];
+    float err[3];
+    for (int k = 0; k < 3; ++k) {
+        const float a = at(n0, k), b = at(n1, k), e = at(n2, k);
+        if (a == 0.0f && b == 0.0f && e == 0.0f) {
+            c[k] = FF{(a * fw.hi + b * fu.hi) + e * fv.hi, 0.0f};
+            err[k] = 0.0f;
+            continue;
+        }
+        const FF t0 = ff_mul(FF{a, 0.0f}, fw), t1 = ff_mul(FF{b, 0.0f}, fu), t2 = ff_mul(FF{e, 0.0f}, fv);
+        c[k] = ff_add(ff_add(t0, t1), t2);
+        err[k] = 0x1p-44f * ((fabsf(t0.hi) + fabsf(t1.hi)) + fabsf(t2.hi)) + 0x1p-120f;
+    }
+    float o[3];
+    if (!unit_ff(c, err, o)) return false;
+    out = V3{o[0], o[1], o[2]};
+    return true;
 }
 
 // Sign of the binary64 dot product of two float vectors, exactly as the

@@ -252,6 +252,25 @@ void emu_dunit(const uint64_t* v, float* ff, float* chain, uint64_t n) {
     }
 }
 
+// surfaceAt's shading normal unit(n0 w + n1 hu + n2 hv): float-float (interp_unit_ff, else
+// the chain) and the binary64 chain alone; per case 9 floats n0,n1,n2 and 2 binary64 weights
+// hu, hv (w = (1 - hu) - hv in binary64, as the device forms it).
+void emu_interp_unit(const float* n, const uint64_t* uv, float* ff, float* chain, uint64_t count) {
+    for (uint64_t i = 0; i < count; ++i) {
+        const V3 n0{n[9 * i], n[9 * i + 1], n[9 * i + 2]}, n1{n[9 * i + 3], n[9 * i + 4], n[9 * i + 5]},
+            n2{n[9 * i + 6], n[9 * i + 7], n[9 * i + 8]};
+        const sf::f64 hu{uv[2 * i]}, hv{uv[2 * i + 1]};
+        const sf::f64 w = sf::sub(sf::sub(sf::one(), hu), hv);
+        const D3 N = dadd(dadd(dscale(widen(n0), w), dscale(widen(n1), hu)), dscale(widen(n2), hv));
+        const V3 fb{7, 7, 7};
+        V3 a;
+        if (!interp_unit_ff(n0, n1, n2, w, hu, hv, a)) a = dunit_ff(N, fb);
+        const V3 b = dunit(N, fb);
+        ff[3 * i] = a.x, ff[3 * i + 1] = a.y, ff[3 * i + 2] = a.z;
+        chain[3 * i] = b.x, chain[3 * i + 1] = b.y, chain[3 * i + 2] = b.z;
+    }
+}
+
 // One coordinate of offsetRay's shifted point: float-float (offset_coord) and the binary64
 // emulation, for coordinate p, scale and normal component n.
 void emu_offset(const float* p, const float* scale, const float* n, float* ff, float* chain, uint64_t count) {

@@ -460,3 +460,57 @@ def test_offset_coordinate_float_float_equals_the_binary64_sum(emu):
     assert len(bad) == 0, [(float(P[i]), float(S[i]), float(N[i]), float(chain[i]), float(want[i])) for i in bad[:5]]
     # the midpoint cases do reach the rounding question
     assert np.count_nonzero(p2 + (2.0 ** -21 * s2) * n2 != p2) > m // 2
+
+
+def test_interpolated_normal_float_float_equals_the_binary64_chain(emu):
+    """surfaceAt's shading normal unit(n0 w + n1 hu + n2 hv) (reference src/geometry.cpp:407-415,
+    binary64 sums) in float-float with the binary64 chain for hard cases equals the chain alone,
+    and the chain equals numpy's binary64 evaluation, on 1.2 M cases: smooth vertex normals,
+    flat triangles (n0 = n1 = n2), axis-aligned normals with signed zero components, nearly
+    opposite vertex normals, exactly opposite ones with w ~ hu (cancellation by up to 2^30, where
+    the binary64 roundings show), and barycentrics at the edges and corners."""
+    fp = C.POINTER(C.c_float)
+    g = np.random.Generator(np.random.PCG64(131))
+    m = 200_000
+
+    def unitv(k):
+        v = g.normal(size=(k, 3))
+        return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+
+    base = unitv(m)
+    smooth = np.stack([base, (base + 0.2 * unitv(m)), (base + 0.2 * unitv(m))], 1).astype(np.float32)
+    flat = np.repeat(unitv(m)[:, None, :], 3, axis=1)
+    axis = np.zeros((m, 3, 3), np.float32)
+    ax = g.integers(0, 3, m)
+    axis[np.arange(m), :, ax] = np.where(g.integers(0, 2, m) == 1, 1.0, -1.0)[:, None]
+    axis[(axis == 0) & (g.integers(0, 2, (m, 3, 3)) == 1)] = -0.0
+    opp = np.stack([base, -base + 1e-3 * unitv(m), base], 1).astype(np.float32)
+    tilt = np.stack([unitv(m), unitv(m), unitv(m)], 1)
+    cancel = np.stack([base, -base, unitv(m)], 1).astype(np.float32)
+    normals = np.concatenate([smooth, flat, axis, opp, tilt, cancel]).reshape(-1, 9)
+    k = len(normals)
+    hu = g.uniform(0, 1, k)
+    hv = g.uniform(0, 1, k) * (1 - hu)
+    corner = g.integers(0, 6, k)
+    hu = np.where(corner == 0, 0.0, np.where(corner == 1, 1.0, hu))
+    hv = np.where(corner == 0, np.where(g.integers(0, 2, k) == 1, 1.0, 0.0), np.where(corner == 1, 0.0, hv))
+    hv = np.where(corner == 2, 1e-12 * g.uniform(size=k), hv)
+    near = slice(5 * m, 6 * m)
+    hv[near] = np.exp2(g.uniform(-40, -10, m))
+    hu[near] = (1 - hv[near]) / 2 + g.normal(size=m) * np.exp2(g.uniform(-45, -20, m))
+    uv = np.ascontiguousarray(np.stack([hu, hv], 1)).view(np.uint64)
+    n32 = np.ascontiguousarray(normals, np.float32)
+    ff = np.empty((k, 3), np.float32)
+    chain = np.empty_like(ff)
+    emu.lib.emu_interp_unit(n32.ctypes.data_as(fp), uv.ctypes.data_as(C.POINTER(C.c_uint64)),
+                            ff.ctypes.data_as(fp), chain.ctypes.data_as(fp), k)
+    bad = np.nonzero(np.any(ff.view(np.uint32) != chain.view(np.uint32), axis=1))[0]
+    assert len(bad) == 0, [(n32[i].tolist(), hu[i], hv[i], ff[i].tolist(), chain[i].tolist()) for i in bad[:3]]
+    nd = n32.astype(np.float64).reshape(-1, 3, 3)
+    w = (1.0 - hu) - hv
+    N = (nd[:, 0] * w[:, None] + nd[:, 1] * hu[:, None]) + nd[:, 2] * hv[:, None]
+    sq = (N[:, 0] * N[:, 0] + N[:, 1] * N[:, 1]) + N[:, 2] * N[:, 2]
+    ok = sq > 0
+    inv = 1.0 / np.sqrt(sq[ok])
+    want = (N[ok] * inv[:, None]).astype(np.float32)
+    np.testing.assert_array_equal(chain[ok].view(np.uint32), want.view(np.uint32))
