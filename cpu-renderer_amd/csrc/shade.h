@@ -17,7 +17,8 @@
 namespace cr {
 
 // ---------------------------------------------------------------- camera
-CR_OUTLINE bool camera_ray(const DevCamera& c, float px, float py, V3& o, V3& d, float& tmin, float& tmax) {
+// Camera::generate with every binary64 step emulated.
+CR_OUTLINE bool camera_ray_chain(const DevCamera& c, float px, float py, V3& o, V3& d, float& tmin, float& tmax) {
     const sf::f64 one = sf::one(), two = sf::two();
     const sf::f64 x = sf::mul(
         sf::mul(sf::sub(sf::div(sf::mul(two, sf::from_float(px)), sf::from_i32(c.width)), one), c.ratio),
@@ -34,6 +35,87 @@ CR_OUTLINE bool camera_ray(const DevCamera& c, float px, float py, V3& o, V3& d,
     tmin = sf::to_float(near_t);
     tmax = sf::to_float(sf::div(sf::from_float(c.far_clip), proj));
     return true;
+}
+
+// hi + lo rounded to float when the reference's binary64 value (within
+// `slack` relative of hi + lo) rounds to the same float: at least
+// 2^-38 + slack (relative) from a binary32 rounding boundary, in the normal range.
+CR_HDI bool ff_round(float hi, float lo, float slack, float& out) {
+    const float f = hi + lo, af = fabsf(f);
+    const float d = fabsf((hi - f) + lo);
+    const float half_ulp = 0.5f * fminf(nextafterf(af, INFINITY) - af, af - nextafterf(af, 0.0f));
+    if (!(af >= 0x1p-100f && af <= 0x1p+126f) || !(half_ulp - d >= (0x1p-38f + slack) * af)) return false;
+    out = f;
+    return true;
+}
+
+// num / den for a float-pair den, as a float pair to about 2^-46 relative.
+CR_HDI FF ff_div(float num, FF den) {
+    const float q0 = num / den.hi;
+    const float r = (fmaf(-q0, den.hi, num)) - q0 * den.lo;
+    return ff_fast_two_sum(q0, r / den.hi);
+}
+
+// Camera::generate in float-float (the binary64 chain below covers the rest):
+// x = (2 px / width - 1) ratio halfHeight and y = (1 - 2 py / height) halfHeight
+// as pairs with absolute error bounds that cover the reference's binary64
+// roundings (the subtraction of 1 can cancel), the direction through unit_ff
+// with those bounds, the projection on f from exact float products, and the
+// clip parameters rounded through ff_round.  False where the chain decides,
+// including every ray the reference rejects.
+CR_HDI bool camera_ray_ff(const DevCamera& c, float px, float py, V3& d, float& tmin, float& tmax) {
+    FF ratio, hh;
+    if (!ff_of(c.ratio, ratio) || !ff_of(c.half_height, hh) || !(fabsf(px) < 0x1p+60f) || !(fabsf(py) < 0x1p+60f) ||
+        c.width <= 0 || c.width > (1 << 24) || c.height <= 0 || c.height > (1 << 24))
+        return false;
+    const float rh = fabsf(ratio.hi) * fabsf(hh.hi);
+    const FF qx = ff_div(2.0f * px, FF{float(c.width), 0.0f}), qy = ff_div(2.0f * py, FF{float(c.height), 0.0f});
+    const FF xm = ff_add(qx, FF{-1.0f, 0.0f}), ym = ff_add(FF{1.0f, 0.0f}, FF{-qy.hi, -qy.lo});
+    const FF x = ff_mul(ff_mul(xm, ratio), hh), y = ff_mul(ym, hh);
+    const float ex = 0x1p-44f * (fabsf(qx.hi) + fabsf(xm.hi)) * rh + 0x1p-120f;
+    const float ey = 0x1p-44f * (fabsf(qy.hi) + fabsf(ym.hi)) * fabsf(hh.hi) + 0x1p-120f;
+    FF v[3];
+    float err[3];
+    for (int k = 0; k < 3; ++k) {
+        const float fk = c.f[k], rk = c.r[k], uk = c.u[k];
+        const FF t1 = ff_mul(FF{rk, 0.0f}, x), t2 = ff_mul(FF{uk, 0.0f}, y);
+        v[k] = ff_add(ff_add(FF{fk, 0.0f}, t1), t2);
+        err[k] = fabsf(rk) * ex + fabsf(uk) * ey + 0x1p-44f * ((fabsf(fk) + fabsf(t1.hi)) + fabsf(t2.hi)) + 0x1p-120f;
+    }
+    float dd[3];
+    if (!unit_ff(v, err, dd)) return false;
+    FF proj{0.0f, 0.0f};
+    float mag = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        const float p = dd[k] * c.f[k];
+        proj = ff_add(proj, FF{p, fmaf(dd[k], c.f[k], -p)});
+        mag += fabsf(p);
+    }
+    const float slack = 0x1p-44f * mag / proj.hi;
+    if (!(proj.hi > 0.0f) || !(slack < 0x1p-20f)) return false;
+    float t0, t1;
+    if (c.near_clip == 0.0f) t0 = c.near_clip;
+    else {
+        const FF nt = ff_div(c.near_clip, proj);
+        if (!ff_round(nt.hi, nt.lo, slack, t0)) return false;
+    }
+    if (c.far_clip == INFINITY) t1 = INFINITY;
+    else {
+        const FF ft = ff_div(c.far_clip, proj);
+        if (!ff_round(ft.hi, ft.lo, slack, t1)) return false;
+    }
+    d = V3{dd[0], dd[1], dd[2]};
+    tmin = t0;
+    tmax = t1;
+    return true;
+}
+
+CR_OUTLINE bool camera_ray(const DevCamera& c, float px, float py, V3& o, V3& d, float& tmin, float& tmax) {
+    if (camera_ray_ff(c, px, py, d, tmin, tmax)) {
+        o = V3{c.position[0], c.position[1], c.position[2]};
+        return true;
+    }
+    return camera_ray_chain(c, px, py, o, d, tmin, tmax);
 }
 
 // ---------------------------------------------------------------- offsetRay

@@ -462,6 +462,31 @@ int emu_camera_generate(const crg_camera* c, float px, float py, int w, int h, c
     }
 }
 
+// Camera::generate on the device for many pixels: the float-float path with the chain for hard
+// cases (camera_ray) and the binary64 chain alone; 6 floats per pixel (direction, tmin, tmax,
+// ok). Returns 1 when the camera itself is invalid.
+int emu_camera_pair(const crg_camera* c, const float* px, const float* py, uint64_t n, int w, int h, float* ff,
+                    float* chain) {
+    try {
+        const DevCamera cam = prepare_camera(*c, w, h);
+        for (uint64_t i = 0; i < n; ++i) {
+            V3 o, d{0, 0, 0};
+            float t0 = 0, t1 = 0;
+            bool ok = camera_ray(cam, px[i], py[i], o, d, t0, t1);
+            float* a = ff + 6 * i;
+            a[0] = d.x, a[1] = d.y, a[2] = d.z, a[3] = ok ? t0 : 0, a[4] = ok ? t1 : 0, a[5] = ok;
+            d = V3{0, 0, 0}, t0 = t1 = 0;
+            ok = camera_ray_chain(cam, px[i], py[i], o, d, t0, t1);
+            float* b = chain + 6 * i;
+            b[0] = d.x, b[1] = d.y, b[2] = d.z, b[3] = ok ? t0 : 0, b[4] = ok ? t1 : 0, b[5] = ok;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        lastError = e.what();
+        return 1;
+    }
+}
+
 int emu_render(void* sp, void* ep, const crg_camera* camera, const crg_options* o, float* film, crg_stats* stats) {
     const EmuScene& s = *static_cast<EmuScene*>(sp);
     const EmuEnv& env = *static_cast<EmuEnv*>(ep);

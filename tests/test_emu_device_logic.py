@@ -514,3 +514,50 @@ def test_interpolated_normal_float_float_equals_the_binary64_chain(emu):
     inv = 1.0 / np.sqrt(sq[ok])
     want = (N[ok] * inv[:, None]).astype(np.float32)
     np.testing.assert_array_equal(chain[ok].view(np.uint32), want.view(np.uint32))
+
+
+def test_camera_float_float_equals_the_binary64_chain(emu):
+    """Camera::generate (reference src/geometry.cpp:213-234) in float-float with the binary64
+    chain for hard cases equals the chain alone (direction, near and far parameters, accept)
+    on 1.5 M pixels of ten cameras: axis-aligned and oblique bases, aspect from the image
+    (0) and given, zero, tiny and large near clips, finite and infinite far clips, wide (up to
+    3.141592 rad: corner rays nearly perpendicular to an oblique view axis, where the
+    projection on it cancels) and narrow fields of view,
+    random sub-pixel positions, pixel corners, image centres and points a few ulps off them."""
+    fp = C.POINTER(C.c_float)
+    g = np.random.Generator(np.random.PCG64(151))
+    cams = [
+        (abi.camera(), 640, 480),
+        (abi.camera(near_clip=0.0, far_clip=float("inf")), 1920, 1080),
+        (scenes.look_at((3, 4, 5), (0, 0.5, 0), 0.8), 1920, 1080),
+        (scenes.look_at((-2.5, 1.0, 7.0), (0.3, -0.2, 0.1), 2.9), 333, 777),
+        (abi.camera(yfov=0.01, near_clip=1e-6, far_clip=1e6), 512, 512),
+        (abi.camera(forward=(0.3, -0.4, -0.86), right=(1, 0.2, 0.1), up=(0, 1, -0.5), aspect=2.35,
+                    near_clip=0.5, far_clip=50), 1000, 425),
+        (scenes.look_at((100, 200, -300), (0, 0, 0), 1.2), 4096, 2160),
+        (abi.camera(near_clip=3e4, far_clip=3e38), 64, 64),
+        (abi.camera(yfov=3.14, aspect=3.0, near_clip=0.25, far_clip=9.0), 800, 600),
+        (abi.camera(forward=(0.3, -0.4, -0.86), right=(1, 0.2, 0.1), up=(0, 1, -0.5), yfov=3.141592,
+                    aspect=3.0, near_clip=0.25, far_clip=9.0), 800, 600),
+    ]
+    for cam, w, h in cams:
+        n = 150_000
+        px = (g.uniform(0, 1, n) * w).astype(np.float32)
+        py = (g.uniform(0, 1, n) * h).astype(np.float32)
+        px[:1000] = np.float32(w / 2)
+        py[1000:2000] = np.float32(h / 2)
+        px[2000:3000] = g.integers(0, w + 1, 1000).astype(np.float32)
+        # a few ulps off the centre lines: the subtraction of 1 cancels and the binary64
+        # quotient's rounding becomes visible in the direction
+        steps = g.integers(-64, 65, 20000).astype(np.float32)
+        cx, cy = np.float32(w / 2), np.float32(h / 2)
+        px[3000:23000] = cx + steps * (np.spacing(cx))
+        py[23000:43000] = cy + steps * (np.spacing(cy))
+        ff = np.empty((n, 6), np.float32)
+        chain = np.empty_like(ff)
+        rc = emu.lib.emu_camera_pair(C.byref(cam), px.ctypes.data_as(fp), py.ctypes.data_as(fp), n, w, h,
+                                     ff.ctypes.data_as(fp), chain.ctypes.data_as(fp))
+        assert rc == 0
+        assert chain[:, 5].mean() > 0.99
+        bad = np.nonzero(np.any(ff.view(np.uint32) != chain.view(np.uint32), axis=1))[0]
+        assert len(bad) == 0, (w, h, [(px[i], py[i], ff[i].tolist(), chain[i].tolist()) for i in bad[:3]])
