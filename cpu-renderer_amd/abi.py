@@ -32,7 +32,7 @@ def _no_texture() -> TextureSlot:
 
 
 class Material(C.Structure):
-    """crg_material — defaults follow the reference Material (src/scene.h:187-199)."""
+    """crg_material — defaults follow the reference Material (src/scene.h:58-70)."""
     _fields_ = [("base_color", F4), ("metallic", C.c_float), ("roughness", C.c_float),
                 ("emission", F3), ("transmission", C.c_float), ("ior", C.c_float),
                 ("thickness", C.c_float), ("attenuation_distance", C.c_float),

@@ -3,7 +3,7 @@
 `Context` owns a device context; `render()` mirrors
 `cr::render(scene, bvh, camera, environment, options)` (reference
 src/render.h:53) and `trace()` mirrors `Bvh::intersect` over a ray array
-(reference src/scene.h:255-256).  Errors come back as exceptions carrying the
+(reference src/scene.h:126-127).  Errors come back as exceptions carrying the
 library's message (the reference's wording, e.g. "Pixel (x,y), sample s: ...").
 
 The product path is the gfx950 library only: if lib/libcrgpu.so is missing or
@@ -29,6 +29,29 @@ class CrgError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(message)
         self.code = code
+
+
+def write_scene_cache(scene: abi.SceneData, path, lib: Optional[C.CDLL] = None) -> abi.SceneInfo:
+    """Scene::finalize + Bvh::build on the host, written to a scene cache file
+    without a device (crg_scene_write_cache)."""
+    lib = lib or load_library()
+    desc = scene.descriptor()
+    info = abi.SceneInfo()
+    msg = C.create_string_buffer(1024)
+    rc = lib.crg_scene_write_cache(C.byref(desc), str(path).encode(), C.byref(info), msg, len(msg))
+    if rc != 0:
+        raise CrgError(rc, msg.value.decode(errors="replace"))
+    return info
+
+
+def check_scene_cache(path, lib: Optional[C.CDLL] = None) -> None:
+    """The checks crg_scene_load makes on a cache file, without a device
+    (crg_scene_check); raises CrgError with the reason."""
+    lib = lib or load_library()
+    msg = C.create_string_buffer(1024)
+    rc = lib.crg_scene_check(str(path).encode(), msg, len(msg))
+    if rc != 0:
+        raise CrgError(rc, msg.value.decode(errors="replace"))
 
 
 def load_library(path: Optional[pathlib.Path] = None) -> C.CDLL:
@@ -65,6 +88,9 @@ def load_library(path: Optional[pathlib.Path] = None) -> C.CDLL:
         "crg_kernel_times": (i32, [vp, C.POINTER(abi.KernelTime), i32, C.POINTER(i32)]),
         "crg_scene_save": (i32, [vp, u64, C.c_char_p]),
         "crg_scene_load": (i32, [vp, C.c_char_p, C.POINTER(u64), C.POINTER(abi.SceneInfo)]),
+        "crg_scene_write_cache": (i32, [C.POINTER(abi.SceneDesc), C.c_char_p, C.POINTER(abi.SceneInfo), C.c_char_p,
+                                        C.c_size_t]),
+        "crg_scene_check": (i32, [C.c_char_p, C.c_char_p, C.c_size_t]),
         "crg_film_encode": (i32, [vp, C.POINTER(C.c_float), i32, i32, C.c_float, C.POINTER(C.c_uint8),
                                   C.POINTER(C.c_uint8), u64, C.POINTER(u64)]),
         "crg_film_hdr_bound": (u64, [i32, i32]),

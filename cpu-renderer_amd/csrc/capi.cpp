@@ -65,10 +65,8 @@ struct StatusError : std::runtime_error {
 };
 
 template <class F>
-int guarded(crg_ctx* ctx, F&& f) {
-    if (!ctx) return CRG_EINVAL;
+int run_mapped(F&& f, std::string& msg) {
     int rc = CRG_OK;
-    std::string msg;
     try {
         f();
     } catch (const StatusError& e) {
@@ -90,8 +88,30 @@ int guarded(crg_ctx* ctx, F&& f) {
         rc = CRG_EINVAL;
         msg = e.what();
     }
+    return rc;
+}
+
+template <class F>
+int guarded(crg_ctx* ctx, F&& f) {
+    if (!ctx) return CRG_EINVAL;
+    std::string msg;
+    const int rc = run_mapped(std::forward<F>(f), msg);
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->error = msg;
+    return rc;
+}
+
+// Entry points without a context report their failure text into a caller
+// buffer (truncated to capacity, always terminated when capacity > 0).
+template <class F>
+int guarded_msg(char* message, size_t capacity, F&& f) {
+    std::string msg;
+    const int rc = run_mapped(std::forward<F>(f), msg);
+    if (message && capacity) {
+        const size_t n = std::min(capacity - 1, msg.size());
+        std::memcpy(message, msg.data(), n);
+        message[n] = 0;
+    }
     return rc;
 }
 
@@ -234,7 +254,9 @@ crg_scene_info scene_info(const SceneRes& r) {
 
 // ---- scene cache file: the device-format arrays of a resident scene, so a
 // server can make a scene resident again without finalize and BVH build.
-constexpr char kCacheMagic[8] = {'C', 'R', 'G', 'S', 'C', 'N', '1', 0};
+// The magic's last two characters are the format version: version 2 files are
+// those whose every index the loader checks before anything is uploaded.
+constexpr char kCacheMagic[8] = {'C', 'R', 'G', 'S', 'C', 'N', '0', '2'};
 
 struct CacheWriter {
     std::vector<uint8_t> out;
@@ -368,6 +390,196 @@ std::vector<T> download(const DevBuf& b, uint64_t count, hipStream_t st) {
     return v;
 }
 
+// The device-format arrays of a scene downloaded from the device.
+struct CacheContent {
+    std::vector<WNode> nodes;
+    std::vector<PrimRec> prims;
+    std::vector<float4> prim_ng;
+    std::vector<PrimAux> aux;
+    std::vector<DevMaterial> materials;
+    std::vector<float4> uvs, tangents, texels;
+    std::vector<DevImage> images;
+    std::vector<DevTexture> textures;
+};
+
+SceneSections sections_of(const CacheContent& c) {
+    return {0,
+            section(c.nodes),    section(c.prims),  section(c.prim_ng), section(c.aux),    section(c.materials),
+            section(c.uvs),      section(c.tangents), section(c.texels), section(c.images), section(c.textures)};
+}
+
+std::string cache_label(const char* path) { return std::string("Scene cache '") + path + "'"; }
+
+void put_section(CacheWriter& w, const Section& s) {
+    w.pod(uint64_t(s.count));
+    w.raw(s.p, s.bytes);
+}
+
+// Writes a cache file: magic, record sizes, triangle and binary-node counts,
+// the device-format sections, the leaf-order maps, the light tables, and the
+// checksum of everything before it.
+void write_cache_file(const char* path, uint64_t triangles, uint64_t binary_nodes, const SceneSections& sec,
+                      const std::vector<uint32_t>& prim_to_tri, const std::vector<uint32_t>& tri_to_prim,
+                      const SceneLights& lights) {
+    CacheWriter w;
+    w.out.reserve(4096 + sec.nodes.bytes + sec.prims.bytes + sec.prim_ng.bytes + sec.aux.bytes + sec.materials.bytes +
+                  sec.uvs.bytes + sec.tangents.bytes + sec.texels.bytes + sec.images.bytes + sec.textures.bytes +
+                  8 * (prim_to_tri.size() + tri_to_prim.size()));
+    w.raw(kCacheMagic, 8);
+    w.raw(kRecordSizes, sizeof(kRecordSizes));
+    w.pod(triangles);
+    w.pod(binary_nodes);
+    for (const Section* s : {&sec.nodes, &sec.prims, &sec.prim_ng, &sec.aux, &sec.materials, &sec.uvs, &sec.tangents,
+                             &sec.texels, &sec.images, &sec.textures})
+        put_section(w, *s);
+    w.vec(prim_to_tri);
+    w.vec(tri_to_prim);
+    w.pod(lights.radius2);
+    w.vec(lights.punctual_weight);
+    w.vec(lights.emissive);
+    w.vec(lights.emissive_weight);
+    w.vec(lights.punctual);
+    w.pod(cache_checksum(w.out.data(), w.out.size()));
+    std::FILE* f = std::fopen(path, "wb");
+    const bool ok = f && std::fwrite(w.out.data(), 1, w.out.size(), f) == w.out.size();
+    if (!f || std::fclose(f) != 0 || !ok) throw std::runtime_error(std::string("Failed writing scene cache '") + path + "'");
+}
+
+// A cache file's sections as views into its bytes (nodes ... textures) and
+// copies (maps, lights).
+struct ParsedCache {
+    uint64_t triangles = 0;
+    SceneSections sec;
+    std::vector<uint32_t> order, tri_to_prim;
+    SceneLights lights;
+};
+
+ParsedCache parse_cache(const FileBytes& file, const std::string& label) {
+    const uint8_t* bytes = file.p.get();
+    if (file.n < 8 + sizeof(kRecordSizes) + 24 || std::memcmp(bytes, kCacheMagic, 6) != 0)
+        throw std::runtime_error(label + ": not a scene cache of this library");
+    if (std::memcmp(bytes, kCacheMagic, 8) != 0)
+        throw std::runtime_error(label + ": written in another format version of this library");
+    uint64_t stored;
+    std::memcpy(&stored, bytes + file.n - 8, 8);
+    if (stored != cache_checksum(bytes, file.n - 8)) throw std::runtime_error(label + ": checksum mismatch");
+    CacheReader rd{bytes, file.n - 8, 8, label};
+    uint32_t sizes[8];
+    rd.raw(sizes, sizeof(sizes));
+    if (std::memcmp(sizes, kRecordSizes, sizeof(sizes)) != 0)
+        throw std::runtime_error(label + ": written by a build with other record layouts");
+    ParsedCache pc;
+    pc.triangles = rd.pod<uint64_t>();
+    SceneSections& sec = pc.sec;
+    sec.binary_nodes = rd.pod<uint64_t>();
+    sec.nodes = rd.section<WNode>();
+    sec.prims = rd.section<PrimRec>();
+    sec.prim_ng = rd.section<float4>();
+    sec.aux = rd.section<PrimAux>();
+    sec.materials = rd.section<DevMaterial>();
+    sec.uvs = rd.section<float4>();
+    sec.tangents = rd.section<float4>();
+    sec.texels = rd.section<float4>();
+    sec.images = rd.section<DevImage>();
+    sec.textures = rd.section<DevTexture>();
+    pc.order = rd.vec<uint32_t>();
+    pc.tri_to_prim = rd.vec<uint32_t>();
+    pc.lights.radius2 = rd.pod<double>();
+    pc.lights.punctual_weight = rd.vec<double>();
+    pc.lights.emissive = rd.vec<uint32_t>();
+    pc.lights.emissive_weight = rd.vec<double>();
+    pc.lights.punctual = rd.vec<DevPunctual>();
+    if (rd.pos != rd.size) throw std::runtime_error(label + ": inconsistent section sizes");
+    return pc;
+}
+
+template <class T>
+T record(const Section& s, size_t i) {
+    T v;
+    std::memcpy(&v, static_cast<const uint8_t*>(s.p) + i * sizeof(T), sizeof(T));
+    return v;
+}
+
+// Every index a kernel or the host follows, checked before anything is
+// uploaded, so that a file whose checksum matches but whose contents do not
+// describe a scene (edited by hand, or written with another meaning of a
+// field) is refused instead of being read out of bounds:
+//   section sizes agree with the triangle and primitive counts;
+//   wide nodes: inner children have larger indices than their parent (the
+//   tree is acyclic, so every walk ends), leaves lie inside the primitives;
+//   primitives: triangle ids, materials and flags; the leaf-order maps;
+//   materials: texture slots and UV sets, with UVs (and tangents for normal
+//   maps) present; textures, images and texel ranges; emissive triangles and
+//   punctual light types.
+void validate_cache(const ParsedCache& pc, const std::string& label) {
+    const SceneSections& sec = pc.sec;
+    const uint64_t n = pc.triangles, np = sec.prims.count, nn = sec.nodes.count;
+    auto fail = [&](const std::string& what) { throw std::runtime_error(label + ": " + what); };
+    if (np < n || sec.prim_ng.count != np || sec.aux.count != np || pc.order.size() != np || pc.tri_to_prim.size() != n ||
+        (sec.uvs.count && sec.uvs.count != 3 * np) || (sec.tangents.count && sec.tangents.count != 3 * np) ||
+        (n && !nn) || pc.lights.emissive.size() != pc.lights.emissive_weight.size() ||
+        pc.lights.punctual.size() != pc.lights.punctual_weight.size())
+        fail("inconsistent section sizes");
+    if (np > kLeafMaxFirst + 1 || nn >= kEmptyChild || n >= kNoTriangle) fail("more records than the device format holds");
+    std::atomic<bool> bad_node{false}, bad_prim{false}, bad_map{false};
+    parallel_chunks(nn, size_t(1) << 16, [&](size_t b, size_t e, size_t) {
+        for (size_t i = b; i < e && !bad_node; ++i) {
+            const WNode w = record<WNode>(sec.nodes, i);
+            for (uint32_t c : w.child) {
+                if (c == kEmptyChild) continue;
+                if (ref_is_leaf(c) ? uint64_t(leaf_first(c)) + leaf_count(c) > np : (c <= i || c >= nn)) bad_node = true;
+            }
+        }
+    });
+    if (bad_node) fail("BVH node refers outside the node or primitive arrays");
+    const uint32_t nm = uint32_t(sec.materials.count);
+    constexpr uint32_t known_flags = kPrimCull | kPrimNeverHit | kPrimMaskTexture | (3u << kPrimClassShift);
+    parallel_chunks(np, size_t(1) << 16, [&](size_t b, size_t e, size_t) {
+        for (size_t i = b; i < e; ++i) {
+            const PrimRec p = record<PrimRec>(sec.prims, i);
+            if (p.id >= n || p.material >= nm || (p.flags & ~known_flags) || pc.order[i] != p.id) bad_prim = true;
+        }
+    });
+    if (bad_prim) fail("primitive record refers outside the triangle or material arrays");
+    parallel_chunks(n, size_t(1) << 16, [&](size_t b, size_t e, size_t) {
+        for (size_t t = b; t < e; ++t) {
+            const uint32_t p = pc.tri_to_prim[t];
+            if (p >= np || pc.order[p] != t) bad_map = true;
+        }
+    });
+    if (bad_map) fail("triangle to primitive map inconsistent");
+    const uint64_t nt = sec.texels.count, ni = sec.images.count, nx = sec.textures.count;
+    for (size_t i = 0; i < ni; ++i) {
+        const DevImage im = record<DevImage>(sec.images, i);
+        const uint64_t size = uint64_t(uint32_t(im.width)) * uint64_t(uint32_t(im.height));
+        if (im.width <= 0 || im.height <= 0 || im.offset > nt || size > nt - im.offset || im.srgb_offset > nt ||
+            size > nt - im.srgb_offset)
+            fail("image " + std::to_string(i) + " lies outside the texel array");
+    }
+    for (size_t i = 0; i < nx; ++i) {
+        const DevTexture t = record<DevTexture>(sec.textures, i);
+        if (t.image < 0 || uint64_t(t.image) >= ni || t.wrap_s < 0 || t.wrap_s > 2 || t.wrap_t < 0 || t.wrap_t > 2)
+            fail("texture " + std::to_string(i) + " refers outside the image array");
+    }
+    for (size_t i = 0; i < nm; ++i) {
+        const DevMaterial m = record<DevMaterial>(sec.materials, i);
+        bool any = false;
+        for (const DevSlot* sl : {&m.base_tex, &m.mr_tex, &m.normal_tex, &m.emissive_tex, &m.transmission_tex}) {
+            if (sl->texture < 0) continue;
+            any = true;
+            if (uint64_t(sl->texture) >= nx || sl->texcoord < 0 || sl->texcoord > 1)
+                fail("material " + std::to_string(i) + " refers outside the texture array");
+        }
+        if ((any || m.maps) && sec.uvs.count != 3 * np) fail("material " + std::to_string(i) + " samples textures but the file holds no UVs");
+        if (m.normal_tex.texture >= 0 && sec.tangents.count != 3 * np)
+            fail("material " + std::to_string(i) + " has a normal map but the file holds no tangents");
+    }
+    for (uint32_t t : pc.lights.emissive)
+        if (t >= n) fail("emissive triangle outside the triangle array");
+    for (const DevPunctual& l : pc.lights.punctual)
+        if (l.type > 2) fail("unknown punctual light type");
+}
+
 // The sampling-angle table of `device`: built on the host once per process and
 // uploaded once per device, then kept until the process ends (128 MiB), shared
 // by every context on that device.
@@ -468,7 +680,7 @@ int crg_scene_save(crg_ctx* ctx, uint64_t scene_id, const char* path) {
     return guarded(ctx, [&] {
         if (!path) throw std::invalid_argument("null path");
         std::shared_ptr<SceneRes> r;
-        CacheWriter w;
+        CacheContent c;
         {
             std::lock_guard<std::mutex> lock(ctx->mu);
             auto it = ctx->scenes.find(scene_id);
@@ -476,101 +688,36 @@ int crg_scene_save(crg_ctx* ctx, uint64_t scene_id, const char* path) {
             r = it->second;
             check(hipSetDevice(ctx->device), "hipSetDevice");
             hipStream_t st = ctx->stream;
-            const uint64_t n = r->triangles;
-            w.raw(kCacheMagic, 8);
-            w.raw(kRecordSizes, sizeof(kRecordSizes));
-            w.pod(n);
-            w.pod(r->binary_nodes);
-            auto nodes = download<WNode>(r->nodes, r->host.node_count, st);
             const uint64_t np = r->host.prim_count;
-            auto prims = download<PrimRec>(r->prims, np, st);
-            auto ng = download<float4>(r->prim_ng, np, st);
-            auto aux = download<PrimAux>(r->aux, np, st);
-            auto mats = download<DevMaterial>(r->materials, r->material_count, st);
-            auto uvs = download<float4>(r->uvs, r->host.uvs ? 3 * np : 0, st);
-            auto tans = download<float4>(r->tangents, r->host.tangents ? 3 * np : 0, st);
-            auto texels = download<float4>(r->texels, r->texel_count, st);
-            auto images = download<DevImage>(r->images, r->image_count, st);
-            auto textures = download<DevTexture>(r->textures, r->texture_count, st);
+            c.nodes = download<WNode>(r->nodes, r->host.node_count, st);
+            c.prims = download<PrimRec>(r->prims, np, st);
+            c.prim_ng = download<float4>(r->prim_ng, np, st);
+            c.aux = download<PrimAux>(r->aux, np, st);
+            c.materials = download<DevMaterial>(r->materials, r->material_count, st);
+            c.uvs = download<float4>(r->uvs, r->host.uvs ? 3 * np : 0, st);
+            c.tangents = download<float4>(r->tangents, r->host.tangents ? 3 * np : 0, st);
+            c.texels = download<float4>(r->texels, r->texel_count, st);
+            c.images = download<DevImage>(r->images, r->image_count, st);
+            c.textures = download<DevTexture>(r->textures, r->texture_count, st);
             check(hipStreamSynchronize(st), "scene download");
-            w.out.reserve(4096 + (nodes.size() * sizeof(WNode) + np * (sizeof(PrimRec) + sizeof(PrimAux) + 24) +
-                                              (uvs.size() + tans.size() + ng.size() + texels.size()) * sizeof(float4) +
-                                              mats.size() * sizeof(DevMaterial) + images.size() * sizeof(DevImage) +
-                                              textures.size() * sizeof(DevTexture)));
-            w.vec(nodes);
-            w.vec(prims);
-            w.vec(ng);
-            w.vec(aux);
-            w.vec(mats);
-            w.vec(uvs);
-            w.vec(tans);
-            w.vec(texels);
-            w.vec(images);
-            w.vec(textures);
         }
-        w.vec(r->prim_to_tri);
-        w.vec(r->tri_to_prim);
-        w.pod(r->lights.radius2);
-        w.vec(r->lights.punctual_weight);
-        w.vec(r->lights.emissive);
-        w.vec(r->lights.emissive_weight);
-        w.vec(r->lights.punctual);
-        w.pod(cache_checksum(w.out.data(), w.out.size()));
-        std::FILE* f = std::fopen(path, "wb");
-        const bool ok = f && std::fwrite(w.out.data(), 1, w.out.size(), f) == w.out.size();
-        if (!f || std::fclose(f) != 0 || !ok) throw std::runtime_error(std::string("Failed writing scene cache '") + path + "'");
+        write_cache_file(path, r->triangles, r->binary_nodes, sections_of(c), r->prim_to_tri, r->tri_to_prim,
+                         r->lights);
     });
 }
 
 int crg_scene_load(crg_ctx* ctx, const char* path, uint64_t* scene_id, crg_scene_info* info) {
     return guarded(ctx, [&] {
         if (!path || !scene_id) throw std::invalid_argument("null path or scene id");
-        const std::string label = std::string("Scene cache '") + path + "'";
         const auto t0 = std::chrono::steady_clock::now();
-        const FileBytes file = read_file(path, label);
-        const uint8_t* bytes = file.p.get();
-        if (file.n < 8 + sizeof(kRecordSizes) + 24 || std::memcmp(bytes, kCacheMagic, 8) != 0)
-            throw std::runtime_error(label + ": not a scene cache of this library");
-        uint64_t stored;
-        std::memcpy(&stored, bytes + file.n - 8, 8);
-        if (stored != cache_checksum(bytes, file.n - 8)) throw std::runtime_error(label + ": checksum mismatch");
-        CacheReader rd{bytes, file.n - 8, 8, label};
-        uint32_t sizes[8];
-        rd.raw(sizes, sizeof(sizes));
-        if (std::memcmp(sizes, kRecordSizes, sizeof(sizes)) != 0)
-            throw std::runtime_error(label + ": written by a build with other record layouts");
-        const uint64_t n = rd.pod<uint64_t>();
-        SceneSections sec;
-        sec.binary_nodes = rd.pod<uint64_t>();
-        sec.nodes = rd.section<WNode>();
-        sec.prims = rd.section<PrimRec>();
-        sec.prim_ng = rd.section<float4>();
-        sec.aux = rd.section<PrimAux>();
-        sec.materials = rd.section<DevMaterial>();
-        sec.uvs = rd.section<float4>();
-        sec.tangents = rd.section<float4>();
-        sec.texels = rd.section<float4>();
-        sec.images = rd.section<DevImage>();
-        sec.textures = rd.section<DevTexture>();
-        std::vector<uint32_t> order = rd.vec<uint32_t>();
-        std::vector<uint32_t> tri_to_prim = rd.vec<uint32_t>();
-        SceneLights lights;
-        lights.radius2 = rd.pod<double>();
-        lights.punctual_weight = rd.vec<double>();
-        lights.emissive = rd.vec<uint32_t>();
-        lights.emissive_weight = rd.vec<double>();
-        lights.punctual = rd.vec<DevPunctual>();
-        const uint64_t np = sec.prims.count;  // leaf-order primitives (split triangles repeat)
-        if (rd.pos != rd.size || np < n || sec.prim_ng.count != np || sec.aux.count != np || order.size() != np ||
-            tri_to_prim.size() != n || (sec.uvs.count && sec.uvs.count != 3 * np) ||
-            (sec.tangents.count && sec.tangents.count != 3 * np) || (n && !sec.nodes.count) ||
-            lights.emissive.size() != lights.emissive_weight.size() ||
-            lights.punctual.size() != lights.punctual_weight.size())
-            throw std::runtime_error(label + ": inconsistent section sizes");
+        const FileBytes file = read_file(path, cache_label(path));
+        ParsedCache pc = parse_cache(file, cache_label(path));
+        validate_cache(pc, cache_label(path));
         check(hipSetDevice(ctx->device), "hipSetDevice");
         const auto t1 = std::chrono::steady_clock::now();
         std::lock_guard<std::mutex> lock(ctx->mu);
-        auto res = make_resident(ctx, sec, std::move(order), std::move(tri_to_prim), std::move(lights), n);
+        auto res = make_resident(ctx, pc.sec, std::move(pc.order), std::move(pc.tri_to_prim), std::move(pc.lights),
+                                 pc.triangles);
         const auto t2 = std::chrono::steady_clock::now();
         if (info) {
             *info = scene_info(*res);
@@ -578,6 +725,37 @@ int crg_scene_load(crg_ctx* ctx, const char* path, uint64_t* scene_id, crg_scene
             info->upload_seconds = std::chrono::duration<double>(t2 - t1).count();
         }
         *scene_id = add_scene(ctx, std::move(res));
+    });
+}
+
+int crg_scene_write_cache(const crg_scene_desc* desc, const char* path, crg_scene_info* info, char* message,
+                          size_t capacity) {
+    return guarded_msg(message, capacity, [&] {
+        if (!desc || !path) throw std::invalid_argument("null scene descriptor or path");
+        if (desc->triangle_count && (!desc->positions || !desc->material_ids))
+            throw std::invalid_argument("scene descriptor without positions or material ids");
+        const auto t0 = std::chrono::steady_clock::now();
+        HostScene hs = finalize_scene(*desc);
+        SceneArrays arr = scene_arrays(hs);
+        SceneLights lights = scene_lights(hs);
+        const auto t1 = std::chrono::steady_clock::now();
+        write_cache_file(path, hs.tris.size(), arr.bvh.binary_nodes, sections_of(arr), arr.bvh.order, arr.tri_to_prim,
+                         lights);
+        if (info) {
+            *info = crg_scene_info{};
+            info->triangles = hs.tris.size();
+            info->bvh_nodes = arr.bvh.binary_nodes;
+            info->wide_nodes = arr.bvh.nodes.size();
+            info->build_seconds = std::chrono::duration<double>(t1 - t0).count();
+        }
+    });
+}
+
+int crg_scene_check(const char* path, char* message, size_t capacity) {
+    return guarded_msg(message, capacity, [&] {
+        if (!path) throw std::invalid_argument("null path");
+        const FileBytes file = read_file(path, cache_label(path));
+        validate_cache(parse_cache(file, cache_label(path)), cache_label(path));
     });
 }
 

@@ -105,7 +105,7 @@ void validate_options(const crg_options& o);
 // reference's rethrown exceptions (reference src/render.cpp:437-438, :310-323).
 std::string format_path_error(const ErrorRec& e, int width, int spp);
 
-// Material table with precomputed absorption (reference src/bsdf.cpp:63-68)
+// Material table with precomputed absorption (reference src/bsdf.cpp:29-34)
 // and texture slots with their rotation's binary64 cos/sin.
 std::vector<DevMaterial> device_materials(const HostScene& s);
 

@@ -124,7 +124,7 @@ struct alignas(16) PrimAux {
 };
 static_assert(sizeof(PrimAux) == 64, "PrimAux is one 64-byte record");
 
-// Texture binding (reference TextureSlot, src/scene.h:181-184) with the
+// Texture binding (reference TextureSlot, src/scene.h:50-53) with the
 // binary64 cos/sin of its rotation computed on the host (std::cos/std::sin on
 // the same value the reference converts).
 struct DevSlot {

@@ -591,7 +591,7 @@ CR_HDI float power_heuristic_ff(float a, float b) {
     return f;
 }
 
-// Beer-Lambert transmittance over `distance` (reference src/bsdf.cpp:71-74).
+// Beer-Lambert transmittance over `distance` (reference src/bsdf.cpp:37-40).
 CR_HDI V3 beer_lambert(V3 sigma, float distance) {
     if (distance == kInf) return V3{sigma.x > 0 ? 0.0f : 1.0f, sigma.y > 0 ? 0.0f : 1.0f, sigma.z > 0 ? 0.0f : 1.0f};
     const V3 a = sigma * (-fmax_(0.0f, distance));

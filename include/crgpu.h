@@ -6,8 +6,8 @@
  *                           const Environment&, const RenderOptions&, Progress)
  *                                                   (reference src/render.h:53)
  *   bool Bvh::intersect(const Scene&, const Ray&, Hit&, bool any, uint32_t ignore)
- *                                                   (reference src/scene.h:255-256)
- *   void Bvh::build(const Scene&)                   (reference src/scene.h:252)
+ *                                                   (reference src/scene.h:126-127)
+ *   void Bvh::build(const Scene&)                   (reference src/scene.h:123)
  *
  * Every entry point takes plain pointers and sizes, never throws, and returns
  * an int status (CRG_OK or one of the CRG_E* codes).  The text of the most
@@ -16,7 +16,7 @@
  * reference src/render.cpp:437-438).
  *
  * Descriptor structs below mirror the reference scene model after import
- * (reference src/scene.h:139-233, src/render.h:8-39) as flat arrays.  The same
+ * (reference src/scene.h:10-117, src/render.h:8-39) as flat arrays.  The same
  * descriptors are consumed by the CPU oracle (oracle/) so that every parity
  * test feeds identical inputs to both sides.
  */
@@ -40,13 +40,13 @@ extern "C" {
 #define CRG_EBUSY     6  /* request still in flight (crg_wait timed out)    */
 
 /* ---- scene model ------------------------------------------------------- */
-/* Texture binding: texture = -1 means "no map" (reference src/scene.h:179-182). */
+/* Texture binding: texture = -1 means "no map" (reference src/scene.h:49-53). */
 typedef struct crg_texture_slot {
     int32_t texture, texcoord;
     float offset[2], scale[2], rotation;
 } crg_texture_slot;
 
-/* Material factors (reference src/scene.h:187-199).  alpha_mode: 0 OPAQUE, 1 MASK. */
+/* Material factors (reference src/scene.h:58-70).  alpha_mode: 0 OPAQUE, 1 MASK. */
 typedef struct crg_material {
     float base_color[4];
     float metallic, roughness;
@@ -60,27 +60,27 @@ typedef struct crg_material {
     crg_texture_slot base_color_tex, metallic_roughness_tex, normal_tex, emissive_tex, transmission_tex;
 } crg_material;
 
-/* Punctual light (reference src/scene.h:208-215). type: 0 point, 1 directional, 2 spot. */
+/* Punctual light (reference src/scene.h:79-86). type: 0 point, 1 directional, 2 spot. */
 typedef struct crg_light {
     int32_t type;
     float position[3], direction[3], color[3];
     float intensity, range, inner_cone, outer_cone;
 } crg_light;
 
-/* Decoded image: rgba is width*height*4 floats (reference src/scene.h:176). */
+/* Decoded image: rgba is width*height*4 floats (reference src/scene.h:47). */
 typedef struct crg_image {
     int32_t width, height;
     const float* rgba;
 } crg_image;
 
-/* Sampler binding of an image. wrap: 0 repeat, 1 clamp, 2 mirror (reference src/scene.h:173-177). */
+/* Sampler binding of an image. wrap: 0 repeat, 1 clamp, 2 mirror (reference src/scene.h:44-48). */
 typedef struct crg_texture {
     int32_t image, wrap_s, wrap_t, linear;
 } crg_texture;
 
 /*
  * Triangle soup in world space, before Scene::finalize (reference
- * src/scene.h:155-163).  positions: 9 floats per triangle (v0,v1,v2).
+ * src/scene.h:24-34).  positions: 9 floats per triangle (v0,v1,v2).
  * normals: 9 floats per triangle or NULL (zero normals are repaired by
  * finalize).  tangents: 12 floats per triangle (xyzw per vertex) or NULL for
  * (0,0,0,1).  uvs: 12 floats per triangle laid out [vertex][set][uv] or NULL.
@@ -102,7 +102,7 @@ typedef struct crg_scene_desc {
     const crg_texture* textures;
 } crg_scene_desc;
 
-/* Perspective camera (reference src/scene.h:202-207). */
+/* Perspective camera (reference src/scene.h:71-78). */
 typedef struct crg_camera {
     float position[3], forward[3], right[3], up[3];
     float yfov, aspect, near_clip, far_clip;
@@ -149,9 +149,9 @@ typedef struct crg_stats {
 } crg_stats;
 
 /*
- * Ray query for the traversal parity harness (reference src/scene.h:139-143):
+ * Ray query for the traversal parity harness (reference src/scene.h:8-14):
  * the ray is origin + origin_low + t*direction on [tmin, tmax]; `ignore` is
- * the excluded source triangle (0xffffffff = none, reference src/scene.h:255).
+ * the excluded source triangle (0xffffffff = none, reference src/scene.h:124-127).
  */
 typedef struct crg_ray {
     float origin[3], direction[3];
@@ -161,7 +161,7 @@ typedef struct crg_ray {
 } crg_ray;
 
 /* Closest/any-hit result.  t,u,v are the binary64 values the reference's
- * Hit holds (reference src/scene.h:166); the device computes them with an
+ * Hit holds (reference src/scene.h:35-37); the device computes them with an
  * exact integer emulation of binary64 and never executes FP64 instructions. */
 typedef struct crg_hit {
     uint32_t hit, triangle;
@@ -202,7 +202,21 @@ int crg_scene_upload(crg_ctx* ctx, const crg_scene_desc* desc, uint64_t* scene_i
  * A file is only valid for builds with the same record layouts.
  */
 int crg_scene_save(crg_ctx* ctx, uint64_t scene_id, const char* path);
+/* Loading checks every index in the file (node children, leaf ranges,
+ * triangle ids, materials, textures, images, texel ranges, lights, the
+ * leaf-order maps) before anything is uploaded, and refuses a file that fails
+ * with a message naming what is wrong. */
 int crg_scene_load(crg_ctx* ctx, const char* path, uint64_t* scene_id, crg_scene_info* info);
+/* Scene::finalize + Bvh::build + flattening on the host (reference
+ * src/geometry.cpp:296-397, :463-550) written straight to a cache file, with
+ * no device: scenes can be prepared offline and made resident later by
+ * crg_scene_load.  info->device_bytes is 0.  Without a context, failure text
+ * goes to message (capacity bytes, always terminated; may be NULL). */
+int crg_scene_write_cache(const crg_scene_desc* desc, const char* path, crg_scene_info* info, char* message,
+                          size_t capacity);
+/* The checks crg_scene_load makes (format, checksum, record layouts, every
+ * index), without a device: CRG_OK, or CRG_EINVAL with the reason in message. */
+int crg_scene_check(const char* path, char* message, size_t capacity);
 int crg_scene_release(crg_ctx* ctx, uint64_t scene_id);
 /* Environment construction (reference src/render.cpp:12-42). */
 int crg_env_upload(crg_ctx* ctx, const crg_env_desc* desc, uint64_t* env_id, uint64_t* device_bytes);

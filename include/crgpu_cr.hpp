@@ -3,7 +3,7 @@
 // libcrgpu.so; `cr::gpu::render(...)` then replaces `cr::render(...)` with the
 // same arguments and result type (reference src/render.h:53), and
 // `cr::gpu::intersect(...)` replaces `Bvh::intersect` for ray batches
-// (reference src/scene.h:255-256).
+// (reference src/scene.h:126-127).
 //
 // The adapter is a template over the reference types so this header compiles
 // on its own; it is instantiated where the reference's render.h is included.

@@ -17,7 +17,7 @@
 extern "C" {
 #endif
 
-/* Mirror of the reference Surface (reference src/scene.h:236-244). */
+/* Mirror of the reference Surface (reference src/scene.h:105-115). */
 typedef struct probe_surface {
     float position[3], position_low[3], geometric_normal[3], normal[3], tangent[3], bitangent[3];
     float base_color[3], emission[3], attenuation_color[3];

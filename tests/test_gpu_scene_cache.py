@@ -76,3 +76,33 @@ def test_damaged_files_are_refused(gpu, tmp_path):
         gpu.load_scene(bad)
     with pytest.raises(CrgError, match="cannot open"):
         gpu.load_scene(tmp_path / "missing.crgscene")
+
+
+def test_host_written_cache_loads_and_renders_like_an_upload(gpu, tmp_path):
+    """crg_scene_write_cache (no device) writes the file crg_scene_load makes
+    resident; its renders equal those of the uploaded scene, and a file whose
+    index was edited (checksum recomputed) is refused by the loader too."""
+    import ref_scenes as rs
+    import test_scene_cache_host as th
+    from cr_amd import abi, crgpu
+    sc, env, cam = rs.textured_factors()
+    opt = abi.options(40, 32, 4, 6, threads=1, seed=5)
+    path = tmp_path / "host.crgscene"
+    crgpu.write_scene_cache(sc, path)
+    sid, _ = gpu.upload_scene(sc)
+    eid, _ = gpu.upload_env(env)
+    lid, _ = gpu.load_scene(path)
+    try:
+        a = gpu.render(sid, eid, cam, opt)
+        b = gpu.render(lid, eid, cam, opt)
+        np.testing.assert_array_equal(a.pixels.view(np.uint32), b.pixels.view(np.uint32))
+    finally:
+        gpu.release_scene(lid)
+        gpu.release_scene(sid)
+        gpu.release_env(eid)
+    f = th.CacheFile(path.read_bytes())
+    f.put_u32("prims", 0, 44, 77)
+    bad = tmp_path / "bad.crgscene"
+    bad.write_bytes(f.sealed())
+    with pytest.raises(crgpu.CrgError, match="primitive record refers outside"):
+        gpu.load_scene(bad)
