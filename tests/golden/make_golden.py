@@ -453,6 +453,84 @@ def film_error_cases():
             ("exposure_nan", good, float("nan"))]
 
 
+# ---------------------------------------------------------------- batched / served requests
+BATCH_SEEDS = (1000, 1001, 1002, 1003, 1005, 1008)
+
+
+def batch_cases():
+    """(name, workload, options) render parity cases for the batched (C4) and served (C5)
+    forms: requests of the bench's mix generator (scenes.batch_request, the same seeds as the
+    64-request mix) capped at 50k triangles, at 32^2..64^2 px, 4..16 spp and the request's own
+    8..16 bounces.  Terrain, spheres and soup scenes all occur."""
+    sky = scenes.synthetic_sky()
+    out = []
+    for seed in BATCH_SEEDS:
+        w = scenes.batch_request(seed, sky=sky, max_triangles=50_000)
+        side = 32 + 8 * (seed % 5)
+        spp = max(4, min(16, w.samples))
+        out.append((w.name, w, w.options(width=side, height=side, samples=spp)))
+    return out
+
+
+def gen_render_batch(ref):
+    out = {}
+    for name, w, opt in batch_cases():
+        ps, pe = ref.scene(w.scene), ref.env(w.env)
+        film, stats = ps.render(pe, w.camera, opt)
+        out[f"{name}/film"] = film
+        out[f"{name}/retries"] = np.array(stats.boundary_retries)
+        out[f"{name}/scene_sha"] = np.array(w.scene.bytes_sha256())
+        out[f"{name}/options"] = np.array([opt.width, opt.height, opt.samples, opt.bounces, opt.seed], np.int64)
+        print(f"  {name}: {int(w.scene.triangle_count)} tris {opt.width}x{opt.height}x{opt.samples} "
+              f"b{opt.bounces} mean {film.mean():.5f} retries {stats.boundary_retries} {stats.seconds:.2f}s")
+    np.savez_compressed(OUT / "render_batch.npz", **out)
+
+
+# ---------------------------------------------------------------- configs 2 and 3 at full scene size
+FULL_NAMES = ("c2_full", "c3_full")
+
+
+def full_case(name):
+    """(name, workload, options, rays) at the benchmark scenes' real sizes: the 3,117,058-triangle
+    config-2 scene at 160x90, 4 spp, 16 bounces and the 8,000,048-triangle config-3 scene at
+    128x72, 4 spp, 24 bounces, each with a 20k-ray traversal set."""
+    if name == "c2_full":
+        w = scenes.config2()
+        return name, w, w.options(width=160, height=90, samples=4), scenes.random_rays(w.scene, 20000, seed=21)
+    w = scenes.config3()
+    return name, w, w.options(width=128, height=72, samples=4), scenes.random_rays(w.scene, 20000, seed=22, spread=1.2)
+
+
+def full_cases():
+    for name in FULL_NAMES:
+        yield full_case(name)
+
+
+def gen_full(ref):
+    out = {}
+    for name, w, opt, rays in full_cases():
+        import time
+        t = time.time()
+        ps, pe = ref.scene(w.scene), ref.env(w.env)
+        build = time.time() - t
+        film, stats = ps.render(pe, w.camera, opt)
+        hits = ps.trace(rays, any_hit=False)
+        anyh = ps.trace(rays, any_hit=True)
+        out[f"{name}/film"] = film
+        out[f"{name}/retries"] = np.array(stats.boundary_retries)
+        out[f"{name}/scene_sha"] = np.array(w.scene.bytes_sha256())
+        out[f"{name}/rays_sha"] = np.array(sha(rays))
+        out[f"{name}/triangles"] = np.array(int(w.scene.triangle_count))
+        for k, v in hits_arrays(hits).items():
+            out[f"{name}/{k}"] = v
+        out[f"{name}/any"] = anyh["hit"].copy()
+        print(f"  {name}: {int(w.scene.triangle_count)} tris build {build:.1f}s {opt.width}x{opt.height}x"
+              f"{opt.samples} mean {film.mean():.5f} retries {stats.boundary_retries} {stats.seconds:.2f}s, "
+              f"{int(hits['hit'].sum())}/{len(rays)} rays hit")
+        del ps
+    np.savez_compressed(OUT / "render_full.npz", **out)
+
+
 def load_golden_render(name):
     g = np.load(OUT / "render.npz")
     return g[f"{name}/film"]
@@ -484,7 +562,7 @@ def main():
     ref = probe.Probe("ref")
     only = set(sys.argv[1:])
     for fn in (gen_gltf, gen_rng_camera, gen_trace, gen_offset_surface, gen_bsdf, gen_env, gen_render,
-               gen_textured_surface, gen_film_encode):
+               gen_textured_surface, gen_film_encode, gen_render_batch, gen_full):
         if only and fn.__name__ not in only:
             continue
         print(fn.__name__)

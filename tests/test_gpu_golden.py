@@ -86,6 +86,7 @@ def test_render_matches_reference(gpu, case):
     if crop is not None:
         y0, y1, x0, x1 = crop
         film = film[y0:y1, x0:x1]
+    assert scene.bytes_sha256() == str(g[f"{name}/scene_sha"]), f"{name}: scene generator drifted"
     ref = g[f"{name}/film"]
     ok = within(film, ref)
     frac = ok.mean()
@@ -95,4 +96,40 @@ def test_render_matches_reference(gpu, case):
         assert frac >= PIXEL_FRACTION, f"{name}: only {frac * 100:.3f}% of pixels within {REL_TOL}"
     else:
         assert ok.all(), f"{name}: {film.reshape(-1, 3)[~ok.reshape(-1)]} vs {ref.reshape(-1, 3)[~ok.reshape(-1)]}"
+    assert res.boundary_retries == int(g[f"{name}/retries"])
+
+
+@pytest.mark.parametrize("name", ["c2_full", "c3_full"])
+def test_full_size_scene_matches_reference(gpu, name):
+    """Configs 2 and 3 at their real scene sizes (3,117,058 and 8,000,048 triangles): the
+    reference's render (reference src/render.cpp:352-456) at 160x90 / 128x72, 4 spp, and a
+    20k-ray traversal set (reference src/geometry.cpp:554-639).  At these sizes the traversal
+    meets deep stacks (LDS ring overflow into the private array), many-candidate exact passes
+    and leaf indices far above 2^20, which the small scenes never reach."""
+    import sys
+    from conftest import GOLDEN
+    from parity import assert_parity
+    sys.path.insert(0, str(GOLDEN))
+    import make_golden
+    g = load_golden("render_full.npz")
+    _, w, opt, rays = make_golden.full_case(name)
+    assert w.scene.bytes_sha256() == str(g[f"{name}/scene_sha"]), f"{name}: scene generator drifted"
+    assert make_golden.sha(rays) == str(g[f"{name}/rays_sha"]), f"{name}: ray generator drifted"
+    sid, info = gpu.upload_scene(w.scene)
+    eid, _ = gpu.upload_env(w.env)
+    try:
+        assert info.triangles == int(g[f"{name}/triangles"])
+        res = gpu.render(sid, eid, w.camera, opt)
+        hits = gpu.trace(sid, rays, any_hit=False)
+        anyh = gpu.trace(sid, rays, any_hit=True)
+    finally:
+        gpu.release_scene(sid)
+        gpu.release_env(eid)
+    np.testing.assert_array_equal(hits["hit"], g[f"{name}/hit"], err_msg=f"{name}: hit flags")
+    np.testing.assert_array_equal(hits["triangle"], g[f"{name}/triangle"], err_msg=f"{name}: triangle ids")
+    for k in ("t", "u", "v"):
+        np.testing.assert_array_equal(hits[k].view(np.uint64), g[f"{name}/{k}"].view(np.uint64),
+                                      err_msg=f"{name}: binary64 {k}")
+    np.testing.assert_array_equal(anyh["hit"], g[f"{name}/any"], err_msg=f"{name}: any-hit flags")
+    assert_parity(res.pixels, g[f"{name}/film"], name)
     assert res.boundary_retries == int(g[f"{name}/retries"])

@@ -141,3 +141,15 @@ def test_film_encode(oracle, name):
         return
     np.testing.assert_array_equal(oracle.film_png8(film, ex), g[f"{name}/png8"])
     assert oracle.film_hdr(film) == g[f"{name}/hdr"].tobytes()
+
+
+@pytest.mark.parametrize("name", [c[0] for c in make_golden.batch_cases()])
+def test_render_batch_requests(oracle, name):
+    """Requests of the batched/served mix generator: the oracle's films equal the
+    reference's bit for bit (the device forms are compared with the same films)."""
+    g = load_golden("render_batch.npz")
+    _, w, opt = next(c for c in make_golden.batch_cases() if c[0] == name)
+    assert w.scene.bytes_sha256() == str(g[f"{name}/scene_sha"]), "scene generator drifted"
+    film, stats = oracle.scene(w.scene).render(oracle.env(w.env), w.camera, opt)
+    np.testing.assert_array_equal(film.view(np.uint32), g[f"{name}/film"].view(np.uint32))
+    assert stats.boundary_retries == int(g[f"{name}/retries"])
