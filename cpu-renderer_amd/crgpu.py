@@ -76,6 +76,7 @@ def load_library(path: Optional[pathlib.Path] = None) -> C.CDLL:
         "crg_render": (i32, [vp, u64, u64, C.POINTER(abi.Camera), C.POINTER(abi.Options), C.POINTER(C.c_float),
                              C.POINTER(abi.Stats)]),
         "crg_trace_rays": (i32, [vp, u64, vp, u64, vp, i32]),
+        "crg_trace_rays_noalloc": (i32, [vp, u64, vp, u64, vp, i32]),
         "crg_render_batch": (i32, [vp, C.POINTER(abi.Request), C.c_uint32, C.POINTER(i32), C.POINTER(abi.Stats)]),
         "crg_submit": (i32, [vp, C.POINTER(abi.Request), C.POINTER(u64)]),
         "crg_wait": (i32, [vp, u64, C.c_double, C.POINTER(abi.Stats)]),
@@ -312,6 +313,15 @@ class Context:
         hits = np.zeros(len(rays), abi.HIT_DTYPE)
         self._check(self.lib.crg_trace_rays(self.handle, scene_id, rays.ctypes.data, len(rays), hits.ctypes.data,
                                             int(bool(any_hit))))
+        return hits
+
+    def trace_noalloc(self, scene_id: int, rays: np.ndarray, any_hit: bool = False) -> np.ndarray:
+        """crg_trace_rays_noalloc: the same queries through the library's fixed
+        per-device scratch (at most 65536 rays per call, no device allocation)."""
+        rays = np.ascontiguousarray(rays, abi.RAY_DTYPE)
+        hits = np.zeros(len(rays), abi.HIT_DTYPE)
+        self._check(self.lib.crg_trace_rays_noalloc(self.handle, scene_id, rays.ctypes.data, len(rays),
+                                                    hits.ctypes.data, int(bool(any_hit))))
         return hits
 
 

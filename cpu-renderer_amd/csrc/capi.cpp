@@ -48,6 +48,7 @@ struct crg_ctx {
     std::map<uint64_t, std::shared_ptr<EnvRes>> envs;
     std::unique_ptr<Engine> engine;
     DevBuf ctl, rays, hits;
+    void* scratch = nullptr;       // this device's fixed scratch (crg_trace_rays_noalloc)
     // film encode
     DevBuf film, ldr, planes, streams, lens, thresholds, first_bad;
     float cached_exposure = std::numeric_limits<float>::quiet_NaN();
@@ -626,6 +627,7 @@ int crg_context_create(int device, crg_ctx** out) {
     try {
         check(hipSetDevice(device), "hipSetDevice");
         ensure_turn_table(device);
+        check(noalloc_scratch(device, &ctx->scratch), "scratch symbol address");
         check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
         ctx->engine = std::make_unique<Engine>(device, unsigned(prop.multiProcessorCount));
     } catch (const std::exception&) {
@@ -839,6 +841,36 @@ int crg_trace_rays(crg_ctx* ctx, uint64_t scene_id, const crg_ray* rays, uint64_
                                 any, ctx->ctl.as<Control>(), st),
               "trace launch");
         if (count) check(hipMemcpyAsync(hits, ctx->hits.p, count * sizeof(crg_hit), hipMemcpyDeviceToHost, st), "copy");
+        check(hipStreamSynchronize(st), "trace");
+        for (uint64_t i = 0; i < count; ++i) {
+            if (hits[i].hit == 2u) throw std::logic_error("BVH traversal exceeded its bounded stack");
+            if (hits[i].hit == 3u) throw std::runtime_error("Non-finite texture coordinates or transform");
+        }
+    });
+}
+
+int crg_trace_rays_noalloc(crg_ctx* ctx, uint64_t scene_id, const crg_ray* rays, uint64_t count, crg_hit* hits,
+                           int any) {
+    return guarded(ctx, [&] {
+        if (count > kNoAllocMaxRays)
+            throw std::invalid_argument("crg_trace_rays_noalloc: " + std::to_string(count) + " rays exceed the " +
+                                        std::to_string(kNoAllocMaxRays) + "-ray scratch; split the batch");
+        if (count && (!rays || !hits)) throw std::invalid_argument("null ray or hit array");
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        auto it = ctx->scenes.find(scene_id);
+        if (it == ctx->scenes.end()) throw std::out_of_range("unknown scene handle");
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        hipStream_t st = ctx->stream;
+        uint8_t* base = static_cast<uint8_t*>(ctx->scratch);
+        Control* ctl = reinterpret_cast<Control*>(base);
+        RayIn* drays = reinterpret_cast<RayIn*>(base + kNoAllocCtlBytes);
+        HitOut* dhits = reinterpret_cast<HitOut*>(base + kNoAllocCtlBytes + kNoAllocMaxRays * sizeof(RayIn));
+        // The control block's counters start at zero on every call (the array
+        // keeps the previous call's contents).
+        check(launch_clear_words(reinterpret_cast<uint32_t*>(ctl), uint32_t(sizeof(Control) / 4), st), "clear");
+        if (count) check(hipMemcpyAsync(drays, rays, count * sizeof(crg_ray), hipMemcpyHostToDevice, st), "copy");
+        check(launch_trace_rays(it->second->desc.as<DevScene>(), drays, count, dhits, any, ctl, st), "trace launch");
+        if (count) check(hipMemcpyAsync(hits, dhits, count * sizeof(crg_hit), hipMemcpyDeviceToHost, st), "copy");
         check(hipStreamSynchronize(st), "trace");
         for (uint64_t i = 0; i < count; ++i) {
             if (hits[i].hit == 2u) throw std::logic_error("BVH traversal exceeded its bounded stack");

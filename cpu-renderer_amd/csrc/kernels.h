@@ -22,6 +22,18 @@ static_assert(sizeof(HitOut) == 32, "crg_hit layout");
 
 hipError_t launch_trace_rays(const DevScene* scene, const RayIn* rays, uint64_t n, HitOut* hits, int any,
                              Control* ctl, hipStream_t st);
+
+// Fixed per-device scratch of the allocation-free ray query: the control block
+// (rounded up to 256 bytes), then the rays, then the hits of at most
+// kNoAllocMaxRays queries (5.3 MB in all).
+constexpr uint64_t kNoAllocMaxRays = uint64_t(1) << 16;
+constexpr uint64_t kNoAllocCtlBytes = (sizeof(Control) + 255) / 256 * 256;
+constexpr uint64_t kNoAllocScratchBytes = kNoAllocCtlBytes + kNoAllocMaxRays * (sizeof(RayIn) + sizeof(HitOut));
+// Device address of that scratch on `device`, which must be the current
+// device on the first call for it (hipGetSymbolAddress once per device).
+hipError_t noalloc_scratch(int device, void** out);
+// Zeroes n 32-bit words with a kernel (no memset node).
+hipError_t launch_clear_words(uint32_t* p, uint32_t n, hipStream_t st);
 // The kernels of one wavefront iteration, in launch order.
 enum Stage {
     kStageExtend = 0,

@@ -14,6 +14,10 @@
 // reference's per-pixel double accumulation (reference src/render.cpp:384-415).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <map>
+#include <mutex>
+
 #include "kernels.h"
 #include "film.h"
 #include "path.h"
@@ -789,6 +793,42 @@ hipError_t launch_trace_rays(const DevScene* scene, const RayIn* rays, uint64_t 
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_trace_rays, dim3(blocks_for(n, 256)), dim3(256), 0, st, scene, rays, n, hits, any, ctl);
     return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ allocation-free ray queries
+// Fixed device scratch of crg_trace_rays_noalloc: the control block, then the
+// staged rays and hits of at most kNoAllocMaxRays queries.  Every device the
+// library runs on holds its own copy for the life of the process; the host
+// reaches it only through the address hipGetSymbolAddress returns for that
+// device (noalloc_scratch), never through the symbol's name.
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+__global__ void k_clear_words(uint32_t* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+hipError_t launch_clear_words(uint32_t* p, uint32_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_clear_words, dim3(std::min<uint32_t>(64, (n + 255) / 256)), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+hipError_t noalloc_scratch(int device, void** out) {
+    static std::mutex mu;
+    static std::map<int, void*> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(device);
+    if (it == addr.end()) {
+        int current = -1;
+        hipError_t e = hipGetDevice(&current);
+        if (e != hipSuccess) return e;
+        if (current != device) return hipErrorInvalidDevice;  // the caller makes `device` current first
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        it = addr.emplace(device, p).first;
+    }
+    *out = it->second;
+    return hipSuccess;
 }
 
 hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const DevJob* jobs, const Queues& Q,

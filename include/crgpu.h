@@ -233,6 +233,18 @@ int crg_render(crg_ctx* ctx, uint64_t scene_id, uint64_t env_id, const crg_camer
 int crg_trace_rays(crg_ctx* ctx, uint64_t scene_id, const crg_ray* rays, uint64_t count,
                    crg_hit* hits, int any);
 
+/* crg_trace_rays without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * staging (control block, rays, hits) lives in a fixed 5.3 MB scratch array
+ * that each device holds for the life of the process; at most
+ * CRG_NOALLOC_MAX_RAYS rays per call (more: CRG_EINVAL, nothing launched).
+ * Results equal crg_trace_rays's.  Calls on one context are serialized;
+ * calls from two contexts on the same device share the array and race. */
+#define CRG_NOALLOC_MAX_RAYS 65536u
+int crg_trace_rays_noalloc(crg_ctx* ctx, uint64_t scene_id, const crg_ray* rays, uint64_t count,
+                           crg_hit* hits, int any);
+
 /* ---- batched and served forms ------------------------------------------
  * Requests over resident scenes share one wavefront path pool: their camera
  * samples are work items of one global stream, so bounces of renders of
