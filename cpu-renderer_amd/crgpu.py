@@ -81,6 +81,7 @@ def load_library(path: Optional[pathlib.Path] = None) -> C.CDLL:
         "crg_submit": (i32, [vp, C.POINTER(abi.Request), C.POINTER(u64)]),
         "crg_wait": (i32, [vp, u64, C.c_double, C.POINTER(abi.Stats)]),
         "crg_set_memory_budget": (i32, [vp, u64]),
+        "crg_request_progress": (i32, [vp, u64, C.POINTER(u64), C.POINTER(u64)]),
         "crg_engine_info_get": (i32, [vp, C.POINTER(abi.EngineInfo)]),
         "crg_get_tuning": (i32, [vp, C.POINTER(abi.Tuning)]),
         "crg_set_tuning": (i32, [vp, C.POINTER(abi.Tuning)]),
@@ -232,6 +233,12 @@ class Context:
             return None
         self._check(rc)
         return RenderResult.from_stats(film, stats)
+
+    def progress(self, ticket: int):
+        """(camera samples finished, total) of a submitted request."""
+        done, total = C.c_uint64(), C.c_uint64()
+        self._check(self.lib.crg_request_progress(self.handle, ticket, C.byref(done), C.byref(total)))
+        return done.value, total.value
 
     def set_memory_budget(self, nbytes: int):
         self._check(self.lib.crg_set_memory_budget(self.handle, int(nbytes)))

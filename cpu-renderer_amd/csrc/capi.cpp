@@ -119,8 +119,8 @@ int guarded_msg(char* message, size_t capacity, F&& f) {
 // Request from C arguments: option/camera validation and the light table,
 // on the calling thread (reference src/render.cpp:356-372).
 Request make_request(crg_ctx* ctx, const crg_request& r) {
+    validate_options(r.options);  // the reference's messages first, as render() validates before allocating
     if (!r.film_rgb) throw std::invalid_argument("null film buffer");
-    validate_options(r.options);
     Request q;
     {
         std::lock_guard<std::mutex> lock(ctx->mu);
@@ -1018,7 +1018,9 @@ int crg_render(crg_ctx* ctx, uint64_t scene_id, uint64_t env_id, const crg_camer
                float* film_rgb, crg_stats* stats) {
     int rc = CRG_OK;
     const int g = guarded(ctx, [&] {
-        if (!camera || !options || !film_rgb) throw std::invalid_argument("null render argument");
+        if (!camera || !options) throw std::invalid_argument("null render argument");
+        validate_options(*options);
+        if (!film_rgb) throw std::invalid_argument("null film buffer");
         crg_request r{};
         r.scene = scene_id;
         r.env = env_id;
@@ -1083,6 +1085,13 @@ int crg_wait(crg_ctx* ctx, uint64_t ticket, double timeout_seconds, crg_stats* s
     int rc = CRG_OK;
     const int g = guarded(ctx, [&] { rc = status_of_wait(*ctx->engine, ticket, timeout_seconds, stats); });
     return g != CRG_OK ? g : rc;
+}
+
+int crg_request_progress(crg_ctx* ctx, uint64_t ticket, uint64_t* complete, uint64_t* total) {
+    return guarded(ctx, [&] {
+        if (!complete || !total) throw std::invalid_argument("null progress output");
+        ctx->engine->progress(ticket, complete, total);
+    });
 }
 
 int crg_set_memory_budget(crg_ctx* ctx, uint64_t bytes) {

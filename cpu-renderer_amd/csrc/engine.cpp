@@ -32,6 +32,7 @@ struct Job {
     uint64_t items = 0, npix = 0, bytes = 0, item_base = 0;
     uint32_t seq = 0, slot = 0;
     uint64_t first_step = 0;
+    uint64_t retired = 0;  // camera samples finished, as of the last step read back
     void* d_samples = nullptr;
     float* d_film = nullptr;
     DevLightEntry* d_lights = nullptr;
@@ -176,6 +177,15 @@ int Engine::wait(uint64_t ticket, double timeout_seconds, crg_stats* stats, std:
     if (stats) *stats = j->stats;
     if (err) *err = j->error;
     return j->status;
+}
+
+void Engine::progress(uint64_t ticket, uint64_t* complete, uint64_t* total) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = tickets_.find(ticket);
+    if (it == tickets_.end()) throw std::out_of_range("unknown request ticket");
+    const Job& j = *it->second;
+    *total = j.items;
+    *complete = j.state == Job::kFinishing || j.state == Job::kDone ? j.items : j.retired;
 }
 
 void Engine::set_budget(uint64_t bytes) {
@@ -435,6 +445,7 @@ void Engine::process_locked(const Readback& rb) {
         Job& j = *jp;
         if (rb.step >= j.first_step) {
             progress += n.retired[j.slot];
+            j.retired = std::min<uint64_t>(n.retired[j.slot], j.items);
             if (n.retired[j.slot] >= j.items) {
                 j.stats.iterations = int32_t((rb.step + 1 - j.first_step) * tune_.iterations_per_graph);
                 complete_locked(jp, n);

@@ -95,6 +95,9 @@ public:
     uint64_t submit(Request r);
     // CRG_EBUSY on timeout, else the request's status (error text in *err).
     int wait(uint64_t ticket, double timeout_seconds, crg_stats* stats, std::string* err);
+    // Camera samples of a submitted request finished so far (as of the last
+    // step read back), and its total.
+    void progress(uint64_t ticket, uint64_t* complete, uint64_t* total);
 
     void set_budget(uint64_t bytes);
     void add_resident(int64_t delta_bytes);

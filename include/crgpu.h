@@ -280,6 +280,11 @@ int crg_submit(crg_ctx* ctx, const crg_request* request, uint64_t* ticket);
  * is written on CRG_OK); the ticket is consumed unless CRG_EBUSY. */
 int crg_wait(crg_ctx* ctx, uint64_t ticket, double timeout_seconds, crg_stats* stats);
 
+/* Camera samples of a submitted (not yet waited-for) request finished so far
+ * and its total, as of the engine's last step: the reference's Progress
+ * callback reports the same pair (reference src/render.cpp:419-427). */
+int crg_request_progress(crg_ctx* ctx, uint64_t ticket, uint64_t* complete, uint64_t* total);
+
 /* Device memory budget for admission (default 200 GiB, capped by the device). */
 int crg_set_memory_budget(crg_ctx* ctx, uint64_t bytes);
 

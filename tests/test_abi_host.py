@@ -142,7 +142,8 @@ def test_degenerate_triangles_are_removed(emu, oracle):
 
 
 def test_reference_side_adapter_compiles_against_reference_headers(tmp_path):
-    """include/crgpu_cr.hpp instantiated with the reference's cr:: types links against libcrgpu.so."""
+    """include/crgpu_render.hpp (the exact-signature drop-in) instantiated with the reference's
+    cr:: types compiles and links against libcrgpu.so; the device render is tests/test_gpu_adapter.py."""
     import pathlib
     ref = pathlib.Path("/root/reference/src")
     if not (ref / "render.h").exists():
@@ -153,5 +154,5 @@ def test_reference_side_adapter_compiles_against_reference_headers(tmp_path):
     subprocess.run(["g++", "-std=c++20", f"-I{ref}", f"-I{ROOT / 'include'}", "-o", str(exe),
                     str(ROOT / "tests" / "csrc" / "adapter_check.cpp"), *srcs, f"-L{lib}", "-lcrgpu",
                     f"-Wl,-rpath,{lib}", "-pthread"], check=True)
-    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
-    assert "rendered 16 px" in out or "no usable gfx950 device" in out
+    out = subprocess.run([str(exe), "--compile-only"], capture_output=True, text=True, check=True).stdout
+    assert "instantiated" in out
