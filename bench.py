@@ -475,14 +475,20 @@ def main():
     samples = w.camera_samples
     value = samples / sec_per_step / 1e6
 
-    # Profiled render: per-kernel device time from HIP events on the library stream.
+    # Work per query from a render with the counting traversal kernels (same seed: the
+    # same queries), then per-kernel device time of the timed kernels (no counters) from
+    # HIP events on the library stream, kernels launched one by one.
+    ctx.set_work_counters(True)
+    cr_ = ctx.render(sid, eid, w.camera, opt)
+    ctx.set_work_counters(False)
     ctx.set_profiling(True)
     pr = ctx.render(sid, eid, w.camera, opt)
     kt = ctx.kernel_times()
     ctx.set_profiling(False)
+    assert (cr_.extend_rays, cr_.shadow_rays) == (pr.extend_rays, pr.shadow_rays)
     ext_launches, ext_ms = kt["k_extend"]
-    n_node = pr.extend_nodes / max(1, pr.extend_rays)
-    n_tri = pr.extend_tris / max(1, pr.extend_rays)
+    n_node = cr_.extend_nodes / max(1, cr_.extend_rays)
+    n_tri = cr_.extend_tris / max(1, cr_.extend_rays)
     bytes_per_ray = 48 + 128 * n_node + 48 * n_tri
     achieved = bytes_per_ray * pr.extend_rays / (ext_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
@@ -569,6 +575,9 @@ def traffic_probe(name: str):
     ctx = crgpu.Context(0)
     sid, _ = ctx.upload_scene(w.scene)
     eid, _ = ctx.upload_env(w.env)
+    ctx.set_work_counters(True)  # work per query (its own render; the counted kernels are not the timed ones)
+    c = ctx.render(sid, eid, w.camera, w.options())
+    ctx.set_work_counters(False)
     ctx.set_profiling(True)  # individual launches (no graph) so every dispatch is counted
     r = ctx.render(sid, eid, w.camera, w.options())
     ctx.set_profiling(False)
@@ -579,8 +588,9 @@ def traffic_probe(name: str):
     torch.cuda.synchronize()
     b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"workload": w.name, "extend_rays": r.extend_rays, "extend_nodes": r.extend_nodes,
-                      "extend_tris": r.extend_tris, "k_extend_launches": kt["k_extend"][0],
+    print(json.dumps({"workload": w.name, "extend_rays": r.extend_rays, "extend_nodes": c.extend_nodes,
+                      "extend_tris": c.extend_tris, "shadow_rays": r.shadow_rays, "shadow_nodes": c.shadow_nodes,
+                      "shadow_tris": c.shadow_tris, "k_extend_launches": kt["k_extend"][0],
                       "copy_bytes": 1 << 30}), flush=True)
     ctx.close()
 

@@ -87,6 +87,7 @@ def load_library(path: Optional[pathlib.Path] = None) -> C.CDLL:
         "crg_set_tuning": (i32, [vp, C.POINTER(abi.Tuning)]),
         "crg_autotune": (i32, [vp, C.POINTER(abi.Request), C.POINTER(abi.Tuning), C.POINTER(C.c_double)]),
         "crg_set_profiling": (i32, [vp, i32]),
+        "crg_set_work_counters": (i32, [vp, i32]),
         "crg_kernel_times": (i32, [vp, C.POINTER(abi.KernelTime), i32, C.POINTER(i32)]),
         "crg_scene_save": (i32, [vp, u64, C.c_char_p]),
         "crg_scene_load": (i32, [vp, C.c_char_p, C.POINTER(u64), C.POINTER(abi.SceneInfo)]),
@@ -265,6 +266,10 @@ class Context:
 
     def set_profiling(self, enable: bool):
         self._check(self.lib.crg_set_profiling(self.handle, int(bool(enable))))
+
+    def set_work_counters(self, enable: bool):
+        """Per-query work counters (node visits, triangle tests) in the stats of later renders."""
+        self._check(self.lib.crg_set_work_counters(self.handle, int(bool(enable))))
 
     def kernel_times(self) -> dict:
         arr = (abi.KernelTime * 16)()

@@ -163,11 +163,11 @@ Section section(const std::vector<T>& v) {
 // The device-format arrays of a scene, wherever they are held on the host.
 struct SceneSections {
     uint64_t binary_nodes = 0;
-    Section nodes, prims, prim_ng, aux, materials, uvs, tangents, texels, images, textures;
+    Section nodes4, nodes8, prims, prim_ng, aux, materials, uvs, tangents, texels, images, textures;
 };
 
 SceneSections sections_of(const SceneArrays& a) {
-    return {a.bvh.binary_nodes, section(a.bvh.nodes), section(a.prims),  section(a.prim_ng),
+    return {a.bvh.binary_nodes, section(a.bvh.q4), section(a.bvh.q8), section(a.prims),  section(a.prim_ng),
             section(a.aux),     section(a.materials), section(a.uvs),    section(a.tangents),
             section(a.texels),  section(a.images),    section(a.textures)};
 }
@@ -207,7 +207,8 @@ std::shared_ptr<SceneRes> make_resident(crg_ctx* ctx, const SceneSections& a, st
     });
     res->masks = masks;
     hipStream_t st = ctx->stream;
-    upload_section(res->nodes, a.nodes, st);
+    upload_section(res->nodes4, a.nodes4, st);
+    upload_section(res->nodes8, a.nodes8, st);
     upload_section(res->prims, a.prims, st);
     upload_section(res->prim_ng, a.prim_ng, st);
     upload_section(res->aux, a.aux, st);
@@ -219,7 +220,8 @@ std::shared_ptr<SceneRes> make_resident(crg_ctx* ctx, const SceneSections& a, st
     upload_section(res->texels, a.texels, st);
     upload_section(res->images, a.images, st);
     upload_section(res->textures, a.textures, st);
-    res->host = DevScene{res->nodes.as<WNode>(),
+    res->host = DevScene{res->nodes4.as<QNode4>(),
+                         res->nodes8.as<QNode8>(),
                          res->prims.as<PrimRec>(),
                          res->prim_ng.as<float4>(),
                          res->aux.as<PrimAux>(),
@@ -231,14 +233,14 @@ std::shared_ptr<SceneRes> make_resident(crg_ctx* ctx, const SceneSections& a, st
                          res->texels.as<float4>(),
                          res->images.as<DevImage>(),
                          res->textures.as<DevTexture>(),
-                         uint32_t(a.nodes.count),
+                         uint32_t(a.nodes4.count),
                          uint32_t(a.prims.count),
                          uint32_t(res->lights.punctual.size()),
-                         0};
+                         uint32_t(a.nodes8.count)};
     res->desc.alloc(sizeof(DevScene));
     check(hipMemcpyAsync(res->desc.p, &res->host, sizeof(DevScene), hipMemcpyHostToDevice, st), "upload");
     check(hipStreamSynchronize(st), "scene upload");
-    res->bytes = res->nodes.bytes + res->prims.bytes + res->prim_ng.bytes + res->aux.bytes + res->materials.bytes +
+    res->bytes = res->nodes4.bytes + res->nodes8.bytes + res->prims.bytes + res->prim_ng.bytes + res->aux.bytes + res->materials.bytes +
                  res->light_rank.bytes + res->punctual.bytes + res->uvs.bytes + res->tangents.bytes +
                  res->texels.bytes + res->images.bytes + res->textures.bytes + res->desc.bytes;
     return res;
@@ -248,7 +250,7 @@ crg_scene_info scene_info(const SceneRes& r) {
     crg_scene_info info{};
     info.triangles = r.triangles;
     info.bvh_nodes = r.binary_nodes;
-    info.wide_nodes = r.host.node_count;
+    info.wide_nodes = r.host.node_count4;
     info.device_bytes = r.bytes;
     return info;
 }
@@ -381,8 +383,9 @@ FileBytes read_file(const char* path, const std::string& label) {
     return f;
 }
 
-const uint32_t kRecordSizes[8] = {sizeof(WNode), sizeof(PrimRec), sizeof(PrimAux), sizeof(DevMaterial),
-                                  sizeof(DevPunctual), sizeof(DevImage), sizeof(DevTexture), sizeof(float4)};
+const uint32_t kRecordSizes[9] = {sizeof(QNode4), sizeof(QNode8),     sizeof(PrimRec),
+                                  sizeof(PrimAux), sizeof(DevMaterial), sizeof(DevPunctual),
+                                  sizeof(DevImage), sizeof(DevTexture), sizeof(float4)};
 
 template <class T>
 std::vector<T> download(const DevBuf& b, uint64_t count, hipStream_t st) {
@@ -393,7 +396,8 @@ std::vector<T> download(const DevBuf& b, uint64_t count, hipStream_t st) {
 
 // The device-format arrays of a scene downloaded from the device.
 struct CacheContent {
-    std::vector<WNode> nodes;
+    std::vector<QNode4> nodes4;
+    std::vector<QNode8> nodes8;
     std::vector<PrimRec> prims;
     std::vector<float4> prim_ng;
     std::vector<PrimAux> aux;
@@ -405,7 +409,8 @@ struct CacheContent {
 
 SceneSections sections_of(const CacheContent& c) {
     return {0,
-            section(c.nodes),    section(c.prims),  section(c.prim_ng), section(c.aux),    section(c.materials),
+            section(c.nodes4),   section(c.nodes8), section(c.prims),  section(c.prim_ng), section(c.aux),
+            section(c.materials),
             section(c.uvs),      section(c.tangents), section(c.texels), section(c.images), section(c.textures)};
 }
 
@@ -423,14 +428,14 @@ void write_cache_file(const char* path, uint64_t triangles, uint64_t binary_node
                       const std::vector<uint32_t>& prim_to_tri, const std::vector<uint32_t>& tri_to_prim,
                       const SceneLights& lights) {
     CacheWriter w;
-    w.out.reserve(4096 + sec.nodes.bytes + sec.prims.bytes + sec.prim_ng.bytes + sec.aux.bytes + sec.materials.bytes +
+    w.out.reserve(4096 + sec.nodes4.bytes + sec.nodes8.bytes + sec.prims.bytes + sec.prim_ng.bytes + sec.aux.bytes + sec.materials.bytes +
                   sec.uvs.bytes + sec.tangents.bytes + sec.texels.bytes + sec.images.bytes + sec.textures.bytes +
                   8 * (prim_to_tri.size() + tri_to_prim.size()));
     w.raw(kCacheMagic, 8);
     w.raw(kRecordSizes, sizeof(kRecordSizes));
     w.pod(triangles);
     w.pod(binary_nodes);
-    for (const Section* s : {&sec.nodes, &sec.prims, &sec.prim_ng, &sec.aux, &sec.materials, &sec.uvs, &sec.tangents,
+    for (const Section* s : {&sec.nodes4, &sec.nodes8, &sec.prims, &sec.prim_ng, &sec.aux, &sec.materials, &sec.uvs, &sec.tangents,
                              &sec.texels, &sec.images, &sec.textures})
         put_section(w, *s);
     w.vec(prim_to_tri);
@@ -465,7 +470,7 @@ ParsedCache parse_cache(const FileBytes& file, const std::string& label) {
     std::memcpy(&stored, bytes + file.n - 8, 8);
     if (stored != cache_checksum(bytes, file.n - 8)) throw std::runtime_error(label + ": checksum mismatch");
     CacheReader rd{bytes, file.n - 8, 8, label};
-    uint32_t sizes[8];
+    uint32_t sizes[9];
     rd.raw(sizes, sizeof(sizes));
     if (std::memcmp(sizes, kRecordSizes, sizeof(sizes)) != 0)
         throw std::runtime_error(label + ": written by a build with other record layouts");
@@ -473,7 +478,8 @@ ParsedCache parse_cache(const FileBytes& file, const std::string& label) {
     pc.triangles = rd.pod<uint64_t>();
     SceneSections& sec = pc.sec;
     sec.binary_nodes = rd.pod<uint64_t>();
-    sec.nodes = rd.section<WNode>();
+    sec.nodes4 = rd.section<QNode4>();
+    sec.nodes8 = rd.section<QNode8>();
     sec.prims = rd.section<PrimRec>();
     sec.prim_ng = rd.section<float4>();
     sec.aux = rd.section<PrimAux>();
@@ -514,24 +520,44 @@ T record(const Section& s, size_t i) {
 //   punctual light types.
 void validate_cache(const ParsedCache& pc, const std::string& label) {
     const SceneSections& sec = pc.sec;
-    const uint64_t n = pc.triangles, np = sec.prims.count, nn = sec.nodes.count;
+    const uint64_t n = pc.triangles, np = sec.prims.count, nn = sec.nodes4.count, nn8 = sec.nodes8.count;
     auto fail = [&](const std::string& what) { throw std::runtime_error(label + ": " + what); };
     if (np < n || sec.prim_ng.count != np || sec.aux.count != np || pc.order.size() != np || pc.tri_to_prim.size() != n ||
         (sec.uvs.count && sec.uvs.count != 3 * np) || (sec.tangents.count && sec.tangents.count != 3 * np) ||
-        (n && !nn) || pc.lights.emissive.size() != pc.lights.emissive_weight.size() ||
+        (n && (!nn || !nn8)) || (!nn != !nn8) || pc.lights.emissive.size() != pc.lights.emissive_weight.size() ||
         pc.lights.punctual.size() != pc.lights.punctual_weight.size())
         fail("inconsistent section sizes");
-    if (np > kLeafMaxFirst + 1 || nn >= kEmptyChild || n >= kNoTriangle) fail("more records than the device format holds");
+    if (np > kLeafMaxFirst + 1 || nn >= kEmptyChild || nn8 >= kEmptyChild || n >= kNoTriangle)
+        fail("more records than the device format holds");
     std::atomic<bool> bad_node{false}, bad_prim{false}, bad_map{false};
-    parallel_chunks(nn, size_t(1) << 16, [&](size_t b, size_t e, size_t) {
-        for (size_t i = b; i < e && !bad_node; ++i) {
-            const WNode w = record<WNode>(sec.nodes, i);
-            for (uint32_t c : w.child) {
-                if (c == kEmptyChild) continue;
-                if (ref_is_leaf(c) ? uint64_t(leaf_first(c)) + leaf_count(c) > np : (c <= i || c >= nn)) bad_node = true;
+    auto check_tree = [&](const Section& nodes, auto tag) {
+        using Node = decltype(tag);
+        constexpr uint32_t width = sizeof(Node{}.child) / sizeof(uint32_t);
+        const uint64_t count = nodes.count;
+        parallel_chunks(count, size_t(1) << 16, [&](size_t b, size_t e, size_t) {
+            for (size_t i = b; i < e && !bad_node; ++i) {
+                const Node w = record<Node>(nodes, i);
+                const uint32_t kids = w.exps >> 24;
+                for (uint32_t e8 = 0; e8 < 3; ++e8) {
+                    const uint32_t ex = (w.exps >> (8 * e8)) & 255u;
+                    if (ex < 1 || ex > 254) bad_node = true;
+                }
+                if (kids > width) bad_node = true;
+                for (uint32_t k = 0; k < width; ++k) {
+                    const uint32_t c = w.child[k];
+                    if (c == kEmptyChild) {
+                        if (k < kids) bad_node = true;
+                        continue;
+                    }
+                    if (k >= kids ||
+                        (ref_is_leaf(c) ? uint64_t(leaf_first(c)) + leaf_count(c) > np : (c <= i || c >= count)))
+                        bad_node = true;
+                }
             }
-        }
-    });
+        });
+    };
+    check_tree(sec.nodes4, QNode4{});
+    check_tree(sec.nodes8, QNode8{});
     if (bad_node) fail("BVH node refers outside the node or primitive arrays");
     const uint32_t nm = uint32_t(sec.materials.count);
     constexpr uint32_t known_flags = kPrimCull | kPrimNeverHit | kPrimMaskTexture | (3u << kPrimClassShift);
@@ -691,7 +717,8 @@ int crg_scene_save(crg_ctx* ctx, uint64_t scene_id, const char* path) {
             check(hipSetDevice(ctx->device), "hipSetDevice");
             hipStream_t st = ctx->stream;
             const uint64_t np = r->host.prim_count;
-            c.nodes = download<WNode>(r->nodes, r->host.node_count, st);
+            c.nodes4 = download<QNode4>(r->nodes4, r->host.node_count4, st);
+            c.nodes8 = download<QNode8>(r->nodes8, r->host.node_count8, st);
             c.prims = download<PrimRec>(r->prims, np, st);
             c.prim_ng = download<float4>(r->prim_ng, np, st);
             c.aux = download<PrimAux>(r->aux, np, st);
@@ -747,7 +774,7 @@ int crg_scene_write_cache(const crg_scene_desc* desc, const char* path, crg_scen
             *info = crg_scene_info{};
             info->triangles = hs.tris.size();
             info->bvh_nodes = arr.bvh.binary_nodes;
-            info->wide_nodes = arr.bvh.nodes.size();
+            info->wide_nodes = arr.bvh.q4.size();
             info->build_seconds = std::chrono::duration<double>(t1 - t0).count();
         }
     });
@@ -838,7 +865,7 @@ int crg_trace_rays(crg_ctx* ctx, uint64_t scene_id, const crg_ray* rays, uint64_
         check(hipMemsetAsync(ctx->ctl.p, 0, sizeof(Control), st), "memset");
         if (count) check(hipMemcpyAsync(ctx->rays.p, rays, count * sizeof(crg_ray), hipMemcpyHostToDevice, st), "copy");
         check(launch_trace_rays(it->second->desc.as<DevScene>(), ctx->rays.as<RayIn>(), count, ctx->hits.as<HitOut>(),
-                                any, ctx->ctl.as<Control>(), st),
+                                any, ctx->ctl.as<Control>(), int(ctx->engine->tuning().bvh_width), st),
               "trace launch");
         if (count) check(hipMemcpyAsync(hits, ctx->hits.p, count * sizeof(crg_hit), hipMemcpyDeviceToHost, st), "copy");
         check(hipStreamSynchronize(st), "trace");
@@ -869,7 +896,9 @@ int crg_trace_rays_noalloc(crg_ctx* ctx, uint64_t scene_id, const crg_ray* rays,
         // keeps the previous call's contents).
         check(launch_clear_words(reinterpret_cast<uint32_t*>(ctl), uint32_t(sizeof(Control) / 4), st), "clear");
         if (count) check(hipMemcpyAsync(drays, rays, count * sizeof(crg_ray), hipMemcpyHostToDevice, st), "copy");
-        check(launch_trace_rays(it->second->desc.as<DevScene>(), drays, count, dhits, any, ctl, st), "trace launch");
+        check(launch_trace_rays(it->second->desc.as<DevScene>(), drays, count, dhits, any, ctl,
+                                int(ctx->engine->tuning().bvh_width), st),
+              "trace launch");
         if (count) check(hipMemcpyAsync(hits, dhits, count * sizeof(crg_hit), hipMemcpyDeviceToHost, st), "copy");
         check(hipStreamSynchronize(st), "trace");
         for (uint64_t i = 0; i < count; ++i) {
@@ -1152,6 +1181,7 @@ int crg_autotune(crg_ctx* ctx, const crg_request* sample, crg_tuning* out, doubl
                 }
             }
         };
+        try_field(&crg_tuning::bvh_width, {4, 8});
         try_field(&crg_tuning::pool_paths, {1u << 20, 1u << 21, 1u << 22, 1u << 23});
         try_field(&crg_tuning::groups_per_cu, {4, 8, 16});
         try_field(&crg_tuning::workgroup, {64, 128, 256});
@@ -1160,6 +1190,10 @@ int crg_autotune(crg_ctx* ctx, const crg_request* sample, crg_tuning* out, doubl
         if (out) *out = best;
         if (samples_per_second) *samples_per_second = double(items) / best_s;
     });
+}
+
+int crg_set_work_counters(crg_ctx* ctx, int enable) {
+    return guarded(ctx, [&] { ctx->engine->set_counting(enable != 0); });
 }
 
 int crg_set_profiling(crg_ctx* ctx, int enable) {

@@ -224,12 +224,21 @@ void Engine::set_tuning(const crg_tuning& t) {
     if (t.groups_per_cu < 1 || t.groups_per_cu > 64) throw std::invalid_argument("groups_per_cu out of range");
     if (t.iterations_per_graph < 2 || t.iterations_per_graph > 64 || (t.iterations_per_graph & 1))
         throw std::invalid_argument("iterations_per_graph must be even and in [2, 64]");
-    if (t.bvh_width != 4) throw std::invalid_argument("bvh_width: only 4-wide nodes are built");
+    if (t.bvh_width != 4 && t.bvh_width != 8) throw std::invalid_argument("bvh_width must be 4 or 8");
     std::unique_lock<std::mutex> lk(mu_);
     cv_done_.wait(lk, [&] { return pending_.empty() && running_.empty() && finishing_.empty(); });
     if (std::memcmp(&t, &tune_, sizeof(t)) != 0) {
         tune_ = t;
         dirty_ = true;
+    }
+}
+
+void Engine::set_counting(bool on) {
+    std::unique_lock<std::mutex> lk(mu_);
+    cv_done_.wait(lk, [&] { return pending_.empty() && running_.empty() && finishing_.empty(); });
+    if (on != count_) {
+        count_ = on;
+        dirty_ = true;  // the graph holds the other traversal variants
     }
 }
 
@@ -302,7 +311,7 @@ void Engine::configure() {
     err_read_ = 0;
     pool_bytes_ = pool_bytes_for(n);
 
-    const LaunchShape shape{cus_ * tune_.groups_per_cu, tune_.workgroup};
+    const LaunchShape shape{cus_ * tune_.groups_per_cu, tune_.workgroup, count_, int(tune_.bvh_width)};
     check(hipStreamSynchronize(st_), "configure");
     check(hipStreamBeginCapture(st_, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
     hipError_t le = hipSuccess;
@@ -385,7 +394,7 @@ void Engine::admit_locked() {
         update_masks_locked();
         // Idle slots start the new items right away (same reset + refill the
         // graph ends with; both are idempotent at a step boundary).
-        const LaunchShape shape{cus_ * tune_.groups_per_cu, tune_.workgroup};
+        const LaunchShape shape{cus_ * tune_.groups_per_cu, tune_.workgroup, count_, int(tune_.bvh_width)};
         for (Stage s : {kStageReset, kStageRefill})
             check(launch_stage(s, P_, S_, jobs_.as<DevJob>(), Q_, 1, ctl_.as<Control>(), cnt_.as<JobCounters>(), shape,
                                st_),
@@ -403,7 +412,7 @@ void Engine::update_masks_locked() {
 
 void Engine::launch_step(Readback& rb) {
     if (prof_.on) {
-        const LaunchShape shape{cus_ * tune_.groups_per_cu, tune_.workgroup};
+        const LaunchShape shape{cus_ * tune_.groups_per_cu, tune_.workgroup, count_, int(tune_.bvh_width)};
         for (uint32_t k = 0; k < tune_.iterations_per_graph; ++k)
             for (int stage = 0; stage < kStageCount; ++stage) {
                 prof_.begin(stage, st_);

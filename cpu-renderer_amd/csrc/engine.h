@@ -35,7 +35,7 @@ namespace cr {
 using Clock = std::chrono::steady_clock;
 
 struct SceneRes {
-    DevBuf nodes, prims, prim_ng, aux, materials, light_rank, punctual, uvs, tangents, texels, images, textures, desc;
+    DevBuf nodes4, nodes8, prims, prim_ng, aux, materials, light_rank, punctual, uvs, tangents, texels, images, textures, desc;
     DevScene host{};
     SceneLights lights;
     std::vector<uint32_t> tri_to_prim;
@@ -105,6 +105,9 @@ public:
     crg_tuning tuning();
     void set_tuning(const crg_tuning& t);  // waits until idle
     void set_profiling(bool on);
+    // Traversal kernels with per-query work counters (node visits, triangle
+    // tests) for the requests that follow; waits until idle.
+    void set_counting(bool on);
     void kernel_times(crg_kernel_time* out, int max, int* count);
 
     static uint64_t footprint(const Request& r);
@@ -144,6 +147,7 @@ private:
     bool dirty_ = true;
     bool configured_ = false;
     bool masks_ = false;  // ctl->masks as last set
+    bool count_ = false;  // work-counting traversal variants
     hipGraphExec_t graph_ = nullptr;
     Profiler prof_;
 

@@ -327,27 +327,76 @@ CR_HDI bool prim_accepts(const DevScene& s, uint32_t prim, uint32_t flags, V3 d)
     return true;
 }
 
-// Slab test against the four children of a node.  Returns a 4-bit hit mask and
-// the (widened) entry distances.
-CR_HDI uint32_t node_hit(const RayQuery& q, const CR_GLOBAL WNode& n, float limit, float near_out[4]) {
+// Slab test against the children of a W-wide quantized node.  Each child's
+// planes are decoded exactly as the builder evaluated them (layout.h QNode),
+// giving a box that contains the child's float box, and then tested like a
+// float box: FP32 slabs widened by 4 ulp relative plus the split-origin slack,
+// so the test can only over-include.  Returns the hit mask and the widened
+// entry distances.
+template <int W>
+CR_HDI uint32_t node_hit(const RayQuery& q, const CR_GLOBAL QNode<W>& n, float limit, float near_out[W]) {
+    const uint32_t ex = n.exps;
+    const uint32_t count = ex >> 24;
+    const float sx = qscale(ex & 255u), sy = qscale((ex >> 8) & 255u), sz = qscale((ex >> 16) & 255u);
+    const float px = n.px, py = n.py, pz = n.pz;
+    const bool dx = !(q.inv[0] < 0), dy = !(q.inv[1] < 0), dz = !(q.inv[2] < 0);
     uint32_t mask = 0;
-    const bool px = !(q.inv[0] < 0), py = !(q.inv[1] < 0), pz = !(q.inv[2] < 0);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const float nx = ((px ? n.lox[c] : n.hix[c]) - q.o.x) * q.inv[0];
-        const float fx = ((px ? n.hix[c] : n.lox[c]) - q.o.x) * q.inv[0];
-        const float ny = ((py ? n.loy[c] : n.hiy[c]) - q.o.y) * q.inv[1];
-        const float fy = ((py ? n.hiy[c] : n.loy[c]) - q.o.y) * q.inv[1];
-        const float nz = ((pz ? n.loz[c] : n.hiz[c]) - q.o.z) * q.inv[2];
-        const float fz = ((pz ? n.hiz[c] : n.loz[c]) - q.o.z) * q.inv[2];
+    for (int c = 0; c < W; ++c) {
+        const int w = c / 4, sh = 8 * (c % 4);
+        const uint32_t lx = (n.qlo[0][w] >> sh) & 255u, hx = (n.qhi[0][w] >> sh) & 255u;
+        const uint32_t ly = (n.qlo[1][w] >> sh) & 255u, hy = (n.qhi[1][w] >> sh) & 255u;
+        const uint32_t lz = (n.qlo[2][w] >> sh) & 255u, hz = (n.qhi[2][w] >> sh) & 255u;
+        const float nx = (qdecode(px, dx ? lx : hx, sx) - q.o.x) * q.inv[0];
+        const float fx = (qdecode(px, dx ? hx : lx, sx) - q.o.x) * q.inv[0];
+        const float ny = (qdecode(py, dy ? ly : hy, sy) - q.o.y) * q.inv[1];
+        const float fy = (qdecode(py, dy ? hy : ly, sy) - q.o.y) * q.inv[1];
+        const float nz = (qdecode(pz, dz ? lz : hz, sz) - q.o.z) * q.inv[2];
+        const float fz = (qdecode(pz, dz ? hz : lz, sz) - q.o.z) * q.inv[2];
         float tn = fmaxf(fmaxf(q.tmin, nx), fmaxf(ny, nz));
         float tf = fminf(fminf(limit, fx), fminf(fy, fz));
         tn = tn - (fabsf(tn) * (4.0f * kU) + q.slack);
         tf = tf + (fabsf(tf) * (4.0f * kU) + q.slack);
         near_out[c] = tn;
-        if (tn <= tf && n.child[c] != kEmptyChild) mask |= 1u << c;
+        if (tn <= tf && uint32_t(c) < count) mask |= 1u << c;
     }
     return mask;
+}
+
+// Entry distances as order-preserving keys (larger float -> larger key) and back.
+CR_HDI uint32_t near_key(float f) {
+    const uint32_t b = sf::fbits(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+CR_HDI float key_near(uint32_t k) { return sf::bitsf((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k); }
+
+// One comparator of a sorting network over (key, ref) pairs: ascending keys.
+CR_HDI void sort2(uint32_t& ka, uint32_t& ra, uint32_t& kb, uint32_t& rb) {
+    const bool swap = kb < ka;
+    const uint32_t k0 = swap ? kb : ka, k1 = swap ? ka : kb;
+    const uint32_t r0 = swap ? rb : ra, r1 = swap ? ra : rb;
+    ka = k0;
+    kb = k1;
+    ra = r0;
+    rb = r1;
+}
+
+// Ascending sort of W keys with their refs (W = 4: 5 comparators; W = 8: the
+// 19-comparator network).
+template <int W>
+CR_HDI void sort_children(uint32_t (&k)[W], uint32_t (&r)[W]) {
+    if constexpr (W == 4) {
+        sort2(k[0], r[0], k[1], r[1]);
+        sort2(k[2], r[2], k[3], r[3]);
+        sort2(k[0], r[0], k[2], r[2]);
+        sort2(k[1], r[1], k[3], r[3]);
+        sort2(k[1], r[1], k[2], r[2]);
+    } else {
+        constexpr int net[19][2] = {{0, 2}, {1, 3}, {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}, {0, 1}, {2, 3},
+                                    {4, 5}, {6, 7}, {2, 4}, {3, 5}, {1, 4}, {3, 6}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+        for (int i = 0; i < 19; ++i) sort2(k[net[i][0]], r[net[i][0]], k[net[i][1]], r[net[i][1]]);
+    }
 }
 
 constexpr int kStackSize = 160;
@@ -559,10 +608,6 @@ CR_HD inline bool leaf_prims(const DevScene& s, const PrimRec* prims, RayQuery& 
 #ifndef CR_LEAF_GROUP
 #define CR_LEAF_GROUP 4
 #endif
-// Child order of a node visit by ranks (1) or by an insertion sort (0).
-#ifndef CR_RANK_PUSH
-#define CR_RANK_PUSH 1
-#endif
 // Keep the scene's node and primitive array pointers of a query in registers
 // (1) or read them from the scene record at each visit (0).
 #ifndef CR_CACHE_PTRS
@@ -621,7 +666,8 @@ struct LocalRing {
     }
 };
 
-template <bool ANY, bool EXACT = true, class Ring = LocalRing, bool PRELOAD = (CR_LEAF_PRELOAD != 0), bool MASK = true>
+template <bool ANY, bool EXACT = true, class Ring = LocalRing, bool PRELOAD = (CR_LEAF_PRELOAD != 0), bool MASK = true,
+          int W = 4>
 struct Traversal {
     ClosestResult best;
     StackEntry stack[kStackSize];  // overflow: entries older than the ring's
@@ -631,7 +677,7 @@ struct Traversal {
     int nr;    // entries in the ring
     uint32_t ref;
     uint32_t leaf;  // postponed leaf (kEmptyChild: none), see inner()/leaves()
-    const WNode* nodes;  // the scene's arrays, read once per query
+    const QNode<W>* nodes;  // the scene's arrays, read once per query
     const PrimRec* prims;
 
     CR_HDI void push(uint32_t r, float n) {
@@ -676,9 +722,10 @@ struct Traversal {
         nr = 0;
         ref = 0;
         leaf = kEmptyChild;
-        nodes = s.nodes;
+        if constexpr (W == 8) nodes = s.nodes8;
+        else nodes = s.nodes4;
         prims = s.prims;
-        return !q.valid || s.node_count == 0 || q.tmax < q.tmin;
+        return !q.valid || s.node_count4 == 0 || q.tmax < q.tmin;
     }
 
     CR_HDI bool test_leaf(const DevScene& s, RayQuery& q, uint32_t r, TraceCount& tc) {
@@ -691,13 +738,10 @@ struct Traversal {
     // the nearest one (or to the next stack entry).  Returns false when the
     // stack bound is exceeded (best.overflow set).
     CR_HDI bool visit_node(const DevScene& s, RayQuery& q, TraceCount& tc) {
-        const CR_GLOBAL WNode& n = gref((CR_CACHE_PTRS ? nodes : s.nodes) + ref);
+        const CR_GLOBAL QNode<W>& n = gref(nodes + ref);
         ++tc.nodes;
-        float nears[4];
-        uint32_t mask = node_hit(q, n, best.thi, nears);
-#if CR_RANK_PUSH
-        // Order by rank instead of an insertion sort (no dynamically indexed
-        // arrays): pos[c] = hit children farther than c, ties broken by index.
+        float nears[W];
+        const uint32_t mask = node_hit<W>(q, n, best.thi, nears);
         if (!mask) {
             next_ref();
             return true;
@@ -707,62 +751,22 @@ struct Traversal {
             best.overflow = true;
             return false;
         }
-        int pos[4];
-        uint32_t ch[4];
+        // Hit children sorted by entry distance (missed ones last): visit the
+        // nearest, push the others farthest first so the next pop is the
+        // second nearest.  The order only affects how soon the bound shrinks,
+        // never the result.
+        uint32_t key[W], ch[W];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < W; ++c) {
+            key[c] = ((mask >> c) & 1u) ? near_key(nears[c]) : 0xffffffffu;
             ch[c] = n.child[c];
-            int p = 0;
-#pragma unroll
-            for (int o = 0; o < 4; ++o)
-                if (o != c && ((mask >> o) & 1u) && (nears[o] > nears[c] || (nears[o] == nears[c] && o > c))) ++p;
-            pos[c] = p;
         }
+        sort_children<W>(key, ch);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            if (k < hits - 1) {
-                uint32_t r = 0;
-                float nk = 0;
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    if (((mask >> c) & 1u) && pos[c] == k) {
-                        r = ch[c];
-                        nk = nears[c];
-                    }
-                push(r, nk);
-            }
-        }
-        uint32_t nearest = 0;
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-            if (((mask >> c) & 1u) && pos[c] == hits - 1) nearest = ch[c];
-        ref = nearest;
+        for (int k = W - 1; k >= 1; --k)
+            if (k < hits) push(ch[k], key_near(key[k]));
+        ref = ch[0];
         return true;
-#else
-        int order[4];
-        int cnt = 0;
-        while (mask) {
-            const int c = __builtin_ctz(mask);
-            mask &= mask - 1;
-            int k = cnt++;
-            while (k > 0 && nears[order[k - 1]] < nears[c]) {
-                order[k] = order[k - 1];
-                --k;
-            }
-            order[k] = c;  // descending near: order[cnt-1] is the nearest
-        }
-        if (cnt > 0) {
-            if (sp + nr + cnt - 1 > kStackSize) {
-                best.overflow = true;
-                return false;
-            }
-            for (int k = 0; k < cnt - 1; ++k) push(n.child[order[k]], nears[order[k]]);
-            ref = n.child[order[cnt - 1]];
-        } else {
-            next_ref();
-        }
-        return true;
-#endif
     }
 
     // Next stack entry that may still hold a nearer hit (kEmptyChild when none).
@@ -823,9 +827,9 @@ struct Traversal {
     }
 };
 
-template <bool ANY, bool EXACT = true>
+template <bool ANY, bool EXACT = true, int W = 4>
 CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, TraceCount& tc) {
-    Traversal<ANY, EXACT> tr;
+    Traversal<ANY, EXACT, LocalRing, (CR_LEAF_PRELOAD != 0), true, W> tr;
     if (!tr.begin(s, q))
         while (!tr.step(s, q, tc)) {
         }

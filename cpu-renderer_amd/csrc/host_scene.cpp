@@ -662,6 +662,153 @@ void split_large(const HostScene& s, std::vector<BPrim>& prims) {
 
 }  // namespace
 
+namespace {
+
+// ---- quantized device trees (layout.h QNode): per-axis origin and scale of a
+// node, and child planes rounded outward in the decode arithmetic itself.
+struct QAxis {
+    float p;
+    uint32_t e;
+    float s;
+};
+
+// Smallest scale exponent whose 255 steps reach `hi` from `lo` when decoded
+// (e = 254 decodes 255 steps to +inf, which covers any hi).
+QAxis quant_axis(float lo, float hi) {
+    QAxis a{lo, 1, qscale(1)};
+    if (!(hi > lo)) return a;
+    int e = 127 + int(std::ceil(std::log2((double(hi) - double(lo)) / 255.0)));
+    e = std::clamp(e, 1, 254);
+    while (e < 254 && !(qdecode(lo, 255, qscale(uint32_t(e))) >= hi)) ++e;
+    while (e > 1 && qdecode(lo, 255, qscale(uint32_t(e - 1))) >= hi) --e;
+    a.e = uint32_t(e);
+    a.s = qscale(a.e);
+    return a;
+}
+
+// Largest q with decode(q) <= v (q = 0 decodes to the origin, the node's low
+// corner, which is <= v), and smallest q with decode(q) >= v.
+uint32_t quant_lo(const QAxis& a, float v) {
+    const double q = std::floor((double(v) - double(a.p)) / double(a.s));
+    uint32_t k = q <= 0 ? 0u : (q >= 255 ? 255u : uint32_t(q));
+    while (k > 0 && !(qdecode(a.p, k, a.s) <= v)) --k;
+    return k;
+}
+uint32_t quant_hi(const QAxis& a, float v) {
+    const double q = std::ceil((double(v) - double(a.p)) / double(a.s));
+    uint32_t k = q <= 0 ? 0u : (q >= 255 ? 255u : uint32_t(q));
+    while (k < 255 && !(qdecode(a.p, k, a.s) >= v)) ++k;
+    return k;
+}
+
+struct QChild {
+    uint32_t ref;
+    Box box;
+};
+
+template <int W>
+QNode<W> quantize_node(const QChild* kids, int n) {
+    QNode<W> q{};
+    Box all;
+    for (int c = 0; c < n; ++c) all.grow(kids[c].box);
+    QAxis ax[3];
+    for (int a = 0; a < 3; ++a) ax[a] = n ? quant_axis(all.lo[a], all.hi[a]) : QAxis{0, 1, qscale(1)};
+    q.px = ax[0].p;
+    q.py = ax[1].p;
+    q.pz = ax[2].p;
+    q.exps = ax[0].e | (ax[1].e << 8) | (ax[2].e << 16) | (uint32_t(n) << 24);
+    for (int c = 0; c < W; ++c) {
+        q.child[c] = c < n ? kids[c].ref : kEmptyChild;
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t lo = c < n ? quant_lo(ax[a], kids[c].box.lo[a]) : 0u;
+            const uint32_t hi = c < n ? quant_hi(ax[a], kids[c].box.hi[a]) : 0u;
+            q.qlo[a][c / 4] |= lo << (8 * (c % 4));
+            q.qhi[a][c / 4] |= hi << (8 * (c % 4));
+        }
+    }
+    return q;
+}
+
+int wnode_children(const WNode& w, QChild* out) {
+    int n = 0;
+    for (int c = 0; c < 4; ++c) {
+        if (w.child[c] == kEmptyChild) continue;
+        Box b;
+        b.lo[0] = w.lox[c], b.lo[1] = w.loy[c], b.lo[2] = w.loz[c];
+        b.hi[0] = w.hix[c], b.hi[1] = w.hiy[c], b.hi[2] = w.hiz[c];
+        out[n++] = QChild{w.child[c], b};
+    }
+    return n;
+}
+
+// The 8-wide tree over the 4-wide tree's leaves: a node takes its 4-wide
+// node's children, then repeatedly replaces the inner child of largest area
+// by that child's own children while they fit in eight slots.  Parents are
+// emitted before their children.
+struct Collapse8 {
+    const std::vector<WNode>& n4;
+    std::vector<QNode8>& out;
+    uint32_t emit(uint32_t i4) {
+        QChild kids[8];
+        int n = wnode_children(n4[i4], kids);
+        for (;;) {
+            int pick = -1;
+            double best = -1;
+            for (int c = 0; c < n; ++c) {
+                if (ref_is_leaf(kids[c].ref)) continue;
+                QChild sub[4];
+                const int k = wnode_children(n4[kids[c].ref], sub);
+                if (n - 1 + k > 8) continue;
+                const double a = kids[c].box.area();
+                if (a > best) {
+                    best = a;
+                    pick = c;
+                }
+            }
+            if (pick < 0) break;
+            QChild sub[4];
+            const int k = wnode_children(n4[kids[pick].ref], sub);
+            kids[pick] = sub[0];
+            for (int j = 1; j < k; ++j) kids[n++] = sub[j];
+        }
+        const uint32_t me = uint32_t(out.size());
+        out.emplace_back();
+        for (int c = 0; c < n; ++c)
+            if (!ref_is_leaf(kids[c].ref)) kids[c].ref = emit(kids[c].ref);
+        out[me] = quantize_node<8>(kids, n);
+        return me;
+    }
+};
+
+}  // namespace
+
+QNode4 quantize_boxes4(const float* lo, const float* hi, const uint32_t* refs, int n) {
+    QChild kids[4];
+    for (int c = 0; c < n && c < 4; ++c) {
+        kids[c].ref = refs[c];
+        for (int a = 0; a < 3; ++a) {
+            kids[c].box.lo[a] = lo[3 * c + a];
+            kids[c].box.hi[a] = hi[3 * c + a];
+        }
+    }
+    return quantize_node<4>(kids, n < 4 ? n : 4);
+}
+
+void quantize_trees(WideBvh& b) {
+    b.q4.resize(b.nodes.size());
+    parallel_chunks(b.nodes.size(), 8192, [&](size_t lo, size_t hi, size_t) {
+        for (size_t i = lo; i < hi; ++i) {
+            QChild kids[4];
+            const int n = wnode_children(b.nodes[i], kids);
+            b.q4[i] = quantize_node<4>(kids, n);
+        }
+    });
+    b.q8.clear();
+    if (b.nodes.empty()) return;
+    b.q8.reserve(b.nodes.size() / 2 + 1);
+    Collapse8{b.nodes, b.q8}.emit(0);
+}
+
 WideBvh build_wide_bvh(const HostScene& s) {
     WideBvh out;
     const size_t n = s.tris.size();
@@ -693,6 +840,7 @@ WideBvh build_wide_bvh(const HostScene& s) {
     for (size_t i = 0; i < m; ++i) out.order[i] = prims[i].id;
     out.nodes.resize(b.next.load());
     out.binary_nodes = 2 * b.leaves.load() - 1;
+    quantize_trees(out);
     return out;
 }
 

@@ -47,13 +47,20 @@ struct HostScene {
 HostScene finalize_scene(const crg_scene_desc& desc);
 
 struct WideBvh {
-    std::vector<WNode> nodes;
+    std::vector<WNode> nodes;   // 4-wide float tree (host)
+    std::vector<QNode4> q4;     // device: the same tree with quantized child boxes
+    std::vector<QNode8> q8;     // device: 8-wide collapse of it over the same leaves
     std::vector<uint32_t> order;  // prim i -> finalized triangle index (a split triangle has several prims)
     uint64_t binary_nodes = 0;    // equivalent binary node count (2*leaves-1)
 };
 
-// Binned SAH (16 bins, leaves <= 4, depth <= 48) emitted as 4-wide nodes.
+// Binned SAH (16 bins, leaves <= 4, depth <= 48) emitted as 4-wide nodes,
+// then quantized (q4) and collapsed to 8-wide nodes (q8).
 WideBvh build_wide_bvh(const HostScene& s);
+// q4 and q8 of b.nodes.
+void quantize_trees(WideBvh& b);
+// One quantized 4-wide node over n <= 4 child boxes (lo/hi: 3 floats per child).
+QNode4 quantize_boxes4(const float* lo, const float* hi, const uint32_t* refs, int n);
 
 // Environment-independent part of the light mixture of one scene.
 struct SceneLights {

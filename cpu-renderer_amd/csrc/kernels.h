@@ -20,8 +20,9 @@ struct HitOut {
 static_assert(sizeof(RayIn) == 48, "crg_ray layout");
 static_assert(sizeof(HitOut) == 32, "crg_hit layout");
 
+// Bvh::intersect per ray over the scene's `width`-wide tree (4 or 8).
 hipError_t launch_trace_rays(const DevScene* scene, const RayIn* rays, uint64_t n, HitOut* hits, int any,
-                             Control* ctl, hipStream_t st);
+                             Control* ctl, int width, hipStream_t st);
 
 // Fixed per-device scratch of the allocation-free ray query: the control block
 // (rounded up to 256 bytes), then the rays, then the hits of at most
@@ -66,6 +67,8 @@ struct Queues {
 struct LaunchShape {
     unsigned grid;   // workgroups
     unsigned block;  // threads per workgroup: 64, 128 or 256
+    bool count;      // traversal kernels with per-query work counters
+    int width;       // BVH width the traversal kernels walk: 4 or 8
 };
 
 hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const DevJob* jobs, const Queues& Q,

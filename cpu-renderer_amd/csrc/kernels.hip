@@ -92,6 +92,7 @@ __device__ inline V3 xyz(float4 v) { return V3{v.x, v.y, v.z}; }
 
 // ------------------------------------------------------------------ ray queries
 // Bvh::intersect over an array of rays (parity harness).  One lane per ray.
+template <int W>
 __global__ void __launch_bounds__(256) k_trace_rays(const DevScene* scene, const RayIn* rays, uint64_t n,
                                                     HitOut* hits, int any, Control* ctl) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -102,7 +103,7 @@ __global__ void __launch_bounds__(256) k_trace_rays(const DevScene* scene, const
     setup_query(q, V3{r.o[0], r.o[1], r.o[2]}, V3{r.d[0], r.d[1], r.d[2]}, r.tmin, r.tmax,
                 V3{r.lo[0], r.lo[1], r.lo[2]}, r.ignore);
     TraceCount tc{0, 0, 0};
-    ClosestResult best = any ? traverse<true>(s, q, tc) : traverse<false>(s, q, tc);
+    ClosestResult best = any ? traverse<true, true, W>(s, q, tc) : traverse<false, true, W>(s, q, tc);
     HitOut h;
     h.hit = best.prim != kNoTriangle ? 1u : 0u;
     h.triangle = h.hit ? best.id : kNoTriangle;
@@ -334,14 +335,16 @@ __device__ inline uint32_t encode_hit(const DevScene& s, const ClosestResult& b)
 // Both traversal kernels come in two variants, with and without the FP32
 // alpha-mask tests (MASK); the one that does not match ctl->masks returns at
 // once, so an iteration runs the mask variant only while a live job needs it.
-template <bool MASK>
+// COUNT adds the per-query work counters (wide nodes visited, triangles
+// tested); the timed variant carries no counter state in its loop.
+template <bool MASK, bool COUNT, int W>
 __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_extend(PathSoA P, const DevJob* jobs, const uint32_t* queue, uint32_t* xq,
                                                 Pending* xpend, int parity, Control* ctl, JobCounters* cnt) {
     if ((ctl->masks != 0) != MASK) return;
     const uint32_t n = ctl->qcount[parity];
     __shared__ uint32_t ring_ref[CR_LDS_RING * 256];
     __shared__ float ring_near[CR_LDS_RING * 256];
-    Traversal<false, false, TraverseRing, (CR_LEAF_PRELOAD != 0), MASK> tr;
+    Traversal<false, false, TraverseRing, (CR_LEAF_PRELOAD != 0), MASK, W> tr;
     tr.ring = TraverseRing{(lds_u32*)(ring_ref + threadIdx.x), (lds_f32*)(ring_near + threadIdx.x), blockDim.x};
     RayQuery q;
     const DevScene* sc = nullptr;
@@ -376,8 +379,10 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_extend(PathSoA 
             xq[pos] = path;
             xpend[pos] = pending_of(tr.best, work);
         }
-        wave_add_keyed(cnt->extend_nodes, job, fin ? tc.nodes : 0u);
-        wave_add_keyed(cnt->extend_tris, job, fin ? tc.tris : 0u);
+        if constexpr (COUNT) {
+            wave_add_keyed(cnt->extend_nodes, job, fin ? tc.nodes : 0u);
+            wave_add_keyed(cnt->extend_tris, job, fin ? tc.tris : 0u);
+        }
     }
 }
 
@@ -558,14 +563,14 @@ __device__ inline void shadow_query(RayQuery& q, const ShadowSoA& S, uint32_t i)
 
 // Persistent FP32-only visibility (same scheme as k_extend); undecided
 // segments without a certain blocker go to the exact queue.
-template <bool MASK>
+template <bool MASK, bool COUNT, int W>
 __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_shadow(PathSoA P, ShadowSoA S, const DevJob* jobs, uint32_t* xsq,
                                                 Pending* xspend, Control* ctl, JobCounters* cnt) {
     if ((ctl->masks != 0) != MASK) return;
     const uint32_t n = ctl->scount;
     __shared__ uint32_t ring_ref[CR_LDS_RING * 256];
     __shared__ float ring_near[CR_LDS_RING * 256];
-    Traversal<true, false, TraverseRing, (CR_LEAF_PRELOAD != 0), MASK> tr;
+    Traversal<true, false, TraverseRing, (CR_LEAF_PRELOAD != 0), MASK, W> tr;
     tr.ring = TraverseRing{(lds_u32*)(ring_ref + threadIdx.x), (lds_f32*)(ring_near + threadIdx.x), blockDim.x};
     RayQuery q;
     const DevScene* sc = nullptr;
@@ -602,8 +607,10 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_shadow(PathSoA 
             xsq[pos] = i;
             xspend[pos] = pending_of(tr.best, work);
         }
-        wave_add_keyed(cnt->shadow_nodes, job, fin ? tc.nodes : 0u);
-        wave_add_keyed(cnt->shadow_tris, job, fin ? tc.tris : 0u);
+        if constexpr (COUNT) {
+            wave_add_keyed(cnt->shadow_nodes, job, fin ? tc.nodes : 0u);
+            wave_add_keyed(cnt->shadow_tris, job, fin ? tc.tris : 0u);
+        }
     }
 }
 
@@ -789,9 +796,12 @@ __global__ void __launch_bounds__(256) k_accumulate(const DevJob* jobs, uint32_t
 static inline unsigned blocks_for(uint64_t n, unsigned bs) { return unsigned((n + bs - 1) / bs); }
 
 hipError_t launch_trace_rays(const DevScene* scene, const RayIn* rays, uint64_t n, HitOut* hits, int any, Control* ctl,
-                             hipStream_t st) {
+                             int width, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_trace_rays, dim3(blocks_for(n, 256)), dim3(256), 0, st, scene, rays, n, hits, any, ctl);
+    if (width == 8)
+        hipLaunchKernelGGL(k_trace_rays<8>, dim3(blocks_for(n, 256)), dim3(256), 0, st, scene, rays, n, hits, any, ctl);
+    else
+        hipLaunchKernelGGL(k_trace_rays<4>, dim3(blocks_for(n, 256)), dim3(256), 0, st, scene, rays, n, hits, any, ctl);
     return hipGetLastError();
 }
 
@@ -831,13 +841,41 @@ hipError_t noalloc_scratch(int device, void** out) {
     return hipSuccess;
 }
 
+// Both alpha-mask variants of a traversal stage (the one not matching
+// ctl->masks returns at once), with or without work counters, for W-wide nodes.
+template <int W, bool SHADOW>
+void launch_traversal(bool count, dim3 g, dim3 b, hipStream_t st, const PathSoA& P, const ShadowSoA& S,
+                      const DevJob* jobs, const Queues& Q, int parity, Control* ctl, JobCounters* cnt) {
+    if constexpr (SHADOW) {
+        if (count) {
+            hipLaunchKernelGGL((k_shadow<false, true, W>), g, b, 0, st, P, S, jobs, Q.xsq, Q.xspend, ctl, cnt);
+            hipLaunchKernelGGL((k_shadow<true, true, W>), g, b, 0, st, P, S, jobs, Q.xsq, Q.xspend, ctl, cnt);
+        } else {
+            hipLaunchKernelGGL((k_shadow<false, false, W>), g, b, 0, st, P, S, jobs, Q.xsq, Q.xspend, ctl, cnt);
+            hipLaunchKernelGGL((k_shadow<true, false, W>), g, b, 0, st, P, S, jobs, Q.xsq, Q.xspend, ctl, cnt);
+        }
+    } else {
+        if (count) {
+            hipLaunchKernelGGL((k_extend<false, true, W>), g, b, 0, st, P, jobs, Q.q[parity], Q.xq, Q.xpend, parity,
+                               ctl, cnt);
+            hipLaunchKernelGGL((k_extend<true, true, W>), g, b, 0, st, P, jobs, Q.q[parity], Q.xq, Q.xpend, parity,
+                               ctl, cnt);
+        } else {
+            hipLaunchKernelGGL((k_extend<false, false, W>), g, b, 0, st, P, jobs, Q.q[parity], Q.xq, Q.xpend, parity,
+                               ctl, cnt);
+            hipLaunchKernelGGL((k_extend<true, false, W>), g, b, 0, st, P, jobs, Q.q[parity], Q.xq, Q.xpend, parity,
+                               ctl, cnt);
+        }
+    }
+}
+
 hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const DevJob* jobs, const Queues& Q,
                         int parity, Control* ctl, JobCounters* cnt, LaunchShape shape, hipStream_t st) {
     const dim3 g(shape.grid), b(shape.block);
     switch (stage) {
         case kStageExtend:
-            hipLaunchKernelGGL(k_extend<false>, g, b, 0, st, P, jobs, Q.q[parity], Q.xq, Q.xpend, parity, ctl, cnt);
-            hipLaunchKernelGGL(k_extend<true>, g, b, 0, st, P, jobs, Q.q[parity], Q.xq, Q.xpend, parity, ctl, cnt);
+            if (shape.width == 8) launch_traversal<8, false>(shape.count, g, b, st, P, S, jobs, Q, parity, ctl, cnt);
+            else launch_traversal<4, false>(shape.count, g, b, st, P, S, jobs, Q, parity, ctl, cnt);
             break;
         case kStageExtendExact:
             hipLaunchKernelGGL(k_extend_exact, g, b, 0, st, P, jobs, Q.xq, Q.xpend, ctl, cnt);
@@ -847,8 +885,8 @@ hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const
                                cnt);
             break;
         case kStageShadow:
-            hipLaunchKernelGGL(k_shadow<false>, g, b, 0, st, P, S, jobs, Q.xsq, Q.xspend, ctl, cnt);
-            hipLaunchKernelGGL(k_shadow<true>, g, b, 0, st, P, S, jobs, Q.xsq, Q.xspend, ctl, cnt);
+            if (shape.width == 8) launch_traversal<8, true>(shape.count, g, b, st, P, S, jobs, Q, parity, ctl, cnt);
+            else launch_traversal<4, true>(shape.count, g, b, st, P, S, jobs, Q, parity, ctl, cnt);
             break;
         case kStageShadowExact:
             hipLaunchKernelGGL(k_shadow_exact, g, b, 0, st, P, S, jobs, Q.xsq, Q.xspend, ctl, cnt);

@@ -82,12 +82,42 @@ CR_HDI uint32_t leaf_first(uint32_t r) { return r & kLeafMaxFirst; }
 CR_HDI uint32_t leaf_count(uint32_t r) { return ((r >> 27) & 15u) + 1; }
 CR_HDI uint32_t make_leaf(uint32_t first, uint32_t count) { return kLeafBit | ((count - 1) << 27) | first; }
 
+// 4-wide node with float child boxes: the host builder's tree, from which the
+// device's quantized trees are made.
 struct alignas(128) WNode {
     float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
     uint32_t child[4];
     uint32_t pad[4];
 };
 static_assert(sizeof(WNode) == 128, "WNode must be one 128-byte line");
+
+// Device node of a W-wide tree (W = 4: 64 bytes, W = 8: 128 bytes) with child
+// boxes quantized to 8 bits per plane.  Per axis a the node stores an origin
+// p_a (the node box's low corner) and a power-of-two scale s_a = 2^(e_a - 127)
+// (the float whose exponent field is e_a, 1 <= e_a <= 254); a child's planes
+// decode as
+//     lo = p_a + float(qlo) * s_a,   hi = p_a + float(qhi) * s_a
+// where the product is exact (q < 2^8, s_a a power of two) and the sum rounds
+// once in binary32.  The builder evaluates exactly this expression and picks
+// qlo / qhi so that the decoded box contains the child's float box, so the
+// slab test on decoded boxes over-includes as the float boxes did.  Byte k of
+// word qlo[a][c / 4] is child c's plane (c % 4 == k).  Empty children have
+// child ref kEmptyChild.
+template <int W>
+struct alignas(16 * W) QNode {
+    float px, py, pz;
+    uint32_t exps;  // e_x | e_y << 8 | e_z << 16 | child count << 24
+    uint32_t qlo[3][W / 4];
+    uint32_t qhi[3][W / 4];
+    uint32_t child[W];
+};
+using QNode4 = QNode<4>;
+using QNode8 = QNode<8>;
+static_assert(sizeof(QNode4) == 64, "4-wide node is one 64-byte block");
+static_assert(sizeof(QNode8) == 128, "8-wide node is one 128-byte line");
+
+CR_HDI float qscale(uint32_t e) { return sf::bitsf(e << 23); }
+CR_HDI float qdecode(float p, uint32_t q, float s) { return p + float(q) * s; }
 
 enum PrimFlags : uint32_t {
     kPrimCull = 1u,        // single-sided opaque: reject when dot(Ng, d) >= 0
@@ -172,7 +202,8 @@ struct DevPunctual {
 };
 
 struct DevScene {
-    const WNode* nodes;
+    const QNode4* nodes4;  // the 4-wide tree
+    const QNode8* nodes8;  // the 8-wide tree over the same leaves (a collapse of the 4-wide one)
     const PrimRec* prims;
     const float4* prim_ng;        // geometric normal per prim (cull test)
     const PrimAux* aux;
@@ -184,7 +215,7 @@ struct DevScene {
     const float4* texels;    // all images, row-major
     const DevImage* images;
     const DevTexture* textures;
-    uint32_t node_count, prim_count, punctual_count, pad;
+    uint32_t node_count4, prim_count, punctual_count, node_count8;
 };
 
 // Light-selection mixture of one (scene, environment) pair.

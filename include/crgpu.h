@@ -136,11 +136,12 @@ typedef struct crg_stats {
     uint64_t shadow_rays;      /* any-hit queries                              */
     int32_t workers;           /* 0 = device                                   */
     int32_t iterations;        /* wavefront iterations executed                */
-    /* Work counters: they depend on how queries share waves, so a request's
+    /* Work counters, filled only while crg_set_work_counters is on (0
+     * otherwise).  They depend on how queries share waves, so a request's
      * values differ slightly between runs alone and in a batch. */
-    uint64_t extend_nodes;     /* 4-wide nodes visited by closest-hit queries  */
+    uint64_t extend_nodes;     /* wide nodes visited by closest-hit queries    */
     uint64_t extend_tris;      /* triangles tested by closest-hit queries      */
-    uint64_t shadow_nodes;     /* 4-wide nodes visited by any-hit queries      */
+    uint64_t shadow_nodes;     /* wide nodes visited by any-hit queries        */
     uint64_t shadow_tris;      /* triangles tested by any-hit queries          */
     uint64_t exact_fallbacks;  /* triangle tests decided in emulated binary64  */
     uint64_t exact_rays;       /* queries rerun by the exact traversal         */
@@ -172,7 +173,8 @@ typedef struct crg_hit {
 typedef struct crg_scene_info {
     uint64_t triangles;        /* after finalize (degenerates removed) */
     uint64_t bvh_nodes;        /* binary SAH nodes built on the host   */
-    uint64_t wide_nodes;       /* 4-wide 128-byte device nodes         */
+    uint64_t wide_nodes;       /* 4-wide device nodes (64 B, quantized boxes); the
+                                  8-wide collapse has about 0.7 as many */
     uint64_t device_bytes;     /* resident HBM footprint               */
     double build_seconds;      /* finalize + SAH build + flatten       */
     double upload_seconds;
@@ -308,7 +310,8 @@ typedef struct crg_tuning {
     uint32_t groups_per_cu;         /* workgroups per compute unit (grid size)   */
     uint32_t iterations_per_graph;  /* wavefront iterations per hipGraph launch
                                        (even, >= 2)                              */
-    uint32_t bvh_width;             /* children per BVH node (4)                 */
+    uint32_t bvh_width;             /* children per BVH node walked: 4 or 8 (every
+                                       resident scene holds both trees)          */
 } crg_tuning;
 int crg_get_tuning(crg_ctx* ctx, crg_tuning* out);
 /* Waits for the requests in flight, then applies the configuration. */
@@ -353,6 +356,10 @@ typedef struct crg_kernel_time {
 } crg_kernel_time;
 
 int crg_set_profiling(crg_ctx* ctx, int enable);
+/* Traversal kernels that count their work per query (the crg_stats work
+ * counters) for the requests that follow; off by default, so the timed
+ * kernels carry no counters.  Waits for the requests in flight. */
+int crg_set_work_counters(crg_ctx* ctx, int enable);
 /* Copies up to `max` entries; *count receives the number of kernels tracked. */
 int crg_kernel_times(crg_ctx* ctx, crg_kernel_time* out, int max, int* count);
 

@@ -7,6 +7,8 @@
 // emulated binary64.  Used by the CPU test-suite to check the device logic
 // against the oracle without a GPU, and to reproduce device behaviour on the
 // host when a GPU result needs explaining.
+#include <array>
+#include <map>
 #include <algorithm>
 #include <cmath>
 #include <random>
@@ -57,9 +59,9 @@ struct EmuEnv {
 // rounds placed by a pseudo-random schedule: any interleaving must give the
 // same result as step().  Odd schedules use the plain leaf test, even ones the
 // preloading one.
-template <bool ANY, bool PRELOAD, bool MASK = true>
+template <bool ANY, bool PRELOAD, bool MASK = true, int W = 4>
 ClosestResult phased(const DevScene& s, RayQuery& q, TraceCount& tc, uint64_t& rng) {
-    Traversal<ANY, false, LocalRing, PRELOAD, MASK> tr;
+    Traversal<ANY, false, LocalRing, PRELOAD, MASK, W> tr;
     if (tr.begin(s, q)) return tr.best;
     for (;;) {
         if (tr.inner(s, q, tc)) break;
@@ -68,6 +70,46 @@ ClosestResult phased(const DevScene& s, RayQuery& q, TraceCount& tc, uint64_t& r
         if (go && tr.has_leaf() && tr.leaves(s, q, tc)) break;
     }
     return tr.best;
+}
+
+// 1: a leaf's float box leaves the decoded boxes on its path; 2: the leaf set
+// differs from the float tree's.
+template <int W>
+int check_quantized(const std::vector<QNode<W>>& q, const std::map<uint32_t, std::array<float, 6>>& leaf_box) {
+    if (q.empty()) return leaf_box.empty() ? 0 : 2;
+    struct Item {
+        uint32_t ref;
+        float lo[3], hi[3];
+    };
+    std::vector<Item> stack{{0, {-INFINITY, -INFINITY, -INFINITY}, {INFINITY, INFINITY, INFINITY}}};
+    size_t leaves = 0;
+    while (!stack.empty()) {
+        const Item it = stack.back();
+        stack.pop_back();
+        if (ref_is_leaf(it.ref)) {
+            auto f = leaf_box.find(it.ref);
+            if (f == leaf_box.end()) return 2;
+            ++leaves;
+            for (int a = 0; a < 3; ++a)
+                if (f->second[a] < it.lo[a] || f->second[3 + a] > it.hi[a]) return 1;
+            continue;
+        }
+        if (it.ref >= q.size()) return 2;
+        const QNode<W>& nd = q[it.ref];
+        const float p[3] = {nd.px, nd.py, nd.pz};
+        for (int c = 0; c < int(nd.exps >> 24); ++c) {
+            Item ch{nd.child[c], {}, {}};
+            for (int a = 0; a < 3; ++a) {
+                const float sc = qscale((nd.exps >> (8 * a)) & 255u);
+                const uint32_t ql = (nd.qlo[a][c / 4] >> (8 * (c % 4))) & 255u;
+                const uint32_t qh = (nd.qhi[a][c / 4] >> (8 * (c % 4))) & 255u;
+                ch.lo[a] = std::max(it.lo[a], qdecode(p[a], ql, sc));
+                ch.hi[a] = std::min(it.hi[a], qdecode(p[a], qh, sc));
+            }
+            stack.push_back(ch);
+        }
+    }
+    return leaves == leaf_box.size() ? 0 : 2;
 }
 
 extern "C" {
@@ -83,11 +125,11 @@ void* emu_scene_create(const crg_scene_desc* d) {
         s->lights = scene_lights(s->hs);
         s->ranks = light_ranks(s->lights, s->arr.bvh.order, n);
         SceneArrays& a = s->arr;
-        s->dev = DevScene{a.bvh.nodes.data(), a.prims.data(), a.prim_ng.data(), a.aux.data(), a.materials.data(),
+        s->dev = DevScene{a.bvh.q4.data(), a.bvh.q8.data(), a.prims.data(), a.prim_ng.data(), a.aux.data(), a.materials.data(),
                           s->ranks.data(), s->lights.punctual.data(), a.uvs.empty() ? nullptr : a.uvs.data(),
                           a.tangents.empty() ? nullptr : a.tangents.data(), a.texels.data(), a.images.data(),
-                          a.textures.data(), uint32_t(a.bvh.nodes.size()), uint32_t(a.prims.size()),
-                          uint32_t(s->lights.punctual.size()), 0};
+                          a.textures.data(), uint32_t(a.bvh.q4.size()), uint32_t(a.prims.size()),
+                          uint32_t(s->lights.punctual.size()), uint32_t(a.bvh.q8.size())};
         return s.release();
     } catch (const std::exception& e) {
         lastError = e.what();
@@ -153,6 +195,17 @@ int emu_scene_check(void* sp) {
     }
     for (size_t i = 0; i < n; ++i)
         if (seen[i] != 1) return 6;
+    // Quantized device trees: both hold exactly the float tree's leaves, and
+    // along every root-to-leaf path the decoded child boxes contain the leaf's
+    // float box (so the slab test over decoded boxes never culls a leaf the
+    // float boxes would visit).
+    std::map<uint32_t, std::array<float, 6>> leaf_box;
+    for (const WNode& w : nodes)
+        for (int c = 0; c < 4; ++c)
+            if (w.child[c] != kEmptyChild && ref_is_leaf(w.child[c]))
+                leaf_box[w.child[c]] = {w.lox[c], w.loy[c], w.loz[c], w.hix[c], w.hiy[c], w.hiz[c]};
+    if (const int r = check_quantized(s.arr.bvh.q4, leaf_box)) return 8 + r;
+    if (const int r = check_quantized(s.arr.bvh.q8, leaf_box)) return 10 + r;
     return 0;
 }
 
@@ -302,22 +355,15 @@ void emu_expf(const float* x, float* out, uint64_t n) {
     for (uint64_t i = 0; i < n; ++i) out[i] = expf_ff(x[i]);
 }
 
-// Slab test of one box (child 0 of a node whose other children are empty):
-// 1 when node_hit reports the box as hit within [tmin, tmax].
+// Slab test of one box, quantized by the builder as the only child of a
+// 4-wide node: 1 when node_hit reports it as hit within [tmin, tmax].
 int emu_box_hit(const float* o, const float* d, float tmin, float tmax, const float* lo, const float* hi) {
     RayQuery q;
     setup_query(q, V3{o[0], o[1], o[2]}, V3{d[0], d[1], d[2]}, tmin, tmax, V3{0, 0, 0}, kNoTriangle);
-    WNode n{};
-    for (int c = 0; c < 4; ++c) {
-        n.lox[c] = n.loy[c] = n.loz[c] = INFINITY;
-        n.hix[c] = n.hiy[c] = n.hiz[c] = -INFINITY;
-        n.child[c] = kEmptyChild;
-    }
-    n.lox[0] = lo[0], n.loy[0] = lo[1], n.loz[0] = lo[2];
-    n.hix[0] = hi[0], n.hiy[0] = hi[1], n.hiz[0] = hi[2];
-    n.child[0] = 1;
+    const uint32_t ref = 1;
+    const QNode4 n = quantize_boxes4(lo, hi, &ref, 1);
     float nears[4];
-    return int(node_hit(q, n, tmax, nears) & 1u);
+    return int(node_hit<4>(q, n, tmax, nears) & 1u);
 }
 
 // Film encoders on the CPU with the device's shared code (film.h) and the host
@@ -398,12 +444,20 @@ int emu_trace(void* sp, const crg_ray* rays, uint64_t n, crg_hit* hits, int any,
                     r.tmin, r.tmax, V3{r.origin_low[0], r.origin_low[1], r.origin_low[2]}, r.ignore);
         TraceCount tc{0, 0, 0};
         // FP32-only pass first, exact rerun only when flagged (the kernels' order).
+        // Schedule bit 3 walks the 8-wide tree instead of the 4-wide one.
         uint64_t rng = schedule * 0x9e3779b97f4a7c15ull + i;
         ClosestResult b;
-        if (!schedule) b = any ? traverse<true, false>(s.dev, q, tc) : traverse<false, false>(s.dev, q, tc);
-        else if (schedule & 4)  // the traversal variant without FP32 alpha-mask tests
+        const uint64_t sched = schedule & ~uint64_t(8);
+        if (schedule & 8) {
+            if (sched & 4)
+                b = any ? phased<true, false, false, 8>(s.dev, q, tc, rng)
+                        : phased<false, false, false, 8>(s.dev, q, tc, rng);
+            else
+                b = any ? phased<true, false, true, 8>(s.dev, q, tc, rng) : phased<false, false, true, 8>(s.dev, q, tc, rng);
+        } else if (!sched) b = any ? traverse<true, false>(s.dev, q, tc) : traverse<false, false>(s.dev, q, tc);
+        else if (sched & 4)  // the traversal variant without FP32 alpha-mask tests
             b = any ? phased<true, false, false>(s.dev, q, tc, rng) : phased<false, false, false>(s.dev, q, tc, rng);
-        else if (schedule & 1) b = any ? phased<true, false>(s.dev, q, tc, rng) : phased<false, false>(s.dev, q, tc, rng);
+        else if (sched & 1) b = any ? phased<true, false>(s.dev, q, tc, rng) : phased<false, false>(s.dev, q, tc, rng);
         else b = any ? phased<true, true>(s.dev, q, tc, rng) : phased<false, true>(s.dev, q, tc, rng);
         const int work = any ? exact_work<true>(b) : exact_work<false>(b);
         const bool rerun = work != 0;

@@ -3,6 +3,7 @@
 from __future__ import annotations
 
 import ctypes as C
+import os
 import pathlib
 import subprocess
 
@@ -20,7 +21,7 @@ def build() -> pathlib.Path:
     if OUT.exists() and OUT.stat().st_mtime >= max(p.stat().st_mtime for p in DEPS):
         return OUT
     OUT.parent.mkdir(exist_ok=True)
-    tmp = OUT.with_suffix(".so.tmp")
+    tmp = OUT.with_suffix(f".so.tmp{os.getpid()}")  # concurrent test workers each build their own
     subprocess.run(["clang++", "-std=c++20", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
                     "-pthread", "-o", str(tmp), *map(str, SRC), "-lz"], check=True)
     tmp.replace(OUT)

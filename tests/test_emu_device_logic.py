@@ -38,7 +38,7 @@ def test_emu_trace_bit_exact(emu, name):
     g = load_golden("trace.npz")
     _, scene, rays, _ = next(s for s in make_golden.trace_sets() if s[0] == name)
     es = emu.scene(scene)
-    assert es.check() == 0, "4-wide BVH structure"
+    assert es.check() == 0, "4-wide BVH structure and its quantized 4- and 8-wide trees"
     assert es.triangles == int(g[f"{name}/triangles"])
     h = es.trace(rays)
     np.testing.assert_array_equal(h["hit"], g[f"{name}/hit"])
@@ -48,7 +48,8 @@ def test_emu_trace_bit_exact(emu, name):
     np.testing.assert_array_equal(es.trace(rays, True)["hit"], g[f"{name}/any"])
     # Postponed-leaf traversal (plain and preloaded leaf tests): any
     # interleaving of inner and leaf rounds gives the same closest and any hits.
-    for schedule in (1, 2, 5):
+    # Schedules 9 and 13 walk the 8-wide tree (with and without the FP32 mask tests).
+    for schedule in (1, 2, 5, 9, 13):
         hp = es.trace(rays, schedule=schedule)
         np.testing.assert_array_equal(hp, h)
         np.testing.assert_array_equal(es.trace(rays, True, schedule=schedule)["hit"], g[f"{name}/any"])

@@ -37,6 +37,19 @@ def _sets():
     return make_golden.trace_sets()
 
 
+def _traced(gpu, sid, rays, width):
+    """Closest and any hits over the scene's `width`-wide tree (crg_trace_rays walks the
+    width of the context's tuning)."""
+    default = gpu.get_tuning()
+    t = gpu.get_tuning()
+    t.bvh_width = width
+    gpu.set_tuning(t)
+    try:
+        return gpu.trace(sid, rays, any_hit=False), gpu.trace(sid, rays, any_hit=True)
+    finally:
+        gpu.set_tuning(default)
+
+
 @pytest.mark.parametrize("case", [s[0] for s in _sets()])
 def test_trace_matches_reference(gpu, trace_golden, case):
     name, scene, rays, stored = next(s for s in _sets() if s[0] == case)
@@ -44,16 +57,17 @@ def test_trace_matches_reference(gpu, trace_golden, case):
     sid, info = gpu.upload_scene(scene)
     try:
         assert info.triangles == int(g[f"{name}/triangles"])
-        hits = gpu.trace(sid, rays, any_hit=False)
-        anyh = gpu.trace(sid, rays, any_hit=True)
+        walks = {w: _traced(gpu, sid, rays, w) for w in (4, 8)}
     finally:
         gpu.release_scene(sid)
-    np.testing.assert_array_equal(hits["hit"], g[f"{name}/hit"], err_msg=f"{name}: hit flags")
-    np.testing.assert_array_equal(hits["triangle"], g[f"{name}/triangle"], err_msg=f"{name}: triangle ids")
-    for k in ("t", "u", "v"):
-        np.testing.assert_array_equal(hits[k].view(np.uint64), g[f"{name}/{k}"].view(np.uint64),
-                                      err_msg=f"{name}: binary64 {k}")
-    np.testing.assert_array_equal(anyh["hit"], g[f"{name}/any"], err_msg=f"{name}: any-hit flags")
+    for width, (hits, anyh) in walks.items():
+        what = f"{name} ({width}-wide)"
+        np.testing.assert_array_equal(hits["hit"], g[f"{name}/hit"], err_msg=f"{what}: hit flags")
+        np.testing.assert_array_equal(hits["triangle"], g[f"{name}/triangle"], err_msg=f"{what}: triangle ids")
+        for k in ("t", "u", "v"):
+            np.testing.assert_array_equal(hits[k].view(np.uint64), g[f"{name}/{k}"].view(np.uint64),
+                                          err_msg=f"{what}: binary64 {k}")
+        np.testing.assert_array_equal(anyh["hit"], g[f"{name}/any"], err_msg=f"{what}: any-hit flags")
 
 
 def _render_cases():
@@ -120,16 +134,17 @@ def test_full_size_scene_matches_reference(gpu, name):
     try:
         assert info.triangles == int(g[f"{name}/triangles"])
         res = gpu.render(sid, eid, w.camera, opt)
-        hits = gpu.trace(sid, rays, any_hit=False)
-        anyh = gpu.trace(sid, rays, any_hit=True)
+        walks = {wd: _traced(gpu, sid, rays, wd) for wd in (4, 8)}
     finally:
         gpu.release_scene(sid)
         gpu.release_env(eid)
-    np.testing.assert_array_equal(hits["hit"], g[f"{name}/hit"], err_msg=f"{name}: hit flags")
-    np.testing.assert_array_equal(hits["triangle"], g[f"{name}/triangle"], err_msg=f"{name}: triangle ids")
-    for k in ("t", "u", "v"):
-        np.testing.assert_array_equal(hits[k].view(np.uint64), g[f"{name}/{k}"].view(np.uint64),
-                                      err_msg=f"{name}: binary64 {k}")
-    np.testing.assert_array_equal(anyh["hit"], g[f"{name}/any"], err_msg=f"{name}: any-hit flags")
+    for width, (hits, anyh) in walks.items():
+        what = f"{name} ({width}-wide)"
+        np.testing.assert_array_equal(hits["hit"], g[f"{name}/hit"], err_msg=f"{what}: hit flags")
+        np.testing.assert_array_equal(hits["triangle"], g[f"{name}/triangle"], err_msg=f"{what}: triangle ids")
+        for k in ("t", "u", "v"):
+            np.testing.assert_array_equal(hits[k].view(np.uint64), g[f"{name}/{k}"].view(np.uint64),
+                                          err_msg=f"{what}: binary64 {k}")
+        np.testing.assert_array_equal(anyh["hit"], g[f"{name}/any"], err_msg=f"{what}: any-hit flags")
     assert_parity(res.pixels, g[f"{name}/film"], name)
     assert res.boundary_retries == int(g[f"{name}/retries"])

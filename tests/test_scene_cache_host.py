@@ -21,7 +21,8 @@ import pytest
 import ref_scenes
 from conftest import abi
 
-MAGIC_LEN, RECORD_SIZES = 8, 8
+MAGIC_LEN, RECORD_SIZES = 8, 9
+CHILD4 = 40  # byte offset of a 4-wide node's child refs (origin, exponents, 12 + 12 quantized planes)
 
 
 def _fnv_step(h, w):
@@ -57,15 +58,17 @@ def cache_checksum(body: bytes) -> int:
 class CacheFile:
     """Section offsets of a cache file, parsed from its own header."""
 
-    NAMES = ("nodes", "prims", "prim_ng", "aux", "materials", "uvs", "tangents", "texels", "images", "textures")
+    NAMES = ("nodes", "nodes8", "prims", "prim_ng", "aux", "materials", "uvs", "tangents", "texels", "images",
+             "textures")
 
     def __init__(self, data: bytes):
         self.data = bytearray(data)
-        sizes = struct.unpack_from("<8I", data, MAGIC_LEN)
-        rec = {"nodes": sizes[0], "prims": sizes[1], "aux": sizes[2], "materials": sizes[3], "images": sizes[5],
-               "textures": sizes[6], "prim_ng": sizes[7], "uvs": sizes[7], "tangents": sizes[7], "texels": sizes[7]}
+        sizes = struct.unpack_from("<9I", data, MAGIC_LEN)
+        rec = {"nodes": sizes[0], "nodes8": sizes[1], "prims": sizes[2], "aux": sizes[3], "materials": sizes[4],
+               "images": sizes[6], "textures": sizes[7], "prim_ng": sizes[8], "uvs": sizes[8], "tangents": sizes[8],
+               "texels": sizes[8]}
         self.rec = rec
-        self.punctual_size = sizes[4]
+        self.punctual_size = sizes[5]
         pos = MAGIC_LEN + 4 * RECORD_SIZES
         self.triangles, self.binary_nodes = struct.unpack_from("<QQ", data, pos)
         pos += 16
@@ -135,7 +138,7 @@ def _tampered(written, tmp_path, edit):
 def _inner_or_leaf(f, want_leaf):
     for i in range(f.sec["nodes"][1]):
         for c in range(4):
-            r = f.get_u32("nodes", i, 96 + 4 * c)
+            r = f.get_u32("nodes", i, CHILD4 + 4 * c)
             if r != 0x7FFFFFFF and bool(r & 0x80000000) == want_leaf:
                 return i, c, r
     return None
@@ -144,13 +147,17 @@ def _inner_or_leaf(f, want_leaf):
 def _stretch_leaf(f):
     """A leaf ref whose 16-primitive range starts at the last primitive."""
     node, child, _ = _inner_or_leaf(f, True)
-    f.put_u32("nodes", node, 96 + 4 * child, 0x80000000 | (15 << 27) | (f.sec["prims"][1] - 1))
+    f.put_u32("nodes", node, CHILD4 + 4 * child, 0x80000000 | (15 << 27) | (f.sec["prims"][1] - 1))
 
 
 EDITS = {
     "leaf range past the primitives": (lambda f: _stretch_leaf(f), "BVH node refers outside"),
     "node child back to the root": (
-        lambda f: f.put_u32("nodes", 0, 96, 0), "BVH node refers outside"),
+        lambda f: f.put_u32("nodes", 0, CHILD4, 0), "BVH node refers outside"),
+    "8-wide node child past the nodes": (
+        lambda f: f.put_u32("nodes8", 0, 64, f.sec["nodes8"][1] + 3), "BVH node refers outside"),
+    "scale exponent out of range": (
+        lambda f: f.put_u32("nodes", 0, 12, f.get_u32("nodes", 0, 12) & 0xFFFFFF00), "BVH node refers outside"),
     "triangle id past the triangles": (
         lambda f: f.put_u32("prims", 1, 12, f.triangles), "primitive record refers outside"),
     "material past the materials": (
