@@ -69,8 +69,22 @@ def workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(w, budget_s: float, threads: int) -> dict:
-    """Time the reference CPU renderer on full-frame renders of fewer samples."""
+def _compare(gpu_film, cpu_film) -> dict:
+    """Per-pixel agreement of the device's film with the reference's on the same samples."""
+    import numpy as np
+    a, r = gpu_film.astype(np.float64), cpu_film.astype(np.float64)
+    d = np.abs(a - r)
+    ok = np.all((d <= 1e-4 * np.abs(r)) | (d <= 1e-7), axis=-1)
+    exact = np.all(gpu_film.view(np.uint32) == cpu_film.view(np.uint32), axis=-1)
+    return {"pixels": int(ok.size), "within_1e-4": round(float(ok.mean()), 6),
+            "bit_exact": round(float(exact.mean()), 6), "diverged_pixels": int((~ok).sum())}
+
+
+def cpu_baseline(ctx, sid, eid, w, budget_s: float, threads: int, max_spp: int = 64) -> dict:
+    """Time the reference CPU renderer on full-frame renders of fewer samples (its
+    pixel/sample streams are the first samples of the GPU run's), compare its film with
+    the device's render of the same samples, and time it once more at the reference's
+    default thread count (threads = 0: half the logical CPUs, reference src/render.cpp:363)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import probe
     kind = "reference" if probe.available("ref") else "port"
@@ -85,16 +99,31 @@ def cpu_baseline(w, budget_s: float, threads: int) -> dict:
     while True:
         opt = w.options(samples=spp, threads=threads)
         film, st = ps.render(pe, w.camera, opt)
-        samples = w.width * w.height * spp
-        best = (samples, st.seconds, spp)
+        best = (w.width * w.height * spp, st.seconds, spp, film)
         log(f"cpu baseline: {w.width}x{w.height}x{spp} in {st.seconds:.2f}s")
-        if st.seconds * 4 > budget_s or spp * 4 > w.samples:
+        if st.seconds * 4 > budget_s or spp * 4 > min(w.samples, max_spp):
             break
         spp *= 4
-    samples, secs, spp = best
+    samples, secs, spp, film = best
+    g = ctx.render(sid, eid, w.camera, w.options(samples=spp))
+    rays = g.extend_rays + g.shadow_rays
+    parity = _compare(g.pixels, film)
+    log(f"cpu baseline: device film at {spp} spp vs reference: {parity}")
+    d_spp = max(1, spp // 4)
+    _, dst = ps.render(pe, w.camera, w.options(samples=d_spp, threads=0))
+    default = {"threads": "0 (reference default: hardware_concurrency / 2)", "workers": int(dst.workers),
+               "msamples_per_s": round(w.width * w.height * d_spp / dst.seconds / 1e6, 4),
+               "sample": f"full frame at {d_spp} spp"}
+    log(f"cpu baseline at the reference default thread count: {default}")
     return {"value": round(samples / secs / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": kind,
+            "cores_meaning": "logical CPUs in this process's affinity mask, one render thread each",
             "sample": f"{w.width}x{w.height} full frame at {spp} spp (pixel/sample streams identical to the "
                       f"GPU run's first {spp} samples), render() time excluding BVH build",
+            "mrays_per_s": round(rays / secs / 1e6, 3),
+            "mrays_note": "extend + shadow queries of these samples (replays included), counted by the device; "
+                          "the reference issues the same queries",
+            "reference_default_threads": default,
+            "film_vs_device": parity,
             "scene_seconds": round(scene_s, 3)}
 
 
@@ -244,6 +273,40 @@ def quality(ctx) -> dict:
             "relmse_by_spp": ladder, "spp_at_relmse_1e-3": hit[0] if hit else None,
             "msamples_per_s_at_target": round(hit[1], 3) if hit else None,
             "mrays_per_s_at_target": round(hit[2], 2) if hit else None}
+
+
+def quality_config2(ctx, sid, eid, w) -> dict:
+    """relMSE of the headline workload (config 2, seed 0) over two 64x64 windows of its 1080p
+    frame (inside a dielectric sphere; sun-lit terrain), rendered with their full-frame sample
+    streams, against the oracle's binary64-accumulated 32768-spp render of the same windows
+    with another seed: the spp at which each window reaches relMSE <= 1e-3."""
+    import numpy as np
+    f = ROOT / "tests" / "golden" / "converged_c2.npz"
+    if not f.exists() or w.name != "config2_terrain_spheres":
+        return None
+    g = np.load(f, allow_pickle=False)
+    spp_ref, seed_ref = (int(x) for x in g["meta"])
+    out, need = {}, 0
+    for name in ("sphere", "terrain"):
+        win = tuple(int(v) for v in g[f"{name}/window"])
+        ref = g[f"{name}/film"].astype(np.float64)
+        ladder, hit, rate = {}, None, None
+        for spp in (1024, 4096, 16384, 65536):
+            t = time.perf_counter()
+            r = ctx.render_window(sid, eid, w.camera, w.options(samples=spp, seed=0), win)
+            el = time.perf_counter() - t
+            img = r.pixels.astype(np.float64)
+            e = float(np.mean((img - ref) ** 2 / (ref * ref + 1e-2)))
+            ladder[spp] = round(e, 7)
+            rate = win[2] * win[3] * spp / el / 1e6
+            if e <= 1e-3 and hit is None:
+                hit = spp
+        out[name] = {"window": list(win), "relmse_by_spp": ladder, "spp_at_relmse_1e-3": hit,
+                     "window_msamples_per_s": round(rate, 2)}
+        need = max(need, hit or 1 << 30)
+    out["reference"] = f"oracle window render, {spp_ref} spp, seed {seed_ref}, binary64 accumulation"
+    out["spp_at_relmse_1e-3"] = need if need < (1 << 30) else None
+    return out
 
 
 BUILDERS = 4  # host threads preparing scenes concurrently in the batched form
@@ -422,6 +485,101 @@ def served_form(ctx, catalog, env_id, count: int, rate: float) -> dict:
             "scenes": f"{len(catalog)} resident; plain uploads (host BVH build) per request"}
 
 
+def measure_plain(ctx, w, steps: int, warmup: int, cpu_budget) -> dict:
+    """The plain render path on one workload: `steps` timed renders after `warmup`, work per
+    query from a counting render, per-kernel device times (HIP events on the library's stream,
+    kernels launched one by one) with the traversal and shading kernels against the HBM
+    roofline, and the reference CPU renderer beside it."""
+    sid, info = ctx.upload_scene(w.scene)
+    eid, env_bytes = ctx.upload_env(w.env)
+    log(f"{w.name} resident: {info.triangles} tris, {info.wide_nodes} wide nodes, "
+        f"{(info.device_bytes + env_bytes) / 2**20:.1f} MiB, host build {info.build_seconds:.2f}s, "
+        f"upload {info.upload_seconds:.2f}s")
+    opt = w.options()
+    for i in range(warmup):
+        r = ctx.render(sid, eid, w.camera, opt)
+        log(f"{w.name} warmup {i}: {r.seconds:.3f}s")
+    rays, res = 0, None
+    t_start = time.perf_counter()
+    for i in range(steps):
+        t = time.perf_counter()
+        res = ctx.render(sid, eid, w.camera, opt)
+        rays += res.extend_rays + res.shadow_rays
+        log(f"{w.name} step {i}: {time.perf_counter() - t:.3f}s")
+    elapsed = time.perf_counter() - t_start
+    sec_per_step = elapsed / steps
+    samples = w.camera_samples
+    value = samples / sec_per_step / 1e6
+
+    # Work per query from a render with the counting traversal kernels (same seed: the
+    # same queries), then per-kernel device time of the timed kernels (no counters) from
+    # HIP events on the library stream, kernels launched one by one.
+    ctx.set_work_counters(True)
+    cr_ = ctx.render(sid, eid, w.camera, opt)
+    ctx.set_work_counters(False)
+    ctx.set_profiling(True)
+    pr = ctx.render(sid, eid, w.camera, opt)
+    kt = ctx.kernel_times()
+    ctx.set_profiling(False)
+    assert (cr_.extend_rays, cr_.shadow_rays) == (pr.extend_rays, pr.shadow_rays)
+    # Algorithmic bytes per query as the survey defines them: 48 (ray in, hit out) +
+    # 128 per wide-node visit + 48 per triangle test.  The device's 4-wide nodes are
+    # 64-byte quantized records, so the bytes it actually needs per visit are half that
+    # figure (node_bytes below).
+    def traversal_line(kernel, rays_, nodes, tris):
+        launches, ms = kt[kernel]
+        n_node, n_tri = nodes / max(1, rays_), tris / max(1, rays_)
+        per = 48 + 128 * n_node + 48 * n_tri
+        ach = per * rays_ / (ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(ach / PEAK_HBM_GBS, 4), "kernel": kernel,
+                "avg_launch_ms": round(ms / max(1, launches), 4), "launches": launches,
+                "bytes_per_ray": round(per, 1), "nodes_per_ray": round(n_node, 2), "tris_per_ray": round(n_tri, 2),
+                "node_bytes": 64, "frac_at_node_bytes": round((per - 64 * n_node) * rays_ / (ms * 1e-3) / 1e9 /
+                                                              PEAK_HBM_GBS, 4)}
+    roofline = traversal_line("k_extend", pr.extend_rays, cr_.extend_nodes, cr_.extend_tris)
+    roofline["traffic"] = _traffic(w.name)
+    shadow_line = traversal_line("k_shadow", pr.shadow_rays, cr_.shadow_nodes, cr_.shadow_tris)
+    # Shading: per shaded path the ray record (64 B) and state record (128 B) read and
+    # written back, the primitive's traversal (48 B), shading (64 B) and normal (16 B)
+    # records, and a 72-byte shadow record written: 584 B (misses read fewer).
+    sh_launches, sh_ms = kt["k_shade"]
+    sh_ach = 584 * pr.extend_rays / (sh_ms * 1e-3) / 1e9
+    shade_line = {"bound": "valu (integer binary64 emulation), shown against hbm", "achieved": round(sh_ach, 1),
+                  "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(sh_ach / PEAK_HBM_GBS, 4), "kernel": "k_shade",
+                  "avg_launch_ms": round(sh_ms / max(1, sh_launches), 4), "bytes_per_path": 584}
+    n_node = cr_.extend_nodes / max(1, cr_.extend_rays)
+    n_tri = cr_.extend_tris / max(1, cr_.extend_rays)
+    ext_ms = kt["k_extend"][1]
+    # The same kernel against the FP32 vector roofline: 12 FLOP per child slab test of a
+    # 4-wide node and 40 per FP32 triangle test (6 FMA per child slab, the watertight test's operations).
+    flop_per_ray = 12 * 4 * n_node + 40 * n_tri
+    tflops = flop_per_ray * pr.extend_rays / (ext_ms * 1e-3) / 1e12
+    valu = {"achieved": round(tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tflops / PEAK_FP32_TFLOPS, 5), "flop_per_ray": round(flop_per_ray, 1), "kernel": "k_extend"}
+    kernels = {k: {"launches": v[0], "ms": round(v[1], 2)} for k, v in kt.items()}
+    log(f"{w.name} kernels: {kernels}")
+    cpu = None
+    if cpu_budget is not None:
+        cpu = cpu_baseline(ctx, sid, eid, w, cpu_budget, len(os.sched_getaffinity(0)))
+    qual2 = quality_config2(ctx, sid, eid, w)
+    if qual2:
+        log(f"{w.name} quality: {qual2}")
+    ctx.release_scene(sid)
+    ctx.release_env(eid)
+    return {"value": round(value, 3), "ms_per_step": round(sec_per_step * 1e3, 2), "steps": steps,
+            "mrays_per_s": round(rays / elapsed / 1e6, 2),
+            "rays_per_sample": round((res.extend_rays + res.shadow_rays) / samples, 3),
+            "exact_fallbacks_per_mray": round(pr.exact_fallbacks / max(1, pr.extend_rays + pr.shadow_rays) * 1e6, 2),
+            "exact_rays_per_mray": round(pr.exact_rays / max(1, pr.extend_rays + pr.shadow_rays) * 1e6, 2),
+            "roofline": roofline, "roofline_k_shadow": shadow_line, "roofline_k_shade": shade_line,
+            "valu_roofline": valu, "kernels_ms": kernels, "cpu_baseline": cpu, "quality": qual2,
+            "config": {"workload": w.name, "triangles": int(info.triangles), "width": w.width, "height": w.height,
+                       "spp": w.samples, "bounces": w.bounces, "seed": w.seed, "env": "synthetic 1024x512 HDR sky",
+                       "bvh_width": ctx.get_tuning().bvh_width},
+            "_info": info, "_result": res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -430,6 +588,8 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline rendering")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-config3", dest="config3", action="store_false",
+                    help="skip the config-3 (8M-triangle divergence stress) block")
     ap.add_argument("--forms", default="plain,batched,served", help="comma list of plain,batched,served")
     ap.add_argument("--batch", type=int, default=64, help="requests in the batched mix")
     ap.add_argument("--served", type=int, default=200, help="requests in the served stream")
@@ -453,62 +613,12 @@ def main():
     w = workload(args.workload)
     log(f"workload {w.name}: {w.scene.triangle_count} tris generated in {time.time() - t0:.1f}s")
     ctx = crgpu.Context(0)
-    sid, info = ctx.upload_scene(w.scene)
-    eid, env_bytes = ctx.upload_env(w.env)
-    log(f"scene resident: {info.triangles} tris, {info.wide_nodes} wide nodes, "
-        f"{(info.device_bytes + env_bytes) / 2**20:.1f} MiB, host build {info.build_seconds:.2f}s, "
-        f"upload {info.upload_seconds:.2f}s")
-    opt = w.options()
-    for i in range(args.warmup):
-        r = ctx.render(sid, eid, w.camera, opt)
-        log(f"warmup {i}: {r.seconds:.3f}s")
-    times, rays, res = [], 0, None
-    t_start = time.perf_counter()
-    for i in range(args.steps):
-        t = time.perf_counter()
-        res = ctx.render(sid, eid, w.camera, opt)
-        times.append(time.perf_counter() - t)
-        rays += res.extend_rays + res.shadow_rays
-        log(f"step {i}: {times[-1]:.3f}s")
-    elapsed = time.perf_counter() - t_start
-    sec_per_step = elapsed / args.steps
-    samples = w.camera_samples
-    value = samples / sec_per_step / 1e6
+    plain = measure_plain(ctx, w, args.steps, args.warmup, None if args.no_cpu else args.cpu_budget)
+    info, res = plain.pop("_info"), plain.pop("_result")
+    value, sec_per_step = plain["value"], plain["ms_per_step"] / 1e3
 
-    # Work per query from a render with the counting traversal kernels (same seed: the
-    # same queries), then per-kernel device time of the timed kernels (no counters) from
-    # HIP events on the library stream, kernels launched one by one.
-    ctx.set_work_counters(True)
-    cr_ = ctx.render(sid, eid, w.camera, opt)
-    ctx.set_work_counters(False)
-    ctx.set_profiling(True)
-    pr = ctx.render(sid, eid, w.camera, opt)
-    kt = ctx.kernel_times()
-    ctx.set_profiling(False)
-    assert (cr_.extend_rays, cr_.shadow_rays) == (pr.extend_rays, pr.shadow_rays)
-    ext_launches, ext_ms = kt["k_extend"]
-    n_node = cr_.extend_nodes / max(1, cr_.extend_rays)
-    n_tri = cr_.extend_tris / max(1, cr_.extend_rays)
-    bytes_per_ray = 48 + 128 * n_node + 48 * n_tri
-    achieved = bytes_per_ray * pr.extend_rays / (ext_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": _traffic(w.name),
-                "kernel": "k_extend", "avg_launch_ms": round(ext_ms / max(1, ext_launches), 4),
-                "launches": ext_launches, "bytes_per_ray": round(bytes_per_ray, 1),
-                "nodes_per_ray": round(n_node, 2), "tris_per_ray": round(n_tri, 2)}
-    # The same kernel against the FP32 vector roofline: 12 FLOP per child slab test of a
-    # 4-wide node and 40 per FP32 triangle test (6 FMA per child slab, the watertight test's operations).
-    flop_per_ray = 12 * 4 * n_node + 40 * n_tri
-    tflops = flop_per_ray * pr.extend_rays / (ext_ms * 1e-3) / 1e12
-    valu = {"achieved": round(tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(tflops / PEAK_FP32_TFLOPS, 5), "flop_per_ray": round(flop_per_ray, 1), "kernel": "k_extend"}
-    kernels = {k: {"launches": v[0], "ms": round(v[1], 2)} for k, v in kt.items()}
-    log(f"kernels: {kernels}")
-    ctx.release_scene(sid)
-    ctx.release_env(eid)
-
-    qual = quality(ctx)
-    log(f"quality: {qual}")
+    qual = {"config2": plain.pop("quality"), "config1": quality(ctx)}
+    log(f"quality config1: {qual['config1']}")
 
     batched = served = None
     if "batched" in forms or "served" in forms:
@@ -527,30 +637,40 @@ def main():
     asset = real_asset_form(ctx, not args.no_cpu)
     log(f"real asset: {asset}")
 
-    cpu = None
+    cpu = plain.get("cpu_baseline")
     host_build = {"triangles": int(info.triangles), "seconds": round(info.build_seconds, 3),
-                  "what": "finalize + 4-wide SAH build + device-layout flattening on all host threads"}
-    if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0))
-        cpu = cpu_baseline(w, args.cpu_budget, threads)
+                  "what": "finalize + 4-wide SAH build + quantized 4- and 8-wide device trees on all host threads"}
+    if cpu:
         ref_s = cpu.pop("scene_seconds")
         host_build.update({"reference_seconds": ref_s,
                            "reference_what": f"the CPU baseline's Scene::finalize + Bvh::build ({cpu['kind']})",
                            "speedup": round(ref_s / max(info.build_seconds, 1e-9), 2)})
+    config3 = None
+    if args.config3 and args.workload != "c3":
+        t0 = time.time()
+        w3 = workload("c3")
+        log(f"workload {w3.name}: {w3.scene.triangle_count} tris generated in {time.time() - t0:.1f}s")
+        config3 = measure_plain(ctx, w3, min(args.steps, 3), 1, None if args.no_cpu else args.cpu_budget)
+        config3.pop("_result")
+        config3.pop("_info")
+        if config3["cpu_baseline"]:
+            config3["cpu_baseline"].pop("scene_seconds")
+        log(f"config3: {config3}")
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(sec_per_step * 1e3, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": w.name, "triangles": int(info.triangles), "width": w.width, "height": w.height,
-                   "spp": w.samples, "bounces": w.bounces, "seed": w.seed, "env": "synthetic 1024x512 HDR sky"},
-        "mrays_per_s": round(rays / elapsed / 1e6, 2),
-        "rays_per_sample": round((res.extend_rays + res.shadow_rays) / samples, 3),
-        "exact_fallbacks_per_mray": round(pr.exact_fallbacks / max(1, pr.extend_rays + pr.shadow_rays) * 1e6, 2),
-        "exact_rays_per_mray": round(pr.exact_rays / max(1, pr.extend_rays + pr.shadow_rays) * 1e6, 2),
-        "roofline": roofline,
-        "valu_roofline": valu,
-        "kernels_ms": kernels,
+        "config": plain["config"],
+        "mrays_per_s": plain["mrays_per_s"],
+        "rays_per_sample": plain["rays_per_sample"],
+        "exact_fallbacks_per_mray": plain["exact_fallbacks_per_mray"],
+        "exact_rays_per_mray": plain["exact_rays_per_mray"],
+        "roofline": plain["roofline"],
+        "roofline_k_shadow": plain["roofline_k_shadow"],
+        "roofline_k_shade": plain["roofline_k_shade"],
+        "valu_roofline": plain["valu_roofline"],
+        "kernels_ms": plain["kernels_ms"],
         "quality": qual,
         "batched": batched,
         "served": served,
@@ -560,6 +680,7 @@ def main():
         "film_encode": film_enc,
         "scene_cache": cache,
         "real_asset_harbor": asset,
+        "config3": config3,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -76,6 +76,8 @@ def load_library(path: Optional[pathlib.Path] = None) -> C.CDLL:
         "crg_render": (i32, [vp, u64, u64, C.POINTER(abi.Camera), C.POINTER(abi.Options), C.POINTER(C.c_float),
                              C.POINTER(abi.Stats)]),
         "crg_trace_rays": (i32, [vp, u64, vp, u64, vp, i32]),
+        "crg_render_window": (i32, [vp, u64, u64, C.POINTER(abi.Camera), C.POINTER(abi.Options), i32, i32, i32, i32,
+                                    C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(abi.Stats)]),
         "crg_trace_rays_noalloc": (i32, [vp, u64, vp, u64, vp, i32]),
         "crg_render_batch": (i32, [vp, C.POINTER(abi.Request), C.c_uint32, C.POINTER(i32), C.POINTER(abi.Stats)]),
         "crg_submit": (i32, [vp, C.POINTER(abi.Request), C.POINTER(u64)]),
@@ -190,6 +192,22 @@ class Context:
         self._check(self.lib.crg_render(self.handle, scene_id, env_id, C.byref(camera), C.byref(options),
                                         film.ctypes.data_as(C.POINTER(C.c_float)), C.byref(stats)))
         return RenderResult.from_stats(film, stats)
+
+    def render_window(self, scene_id: int, env_id: int, camera: abi.Camera, options: abi.Options, window,
+                      with_samples: bool = False):
+        """crg_render_window: the pixels of window (x0, y0, w, h) of the frame with their
+        full-frame sample streams; with_samples also returns every sample's radiance
+        ([spp, h, w, 3]).  Returns the RenderResult (and the samples)."""
+        x0, y0, w, h = window
+        film = np.zeros((h, w, 3), np.float32)
+        smp = np.zeros((options.samples, h, w, 3), np.float32) if with_samples else None
+        stats = abi.Stats()
+        self._check(self.lib.crg_render_window(self.handle, scene_id, env_id, C.byref(camera), C.byref(options),
+                                               x0, y0, w, h, film.ctypes.data_as(C.POINTER(C.c_float)),
+                                               smp.ctypes.data_as(C.POINTER(C.c_float)) if smp is not None else None,
+                                               C.byref(stats)))
+        r = RenderResult.from_stats(film, stats)
+        return (r, smp) if with_samples else r
 
     # ---- batched and served forms -------------------------------------------------
     @staticmethod

@@ -135,6 +135,10 @@ Request make_request(crg_ctx* ctx, const crg_request& r) {
     q.camera = prepare_camera(r.camera, r.options.width, r.options.height);
     q.lights = build_light_table(q.scene->lights, q.scene->tri_to_prim, q.env->integral);
     q.film = r.film_rgb;
+    q.win[0] = 0;
+    q.win[1] = 0;
+    q.win[2] = r.options.width;
+    q.win[3] = r.options.height;
     return q;
 }
 
@@ -849,6 +853,35 @@ int crg_env_release(crg_ctx* ctx, uint64_t env_id) {
         ctx->engine->add_resident(-int64_t(it->second->bytes));
         ctx->envs.erase(it);
     });
+}
+
+int crg_render_window(crg_ctx* ctx, uint64_t scene_id, uint64_t env_id, const crg_camera* camera,
+                      const crg_options* options, int32_t x0, int32_t y0, int32_t width, int32_t height, float* film_rgb,
+                      float* samples_rgb, crg_stats* stats) {
+    int rc = CRG_OK;
+    const int g = guarded(ctx, [&] {
+        if (!camera || !options) throw std::invalid_argument("null render argument");
+        validate_options(*options);
+        if (!film_rgb) throw std::invalid_argument("null film buffer");
+        if (x0 < 0 || y0 < 0 || width < 1 || height < 1 || int64_t(x0) + width > options->width ||
+            int64_t(y0) + height > options->height)
+            throw std::invalid_argument("Render window must lie inside the image");
+        crg_request r{};
+        r.scene = scene_id;
+        r.env = env_id;
+        r.camera = *camera;
+        r.options = *options;
+        r.film_rgb = film_rgb;
+        Request q = make_request(ctx, r);
+        q.win[0] = x0;
+        q.win[1] = y0;
+        q.win[2] = width;
+        q.win[3] = height;
+        q.samples = samples_rgb;
+        const uint64_t t = ctx->engine->submit(std::move(q));
+        rc = status_of_wait(*ctx->engine, t, -1.0, stats);
+    });
+    return g != CRG_OK ? g : rc;
 }
 
 int crg_trace_rays(crg_ctx* ctx, uint64_t scene_id, const crg_ray* rays, uint64_t count, crg_hit* hits, int any) {

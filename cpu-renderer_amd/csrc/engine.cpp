@@ -141,14 +141,14 @@ Engine::~Engine() {
 }
 
 uint64_t Engine::footprint(const Request& r) {
-    const uint64_t npix = uint64_t(r.options.width) * uint64_t(r.options.height);
+    const uint64_t npix = uint64_t(r.win[2]) * uint64_t(r.win[3]);
     return npix * uint64_t(r.options.samples) * sizeof(float4) + npix * 3 * sizeof(float) +
            std::max<size_t>(r.lights.entries.size(), 1) * sizeof(DevLightEntry);
 }
 
 uint64_t Engine::submit(Request r) {
     auto j = std::make_shared<Job>();
-    j->npix = uint64_t(r.options.width) * uint64_t(r.options.height);
+    j->npix = uint64_t(r.win[2]) * uint64_t(r.win[3]);
     j->items = j->npix * uint64_t(r.options.samples);
     j->bytes = footprint(r);
     j->req = std::move(r);
@@ -370,6 +370,10 @@ void Engine::admit_locked() {
         d.samples = static_cast<float4*>(j.d_samples);
         d.item_base = total_items_;
         d.items = j.items;
+        d.win_x = j.req.win[0];
+        d.win_y = j.req.win[1];
+        d.win_w = j.req.win[2];
+        d.win_h = j.req.win[3];
         if (!j.req.lights.entries.empty())
             check(hipMemcpyAsync(j.d_lights, j.req.lights.entries.data(), j.req.lights.entries.size() * sizeof(DevLightEntry),
                                  hipMemcpyHostToDevice, st_),
@@ -518,6 +522,15 @@ void Engine::finalize_ready_locked(bool block) {
         j.t_done = Clock::now();
         if (j.status == CRG_OK) {
             std::memcpy(j.req.film, j.h_film, j.npix * 3 * sizeof(float));
+            if (j.req.samples) {  // each sample's radiance, for callers comparing samples one by one
+                std::vector<float4> v(j.items);
+                check(hipMemcpy(v.data(), j.d_samples, j.items * sizeof(float4), hipMemcpyDeviceToHost), "samples read");
+                for (uint64_t i = 0; i < j.items; ++i) {
+                    j.req.samples[3 * i] = v[i].x;
+                    j.req.samples[3 * i + 1] = v[i].y;
+                    j.req.samples[3 * i + 2] = v[i].z;
+                }
+            }
             if (j.has_err) {
                 j.status = CRG_EPATH;
                 j.error = format_path_error(j.err, j.req.options.width, j.req.options.samples);

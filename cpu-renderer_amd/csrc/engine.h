@@ -79,7 +79,9 @@ struct Request {
     crg_options options{};
     DevCamera camera{};
     LightTable lights;
-    float* film = nullptr;
+    float* film = nullptr;     // win_w * win_h * 3 floats
+    int32_t win[4] = {0, 0, 0, 0};  // x, y, w, h of the frame window rendered (the whole frame by default)
+    float* samples = nullptr;  // optional: spp * win_w * win_h * 3 floats, each sample's radiance
 };
 
 struct Job;

@@ -183,9 +183,11 @@ __device__ inline void store_state(const PathSoA& P, uint32_t slot, const PathSt
 // Camera sample `local` of job `ji` into `slot`; false if the camera failed.
 __device__ inline bool init_path(const PathSoA& P, uint32_t slot, const DevJob& J, uint32_t ji, uint64_t local,
                                  bool precise, Control* ctl) {
-    const uint64_t npix = uint64_t(J.width) * uint64_t(J.height);
+    const uint64_t npix = uint64_t(J.win_w) * uint64_t(J.win_h);
     const uint32_t sample = uint32_t(local / npix);
-    const uint32_t pixel = uint32_t(local - uint64_t(sample) * npix);
+    const uint32_t lp = uint32_t(local - uint64_t(sample) * npix);  // window pixel
+    const uint32_t ly = lp / uint32_t(J.win_w), lx = lp - ly * uint32_t(J.win_w);
+    const uint32_t pixel = (uint32_t(J.win_y) + ly) * uint32_t(J.width) + uint32_t(J.win_x) + lx;
     PathState st;
     const bool ok = begin_sample(J, pixel, sample, precise, st);
     if (!ok) {  // retired by the next shade with zero radiance
@@ -669,13 +671,15 @@ __global__ void __launch_bounds__(256) k_retire(PathSoA P, const DevJob* jobs, c
             const PathRec& c = P.rec[path];
             const V3 l = xyz(c.rad);
             const uint32_t pixel = c.pixel, sample = c.sample;
-            const uint64_t npix = uint64_t(J.width) * uint64_t(J.height);
+            const uint64_t npix = uint64_t(J.win_w) * uint64_t(J.win_h);
+            const uint32_t gy = pixel / uint32_t(J.width), gx = pixel - gy * uint32_t(J.width);
+            const uint64_t lp = uint64_t(gy - uint32_t(J.win_y)) * uint64_t(J.win_w) + (gx - uint32_t(J.win_x));
             if (!radiance_ok(l)) {
                 report_error(ctl, ji, uint64_t(pixel) * uint64_t(J.spp) + sample, kErrBadRadiance, 0, 0, 0,
                              sf::zero(0), V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0});
             }
-            const uint64_t at = uint64_t(sample) * npix + pixel;
-            if (at < J.items) J.samples[at] = make_float4(l.x, l.y, l.z, 0);
+            const uint64_t at = uint64_t(sample) * npix + lp;
+            if (lp < npix && at < J.items) J.samples[at] = make_float4(l.x, l.y, l.z, 0);
         }
         wave_add_keyed(cnt->retired, ji, live ? 1u : 0u);
         if (live) {
@@ -776,7 +780,7 @@ __global__ void __launch_bounds__(256) k_pool_init(uint32_t* freeq, uint32_t poo
 // film = float(sum_s L_s / spp) with the sum in binary64, samples in order.
 __global__ void __launch_bounds__(256) k_accumulate(const DevJob* jobs, uint32_t job, float* film) {
     const DevJob& J = jobs[job];
-    const uint64_t npix = uint64_t(J.width) * uint64_t(J.height);
+    const uint64_t npix = uint64_t(J.win_w) * uint64_t(J.win_h);
     const uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= npix) return;
     sf::f64 r = sf::zero(0), g = sf::zero(0), b = sf::zero(0);

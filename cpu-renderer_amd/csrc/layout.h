@@ -253,9 +253,13 @@ struct DevJob {
     int32_t width, height, spp, bounces;
     uint64_t seed;
     int32_t nee, rr;
-    float4* samples;     // [sample][pixel] radiance (xyz)
+    float4* samples;     // [sample][window pixel] radiance (xyz)
     uint64_t item_base;  // first global work item of this job
-    uint64_t items;      // width*height*spp
+    uint64_t items;      // win_w*win_h*spp
+    // Window of the frame this job renders (the whole frame: 0, 0, width,
+    // height); pixels keep their frame index, so their sample streams are the
+    // full frame's.
+    int32_t win_x, win_y, win_w, win_h;
 };
 
 enum PathFlag : uint32_t {

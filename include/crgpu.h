@@ -230,6 +230,18 @@ int crg_env_release(crg_ctx* ctx, uint64_t env_id);
 int crg_render(crg_ctx* ctx, uint64_t scene_id, uint64_t env_id, const crg_camera* camera,
                const crg_options* options, float* film_rgb, crg_stats* stats);
 
+/* render() over a window of the frame: the pixels x0 <= x < x0+width,
+ * y0 <= y < y0+height of the options' width x height image, each with its
+ * frame index and hence exactly the sample stream, radiance and film value it
+ * has in the full render (reference src/render.cpp:384-415).  film_rgb
+ * receives width*height*3 floats; samples_rgb (may be NULL) receives every
+ * sample's radiance, spp*width*height*3 floats ordered [sample][pixel].  No
+ * counterpart in the reference: crops of large frames at high sample counts
+ * (converged references, quality measurement) and sample-by-sample comparison. */
+int crg_render_window(crg_ctx* ctx, uint64_t scene_id, uint64_t env_id, const crg_camera* camera,
+                      const crg_options* options, int32_t x0, int32_t y0, int32_t width, int32_t height,
+                      float* film_rgb, float* samples_rgb, crg_stats* stats);
+
 /* Bvh::intersect over a host ray array (reference src/geometry.cpp:554-639).
  * any != 0 selects visibility queries (reference src/geometry.cpp:608). */
 int crg_trace_rays(crg_ctx* ctx, uint64_t scene_id, const crg_ray* rays, uint64_t count,

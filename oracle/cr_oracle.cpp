@@ -1215,10 +1215,19 @@ void validate(const crg_options& o) {
     if (o.threads < 0 || o.threads > 256) throw std::runtime_error("Threads must be 0 (automatic) or in [1, 256]");
 }
 
-RenderOut render(const Scene& sc, const Bvh& bvh, const Cam& cam, const Env& env, const crg_options& o, float* film) {
+// A window (wx, wy, ww, wh) of the frame renders only those pixels, each with
+// its global index and hence its full-frame sample stream; film (and, when
+// given, samples: [sample][window pixel][rgb], the radiance of every sample in
+// order) is indexed by window pixel.  The full frame is the window (0, 0, W, H).
+RenderOut render(const Scene& sc, const Bvh& bvh, const Cam& cam, const Env& env, const crg_options& o, float* film,
+                 int wx = 0, int wy = 0, int ww = -1, int wh = -1, float* samples = nullptr) {
     validate(o);
+    if (ww < 0) ww = o.width;
+    if (wh < 0) wh = o.height;
+    if (wx < 0 || wy < 0 || ww < 1 || wh < 1 || wx + ww > o.width || wy + wh > o.height)
+        throw std::runtime_error("Render window must lie inside the image");
     RenderOut out;
-    const int tile = 16, tx = (o.width + tile - 1) / tile, ty = (o.height + tile - 1) / tile, tiles = tx * ty;
+    const int tile = 16, tx = (ww + tile - 1) / tile, ty = (wh + tile - 1) / tile, tiles = tx * ty;
     out.workers = std::min(tiles, o.threads ? o.threads : int(std::max(1u, std::thread::hardware_concurrency() / 2)));
     const Lights lights(sc, env);
     std::atomic<int> next{0};
@@ -1234,13 +1243,14 @@ RenderOut render(const Scene& sc, const Bvh& bvh, const Cam& cam, const Env& env
             for (;;) {
                 const int t = next.fetch_add(1);
                 if (t >= tiles || stop.load()) break;
-                const int x0 = (t % tx) * tile, y0 = (t / tx) * tile;
-                const int x1 = std::min(x0 + tile, o.width), y1 = std::min(y0 + tile, o.height);
+                const int x0 = wx + (t % tx) * tile, y0 = wy + (t / tx) * tile;
+                const int x1 = std::min(x0 + tile, wx + ww), y1 = std::min(y0 + tile, wy + wh);
                 for (int y = y0; y < y1; ++y)
                     for (int x = x0; x < x1; ++x) {
                         cx = x;
                         cy = y;
                         const uint64_t pixel = uint64_t(y) * o.width + x;
+                        const uint64_t local = uint64_t(y - wy) * ww + uint64_t(x - wx);
                         double r = 0, gsum = 0, b = 0;
                         for (int smp = 0; smp < o.samples; ++smp) {
                             cs = smp;
@@ -1259,10 +1269,16 @@ RenderOut render(const Scene& sc, const Bvh& bvh, const Cam& cam, const Env& env
                             r += l.x;
                             gsum += l.y;
                             b += l.z;
+                            if (samples) {
+                                float* q = samples + 3 * (uint64_t(smp) * uint64_t(ww) * uint64_t(wh) + local);
+                                q[0] = l.x;
+                                q[1] = l.y;
+                                q[2] = l.z;
+                            }
                         }
-                        film[3 * pixel] = float(r / o.samples);
-                        film[3 * pixel + 1] = float(gsum / o.samples);
-                        film[3 * pixel + 2] = float(b / o.samples);
+                        film[3 * local] = float(r / o.samples);
+                        film[3 * local + 1] = float(gsum / o.samples);
+                        film[3 * local + 2] = float(b / o.samples);
                     }
             }
         } catch (const std::exception& e) {
@@ -1638,6 +1654,23 @@ int orc_render(void* scene, void* env, const crg_camera* c, const crg_options* o
             std::memset(st, 0, sizeof(*st));
             st->seconds = r.seconds;
             st->camera_samples = uint64_t(o->width) * o->height * o->samples;
+            st->boundary_retries = r.retries;
+            st->extend_rays = r.extend;
+            st->shadow_rays = r.shadow;
+            st->workers = r.workers;
+        }
+    });
+}
+
+int orc_render_window(void* scene, void* env, const crg_camera* c, const crg_options* o, int x0, int y0, int w, int h,
+                      float* film, float* samples, crg_stats* st) {
+    const SceneBox& b = *static_cast<SceneBox*>(scene);
+    return guard([&] {
+        const RenderOut r = render(b.s, b.bvh, camOf(*c), *static_cast<Env*>(env), *o, film, x0, y0, w, h, samples);
+        if (st) {
+            std::memset(st, 0, sizeof(*st));
+            st->seconds = r.seconds;
+            st->camera_samples = uint64_t(w) * uint64_t(h) * uint64_t(o->samples);
             st->boundary_retries = r.retries;
             st->extend_rays = r.extend;
             st->shadow_rays = r.shadow;

@@ -64,6 +64,12 @@ typedef struct probe_bsdf_sample {
 PROBE_DECLARE(ref)
 PROBE_DECLARE(orc)
 
+/* Oracle only: render() over a window (x0, y0, w, h) of the frame with global
+ * pixel indices (the full-frame sample streams); film is w*h RGB, samples
+ * (NULL or spp*w*h RGB, [sample][window pixel]) the radiance of every sample. */
+int orc_render_window(void* scene, void* env, const crg_camera* camera, const crg_options* options, int x0, int y0,
+                      int w, int h, float* film, float* samples, crg_stats* stats);
+
 /* Reference only: io::writePng and io::writeHdr to the given paths (timing). */
 int ref_film_write(const float* film, int width, int height, float exposure, const char* png_path,
                    const char* hdr_path);

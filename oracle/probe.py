@@ -87,6 +87,9 @@ class Probe:
             "film_png8": (C.c_int, [fp, C.c_int, C.c_int, f, C.POINTER(C.c_uint8)]),
             "film_hdr": (C.c_int, [fp, C.c_int, C.c_int, C.POINTER(C.c_uint8), u64, C.POINTER(u64)]),
         }
+        if kind == "orc":
+            sig["render_window"] = (C.c_int, [vp, vp, C.POINTER(abi.Camera), C.POINTER(abi.Options), C.c_int, C.c_int,
+                                              C.c_int, C.c_int, fp, fp, C.POINTER(abi.Stats)])
         if kind == "ref":
             sig["film_write"] = (C.c_int, [fp, C.c_int, C.c_int, f, C.c_char_p, C.c_char_p])
             sig["gltf_load"] = (vp, [C.c_char_p])
@@ -215,6 +218,21 @@ class ProbeScene:
         if rc:
             raise RuntimeError(self.p.error())
         return s
+
+    def render_window(self, env: "ProbeEnv", camera, options, window, with_samples=False):
+        """Oracle only: the pixels of window (x0, y0, w, h) of the frame, each with its
+        full-frame sample stream; also every sample's radiance ([spp, h, w, 3]) on request."""
+        x0, y0, w, h = window
+        film = np.zeros((h, w, 3), np.float32)
+        smp = np.zeros((options.samples, h, w, 3), np.float32) if with_samples else None
+        stats = abi.Stats()
+        rc = self.p._render_window(self.h, env.h, C.byref(camera), C.byref(options), x0, y0, w, h,
+                                   film.ctypes.data_as(C.POINTER(C.c_float)),
+                                   smp.ctypes.data_as(C.POINTER(C.c_float)) if smp is not None else None,
+                                   C.byref(stats))
+        if rc:
+            raise RuntimeError(self.p.error())
+        return (film, stats, smp) if with_samples else (film, stats)
 
     def render(self, env: "ProbeEnv", camera, options):
         film = np.zeros((options.height, options.width, 3), np.float32)

@@ -27,34 +27,18 @@ import pytest
 import ref_scenes as rs
 from conftest import ROOT, abi, load_golden
 from parity import assert_parity
+from scene_files import write_scene_file
 
 pytestmark = pytest.mark.gpu
 
 EXE = ROOT / "oracle" / "_ref" / "adapter_check"
 
 
-def _write_scene(path: pathlib.Path, scene: abi.SceneData, env: abi.EnvData, cam: abi.Camera, opt: abi.Options):
-    n = scene.triangle_count
-    normals = scene.normals if scene.normals is not None else np.zeros((n, 9), np.float32)
-    with open(path, "wb") as f:
-        f.write(np.array([n, len(scene.materials), len(scene.lights)], np.uint64).tobytes())
-        f.write(np.ascontiguousarray(scene.positions, np.float32).tobytes())
-        f.write(np.ascontiguousarray(normals, np.float32).tobytes())
-        f.write(np.ascontiguousarray(scene.material_ids, np.uint32).tobytes())
-        for m in scene.materials:
-            f.write(bytes(m))
-        for l in scene.lights:
-            f.write(bytes(l))
-        f.write(bytes(cam))
-        f.write(np.array(env.constant, np.float32).tobytes())
-        f.write(bytes(opt))
-
-
 def _run(tmp_path, scene, env, cam, opt):
     if not EXE.exists():
         pytest.fail(f"{EXE} is not built (make -C oracle ref where the reference tree exists)")
     src, film = tmp_path / "scene.bin", tmp_path / "film.bin"
-    _write_scene(src, scene, env, cam, opt)
+    write_scene_file(src, scene, env, cam, opt)
     p = subprocess.run([str(EXE), str(src), str(film)], capture_output=True, text=True, timeout=240)
     print(p.stdout)
     assert p.returncode == 0, p.stdout + p.stderr

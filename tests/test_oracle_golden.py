@@ -153,3 +153,32 @@ def test_render_batch_requests(oracle, name):
     film, stats = oracle.scene(w.scene).render(oracle.env(w.env), w.camera, opt)
     np.testing.assert_array_equal(film.view(np.uint32), g[f"{name}/film"].view(np.uint32))
     assert stats.boundary_retries == int(g[f"{name}/retries"])
+
+
+WINDOW_CASES = ("c2_small", "c3_small", "tex_factors", "fixture4", "plane_1")
+
+
+@pytest.mark.parametrize("name", WINDOW_CASES)
+def test_render_window_matches_reference_crops(oracle, name):
+    """The oracle's window render (the pixels of a window of the frame, each with its frame
+    index, so its full-frame sample stream) equals the same crop of the reference's own
+    full-frame film bit for bit, at several windows including the frame's edges; the
+    per-sample radiance it reports sums, in sample order in binary64, to the film."""
+    g = load_golden("render.npz")
+    _, scene, env, cam, opt, crop = next(c for c in make_golden.render_cases() if c[0] == name)
+    ref = g[f"{name}/film"]
+    ps, pe = oracle.scene(scene), oracle.env(env)
+    W, H = opt.width, opt.height
+    for win in ((0, 0, W, H), (W // 3, H // 4, max(1, W // 2), max(1, H // 2)), (W - 1, H - 1, 1, 1),
+                (0, H // 2, W, H - H // 2)):
+        x0, y0, w, h = win
+        film, st, smp = ps.render_window(pe, cam, opt, win, with_samples=True)
+        np.testing.assert_array_equal(film.view(np.uint32), ref[y0:y0 + h, x0:x0 + w].view(np.uint32),
+                                      err_msg=f"{name} window {win}")
+        assert st.camera_samples == w * h * opt.samples
+        acc = np.zeros((h, w, 3), np.float64)
+        for s in range(opt.samples):
+            acc += smp[s]
+        np.testing.assert_array_equal((acc / opt.samples).astype(np.float32).view(np.uint32), film.view(np.uint32))
+    with pytest.raises(RuntimeError, match="Render window must lie inside the image"):
+        ps.render_window(pe, cam, opt, (W - 1, 0, 2, 1))
