@@ -76,6 +76,7 @@ def load_library(path: Optional[pathlib.Path] = None) -> C.CDLL:
         "crg_render": (i32, [vp, u64, u64, C.POINTER(abi.Camera), C.POINTER(abi.Options), C.POINTER(C.c_float),
                              C.POINTER(abi.Stats)]),
         "crg_trace_rays": (i32, [vp, u64, vp, u64, vp, i32]),
+        "crg_device_probe": (i32, [vp, i32, u64, u64, C.POINTER(abi.Camera), i32, i32, vp, u64, vp]),
         "crg_render_window": (i32, [vp, u64, u64, C.POINTER(abi.Camera), C.POINTER(abi.Options), i32, i32, i32, i32,
                                     C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(abi.Stats)]),
         "crg_trace_rays_noalloc": (i32, [vp, u64, vp, u64, vp, i32]),
@@ -344,6 +345,18 @@ class Context:
         self._check(self.lib.crg_trace_rays(self.handle, scene_id, rays.ctypes.data, len(rays), hits.ctypes.data,
                                             int(bool(any_hit))))
         return hits
+
+    def probe(self, what: int, records: np.ndarray, out_bytes: int, scene_id: int = 0, env_id: int = 0,
+              camera: Optional[abi.Camera] = None, width: int = 0, height: int = 0) -> np.ndarray:
+        """crg_device_probe: one device function over `records` (a contiguous array of the
+        function's input records); returns the output records as raw bytes (n, out_bytes)."""
+        records = np.ascontiguousarray(records)
+        n = len(records)
+        out = np.zeros((n, out_bytes), np.uint8)
+        self._check(self.lib.crg_device_probe(self.handle, what, scene_id, env_id,
+                                              C.byref(camera) if camera is not None else None, width, height,
+                                              records.ctypes.data, n, out.ctypes.data))
+        return out
 
     def trace_noalloc(self, scene_id: int, rays: np.ndarray, any_hit: bool = False) -> np.ndarray:
         """crg_trace_rays_noalloc: the same queries through the library's fixed

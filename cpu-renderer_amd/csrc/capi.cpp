@@ -884,6 +884,76 @@ int crg_render_window(crg_ctx* ctx, uint64_t scene_id, uint64_t env_id, const cr
     return g != CRG_OK ? g : rc;
 }
 
+int crg_device_probe(crg_ctx* ctx, int32_t what, uint64_t scene_id, uint64_t env_id, const crg_camera* camera,
+                     int32_t width, int32_t height, const void* in, uint64_t count, void* out) {
+    return guarded(ctx, [&] {
+        static const size_t in_size[8] = {2 * sizeof(float), sizeof(uint64_t), 3 * sizeof(float), 3 * sizeof(float),
+                                          sizeof(crg_probe_bsdf_in), sizeof(crg_probe_bsdf_in),
+                                          sizeof(crg_probe_offset_in), sizeof(crg_probe_surface_in)};
+        static const size_t out_size[8] = {sizeof(crg_probe_ray), sizeof(crg_probe_env_sample), sizeof(float),
+                                           3 * sizeof(float), sizeof(crg_probe_bsdf_sample), 4 * sizeof(float),
+                                           sizeof(crg_probe_ray), sizeof(crg_surface)};
+        if (what < 0 || what > CRG_PROBE_SURFACE_AT) throw std::invalid_argument("unknown probe function");
+        if (count && (!in || !out)) throw std::invalid_argument("null probe records");
+        ProbeArgs a{};
+        a.what = what;
+        a.n = count;
+        std::shared_ptr<SceneRes> scene;
+        std::shared_ptr<EnvRes> env;
+        {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            if (what == CRG_PROBE_SURFACE_AT) {
+                auto it = ctx->scenes.find(scene_id);
+                if (it == ctx->scenes.end()) throw std::out_of_range("unknown scene handle");
+                scene = it->second;
+            }
+            if (what >= CRG_PROBE_ENV_SAMPLE && what <= CRG_PROBE_ENV_RADIANCE) {
+                auto it = ctx->envs.find(env_id);
+                if (it == ctx->envs.end()) throw std::out_of_range("unknown environment handle");
+                env = it->second;
+            }
+        }
+        if (what == CRG_PROBE_CAMERA) {
+            if (!camera) throw std::invalid_argument("null camera");
+            a.camera = prepare_camera(*camera, width, height);
+        }
+        std::vector<float> ung;
+        std::vector<uint32_t> tri_to_prim_check;
+        if (what == CRG_PROBE_OFFSET_RAY) {
+            ung.resize(3 * count);
+            const auto* q = static_cast<const crg_probe_offset_in*>(in);
+            for (uint64_t i = 0; i < count; ++i) unit_vector(q[i].geometric_normal, &ung[3 * i]);
+        }
+        if (what == CRG_PROBE_SURFACE_AT) {
+            const auto* q = static_cast<const crg_probe_surface_in*>(in);
+            for (uint64_t i = 0; i < count; ++i)
+                if (q[i].hit.triangle >= scene->tri_to_prim.size()) throw std::invalid_argument("hit triangle outside the scene");
+        }
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        hipStream_t st = ctx->stream;
+        DevBuf din, dout, dmap, dung;
+        din.alloc(count * in_size[what]);
+        dout.alloc(count * out_size[what]);
+        if (count) check(hipMemcpyAsync(din.p, in, count * in_size[what], hipMemcpyHostToDevice, st), "copy");
+        if (scene) {
+            upload(dmap, scene->tri_to_prim, st);
+            a.scene = scene->desc.as<DevScene>();
+            a.tri_to_prim = dmap.as<uint32_t>();
+        }
+        if (env) a.env = env->desc.as<DevEnv>();
+        if (!ung.empty()) {
+            upload(dung, ung, st);
+            a.ung = dung.as<float>();
+        }
+        a.in = din.p;
+        a.out = dout.p;
+        check(launch_probe(a, st), "probe launch");
+        if (count) check(hipMemcpyAsync(out, dout.p, count * out_size[what], hipMemcpyDeviceToHost, st), "copy");
+        check(hipStreamSynchronize(st), "probe");
+    });
+}
+
 int crg_trace_rays(crg_ctx* ctx, uint64_t scene_id, const crg_ray* rays, uint64_t count, crg_hit* hits, int any) {
     return guarded(ctx, [&] {
         std::lock_guard<std::mutex> lock(ctx->mu);

@@ -782,6 +782,13 @@ struct Collapse8 {
 
 }  // namespace
 
+void unit_vector(const float* v, float* out) {
+    const HVec3 u = unit_of(DV{v[0], v[1], v[2]});
+    out[0] = u.x;
+    out[1] = u.y;
+    out[2] = u.z;
+}
+
 QNode4 quantize_boxes4(const float* lo, const float* hi, const uint32_t* refs, int n) {
     QChild kids[4];
     for (int c = 0; c < n && c < 4; ++c) {

@@ -59,6 +59,8 @@ struct WideBvh {
 WideBvh build_wide_bvh(const HostScene& s);
 // q4 and q8 of b.nodes.
 void quantize_trees(WideBvh& b);
+// unit() of the reference in binary64, rounded to float (reference src/geometry.cpp:31-37).
+void unit_vector(const float* v, float* out);
 // One quantized 4-wide node over n <= 4 child boxes (lo/hi: 3 floats per child).
 QNode4 quantize_boxes4(const float* lo, const float* hi, const uint32_t* refs, int n);
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "layout.h"
+#include "../../include/crgpu.h"
 
 namespace cr {
 
@@ -33,6 +34,20 @@ constexpr uint64_t kNoAllocScratchBytes = kNoAllocCtlBytes + kNoAllocMaxRays * (
 // Device address of that scratch on `device`, which must be the current
 // device on the first call for it (hipGetSymbolAddress once per device).
 hipError_t noalloc_scratch(int device, void** out);
+// Per-function probe (crg_device_probe): applies one device function of the
+// path to n input records (kinds and record layouts as in crgpu.h).
+struct ProbeArgs {
+    int what;
+    const DevScene* scene;
+    const DevEnv* env;
+    DevCamera camera;
+    const void* in;
+    void* out;
+    uint64_t n;
+    const uint32_t* tri_to_prim;  // surfaceAt: finalized triangle -> leaf-order primitive
+    const float* ung;             // offsetRay: unit(geometric normal) per record, from the host
+};
+hipError_t launch_probe(const ProbeArgs& a, hipStream_t st);
 // Zeroes n 32-bit words with a kernel (no memset node).
 hipError_t launch_clear_words(uint32_t* p, uint32_t n, hipStream_t st);
 // The kernels of one wavefront iteration, in launch order.

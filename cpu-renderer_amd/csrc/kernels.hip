@@ -809,6 +809,147 @@ hipError_t launch_trace_rays(const DevScene* scene, const RayIn* rays, uint64_t 
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ per-function probe
+__device__ inline Surf surf_of(const crg_surface& c) {
+    Surf s;
+    s.position = V3{c.position[0], c.position[1], c.position[2]};
+    s.position_low = V3{c.position_low[0], c.position_low[1], c.position_low[2]};
+    s.ng = V3{c.geometric_normal[0], c.geometric_normal[1], c.geometric_normal[2]};
+    s.normal = V3{c.normal[0], c.normal[1], c.normal[2]};
+    s.base = V3{c.base_color[0], c.base_color[1], c.base_color[2]};
+    s.emission = V3{c.emission[0], c.emission[1], c.emission[2]};
+    s.att_color = V3{c.attenuation_color[0], c.attenuation_color[1], c.attenuation_color[2]};
+    s.metallic = c.metallic;
+    s.roughness = c.roughness;
+    s.transmission = c.transmission;
+    s.ior = c.ior;
+    s.thickness = c.thickness;
+    s.att_distance = c.attenuation_distance;
+    s.material = c.material;
+    s.front = c.front_face != 0;
+    s.double_sided = c.double_sided != 0;
+    s.texture_error = false;
+    return s;
+}
+
+__device__ inline void put3(float* o, V3 v) {
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+}
+
+__device__ inline crg_probe_ray probe_ray(V3 o, V3 d, float tmin, float tmax, int32_t status) {
+    crg_probe_ray r{};
+    put3(r.ray.origin, o);
+    put3(r.ray.direction, d);
+    r.ray.tmin = tmin;
+    r.ray.tmax = tmax;
+    r.ray.ignore = 0xffffffffu;
+    r.status = status;
+    return r;
+}
+
+__global__ void __launch_bounds__(256) k_probe(ProbeArgs a) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    switch (a.what) {
+        case CRG_PROBE_CAMERA: {
+            const float* px = static_cast<const float*>(a.in) + 2 * i;
+            V3 o, d;
+            float tmin = 0, tmax = 0;
+            const bool ok = camera_ray(a.camera, px[0], px[1], o, d, tmin, tmax);
+            static_cast<crg_probe_ray*>(a.out)[i] = probe_ray(o, d, tmin, tmax, ok ? 0 : 1);
+            break;
+        }
+        case CRG_PROBE_ENV_SAMPLE: {
+            Rng rng{static_cast<const uint64_t*>(a.in)[i]};
+            crg_probe_env_sample r{};
+            put3(r.direction, env_sample(*a.env, rng, r.pdf));
+            r.rng_end = rng.state;
+            static_cast<crg_probe_env_sample*>(a.out)[i] = r;
+            break;
+        }
+        case CRG_PROBE_ENV_PDF: {
+            const float* d = static_cast<const float*>(a.in) + 3 * i;
+            static_cast<float*>(a.out)[i] = env_pdf(*a.env, V3{d[0], d[1], d[2]});
+            break;
+        }
+        case CRG_PROBE_ENV_RADIANCE: {
+            const float* d = static_cast<const float*>(a.in) + 3 * i;
+            put3(static_cast<float*>(a.out) + 3 * i, env_radiance(*a.env, V3{d[0], d[1], d[2]}));
+            break;
+        }
+        case CRG_PROBE_BSDF_SAMPLE: {
+            const crg_probe_bsdf_in& q = static_cast<const crg_probe_bsdf_in*>(a.in)[i];
+            Rng rng{q.rng};
+            const BsdfSample b = sample_bsdf(surf_of(q.surface), V3{q.wo[0], q.wo[1], q.wo[2]}, rng);
+            crg_probe_bsdf_sample r{};
+            put3(r.direction, b.direction);
+            put3(r.weight, b.weight);
+            r.pdf = b.pdf;
+            r.eta = b.eta;
+            r.delta = b.delta;
+            r.transmitted = b.transmitted;
+            r.valid = sample_ok(b);
+            r.rng_end = rng.state;
+            static_cast<crg_probe_bsdf_sample*>(a.out)[i] = r;
+            break;
+        }
+        case CRG_PROBE_BSDF_EVAL: {
+            const crg_probe_bsdf_in& q = static_cast<const crg_probe_bsdf_in*>(a.in)[i];
+            const BsdfEval e = evaluate_bsdf(surf_of(q.surface), V3{q.wo[0], q.wo[1], q.wo[2]},
+                                             V3{q.wi[0], q.wi[1], q.wi[2]});
+            float* o = static_cast<float*>(a.out) + 4 * i;
+            put3(o, e.value);
+            o[3] = e.pdf;
+            break;
+        }
+        case CRG_PROBE_OFFSET_RAY: {
+            const crg_probe_offset_in& q = static_cast<const crg_probe_offset_in*>(a.in)[i];
+            V3 o{0, 0, 0}, d{0, 0, 0};
+            float tmax = 0;
+            const uint32_t err = offset_ray(V3{q.position[0], q.position[1], q.position[2]},
+                                            V3{q.geometric_normal[0], q.geometric_normal[1], q.geometric_normal[2]},
+                                            V3{a.ung[3 * i], a.ung[3 * i + 1], a.ung[3 * i + 2]},
+                                            V3{q.direction[0], q.direction[1], q.direction[2]}, q.max_distance, o, d, tmax);
+            static_cast<crg_probe_ray*>(a.out)[i] = probe_ray(o, d, 0.0f, tmax, int32_t(err));
+            break;
+        }
+        default: {  // CRG_PROBE_SURFACE_AT
+            const crg_probe_surface_in& q = static_cast<const crg_probe_surface_in*>(a.in)[i];
+            const uint32_t prim = a.tri_to_prim[q.hit.triangle];
+            const Surf s = surface_at(*a.scene, prim, V3{q.ray.direction[0], q.ray.direction[1], q.ray.direction[2]},
+                                      sf::f64{__builtin_bit_cast(uint64_t, q.hit.u)},
+                                      sf::f64{__builtin_bit_cast(uint64_t, q.hit.v)});
+            crg_surface r{};
+            put3(r.position, s.position);
+            put3(r.position_low, s.position_low);
+            put3(r.geometric_normal, s.ng);
+            put3(r.normal, s.normal);
+            put3(r.base_color, s.base);
+            put3(r.emission, s.emission);
+            put3(r.attenuation_color, s.att_color);
+            r.metallic = s.metallic;
+            r.roughness = s.roughness;
+            r.transmission = s.transmission;
+            r.ior = s.ior;
+            r.thickness = s.thickness;
+            r.attenuation_distance = s.att_distance;
+            r.material = s.material;
+            r.front_face = s.front;
+            r.double_sided = s.double_sided;
+            static_cast<crg_surface*>(a.out)[i] = r;
+            break;
+        }
+    }
+}
+
+hipError_t launch_probe(const ProbeArgs& a, hipStream_t st) {
+    if (a.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_probe, dim3(blocks_for(a.n, 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ allocation-free ray queries
 // Fixed device scratch of crg_trace_rays_noalloc: the control block, then the
 // staged rays and hits of at most kNoAllocMaxRays queries.  Every device the

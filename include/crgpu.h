@@ -357,6 +357,81 @@ int crg_write_png(crg_ctx* ctx, const char* path, const float* film_rgb, int32_t
                   float exposure);
 int crg_write_hdr(crg_ctx* ctx, const char* path, const float* film_rgb, int32_t width, int32_t height);
 
+/* ---- per-function device probe (parity tests) ---------------------------
+ * Runs one function of the device path on gfx950 over an array of input
+ * records, so each can be compared with the reference's own values.  No
+ * counterpart in the reference: test infrastructure for the parity tests of
+ * the functions render() is made of.  Records (in -> out), one per element:
+ *   CRG_PROBE_CAMERA       float px, py              -> crg_probe_ray  (Camera::generate,
+ *                          with `camera`, `width`, `height`; reference src/geometry.cpp:213-234)
+ *   CRG_PROBE_ENV_SAMPLE   uint64_t rng state        -> crg_probe_env_sample (Environment::sample)
+ *   CRG_PROBE_ENV_PDF      float direction[3]        -> float          (Environment::pdf)
+ *   CRG_PROBE_ENV_RADIANCE float direction[3]        -> float rgb[3]   (Environment::radiance)
+ *                          (the three with `env_id`; reference src/render.cpp:45-94)
+ *   CRG_PROBE_BSDF_SAMPLE  crg_probe_bsdf_in         -> crg_probe_bsdf_sample (sampleBsdf,
+ *                          reference src/bsdf.cpp:184-237)
+ *   CRG_PROBE_BSDF_EVAL    crg_probe_bsdf_in         -> float value[3], pdf (evaluateBsdf,
+ *                          reference src/bsdf.cpp:176-181)
+ *   CRG_PROBE_OFFSET_RAY   crg_probe_offset_in       -> crg_probe_ray  (offsetRay,
+ *                          reference src/geometry.cpp:186-209)
+ *   CRG_PROBE_SURFACE_AT   crg_probe_surface_in      -> crg_surface    (surfaceAt over `scene_id`,
+ *                          reference src/geometry.cpp:401-459; tangent/bitangent are not
+ *                          computed: the device's BSDF does not read them) */
+#define CRG_PROBE_CAMERA 0
+#define CRG_PROBE_ENV_SAMPLE 1
+#define CRG_PROBE_ENV_PDF 2
+#define CRG_PROBE_ENV_RADIANCE 3
+#define CRG_PROBE_BSDF_SAMPLE 4
+#define CRG_PROBE_BSDF_EVAL 5
+#define CRG_PROBE_OFFSET_RAY 6
+#define CRG_PROBE_SURFACE_AT 7
+
+/* The reference's Surface (reference src/scene.h:105-115). */
+typedef struct crg_surface {
+    float position[3], position_low[3], geometric_normal[3], normal[3], tangent[3], bitangent[3];
+    float base_color[3], emission[3], attenuation_color[3];
+    float metallic, roughness, transmission, ior, thickness, attenuation_distance;
+    uint32_t material;
+    int32_t front_face, double_sided;
+} crg_surface;
+
+typedef struct crg_probe_ray {
+    crg_ray ray;     /* origin, direction, tmin, tmax (origin_low and ignore zero) */
+    int32_t status;  /* 0, or the failure (camera: invalid; offset: invalid / overflow) */
+    int32_t pad;
+} crg_probe_ray;
+
+typedef struct crg_probe_env_sample {
+    float direction[3], pdf;
+    uint64_t rng_end;  /* generator state after the draws */
+} crg_probe_env_sample;
+
+typedef struct crg_probe_bsdf_in {
+    crg_surface surface;
+    float wo[3], wi[3];
+    uint64_t rng;      /* generator state (sample) */
+} crg_probe_bsdf_in;
+
+/* The reference's BsdfSample (reference src/bsdf.h:19-29); valid = operator bool. */
+typedef struct crg_probe_bsdf_sample {
+    float direction[3], weight[3];
+    float pdf, eta;
+    int32_t delta, transmitted, valid;
+    uint64_t rng_end;
+} crg_probe_bsdf_sample;
+
+typedef struct crg_probe_offset_in {
+    float position[3], geometric_normal[3], direction[3], max_distance;
+} crg_probe_offset_in;
+
+typedef struct crg_probe_surface_in {
+    crg_ray ray;
+    crg_hit hit;  /* triangle: finalized index; t, u, v: binary64 */
+} crg_probe_surface_in;
+
+int crg_device_probe(crg_ctx* ctx, int32_t what, uint64_t scene_id, uint64_t env_id, const crg_camera* camera,
+                     int32_t width, int32_t height, const void* in, uint64_t count, void* out);
+
 /* ---- measurement ------------------------------------------------------- */
 /* Per-kernel device time, measured with HIP events recorded on the context's
  * stream around every launch while profiling is enabled.  No counterpart in
