@@ -355,13 +355,25 @@ void emu_expf(const float* x, float* out, uint64_t n) {
     for (uint64_t i = 0; i < n; ++i) out[i] = expf_ff(x[i]);
 }
 
-// Slab test of one box, quantized by the builder as the only child of a
-// 4-wide node: 1 when node_hit reports it as hit within [tmin, tmax].
-int emu_box_hit(const float* o, const float* d, float tmin, float tmax, const float* lo, const float* hi) {
+// Slab test of one box, quantized by the builder as child 0 of a 4-wide node
+// (with a sibling box as child 1 when sib_lo is not null, which moves the
+// node's origin and coarsens its scale): 1 when node_hit reports child 0 as
+// hit within [tmin, tmax].
+int emu_box_hit(const float* o, const float* d, float tmin, float tmax, const float* lo, const float* hi,
+                const float* sib_lo, const float* sib_hi) {
     RayQuery q;
     setup_query(q, V3{o[0], o[1], o[2]}, V3{d[0], d[1], d[2]}, tmin, tmax, V3{0, 0, 0}, kNoTriangle);
-    const uint32_t ref = 1;
-    const QNode4 n = quantize_boxes4(lo, hi, &ref, 1);
+    const uint32_t refs[2] = {1, 2};
+    float l[6] = {lo[0], lo[1], lo[2], 0, 0, 0}, h[6] = {hi[0], hi[1], hi[2], 0, 0, 0};
+    int count = 1;
+    if (sib_lo) {
+        for (int a = 0; a < 3; ++a) {
+            l[3 + a] = sib_lo[a];
+            h[3 + a] = sib_hi[a];
+        }
+        count = 2;
+    }
+    const QNode4 n = quantize_boxes4(l, h, refs, count);
     float nears[4];
     return int(node_hit<4>(q, n, tmax, nears) & 1u);
 }

@@ -46,7 +46,7 @@ class Emu:
             "emu_env_pdf": (C.c_float, [vp, fp]),
             "emu_env_radiance": (None, [vp, fp, fp]),
             "emu_trace": (C.c_int, [vp, vp, u64, vp, C.c_int, C.POINTER(u64), u64]),
-            "emu_box_hit": (C.c_int, [fp, fp, C.c_float, C.c_float, fp, fp]),
+            "emu_box_hit": (C.c_int, [fp, fp, C.c_float, C.c_float, fp, fp, fp, fp]),
             "emu_expf": (None, [fp, fp, u64]),
             "emu_pow5": (None, [fp, fp, fp, u64]),
             "emu_mis": (None, [fp, fp, fp, fp, u64]),

@@ -100,13 +100,16 @@ def test_slab_test_is_conservative(emu):
     """node_hit never culls a box the ray meets in exact arithmetic (the FP32
     traversal may only over-approximate, reference src/geometry.cpp:565-586):
     grazing boxes whose faces pass through points of the ray, at origin scales
-    from 1e-3 to 1e5, with tiny and zero direction components."""
+    from 1e-3 to 1e5, with tiny and zero direction components.  The box is
+    quantized as a child of a wide node, half the time next to a sibling box
+    up to 1e4 box sizes away (the node's origin far from the box, a coarse
+    scale, the per-node distance terms large against the box's own)."""
     import ctypes as C
     rng = np.random.default_rng(11)
     fp = C.POINTER(C.c_float)
     f32 = lambda v: np.ascontiguousarray(v, np.float32)
     missed = tight_misses = 0
-    for case in range(3000):
+    for case in range(6000):
         scale = [1e-3, 1.0, 1e3, 1e5][case % 4]
         o = f32(rng.uniform(-1, 1, 3) * scale)
         d = rng.normal(size=3)
@@ -130,15 +133,23 @@ def test_slab_test_is_conservative(emu):
                 if nudge else np.float32(p[a])
         lo, hi = f32(np.minimum(lo, hi)), f32(np.maximum(lo, hi))
         tmax = np.float32(np.inf) if case % 3 else np.float32(t0 * (1 + rng.uniform(-1e-6, 1e-6)))
+        sib_lo = sib_hi = None
+        if case % 2:
+            size = float(np.max(hi.astype(np.float64) - lo)) + 1e-30
+            off = rng.normal(size=3) * size * 10.0 ** rng.uniform(0, 4)
+            c = (lo.astype(np.float64) + hi) / 2 + off
+            sib_lo, sib_hi = f32(c - size * rng.uniform(0.1, 3)), f32(c + size * rng.uniform(0.1, 3))
         got = emu.lib.emu_box_hit(o.ctypes.data_as(fp), d.ctypes.data_as(fp), 0.0, float(tmax),
-                                  lo.ctypes.data_as(fp), hi.ctypes.data_as(fp))
+                                  lo.ctypes.data_as(fp), hi.ctypes.data_as(fp),
+                                  sib_lo.ctypes.data_as(fp) if sib_lo is not None else None,
+                                  sib_hi.ctypes.data_as(fp) if sib_hi is not None else None)
         exact = _exact_slab(o, d, 0.0, float(tmax), lo, hi)
         if exact and not got:
             missed += 1
         if not exact:
             tight_misses += 1
     assert missed == 0
-    assert tight_misses > 100  # the set really probes both sides of contact
+    assert tight_misses > 200  # the set really probes both sides of contact
 
 
 def test_emu_camera_bit_exact(emu):
