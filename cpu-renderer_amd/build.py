@@ -35,22 +35,36 @@ def _run(cmd):
     return out
 
 
-def build(verbose: bool = False) -> pathlib.Path:
-    BUILD.mkdir(exist_ok=True)
-    LIB.parent.mkdir(exist_ok=True)
+def build(verbose: bool = False, defines=(), lib: pathlib.Path = LIB, build_dir: pathlib.Path = BUILD) -> pathlib.Path:
+    """Compile and link the library.  `defines` (NAME=VALUE strings) set the
+    kernels' launch and traversal knobs (the CR_* macros of the sources) for a
+    variant build, written to `lib` with objects in `build_dir`."""
+    build_dir.mkdir(exist_ok=True)
+    lib.parent.mkdir(exist_ok=True)
+    dflags = [f"-D{d}" for d in defines]
     jobs = []
     for src, extra in UNITS:
-        obj = BUILD / (src + ".o")
-        jobs.append((obj, [HIPCC, *COMMON, *extra, "-c", str(CSRC / src), "-o", str(obj)]))
+        obj = build_dir / (src + ".o")
+        jobs.append((obj, [HIPCC, *COMMON, *dflags, *extra, "-c", str(CSRC / src), "-o", str(obj)]))
     with ThreadPoolExecutor(len(jobs)) as ex:
         for out in ex.map(lambda j: _run(j[1]), jobs):
             if verbose and out:
                 print(out)
-    tmp = LIB.with_suffix(".so.tmp")
+    tmp = lib.with_suffix(".so.tmp")
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", str(tmp), *[str(j[0]) for j in jobs], "-lpthread", "-lz"])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-v", action="store_true")
+    ap.add_argument("--define", action="append", default=[], help="NAME=VALUE for a variant build")
+    ap.add_argument("--out", help="library path of a variant build")
+    a = ap.parse_args()
+    if a.out:
+        out = pathlib.Path(a.out).resolve()
+        print(build(a.v, a.define, out, HERE / ("_build_" + out.stem)))
+    else:
+        print(build(a.v, a.define))
