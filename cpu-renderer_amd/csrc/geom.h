@@ -648,14 +648,19 @@ CR_HD inline bool leaf_fast(const DevScene& s, const PrimRec* prims, RayQuery& q
 // persistent kernel can interleave several queries per lane.  EXACT=false is
 // the FP32-only fast path (see leaf_prims).
 // Traversal stack = a small ring of the newest entries (in LDS on the device:
-// `LdsRing`; a private array on the host: `LocalRing`) on top of a private
+// `LdsRing`; private arrays otherwise: `LocalRing`) on top of a private
 // overflow array holding the older entries.  One LIFO stack: when the ring is
 // full, its oldest entry moves to the overflow array; pops take the ring
 // first.  Entries are {child ref, entry distance}.
+// The Traversal holds only pointers to the ring and overflow storage, which
+// its caller declares: both are indexed at run time, and an object with such
+// an array inside stays in private (scratch) memory as a whole, so the best
+// hit, its bound and the stack counters would be stored and reloaded there at
+// every visit.  With the arrays outside, that state stays in registers.
 struct LocalRing {
     static constexpr int kSize = 8;
-    uint32_t ref[kSize];
-    float near[kSize];
+    uint32_t* ref;  // kSize entries each, owned by the caller
+    float* near;
     CR_HDI void put(int i, uint32_t r, float n) {
         ref[i] = r;
         near[i] = n;
@@ -670,7 +675,7 @@ template <bool ANY, bool EXACT = true, class Ring = LocalRing, bool PRELOAD = (C
           int W = 4>
 struct Traversal {
     ClosestResult best;
-    StackEntry stack[kStackSize];  // overflow: entries older than the ring's
+    StackEntry* stack;  // overflow (kStackSize entries, owned by the caller): entries older than the ring's
     Ring ring;
     int sp;    // entries in `stack`
     int head;  // ring slot of the next push
@@ -829,7 +834,12 @@ struct Traversal {
 
 template <bool ANY, bool EXACT = true, int W = 4>
 CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, TraceCount& tc) {
+    StackEntry overflow[kStackSize];
+    uint32_t ring_ref[LocalRing::kSize];
+    float ring_near[LocalRing::kSize];
     Traversal<ANY, EXACT, LocalRing, (CR_LEAF_PRELOAD != 0), true, W> tr;
+    tr.stack = overflow;
+    tr.ring = LocalRing{ring_ref, ring_near};
     if (!tr.begin(s, q))
         while (!tr.step(s, q, tc)) {
         }

@@ -23,11 +23,12 @@
 #include "path.h"
 
 // Minimum workgroups of 256 lanes per compute unit the compiler must fit
-// (register budget = occupancy): traversal kernels are latency bound and keep
-// their stacks in scratch anyway; shading needs registers for the emulated
-// binary64 arithmetic.
+// (register budget = occupancy).  Traversal: 4 (128 VGPRs, 4 waves per SIMD)
+// measured fastest on the 3.1 M and 8.0 M triangle renders, ahead of 5 (96
+// VGPRs, heavy spilling) and 3 (no spills, too few waves to hide node-fetch
+// latency); shading needs registers for the emulated binary64 arithmetic.
 #ifndef CR_TRAVERSE_MIN_BLOCKS
-#define CR_TRAVERSE_MIN_BLOCKS 5
+#define CR_TRAVERSE_MIN_BLOCKS 4
 #endif
 #ifndef CR_SHADE_MIN_BLOCKS
 #define CR_SHADE_MIN_BLOCKS 2
@@ -283,9 +284,10 @@ __device__ inline void run_visits(T& tr, const DevScene* sc, RayQuery& q, TraceC
 
 // Newest traversal stack entries of each lane in LDS (lane-minor layout, so a
 // wave's accesses to one ring slot are conflict-free); older entries overflow
-// to the lane's private array.
+// to the lane's private array.  18 entries x 8 B x 256 lanes = 36 KiB per
+// workgroup, 144 KiB of the 160 KiB LDS at 4 workgroups per CU.
 #ifndef CR_LDS_RING
-#define CR_LDS_RING 12
+#define CR_LDS_RING 18
 #endif
 // The ring pointers carry the LDS address space explicitly: through generic
 // pointers the accesses would compile to flat instructions.
@@ -346,7 +348,9 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_extend(PathSoA 
     const uint32_t n = ctl->qcount[parity];
     __shared__ uint32_t ring_ref[CR_LDS_RING * 256];
     __shared__ float ring_near[CR_LDS_RING * 256];
+    StackEntry overflow[kStackSize];
     Traversal<false, false, TraverseRing, (CR_LEAF_PRELOAD != 0), MASK, W> tr;
+    tr.stack = overflow;
     tr.ring = TraverseRing{(lds_u32*)(ring_ref + threadIdx.x), (lds_f32*)(ring_near + threadIdx.x), blockDim.x};
     RayQuery q;
     const DevScene* sc = nullptr;
@@ -572,7 +576,9 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_shadow(PathSoA 
     const uint32_t n = ctl->scount;
     __shared__ uint32_t ring_ref[CR_LDS_RING * 256];
     __shared__ float ring_near[CR_LDS_RING * 256];
+    StackEntry overflow[kStackSize];
     Traversal<true, false, TraverseRing, (CR_LEAF_PRELOAD != 0), MASK, W> tr;
+    tr.stack = overflow;
     tr.ring = TraverseRing{(lds_u32*)(ring_ref + threadIdx.x), (lds_f32*)(ring_near + threadIdx.x), blockDim.x};
     RayQuery q;
     const DevScene* sc = nullptr;

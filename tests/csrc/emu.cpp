@@ -61,7 +61,12 @@ struct EmuEnv {
 // preloading one.
 template <bool ANY, bool PRELOAD, bool MASK = true, int W = 4>
 ClosestResult phased(const DevScene& s, RayQuery& q, TraceCount& tc, uint64_t& rng) {
+    StackEntry overflow[kStackSize];
+    uint32_t ring_ref[LocalRing::kSize];
+    float ring_near[LocalRing::kSize];
     Traversal<ANY, false, LocalRing, PRELOAD, MASK, W> tr;
+    tr.stack = overflow;
+    tr.ring = LocalRing{ring_ref, ring_near};
     if (tr.begin(s, q)) return tr.best;
     for (;;) {
         if (tr.inner(s, q, tc)) break;
