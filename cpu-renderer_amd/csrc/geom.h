@@ -416,8 +416,11 @@ struct ClosestResult {
     sf::f64 t, u, v;
     float tlo, thi;  // FP32 enclosure of the binary64 t (or the query limit)
     uint32_t ncand;  // candidates met by the FP32-only traversal (kCand + 1: too many)
-    uint32_t cand[kCand];
-    float cand_lo[kCand];
+    // The first kCand candidates, in storage the caller owns (the FP32-only
+    // traversal needs it; null elsewhere).  Candidates are rare: held here
+    // they would cost registers in every traversal loop, and as arrays
+    // indexed at run time they would move the whole result to private memory.
+    Pending* cands;
 };
 
 // Replace the current best by an exactly evaluated candidate when it wins the
@@ -437,12 +440,10 @@ CR_HDI void take_exact(ClosestResult& best, uint32_t prim, uint32_t id, const Ex
 // FP32-only traversal: a primitive it cannot decide alone becomes a candidate
 // for the exact pass; the traversal goes on.
 CR_HDI void add_cand(ClosestResult& b, uint32_t prim, float lo) {
-#pragma unroll
-    for (uint32_t k = 0; k < kCand; ++k)
-        if (k == b.ncand) {
-            b.cand[k] = prim;
-            b.cand_lo[k] = lo;
-        }
+    if (b.ncand < kCand) {
+        b.cands->cand[b.ncand] = prim;
+        b.cands->lo[b.ncand] = lo;
+    }
     if (b.ncand <= kCand) ++b.ncand;
     b.need_exact = true;
 }
@@ -502,23 +503,23 @@ CR_HDI int exact_work(const ClosestResult& b) {
     if (b.overflow || b.tex_error || b.ncand == 0) return 0;
     if (ANY && b.prim != kNoTriangle) return 0;  // a certain blocker
     if (b.ncand > kCand) return 2;
-    bool relevant = false;
-#pragma unroll
-    for (uint32_t k = 0; k < kCand; ++k)
-        if (k < b.ncand && (ANY || !(b.cand_lo[k] > b.thi))) relevant = true;
-    return relevant ? 1 : 0;
+    if (ANY) return 1;
+    for (uint32_t k = 0; k < b.ncand; ++k)
+        if (!(b.cands->lo[k] > b.thi)) return 1;
+    return 0;
 }
 
+// The exact pass's record of an FP32-only result with work != 0 (its
+// candidates then sit in b.cands).  Unused candidate slots are zero.
 CR_HDI Pending pending_of(const ClosestResult& b, int work) {
     Pending p;
     p.best = b.prim;
     p.n = b.ncand > kCand ? kCand : b.ncand;
     p.thi = b.thi;
     p.full = work == 2 ? 1u : 0u;
-#pragma unroll
     for (uint32_t k = 0; k < kCand; ++k) {
-        p.cand[k] = b.cand[k];
-        p.lo[k] = b.cand_lo[k];
+        p.cand[k] = k < p.n ? b.cands->cand[k] : 0u;
+        p.lo[k] = k < p.n ? b.cands->lo[k] : 0.0f;
     }
     return p;
 }
@@ -677,6 +678,7 @@ struct Traversal {
     ClosestResult best;
     StackEntry* stack;  // overflow (kStackSize entries, owned by the caller): entries older than the ring's
     Ring ring;
+    Pending* cands;  // candidate storage of an FP32-only traversal (owned by the caller)
     int sp;    // entries in `stack`
     int head;  // ring slot of the next push
     int nr;    // entries in the ring
@@ -719,6 +721,7 @@ struct Traversal {
         best.tex_error = false;
         best.need_exact = false;
         best.ncand = 0;
+        best.cands = cands;
         best.t = sf::from_float(q.tmax);
         best.u = best.v = sf::zero(0);
         best.tlo = best.thi = q.tmax;
@@ -832,14 +835,17 @@ struct Traversal {
     }
 };
 
+// One whole traversal.  EXACT=false (FP32 only) needs `cands`, storage for
+// the candidates its result refers to; the exact traversal keeps none.
 template <bool ANY, bool EXACT = true, int W = 4>
-CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, TraceCount& tc) {
+CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, TraceCount& tc, Pending* cands = nullptr) {
     StackEntry overflow[kStackSize];
     uint32_t ring_ref[LocalRing::kSize];
     float ring_near[LocalRing::kSize];
     Traversal<ANY, EXACT, LocalRing, (CR_LEAF_PRELOAD != 0), true, W> tr;
     tr.stack = overflow;
     tr.ring = LocalRing{ring_ref, ring_near};
+    tr.cands = cands;
     if (!tr.begin(s, q))
         while (!tr.step(s, q, tc)) {
         }
@@ -869,6 +875,7 @@ CR_HD inline ClosestResult resolve_candidates(const DevScene& s, RayQuery& q, co
     best.tex_error = false;
     best.need_exact = false;
     best.ncand = 0;
+    best.cands = nullptr;
     best.t = sf::from_float(q.tmax);
     best.u = best.v = sf::zero(0);
     best.tlo = best.thi = q.tmax;

@@ -349,8 +349,10 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_extend(PathSoA 
     __shared__ uint32_t ring_ref[CR_LDS_RING * 256];
     __shared__ float ring_near[CR_LDS_RING * 256];
     StackEntry overflow[kStackSize];
+    Pending cands;
     Traversal<false, false, TraverseRing, (CR_LEAF_PRELOAD != 0), MASK, W> tr;
     tr.stack = overflow;
+    tr.cands = &cands;
     tr.ring = TraverseRing{(lds_u32*)(ring_ref + threadIdx.x), (lds_f32*)(ring_near + threadIdx.x), blockDim.x};
     RayQuery q;
     const DevScene* sc = nullptr;
@@ -577,8 +579,10 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_shadow(PathSoA 
     __shared__ uint32_t ring_ref[CR_LDS_RING * 256];
     __shared__ float ring_near[CR_LDS_RING * 256];
     StackEntry overflow[kStackSize];
+    Pending cands;
     Traversal<true, false, TraverseRing, (CR_LEAF_PRELOAD != 0), MASK, W> tr;
     tr.stack = overflow;
+    tr.cands = &cands;
     tr.ring = TraverseRing{(lds_u32*)(ring_ref + threadIdx.x), (lds_f32*)(ring_near + threadIdx.x), blockDim.x};
     RayQuery q;
     const DevScene* sc = nullptr;

@@ -60,13 +60,14 @@ struct EmuEnv {
 // same result as step().  Odd schedules use the plain leaf test, even ones the
 // preloading one.
 template <bool ANY, bool PRELOAD, bool MASK = true, int W = 4>
-ClosestResult phased(const DevScene& s, RayQuery& q, TraceCount& tc, uint64_t& rng) {
+ClosestResult phased(const DevScene& s, RayQuery& q, TraceCount& tc, uint64_t& rng, Pending* cands) {
     StackEntry overflow[kStackSize];
     uint32_t ring_ref[LocalRing::kSize];
     float ring_near[LocalRing::kSize];
     Traversal<ANY, false, LocalRing, PRELOAD, MASK, W> tr;
     tr.stack = overflow;
     tr.ring = LocalRing{ring_ref, ring_near};
+    tr.cands = cands;
     if (tr.begin(s, q)) return tr.best;
     for (;;) {
         if (tr.inner(s, q, tc)) break;
@@ -464,18 +465,19 @@ int emu_trace(void* sp, const crg_ray* rays, uint64_t n, crg_hit* hits, int any,
         // Schedule bit 3 walks the 8-wide tree instead of the 4-wide one.
         uint64_t rng = schedule * 0x9e3779b97f4a7c15ull + i;
         ClosestResult b;
+        Pending cands;
         const uint64_t sched = schedule & ~uint64_t(8);
         if (schedule & 8) {
             if (sched & 4)
-                b = any ? phased<true, false, false, 8>(s.dev, q, tc, rng)
-                        : phased<false, false, false, 8>(s.dev, q, tc, rng);
+                b = any ? phased<true, false, false, 8>(s.dev, q, tc, rng, &cands)
+                        : phased<false, false, false, 8>(s.dev, q, tc, rng, &cands);
             else
-                b = any ? phased<true, false, true, 8>(s.dev, q, tc, rng) : phased<false, false, true, 8>(s.dev, q, tc, rng);
-        } else if (!sched) b = any ? traverse<true, false>(s.dev, q, tc) : traverse<false, false>(s.dev, q, tc);
+                b = any ? phased<true, false, true, 8>(s.dev, q, tc, rng, &cands) : phased<false, false, true, 8>(s.dev, q, tc, rng, &cands);
+        } else if (!sched) b = any ? traverse<true, false>(s.dev, q, tc, &cands) : traverse<false, false>(s.dev, q, tc, &cands);
         else if (sched & 4)  // the traversal variant without FP32 alpha-mask tests
-            b = any ? phased<true, false, false>(s.dev, q, tc, rng) : phased<false, false, false>(s.dev, q, tc, rng);
-        else if (sched & 1) b = any ? phased<true, false>(s.dev, q, tc, rng) : phased<false, false>(s.dev, q, tc, rng);
-        else b = any ? phased<true, true>(s.dev, q, tc, rng) : phased<false, true>(s.dev, q, tc, rng);
+            b = any ? phased<true, false, false>(s.dev, q, tc, rng, &cands) : phased<false, false, false>(s.dev, q, tc, rng, &cands);
+        else if (sched & 1) b = any ? phased<true, false>(s.dev, q, tc, rng, &cands) : phased<false, false>(s.dev, q, tc, rng, &cands);
+        else b = any ? phased<true, true>(s.dev, q, tc, rng, &cands) : phased<false, true>(s.dev, q, tc, rng, &cands);
         const int work = any ? exact_work<true>(b) : exact_work<false>(b);
         const bool rerun = work != 0;
         if (rerun) {
@@ -597,7 +599,8 @@ int emu_render(void* sp, void* ep, const crg_camera* camera, const crg_options* 
                         setup_query(q, st.o, st.d, st.tmin, st.tmax, st.lo, st.prev);
                         TraceCount tc{0, 0, 0};
                         // Same two-phase order as k_extend / k_extend_exact.
-                        ClosestResult c = traverse<false, false>(s.dev, q, tc);
+                        Pending cands;
+                        ClosestResult c = traverse<false, false>(s.dev, q, tc, &cands);
                         if (const int work = exact_work<false>(c)) {
                             RayQuery qx;
                             setup_query(qx, st.o, st.d, st.tmin, st.tmax, st.lo, st.prev);
@@ -617,7 +620,7 @@ int emu_render(void* sp, void* ep, const crg_camera* camera, const crg_options* 
                         if (sh.emit) {
                             RayQuery sq;
                             setup_query(sq, sh.o, sh.d, 0, sh.tmax, sh.lo, sh.ignore);
-                            ClosestResult b = traverse<true, false>(s.dev, sq, tc);
+                            ClosestResult b = traverse<true, false>(s.dev, sq, tc, &cands);
                             if (const int work = b.overflow ? 0 : exact_work<true>(b)) {
                                 RayQuery sx;
                                 setup_query(sx, sh.o, sh.d, 0, sh.tmax, sh.lo, sh.ignore);
