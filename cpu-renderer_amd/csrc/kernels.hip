@@ -242,7 +242,10 @@ __device__ inline void start_item(const PathSoA& P, const DevJob* jobs, uint32_t
 #define CR_VISITS 16
 #endif
 constexpr int kVisitsPerRefill = CR_VISITS;
-constexpr int kRefillBelow = 48;  // refill when at most this many lanes are busy
+#ifndef CR_REFILL_BELOW
+#define CR_REFILL_BELOW 48
+#endif
+constexpr int kRefillBelow = CR_REFILL_BELOW;  // refill when at most this many lanes are busy
 
 // kVisitsPerRefill rounds of traversal for the busy lanes of a wave.  With
 // CR_TRAV_PP each round is one inner-node visit per lane, followed by a leaf
@@ -253,7 +256,7 @@ constexpr int kRefillBelow = 48;  // refill when at most this many lanes are bus
 #define CR_TRAV_PP 1
 #endif
 #ifndef CR_LEAF_MIN
-#define CR_LEAF_MIN 24
+#define CR_LEAF_MIN 32
 #endif
 template <class T>
 __device__ inline void run_visits(T& tr, const DevScene* sc, RayQuery& q, TraceCount& tc, bool& busy, bool& fin) {
