@@ -340,19 +340,29 @@ CR_HDI uint32_t node_hit(const RayQuery& q, const CR_GLOBAL QNode<W>& n, float l
     const float sx = qscale(ex & 255u), sy = qscale((ex >> 8) & 255u), sz = qscale((ex >> 16) & 255u);
     const float px = n.px, py = n.py, pz = n.pz;
     const bool dx = !(q.inv[0] < 0), dy = !(q.inv[1] < 0), dz = !(q.inv[2] < 0);
+    // Near and far plane words of each axis, picked once per word by the ray's
+    // direction sign (a positive direction enters at the low plane); each
+    // child's byte then converts straight to float.
+    uint32_t nw[3][W / 4], fw[3][W / 4];
+#pragma unroll
+    for (int w = 0; w < W / 4; ++w) {
+        nw[0][w] = dx ? n.qlo[0][w] : n.qhi[0][w];
+        fw[0][w] = dx ? n.qhi[0][w] : n.qlo[0][w];
+        nw[1][w] = dy ? n.qlo[1][w] : n.qhi[1][w];
+        fw[1][w] = dy ? n.qhi[1][w] : n.qlo[1][w];
+        nw[2][w] = dz ? n.qlo[2][w] : n.qhi[2][w];
+        fw[2][w] = dz ? n.qhi[2][w] : n.qlo[2][w];
+    }
     uint32_t mask = 0;
 #pragma unroll
     for (int c = 0; c < W; ++c) {
         const int w = c / 4, sh = 8 * (c % 4);
-        const uint32_t lx = (n.qlo[0][w] >> sh) & 255u, hx = (n.qhi[0][w] >> sh) & 255u;
-        const uint32_t ly = (n.qlo[1][w] >> sh) & 255u, hy = (n.qhi[1][w] >> sh) & 255u;
-        const uint32_t lz = (n.qlo[2][w] >> sh) & 255u, hz = (n.qhi[2][w] >> sh) & 255u;
-        const float nx = (qdecode(px, dx ? lx : hx, sx) - q.o.x) * q.inv[0];
-        const float fx = (qdecode(px, dx ? hx : lx, sx) - q.o.x) * q.inv[0];
-        const float ny = (qdecode(py, dy ? ly : hy, sy) - q.o.y) * q.inv[1];
-        const float fy = (qdecode(py, dy ? hy : ly, sy) - q.o.y) * q.inv[1];
-        const float nz = (qdecode(pz, dz ? lz : hz, sz) - q.o.z) * q.inv[2];
-        const float fz = (qdecode(pz, dz ? hz : lz, sz) - q.o.z) * q.inv[2];
+        const float nx = (qdecode(px, (nw[0][w] >> sh) & 255u, sx) - q.o.x) * q.inv[0];
+        const float fx = (qdecode(px, (fw[0][w] >> sh) & 255u, sx) - q.o.x) * q.inv[0];
+        const float ny = (qdecode(py, (nw[1][w] >> sh) & 255u, sy) - q.o.y) * q.inv[1];
+        const float fy = (qdecode(py, (fw[1][w] >> sh) & 255u, sy) - q.o.y) * q.inv[1];
+        const float nz = (qdecode(pz, (nw[2][w] >> sh) & 255u, sz) - q.o.z) * q.inv[2];
+        const float fz = (qdecode(pz, (fw[2][w] >> sh) & 255u, sz) - q.o.z) * q.inv[2];
         float tn = fmaxf(fmaxf(q.tmin, nx), fmaxf(ny, nz));
         float tf = fminf(fminf(limit, fx), fminf(fy, fz));
         tn = tn - (fabsf(tn) * (4.0f * kU) + q.slack);

@@ -117,7 +117,10 @@ static_assert(sizeof(QNode4) == 64, "4-wide node is one 64-byte block");
 static_assert(sizeof(QNode8) == 128, "8-wide node is one 128-byte line");
 
 CR_HDI float qscale(uint32_t e) { return sf::bitsf(e << 23); }
-CR_HDI float qdecode(float p, uint32_t q, float s) { return p + float(q) * s; }
+// A child plane: p + q * 2^(e-127).  The product of an 8-bit q and a power of
+// two is exact, so the fused form rounds once, like the plain sum (one
+// instruction on the device); host and device decode with this same function.
+CR_HDI float qdecode(float p, uint32_t q, float s) { return __builtin_fmaf(float(q), s, p); }
 
 enum PrimFlags : uint32_t {
     kPrimCull = 1u,        // single-sided opaque: reject when dot(Ng, d) >= 0
