@@ -365,25 +365,22 @@ CR_HDI uint32_t node_hit(const RayQuery& q, const CR_GLOBAL QNode<W>& n, float l
         const float fz = (qdecode(pz, (fw[2][w] >> sh) & 255u, sz) - q.o.z) * q.inv[2];
         float tn = fmaxf(fmaxf(q.tmin, nx), fmaxf(ny, nz));
         float tf = fminf(fminf(limit, fx), fminf(fy, fz));
-        tn = tn - (fabsf(tn) * (4.0f * kU) + q.slack);
-        tf = tf + (fabsf(tf) * (4.0f * kU) + q.slack);
+        // Widening 4u|t| + slack: the product by 2^-22 is exact, so the fused
+        // form rounds like the separate multiply and add.
+        tn = tn - __builtin_fmaf(fabsf(tn), 4.0f * kU, q.slack);
+        tf = tf + __builtin_fmaf(fabsf(tf), 4.0f * kU, q.slack);
         near_out[c] = tn;
         if (tn <= tf && uint32_t(c) < count) mask |= 1u << c;
     }
     return mask;
 }
 
-// Entry distances as order-preserving keys (larger float -> larger key) and back.
-CR_HDI uint32_t near_key(float f) {
-    const uint32_t b = sf::fbits(f);
-    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-}
-CR_HDI float key_near(uint32_t k) { return sf::bitsf((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k); }
-
-// One comparator of a sorting network over (key, ref) pairs: ascending keys.
-CR_HDI void sort2(uint32_t& ka, uint32_t& ra, uint32_t& kb, uint32_t& rb) {
+// One comparator of a sorting network over (entry distance, ref) pairs:
+// ascending distances.  A hit child's distance is finite (an entry at +inf
+// widens to NaN and fails the test), a missed child's is +inf.
+CR_HDI void sort2(float& ka, uint32_t& ra, float& kb, uint32_t& rb) {
     const bool swap = kb < ka;
-    const uint32_t k0 = swap ? kb : ka, k1 = swap ? ka : kb;
+    const float k0 = swap ? kb : ka, k1 = swap ? ka : kb;
     const uint32_t r0 = swap ? rb : ra, r1 = swap ? ra : rb;
     ka = k0;
     kb = k1;
@@ -394,7 +391,7 @@ CR_HDI void sort2(uint32_t& ka, uint32_t& ra, uint32_t& kb, uint32_t& rb) {
 // Ascending sort of W keys with their refs (W = 4: 5 comparators; W = 8: the
 // 19-comparator network).
 template <int W>
-CR_HDI void sort_children(uint32_t (&k)[W], uint32_t (&r)[W]) {
+CR_HDI void sort_children(float (&k)[W], uint32_t (&r)[W]) {
     if constexpr (W == 4) {
         sort2(k[0], r[0], k[1], r[1]);
         sort2(k[2], r[2], k[3], r[3]);
@@ -773,16 +770,17 @@ struct Traversal {
         // nearest, push the others farthest first so the next pop is the
         // second nearest.  The order only affects how soon the bound shrinks,
         // never the result.
-        uint32_t key[W], ch[W];
+        float key[W];
+        uint32_t ch[W];
 #pragma unroll
         for (int c = 0; c < W; ++c) {
-            key[c] = ((mask >> c) & 1u) ? near_key(nears[c]) : 0xffffffffu;
+            key[c] = ((mask >> c) & 1u) ? nears[c] : kInf;
             ch[c] = n.child[c];
         }
         sort_children<W>(key, ch);
 #pragma unroll
         for (int k = W - 1; k >= 1; --k)
-            if (k < hits) push(ch[k], key_near(key[k]));
+            if (k < hits) push(ch[k], key[k]);
         ref = ch[0];
         return true;
     }
