@@ -333,10 +333,14 @@ CR_HDI bool prim_accepts(const DevScene& s, uint32_t prim, uint32_t flags, V3 d)
 // float box: FP32 slabs widened by 4 ulp relative plus the split-origin slack,
 // so the test can only over-include.  Returns the hit mask and the widened
 // entry distances.
+// A child slot is present when its ref is not kEmptyChild (the builder and
+// the cache-file check keep that equal to slot < child count).  Testing the
+// refs makes the mask depend on them, so their load is issued with the
+// planes' instead of after the mask test (a second round trip to memory on
+// every visit that hits a child).
 template <int W>
 CR_HDI uint32_t node_hit(const RayQuery& q, const CR_GLOBAL QNode<W>& n, float limit, float near_out[W]) {
     const uint32_t ex = n.exps;
-    const uint32_t count = ex >> 24;
     const float sx = qscale(ex & 255u), sy = qscale((ex >> 8) & 255u), sz = qscale((ex >> 16) & 255u);
     const float px = n.px, py = n.py, pz = n.pz;
     const bool dx = !(q.inv[0] < 0), dy = !(q.inv[1] < 0), dz = !(q.inv[2] < 0);
@@ -370,7 +374,7 @@ CR_HDI uint32_t node_hit(const RayQuery& q, const CR_GLOBAL QNode<W>& n, float l
         tn = tn - __builtin_fmaf(fabsf(tn), 4.0f * kU, q.slack);
         tf = tf + __builtin_fmaf(fabsf(tf), 4.0f * kU, q.slack);
         near_out[c] = tn;
-        if (tn <= tf && uint32_t(c) < count) mask |= 1u << c;
+        if (tn <= tf && n.child[c] != kEmptyChild) mask |= 1u << c;
     }
     return mask;
 }
@@ -536,10 +540,10 @@ struct TraceCount {
     uint32_t nodes, tris, fallbacks;
 };
 
-// Tests a leaf's primitives.  Returns true when traversal can stop (any-hit
-// found, or an FP32-only traversal needs the exact rerun).  EXACT=false
-// compiles no binary64 emulation at all: undecided tests flag need_exact.
-template <bool ANY, bool EXACT, bool MASK = true>
+// Tests a leaf's primitives in the exact traversal: FP32 first, the binary64
+// emulation for what FP32 leaves undecided.  Returns true when traversal can
+// stop (any-hit found).  The FP32-only traversal uses leaf_fp32 instead.
+template <bool ANY, bool MASK = true>
 CR_HD inline bool leaf_prims(const DevScene& s, const PrimRec* prims, RayQuery& q, uint32_t first, uint32_t count,
                              ClosestResult& best, TraceCount& tc) {
     for (uint32_t i = first; i < first + count; ++i) {
@@ -550,10 +554,6 @@ CR_HD inline bool leaf_prims(const DevScene& s, const PrimRec* prims, RayQuery& 
         float clo = 0, chi = 0;
         const int fr = fast_triangle(q, va, vb, vc, clo, chi);
         if (fr == kReject) continue;
-        if constexpr (!EXACT) {
-            if (fp32_take<ANY, MASK>(s, q, best, i, p.id, p.flags, fr, clo, chi)) return true;
-            continue;
-        } else {
         if (fr == kAccept && !(p.flags & kPrimMaskTexture)) {
             if (!ANY && clo > best.thi) continue;  // certainly beyond the current limit
             if (!prim_accepts(s, i, p.flags, q.d)) continue;
@@ -602,14 +602,34 @@ CR_HD inline bool leaf_prims(const DevScene& s, const PrimRec* prims, RayQuery& 
         }
         if (best.prim == kNoTriangle || sf::lt(e.t, best.t) || (sf::eq(e.t, best.t) && p.id < best.id))
             take_exact(best, i, p.id, e);
-        }
+    }
+    return false;
+}
+
+// FP32-only leaf test.  Each primitive record is read whole before any of its
+// fields decides anything: the ignored-id and never-hit checks come after the
+// triangle test (both only turn its outcome into a rejection, and fp32_take
+// rejects a never-hit primitive without side effects), so the record's id,
+// vertices and flags are loaded together instead of one after another.
+template <bool ANY, bool MASK = true>
+CR_HD inline bool leaf_fp32(const DevScene& s, const PrimRec* prims, RayQuery& q, uint32_t first, uint32_t count,
+                            ClosestResult& best, TraceCount& tc) {
+    for (uint32_t i = first; i < first + count; ++i) {
+        const CR_GLOBAL PrimRec& p = gref(prims + i);
+        const uint32_t id = p.id, flags = p.flags;
+        float clo = 0, chi = 0;
+        int fr = fast_triangle(q, V3{p.ax, p.ay, p.az}, V3{p.bx, p.by, p.bz}, V3{p.cx, p.cy, p.cz}, clo, chi);
+        tc.tris += id != q.ignore ? 1u : 0u;
+        if (id == q.ignore || (flags & kPrimNeverHit)) fr = kReject;
+        if (fr == kReject) continue;
+        if (fp32_take<ANY, MASK>(s, q, best, i, id, flags, fr, clo, chi)) return true;
     }
     return false;
 }
 
 // FP32-only leaf test with the primitive records of each group of four loaded
 // before any is tested, so their loads are in flight together.  Same decisions
-// as leaf_prims<ANY, false>.
+// as leaf_fp32.
 #ifndef CR_LEAF_PRELOAD
 #define CR_LEAF_PRELOAD 0
 #endif
@@ -746,7 +766,8 @@ struct Traversal {
     CR_HDI bool test_leaf(const DevScene& s, RayQuery& q, uint32_t r, TraceCount& tc) {
         const PrimRec* pr = CR_CACHE_PTRS ? prims : s.prims;
         if constexpr (!EXACT && PRELOAD) return leaf_fast<ANY, MASK>(s, pr, q, leaf_first(r), leaf_count(r), best, tc);
-        else return leaf_prims<ANY, EXACT, MASK>(s, pr, q, leaf_first(r), leaf_count(r), best, tc);
+        else if constexpr (!EXACT) return leaf_fp32<ANY, MASK>(s, pr, q, leaf_first(r), leaf_count(r), best, tc);
+        else return leaf_prims<ANY, MASK>(s, pr, q, leaf_first(r), leaf_count(r), best, tc);
     }
 
     // Visit of the inner node `ref`: pushes the farther hit children, moves to

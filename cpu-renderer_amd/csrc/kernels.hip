@@ -449,10 +449,17 @@ __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, S
 #endif
     const uint32_t n = ctl->qcount[parity];
     const uint32_t per = kShadeChunk / blockDim.x;  // entries per thread and chunk
-    for (uint32_t base = blockIdx.x * kShadeChunk; base < n; base += gridDim.x * kShadeChunk) {
-        const uint32_t m = min(kShadeChunk, n - base);
+    // Workgroups claim chunks one at a time from a queue counter: only two
+    // shading workgroups fit a CU, and with a fixed grid-stride assignment the
+    // last round of chunks left most CUs idle (k_shade 4.8 % slower).
+    __shared__ uint32_t chunk_base;
+    for (;;) {
         if (threadIdx.x < kShadeClasses) counts[threadIdx.x] = cursor[threadIdx.x] = 0;
+        if (threadIdx.x == 0) chunk_base = atomicAdd(&ctl->khead, kShadeChunk);
         __syncthreads();
+        const uint32_t base = chunk_base;  // the same for the whole workgroup
+        if (base >= n) break;
+        const uint32_t m = min(kShadeChunk, n - base);
         for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {  // path ids < 2^25: class fits above
             const uint32_t path = queue[base + k];
             const uint32_t c = P.ray[path].hit >> 30;
@@ -722,6 +729,7 @@ __global__ void k_reset(int parity, Control* ctl) {
     ctl->xscount = 0;
     ctl->ehead = 0;
     ctl->shead = 0;
+    ctl->khead = 0;
     const unsigned long long issued = ctl->issued, total = ctl->total;
     const unsigned long long avail = issued < total ? total - issued : 0ull;
     const uint32_t refill = avail < ctl->fcount ? uint32_t(avail) : ctl->fcount;
@@ -782,7 +790,7 @@ __global__ void __launch_bounds__(256) k_pool_init(uint32_t* freeq, uint32_t poo
         ctl->refill_n = ctl->refill_from = ctl->refill_q = 0;
         ctl->seq_lo = ctl->seq_hi = 0;
         ctl->err_count = 0;
-        ctl->ehead = ctl->shead = 0;
+        ctl->ehead = ctl->shead = ctl->khead = 0;
         ctl->pool = pool;
         ctl->issued = ctl->total = ctl->refill_base = 0;
         ctl->fallbacks = 0;

@@ -352,6 +352,7 @@ struct Control {
     uint32_t seq_lo, seq_hi;   // live job sequence range
     uint32_t err_count;        // monotonic; records go to errors[n % kMaxErrors]
     uint32_t ehead, shead;     // next unclaimed entry of the extend / shadow queue
+    uint32_t khead;            // next unclaimed shading chunk (extend-queue index)
     uint32_t pool;             // path slots (capacity of every queue)
     uint32_t masks;            // a live job's scene has alpha-mask primitives: traversal variant with FP32 mask tests
     unsigned long long issued;
