@@ -356,8 +356,8 @@ void Engine::admit_locked() {
         const crg_options& o = j.req.options;
         DevJob& d = h_jobs_[slot];
         d = DevJob{};
-        d.scene = j.req.scene->desc.as<DevScene>();
-        d.env = j.req.env->desc.as<DevEnv>();
+        d.scene = j.req.scene->host;
+        d.env = j.req.env->host;
         d.lights = DevLights{j.d_lights, uint32_t(j.req.lights.entries.size()), j.req.lights.env_probability};
         d.cam = j.req.camera;
         d.width = o.width;

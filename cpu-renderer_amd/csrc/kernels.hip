@@ -370,7 +370,7 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_extend(PathSoA 
                 path = queue[idx];
                 const PathRay r = P.ray[path];
                 job = r.job;
-                sc = jobs[job].scene;
+                sc = &jobs[job].scene;
                 setup_query(q, xyz(r.o), xyz(r.d), r.o.w, r.d.w, xyz(r.lo), r.prev);
                 tc = TraceCount{0, 0, 0};
                 busy = !tr.begin(*sc, q);
@@ -415,7 +415,7 @@ __global__ void __launch_bounds__(256) k_extend_exact(PathSoA P, const DevJob* j
         const uint32_t path = xq[i];
         const PathRay r = P.ray[path];
         const uint32_t job = r.job;
-        const DevScene& s = *jobs[job].scene;
+        const DevScene& s = jobs[job].scene;
         RayQuery q;
         setup_query(q, xyz(r.o), xyz(r.d), r.o.w, r.d.w, xyz(r.lo), r.prev);
         TraceCount tc{0, 0, 0};
@@ -607,7 +607,7 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_shadow(PathSoA 
             if (!busy && idx < n) {
                 i = idx;
                 job = S.job[i];
-                sc = jobs[job].scene;
+                sc = &jobs[job].scene;
                 shadow_query(q, S, i);
                 tc = TraceCount{0, 0, 0};
                 busy = !tr.begin(*sc, q);
@@ -642,7 +642,7 @@ __global__ void __launch_bounds__(256) k_shadow_exact(PathSoA P, ShadowSoA S, co
     for (uint32_t j = spread_first(); j < n; j += spread_stride()) {
         const uint32_t i = xsq[j];
         const uint32_t job = S.job[i];
-        const DevScene& s = *jobs[job].scene;
+        const DevScene& s = jobs[job].scene;
         RayQuery q;
         shadow_query(q, S, i);
         TraceCount tc{0, 0, 0};

@@ -249,8 +249,11 @@ struct DevCamera {
 
 // One render request: scene/environment/camera/options plus its sample buffer.
 struct DevJob {
-    const DevScene* scene;
-    const DevEnv* env;
+    // Copies of the resident scene's and environment's descriptors: a path
+    // reaches their arrays with the job record's load instead of a second,
+    // dependent load of a descriptor it points to.
+    DevScene scene;
+    DevEnv env;
     DevLights lights;
     DevCamera cam;
     int32_t width, height, spp, bounces;

@@ -83,8 +83,8 @@ CR_HDI bool begin_sample(const DevJob& J, uint32_t pixel, uint32_t sample, bool 
 CR_HDI uint32_t bounce(const DevJob& J, PathState& st, uint32_t prim, ShadowOut& sh, ErrorOut& err) {
     sh.emit = false;
     err.code = 0;
-    const DevScene& sc = *J.scene;
-    const DevEnv& env = *J.env;
+    const DevScene& sc = J.scene;
+    const DevEnv& env = J.env;
     const uint32_t depth = st.flags & kDepthMask;
     const bool last_delta = (st.flags & kLastDelta) != 0, precise = (st.flags & kPrecise) != 0;
     const bool nee = J.nee != 0;

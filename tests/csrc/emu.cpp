@@ -566,8 +566,8 @@ int emu_render(void* sp, void* ep, const crg_camera* camera, const crg_options* 
     try {
         validate_options(*o);
         DevJob J{};
-        J.scene = &s.dev;
-        J.env = &env.dev;
+        J.scene = s.dev;
+        J.env = env.dev;
         const LightTable lt = build_light_table(s.lights, s.arr.tri_to_prim, env.he.integral);
         J.lights = DevLights{lt.entries.data(), uint32_t(lt.entries.size()), lt.env_probability};
         J.cam = prepare_camera(*camera, o->width, o->height);
