@@ -801,6 +801,38 @@ QNode4 quantize_boxes4(const float* lo, const float* hi, const uint32_t* refs, i
     return quantize_node<4>(kids, n < 4 ? n : 4);
 }
 
+// Node layout.  The parallel builder numbers nodes in the order threads
+// create them, so siblings and parents and children lie far apart.  Relabel
+// so that each node's inner children sit in consecutive slots (a 128-byte
+// line holds two 64-byte quantized nodes), blocks in depth-first order from
+// the root; every child still comes after its parent.
+static void relabel_children_contiguous(std::vector<WNode>& nodes) {
+    const size_t n = nodes.size();
+    if (n < 2) return;
+    std::vector<uint32_t> to(n, kEmptyChild);
+    to[0] = 0;
+    uint32_t next = 1;
+    std::vector<uint32_t> stack{0};
+    while (!stack.empty()) {
+        const uint32_t i = stack.back();
+        stack.pop_back();
+        const WNode& w = nodes[i];
+        for (int c = 0; c < 4; ++c)
+            if (w.child[c] != kEmptyChild && !ref_is_leaf(w.child[c])) to[w.child[c]] = next++;
+        for (int c = 3; c >= 0; --c)  // first child's block is laid out first
+            if (w.child[c] != kEmptyChild && !ref_is_leaf(w.child[c])) stack.push_back(w.child[c]);
+    }
+    if (next != n) throw std::logic_error("wide BVH relabel: unreachable nodes");
+    std::vector<WNode> out(n);
+    for (size_t i = 0; i < n; ++i) {
+        WNode w = nodes[i];
+        for (int c = 0; c < 4; ++c)
+            if (w.child[c] != kEmptyChild && !ref_is_leaf(w.child[c])) w.child[c] = to[w.child[c]];
+        out[to[i]] = w;
+    }
+    nodes.swap(out);
+}
+
 void quantize_trees(WideBvh& b) {
     b.q4.resize(b.nodes.size());
     parallel_chunks(b.nodes.size(), 8192, [&](size_t lo, size_t hi, size_t) {
@@ -847,6 +879,7 @@ WideBvh build_wide_bvh(const HostScene& s) {
     for (size_t i = 0; i < m; ++i) out.order[i] = prims[i].id;
     out.nodes.resize(b.next.load());
     out.binary_nodes = 2 * b.leaves.load() - 1;
+    relabel_children_contiguous(out.nodes);
     quantize_trees(out);
     return out;
 }

@@ -274,7 +274,9 @@ CR_HDI uint64_t quotient_sig(uint64_t ma, uint64_t mb) {
     return q | uint64_t(r != 0);
 }
 
-CR_HD inline f64 div(f64 a, f64 b) {
+// One out-of-line copy (fewer instructions to fetch in the shading kernel;
+// measured slightly faster than inlining every division).
+CR_HD CR_NOINLINE inline f64 div(f64 a, f64 b) {
     const uint32_t xa = uint32_t(a.b >> 52) & 0x7ffu, xb = uint32_t(b.b >> 52) & 0x7ffu;
     const uint32_t s = sign_of(a) ^ sign_of(b);
     int32_t e;
