@@ -440,6 +440,67 @@ constexpr uint32_t kShadeChunk = CR_SHADE_CHUNK;
 #define CR_SHADE_OCTSORT 1
 #endif
 
+// One path's bounce given its closest-hit word: the state update, the shadow
+// ray it emits and what becomes of the path.
+struct ShadeResult {
+    uint32_t action, oct;
+    bool replayed;
+    ShadowOut sh;
+};
+
+__device__ inline ShadeResult shade_one(const PathSoA& P, const DevJob& J, uint32_t ji, uint32_t path,
+                                        uint32_t hit_word, Control* ctl) {
+    ShadeResult o;
+    o.replayed = false;
+    o.sh.emit = false;
+    const uint32_t pixel = P.rec[path].pixel, sample = P.rec[path].sample;
+    const uint64_t key = uint64_t(pixel) * uint64_t(J.spp) + sample;
+    const uint32_t hit = hit_word & kHitPrimMask;
+    PathState st = load_state(P, path);
+    if (st.flags & kFailed) {
+        o.action = kRetire;
+    } else if (hit == kHitOverflow || hit == kHitTexture) {
+        report_error(ctl, ji, key, hit == kHitOverflow ? kErrStack : kErrTexture, st.flags & kDepthMask, 0, 0,
+                     sf::zero(0), st.o, st.o, st.d);
+        st.L = v3(0);
+        o.action = kRetire;
+    } else {
+        ErrorOut err;
+        o.action = bounce(J, st, hit == kHitMiss ? kNoTriangle : hit, o.sh, err);
+        if (err.code)
+            report_error(ctl, ji, key, err.code, err.depth, err.triangle, err.material, err.t, err.point, err.origin,
+                         err.direction);
+    }
+    if (o.action == kReplay) {
+        o.replayed = true;
+        begin_sample(J, pixel, sample, true, st);
+        o.action = kContinue;
+    }
+    store_state(P, path, st);
+    o.oct = (st.d.x < 0 ? 1u : 0u) | (st.d.y < 0 ? 2u : 0u) | (st.d.z < 0 ? 4u : 0u);
+    return o;
+}
+
+// Wave-level bookkeeping of shade_one's results: counters, the shadow queue
+// and the retire list (the continuing paths' next-queue slot is the caller's).
+__device__ inline void shade_emit(const ShadowSoA& S, JobCounters* cnt, Control* ctl, uint32_t* retire, bool live,
+                                  uint32_t path, uint32_t ji, const ShadeResult& o) {
+    wave_add_keyed(cnt->extend_rays, ji, live ? 1u : 0u);
+    wave_add_keyed(cnt->shadow_rays, ji, live && o.sh.emit ? 1u : 0u);
+    wave_add_keyed(cnt->replays, ji, live && o.replayed ? 1u : 0u);
+    const uint32_t spos = wave_append(&ctl->scount, live && o.sh.emit);
+    if (live && o.sh.emit) {
+        S.path[spos] = path;
+        S.job[spos] = ji;
+        S.o[spos] = make_float4(o.sh.o.x, o.sh.o.y, o.sh.o.z, 0.0f);
+        S.d[spos] = make_float4(o.sh.d.x, o.sh.d.y, o.sh.d.z, o.sh.tmax);
+        S.lo[spos] = make_float4(o.sh.lo.x, o.sh.lo.y, o.sh.lo.z, __uint_as_float(o.sh.ignore));
+        S.c[spos] = make_float4(o.sh.c.x, o.sh.c.y, o.sh.c.z, 0);
+    }
+    const uint32_t rpos = wave_append(&ctl->rcount, live && o.action == kRetire);
+    if (live && o.action == kRetire) retire[rpos] = path;
+}
+
 __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, ShadowSoA S, const DevJob* jobs,
                                                                   const uint32_t* queue, uint32_t* next_queue,
                                                                   uint32_t* retire, int parity, Control* ctl,
@@ -482,65 +543,29 @@ __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, S
         for (uint32_t r = 0; r < per; ++r) {
             const uint32_t k = r * blockDim.x + threadIdx.x;
             const bool live = k < m;
-            uint32_t path = 0, action = kRetire, ji = 0, oct = 0;
-            bool replayed = false;
-            ShadowOut sh;
-            sh.emit = false;
+            uint32_t path = 0, ji = 0;
+            ShadeResult o;
+            o.action = kRetire;
+            o.oct = 0;
+            o.replayed = false;
+            o.sh.emit = false;
             if (live) {
                 path = sorted[k];
                 const uint32_t hit_word = P.ray[path].hit;
                 ji = P.ray[path].job;
-                const DevJob& J = jobs[ji];
-                const uint32_t pixel = P.rec[path].pixel, sample = P.rec[path].sample;
-                const uint64_t key = uint64_t(pixel) * uint64_t(J.spp) + sample;
-                const uint32_t hit = hit_word & kHitPrimMask;
-                PathState st = load_state(P, path);
-                if (st.flags & kFailed) {
-                    action = kRetire;
-                } else if (hit == kHitOverflow || hit == kHitTexture) {
-                    report_error(ctl, ji, key, hit == kHitOverflow ? kErrStack : kErrTexture, st.flags & kDepthMask, 0,
-                                 0, sf::zero(0), st.o, st.o, st.d);
-                    st.L = v3(0);
-                    action = kRetire;
-                } else {
-                    ErrorOut err;
-                    action = bounce(J, st, hit == kHitMiss ? kNoTriangle : hit, sh, err);
-                    if (err.code)
-                        report_error(ctl, ji, key, err.code, err.depth, err.triangle, err.material, err.t, err.point,
-                                     err.origin, err.direction);
-                }
-                if (action == kReplay) {
-                    replayed = true;
-                    begin_sample(J, pixel, sample, true, st);
-                    action = kContinue;
-                }
-                store_state(P, path, st);
-                oct = (st.d.x < 0 ? 1u : 0u) | (st.d.y < 0 ? 2u : 0u) | (st.d.z < 0 ? 4u : 0u);
+                o = shade_one(P, jobs[ji], ji, path, hit_word, ctl);
             }
-            wave_add_keyed(cnt->extend_rays, ji, live ? 1u : 0u);
-            wave_add_keyed(cnt->shadow_rays, ji, live && sh.emit ? 1u : 0u);
-            wave_add_keyed(cnt->replays, ji, replayed ? 1u : 0u);
-            const uint32_t spos = wave_append(&ctl->scount, live && sh.emit);
-            if (live && sh.emit) {
-                S.path[spos] = path;
-                S.job[spos] = ji;
-                S.o[spos] = make_float4(sh.o.x, sh.o.y, sh.o.z, 0.0f);
-                S.d[spos] = make_float4(sh.d.x, sh.d.y, sh.d.z, sh.tmax);
-                S.lo[spos] = make_float4(sh.lo.x, sh.lo.y, sh.lo.z, __uint_as_float(sh.ignore));
-                S.c[spos] = make_float4(sh.c.x, sh.c.y, sh.c.z, 0);
-            }
+            shade_emit(S, cnt, ctl, retire, live, path, ji, o);
 #if CR_SHADE_OCTSORT
             if (live) {
-                const bool cont = action == kContinue;
-                stage[k] = cont ? (path | (oct << 29)) : 0xffffffffu;
-                if (cont) atomicAdd(&ocount[oct], 1u);
+                const bool cont = o.action == kContinue;
+                stage[k] = cont ? (path | (o.oct << 29)) : 0xffffffffu;
+                if (cont) atomicAdd(&ocount[o.oct], 1u);
             }
 #else
-            const uint32_t qpos = wave_append(&ctl->qcount[parity ^ 1], live && action == kContinue);
-            if (live && action == kContinue) next_queue[qpos] = path;
+            const uint32_t qpos = wave_append(&ctl->qcount[parity ^ 1], live && o.action == kContinue);
+            if (live && o.action == kContinue) next_queue[qpos] = path;
 #endif
-            const uint32_t rpos = wave_append(&ctl->rcount, live && action == kRetire);
-            if (live && action == kRetire) retire[rpos] = path;
         }
 #if CR_SHADE_OCTSORT
         // The chunk's continuing paths go to the next extend queue grouped by
