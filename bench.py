@@ -576,7 +576,7 @@ def measure_plain(ctx, w, steps: int, warmup: int, cpu_budget) -> dict:
             "valu_roofline": valu, "kernels_ms": kernels, "cpu_baseline": cpu, "quality": qual2,
             "config": {"workload": w.name, "triangles": int(info.triangles), "width": w.width, "height": w.height,
                        "spp": w.samples, "bounces": w.bounces, "seed": w.seed, "env": "synthetic 1024x512 HDR sky",
-                       "bvh_width": ctx.get_tuning().bvh_width},
+                       "bvh_width": ctx.get_tuning().bvh_width, "tuning": ctx.get_tuning().as_dict()},
             "_info": info, "_result": res}
 
 
@@ -613,6 +613,7 @@ def main():
     w = workload(args.workload)
     log(f"workload {w.name}: {w.scene.triangle_count} tris generated in {time.time() - t0:.1f}s")
     ctx = crgpu.Context(0)
+    default_tuning = ctx.get_tuning()  # the plain renders run with the library's default configuration
     plain = measure_plain(ctx, w, args.steps, args.warmup, None if args.no_cpu else args.cpu_budget)
     info, res = plain.pop("_info"), plain.pop("_result")
     value, sec_per_step = plain["value"], plain["ms_per_step"] / 1e3
@@ -650,6 +651,7 @@ def main():
         t0 = time.time()
         w3 = workload("c3")
         log(f"workload {w3.name}: {w3.scene.triangle_count} tris generated in {time.time() - t0:.1f}s")
+        ctx.set_tuning(default_tuning)  # the batched form left its autotuned configuration applied
         config3 = measure_plain(ctx, w3, min(args.steps, 3), 1, None if args.no_cpu else args.cpu_budget)
         config3.pop("_result")
         config3.pop("_info")

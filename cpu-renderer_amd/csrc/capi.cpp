@@ -1285,7 +1285,7 @@ int crg_autotune(crg_ctx* ctx, const crg_request* sample, crg_tuning* out, doubl
             }
         };
         try_field(&crg_tuning::bvh_width, {4, 8});
-        try_field(&crg_tuning::pool_paths, {1u << 20, 1u << 21, 1u << 22, 1u << 23});
+        try_field(&crg_tuning::pool_paths, {1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25});
         try_field(&crg_tuning::groups_per_cu, {4, 8, 16});
         try_field(&crg_tuning::workgroup, {64, 128, 256});
         try_field(&crg_tuning::iterations_per_graph, {2, 4, 8});

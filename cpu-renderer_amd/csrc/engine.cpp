@@ -108,7 +108,11 @@ Engine::Engine(int device, unsigned compute_units) : device_(device), cus_(compu
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b)
         budget_ = std::min<uint64_t>(budget_, uint64_t(total_b) - std::min<uint64_t>(total_b / 16, 8ull << 30));
-    tune_ = crg_tuning{1u << 22, 256, 8, 4, 4};
+    // 32 M path slots (12.9 GB): longer persistent traversal runs and a third
+    // of the iterations of a 4 M pool (config-2 render 289 -> 339 Msamples/s);
+    // four 256-lane workgroups per CU (the traversal kernels' occupancy), two
+    // iterations per graph launch.
+    tune_ = crg_tuning{1u << 25, 256, 4, 2, 4};
     slot_job_.resize(kJobSlots);
     worker_ = std::thread([this] { run(); });
 }
