@@ -330,7 +330,7 @@ def batched_form(ctx, count: int) -> dict:
     # autotune on a mid-size request at reduced spp (results never depend on the choice)
     mid = sorted(mix, key=lambda w: w.width * w.height * int(w.scene.triangle_count))[count // 2]
     sid, _ = ctx.upload_scene(mid.scene)
-    tuned, rate = ctx.autotune(sid, env_id, mid.camera, mid.options(samples=4))
+    tuned, rate = ctx.autotune(sid, env_id, mid.camera, mid.options(samples=16))
     ctx.release_scene(sid)
     log(f"batched: autotuned {tuned.as_dict()} ({rate / 1e6:.1f} Msamples/s on the sample)")
     # plain drop-in, one request at a time

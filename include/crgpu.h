@@ -329,7 +329,9 @@ int crg_get_tuning(crg_ctx* ctx, crg_tuning* out);
 /* Waits for the requests in flight, then applies the configuration. */
 int crg_set_tuning(crg_ctx* ctx, const crg_tuning* tuning);
 /* Times short renders of `sample` under candidate configurations, applies the
- * fastest and returns it (with its measured rate in *samples_per_second). */
+ * fastest and returns it (with its measured rate in *samples_per_second).  Pool
+ * sizes are searched only when the sample has at least half as many work items
+ * as the configured pool has slots; otherwise the configured pool is kept. */
 int crg_autotune(crg_ctx* ctx, const crg_request* sample, crg_tuning* out, double* samples_per_second);
 
 /* ---- film encode -------------------------------------------------------- */
