@@ -32,7 +32,7 @@ struct RayQuery {
     float ovec;     // 2.02*|olow|_inf: bound on the split-origin part of FP32 vertex offsets
     bool precise;   // origin_low != 0
     bool valid;
-    bool have_exact;
+    bool have_exact, have_esz;
     sf::f64 esx, esy, esz;  // binary64 shear constants, computed on first use
 };
 
@@ -50,6 +50,7 @@ CR_HDI void setup_query(RayQuery& q, V3 o, V3 d, float tmin, float tmax, V3 olow
     q.valid = valid_ray(o, olow, d, tmin, tmax);
     q.precise = nonzero(olow);
     q.have_exact = false;
+    q.have_esz = false;
     int kz = 0;
     if (fabsf(d.y) > fabsf(at(d, kz))) kz = 1;
     if (fabsf(d.z) > fabsf(at(d, kz))) kz = 2;
@@ -85,14 +86,77 @@ CR_HDI void ensure_exact_shear(RayQuery& q) {
     const sf::f64 dz = sf::from_float(at(q.d, q.kz));
     q.esx = sf::div(sf::from_float(at(q.d, q.kx)), dz);
     q.esy = sf::div(sf::from_float(at(q.d, q.ky)), dz);
-    q.esz = sf::div(sf::one(), dz);
     q.have_exact = true;
+}
+
+CR_HDI void ensure_exact_esz(RayQuery& q) {
+    if (q.have_esz) return;
+    q.esz = sf::div(sf::one(), sf::from_float(at(q.d, q.kz)));
+    q.have_esz = true;
 }
 
 struct ExactHit {
     bool hit;
     sf::f64 t, u, v;
 };
+
+// Origin component o + olow in binary64 when olow is a zero: o itself, with
+// the sign of a zero sum (-0 only when both are -0).
+CR_HDI float origin_of_zero_low(float o, float olow) {
+    return o == 0.0f ? sf::bitsf(sf::fbits(o) & sf::fbits(olow) & 0x80000000u) : o;
+}
+
+// Vertex minus ray origin in binary64, components permuted to (kx, ky, kz).
+// Without a low origin part every component is a difference of two floats.
+CR_HDI void rel_vertex(const RayQuery& q, V3 v, const D3& origin, sf::f64& x, sf::f64& y, sf::f64& z) {
+    if (!q.precise) {
+        x = sf::sub_floats(at(v, q.kx), origin_of_zero_low(at(q.o, q.kx), at(q.olow, q.kx)));
+        y = sf::sub_floats(at(v, q.ky), origin_of_zero_low(at(q.o, q.ky), at(q.olow, q.ky)));
+        z = sf::sub_floats(at(v, q.kz), origin_of_zero_low(at(q.o, q.kz), at(q.olow, q.kz)));
+        return;
+    }
+    const D3 a = dsub(widen(v), origin);
+    x = dat(a, q.kx);
+    y = dat(a, q.ky);
+    z = dat(a, q.kz);
+}
+
+// The reference's sheared edge functions (reference src/geometry.cpp:154-176):
+// the z offsets of the three vertices, w0..w2 and det = w0 + w1 + w2, each in
+// binary64 in the reference's expression order.
+struct EdgeFunctions {
+    sf::f64 az, bz, cz, w0, w1, w2, det;
+};
+
+CR_HDI EdgeFunctions edge_functions(RayQuery& q, V3 va, V3 vb, V3 vc) {
+    ensure_exact_shear(q);
+    D3 origin{sf::zero(0), sf::zero(0), sf::zero(0)};
+    if (q.precise) origin = dadd(widen(q.o), widen(q.olow));
+    EdgeFunctions e;
+    sf::f64 ax, ay, bx, by, cx, cy;
+    rel_vertex(q, va, origin, ax, ay, e.az);
+    rel_vertex(q, vb, origin, bx, by, e.bz);
+    rel_vertex(q, vc, origin, cx, cy, e.cz);
+    ax = sf::sub(ax, sf::mul(q.esx, e.az));
+    ay = sf::sub(ay, sf::mul(q.esy, e.az));
+    bx = sf::sub(bx, sf::mul(q.esx, e.bz));
+    by = sf::sub(by, sf::mul(q.esy, e.bz));
+    cx = sf::sub(cx, sf::mul(q.esx, e.cz));
+    cy = sf::sub(cy, sf::mul(q.esy, e.cz));
+    e.w0 = sf::sub(sf::mul(cx, by), sf::mul(cy, bx));
+    e.w1 = sf::sub(sf::mul(ax, cy), sf::mul(ay, cx));
+    e.w2 = sf::sub(sf::mul(bx, ay), sf::mul(by, ax));
+    e.det = sf::add(sf::add(e.w0, e.w1), e.w2);
+    return e;
+}
+
+// t = (w0*(sz*az) + w1*(sz*bz) + w2*(sz*cz)) / det in binary64.
+CR_HDI sf::f64 edge_t(RayQuery& q, const EdgeFunctions& e) {
+    ensure_exact_esz(q);
+    const sf::f64 num = sf::add(sf::add(sf::mul(e.w0, sf::mul(q.esz, e.az)), sf::mul(e.w1, sf::mul(q.esz, e.bz))),
+                                sf::mul(e.w2, sf::mul(q.esz, e.cz)));
+    return sf::div(num, e.det);
+}
 
 // The reference's intersectTriangle in emulated binary64, same expression order.
 CR_OUTLINE ExactHit exact_triangle(RayQuery& q, V3 va, V3 vb, V3 vc, sf::f64 limit_in) {
@@ -102,32 +166,32 @@ CR_OUTLINE ExactHit exact_triangle(RayQuery& q, V3 va, V3 vb, V3 vc, sf::f64 lim
     const sf::f64 tmin = sf::from_float(q.tmin);
     const sf::f64 limit = sf::min_(sf::from_float(q.tmax), limit_in);
     if (sf::lt(limit, tmin)) return r;
-    ensure_exact_shear(q);
-    const D3 origin = dadd(widen(q.o), widen(q.olow));
-    const D3 a = dsub(widen(va), origin), b = dsub(widen(vb), origin), c = dsub(widen(vc), origin);
-    const int kx = q.kx, ky = q.ky, kz = q.kz;
-    const sf::f64 az = dat(a, kz), bz = dat(b, kz), cz = dat(c, kz);
-    const sf::f64 ax = sf::sub(dat(a, kx), sf::mul(q.esx, az)), ay = sf::sub(dat(a, ky), sf::mul(q.esy, az));
-    const sf::f64 bx = sf::sub(dat(b, kx), sf::mul(q.esx, bz)), by = sf::sub(dat(b, ky), sf::mul(q.esy, bz));
-    const sf::f64 cx = sf::sub(dat(c, kx), sf::mul(q.esx, cz)), cy = sf::sub(dat(c, ky), sf::mul(q.esy, cz));
-    const sf::f64 w0 = sf::sub(sf::mul(cx, by), sf::mul(cy, bx));
-    const sf::f64 w1 = sf::sub(sf::mul(ax, cy), sf::mul(ay, cx));
-    const sf::f64 w2 = sf::sub(sf::mul(bx, ay), sf::mul(by, ax));
+    const EdgeFunctions e = edge_functions(q, va, vb, vc);
     const sf::f64 z = sf::zero(0);
-    const bool neg = sf::lt(w0, z) || sf::lt(w1, z) || sf::lt(w2, z);
-    const bool pos = sf::gt(w0, z) || sf::gt(w1, z) || sf::gt(w2, z);
+    const bool neg = sf::lt(e.w0, z) || sf::lt(e.w1, z) || sf::lt(e.w2, z);
+    const bool pos = sf::gt(e.w0, z) || sf::gt(e.w1, z) || sf::gt(e.w2, z);
     if (neg && pos) return r;
-    const sf::f64 det = sf::add(sf::add(w0, w1), w2);
-    if (sf::is_zero(det) || !sf::is_finite(det)) return r;
-    const sf::f64 num = sf::add(sf::add(sf::mul(w0, sf::mul(q.esz, az)), sf::mul(w1, sf::mul(q.esz, bz))),
-                                sf::mul(w2, sf::mul(q.esz, cz)));
-    const sf::f64 t = sf::div(num, det);
+    if (sf::is_zero(e.det) || !sf::is_finite(e.det)) return r;
+    const sf::f64 t = edge_t(q, e);
     if (!sf::is_finite(t) || sf::gt(sf::fabs_(t), sf::from_float(kFltMax)) || sf::lt(t, tmin) || sf::gt(t, limit))
         return r;
     r.hit = true;
     r.t = t;
-    r.u = sf::div(w1, det);
-    r.v = sf::div(w2, det);
+    r.u = sf::div(e.w1, e.det);
+    r.v = sf::div(e.w2, e.det);
+    return r;
+}
+
+// Barycentrics (and t when `want_t`) of a hit the traversal has already
+// decided: the same binary64 values exact_triangle returns for it, without
+// its acceptance checks and, unless asked, without t's terms.
+CR_OUTLINE ExactHit exact_hit_uv(RayQuery& q, V3 va, V3 vb, V3 vc, bool want_t) {
+    const EdgeFunctions e = edge_functions(q, va, vb, vc);
+    ExactHit r;
+    r.hit = true;
+    r.t = want_t ? edge_t(q, e) : sf::zero(0);
+    r.u = sf::div(e.w1, e.det);
+    r.v = sf::div(e.w2, e.det);
     return r;
 }
 
@@ -885,6 +949,12 @@ CR_HD inline ClosestResult traverse(const DevScene& s, RayQuery& q, TraceCount& 
 CR_HDI ExactHit exact_of(const DevScene& s, RayQuery& q, uint32_t prim) {
     const PrimRec p = s.prims[prim];
     return exact_triangle(q, V3{p.ax, p.ay, p.az}, V3{p.bx, p.by, p.bz}, V3{p.cx, p.cy, p.cz}, sf::inf(0));
+}
+
+// Exact (u, v), and t when `want_t`, of the closest hit a traversal decided.
+CR_HDI ExactHit exact_uv_of(const DevScene& s, RayQuery& q, uint32_t prim, bool want_t) {
+    const PrimRec p = s.prims[prim];
+    return exact_hit_uv(q, V3{p.ax, p.ay, p.az}, V3{p.bx, p.by, p.bz}, V3{p.cx, p.cy, p.cz}, want_t);
 }
 
 // Exact pass over the candidates an FP32-only traversal left (exact_work 1):

@@ -100,15 +100,18 @@ CR_HDI uint32_t bounce(const DevJob& J, PathState& st, uint32_t prim, ShadowOut&
     RayQuery q;
     setup_query(q, ray_o, ray_d, st.tmin, st.tmax, st.lo, st.prev);
     const uint32_t tri_id = sc.prims[prim].id;
-    const ExactHit hit = exact_of(sc, q, prim);
-    const float tf = sf::to_float(hit.t);
+    // t only feeds the medium's attenuation (and error reports, which
+    // recompute it); u and v always feed the surface reconstruction.
+    const bool in_medium = st.medium >= 0;
+    const ExactHit hit = exact_uv_of(sc, q, prim, in_medium);
+    const float tf = in_medium ? sf::to_float(hit.t) : 0.0f;
     const Surf s = surface_at(sc, prim, ray_d, hit.u, hit.v);
     if (s.texture_error) {
         set_error(err, kErrTexture, depth, tri_id, s.material, sf::zero(0), s.position, ray_o, ray_d);
         st.L = v3(0);
         return kRetire;
     }
-    if (st.medium >= 0) T = T * beer_lambert(st.sigma, tf);
+    if (in_medium) T = T * beer_lambert(st.sigma, tf);
     if ((s.front || s.double_sided) && max_component(s.emission) > 0) {
         const float w = last_delta || !nee
                             ? 1
@@ -219,7 +222,8 @@ CR_HDI uint32_t bounce(const DevJob& J, PathState& st, uint32_t prim, ShadowOut&
         if (boundary) {
             sh.emit = false;
             if (!precise) return kReplay;
-            set_error(err, boundary, depth, tri_id, s.material, hit.t, s.position, ray_o, ray_d);
+            set_error(err, boundary, depth, tri_id, s.material, in_medium ? hit.t : exact_of(sc, q, prim).t,
+                      s.position, ray_o, ray_d);
             st.L = v3(0);
             return kRetire;
         }

@@ -428,6 +428,32 @@ CR_HDI f64 from_float(float x) {
     return f64{(uint64_t(s) << 63) | (uint64_t(ex - 127 + 1023) << 52) | (uint64_t(fr) << 29)};
 }
 
+// double(x) - double(y) for binary32 x, y.  When both are normal and their
+// exponents differ by at most 28, the difference of the two 24-bit significands
+// aligned to the smaller exponent has at most 53 bits, so it is formed exactly
+// in a 64-bit integer and packed without rounding; every other case (zeros,
+// subnormals, infinities, NaN, wider exponent gaps) takes the general path.
+CR_HDI f64 sub_floats(float x, float y) {
+    const uint32_t ux = fbits(x), uy = fbits(y);
+    const int32_t ex = int32_t((ux >> 23) & 0xffu), ey = int32_t((uy >> 23) & 0xffu);
+    const int32_t gap = ex - ey;
+    if (uint32_t(ex - 1) < 254u && uint32_t(ey - 1) < 254u && gap <= 28 && gap >= -28) {
+        const int32_t emin = ex < ey ? ex : ey;
+        int64_t X = int64_t((ux & 0x7fffffu) | 0x800000u) << (ex - emin);
+        int64_t Y = int64_t((uy & 0x7fffffu) | 0x800000u) << (ey - emin);
+        if (ux >> 31) X = -X;
+        if (uy >> 31) Y = -Y;
+        const int64_t D = X - Y;  // |D| < 2^53
+        if (D == 0) return zero(0);
+        const uint64_t mag = D < 0 ? uint64_t(-D) : uint64_t(D);
+        const int32_t lead = 63 - clz64(mag);  // <= 52
+        // D * 2^(emin - 150); biased binary64 exponent emin - 150 + lead + 1023
+        return f64{(uint64_t(D < 0) << 63) | (uint64_t(emin + 873 + lead) << 52) |
+                   ((mag << (52 - lead)) & 0x000fffffffffffffull)};
+    }
+    return sub(from_float(x), from_float(y));
+}
+
 // Correctly rounded narrowing double -> float (round-to-nearest-even).
 CR_HD CR_NOINLINE inline float to_float(f64 a) {
     const uint32_t xa = uint32_t(a.b >> 52) & 0x7ffu;

@@ -81,6 +81,41 @@ int main(int argc, char** argv) {
             if (!same(std::sqrt(v), sf::sqrt(x))) report("sqrt-edge", v, 0, std::sqrt(v), sf::sqrt(x));
         }
     }
+    // Differences of two binary32 values (vertex minus ray origin): any bit
+    // patterns, nearby exponents (the integer path), exponent gaps around its
+    // 28-step limit, equal and opposite values, zeros and subnormals.
+    {
+        auto fbits = [](uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; };
+        const float fspecial[] = {0.0f, -0.0f, 1.0f, -1.0f, INFINITY, -INFINITY, NAN, 0x1p-149f, -0x1p-149f,
+                                  0x1p-126f, 0x1.fffffep127f, -0x1.fffffep127f, 3.0f, 0.1f, 1e-30f, 1e30f};
+        for (float a : fspecial) for (float b : fspecial) {
+            const double ref = double(a) - double(b);
+            if (!same(ref, sf::sub_floats(a, b))) report("sub_floats", a, b, ref, sf::sub_floats(a, b));
+        }
+        for (long i = 0; i < n; ++i) {
+            float a, b;
+            const uint32_t r = uint32_t(g());
+            switch (i % 4) {
+                case 0: a = fbits(uint32_t(g())); b = fbits(uint32_t(g())); break;
+                case 1: {  // exponents within 40 of each other
+                    a = fbits(uint32_t(g()));
+                    const int32_t ea = int32_t((sf::fbits(a) >> 23) & 0xff);
+                    const int32_t eb = std::min(254, std::max(0, ea + int32_t(r % 81) - 40));
+                    b = fbits((uint32_t(g()) & 0x807fffffu) | (uint32_t(eb) << 23));
+                    break;
+                }
+                case 2: a = fbits(uint32_t(g())); b = (r & 1) ? a : -a; break;
+                default: {  // neighbours and near-cancelling values
+                    a = float(int32_t(r % 2000001) - 1000000) * 0.001f;
+                    b = std::nextafter(a, (g() & 1) ? INFINITY : -INFINITY);
+                    if (g() & 1) b = std::ldexp(b, int(g() % 61) - 30);
+                    break;
+                }
+            }
+            const double ref = double(a) - double(b);
+            if (!same(ref, sf::sub_floats(a, b))) report("sub_floats", a, b, ref, sf::sub_floats(a, b));
+        }
+    }
     // floor / fmod(x, 2) / to_i64 (texture coordinate wrapping).
     for (long i = 0; i < n / 4; ++i) {
         double x;
