@@ -298,19 +298,22 @@ __device__ inline void run_visits(T& tr, const DevScene* sc, RayQuery& q, TraceC
 // pointers the accesses would compile to flat instructions.
 using lds_u32 = __attribute__((address_space(3))) uint32_t;
 using lds_f32 = __attribute__((address_space(3))) float;
+// Rows are kRingLanes entries apart whatever the workgroup size (at most
+// 256 threads, one column each): a compile-time row stride makes a slot's
+// address a shift instead of a 32-bit multiply (quarter rate) per access.
+constexpr uint32_t kRingLanes = 256;
 template <int N>
 struct LdsRing {
     static constexpr int kSize = N;
-    lds_u32* ref;  // this lane's column: ref[i * stride]
+    lds_u32* ref;  // this lane's column: ref[i * kRingLanes]
     lds_f32* near;
-    uint32_t stride;
     __device__ void put(int i, uint32_t r, float n) {
-        ref[i * stride] = r;
-        near[i * stride] = n;
+        ref[i * kRingLanes] = r;
+        near[i * kRingLanes] = n;
     }
     __device__ void get(int i, uint32_t& r, float& n) const {
-        r = ref[i * stride];
-        n = near[i * stride];
+        r = ref[i * kRingLanes];
+        n = near[i * kRingLanes];
     }
 };
 using TraverseRing = LdsRing<CR_LDS_RING>;
@@ -351,14 +354,14 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_extend(PathSoA 
                                                 Pending* xpend, int parity, Control* ctl, JobCounters* cnt) {
     if ((ctl->masks != 0) != MASK) return;
     const uint32_t n = ctl->qcount[parity];
-    __shared__ uint32_t ring_ref[CR_LDS_RING * 256];
-    __shared__ float ring_near[CR_LDS_RING * 256];
+    __shared__ uint32_t ring_ref[CR_LDS_RING * kRingLanes];
+    __shared__ float ring_near[CR_LDS_RING * kRingLanes];
     StackEntry overflow[kStackSize];
     Pending cands;
     Traversal<false, false, TraverseRing, (CR_LEAF_PRELOAD != 0), MASK, W> tr;
     tr.stack = overflow;
     tr.cands = &cands;
-    tr.ring = TraverseRing{(lds_u32*)(ring_ref + threadIdx.x), (lds_f32*)(ring_near + threadIdx.x), blockDim.x};
+    tr.ring = TraverseRing{(lds_u32*)(ring_ref + threadIdx.x), (lds_f32*)(ring_near + threadIdx.x)};
     RayQuery q;
     const DevScene* sc = nullptr;
     uint32_t path = 0, job = 0;
@@ -613,14 +616,14 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_shadow(PathSoA 
                                                 Pending* xspend, Control* ctl, JobCounters* cnt) {
     if ((ctl->masks != 0) != MASK) return;
     const uint32_t n = ctl->scount;
-    __shared__ uint32_t ring_ref[CR_LDS_RING * 256];
-    __shared__ float ring_near[CR_LDS_RING * 256];
+    __shared__ uint32_t ring_ref[CR_LDS_RING * kRingLanes];
+    __shared__ float ring_near[CR_LDS_RING * kRingLanes];
     StackEntry overflow[kStackSize];
     Pending cands;
     Traversal<true, false, TraverseRing, (CR_LEAF_PRELOAD != 0), MASK, W> tr;
     tr.stack = overflow;
     tr.cands = &cands;
-    tr.ring = TraverseRing{(lds_u32*)(ring_ref + threadIdx.x), (lds_f32*)(ring_near + threadIdx.x), blockDim.x};
+    tr.ring = TraverseRing{(lds_u32*)(ring_ref + threadIdx.x), (lds_f32*)(ring_near + threadIdx.x)};
     RayQuery q;
     const DevScene* sc = nullptr;
     uint32_t i = 0, job = 0;

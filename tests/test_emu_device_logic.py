@@ -152,6 +152,103 @@ def test_slab_test_is_conservative(emu):
     assert tight_misses > 200  # the set really probes both sides of contact
 
 
+def test_slab_test_keeps_corner_contacts(emu):
+    """Boxes the ray meets at (or within rounding of) one corner: it enters
+    through two faces and leaves through the third at nearly the same t, so
+    every box it meets at all it meets only just.  Every plane is exactly
+    decodable (coordinates on a power-of-two grid whose step is the node's
+    scale), so the decoded box is the box itself and nothing but the slab
+    arithmetic's widening keeps the contact.  The ray origin lies deep inside
+    a node spanning 255 grid steps at coordinates up to 2^12 steps, so the
+    contact distance is small against the node's planes and coordinates (a
+    slab arithmetic whose rounding grew with those, not with the distance,
+    would lose such contacts)."""
+    import ctypes as C
+    rng = np.random.default_rng(5)
+    fp = C.POINTER(C.c_float)
+    f32 = lambda v: np.ascontiguousarray(v, np.float32)
+    hits = 0
+    for case in range(6000):
+        g = 2.0 ** int(rng.integers(-12, 8))             # grid step = the node's scale
+        p = rng.integers(-2 ** 12, 2 ** 12, 3) * g      # node origin (low corner)
+        c = p + rng.integers(64, 192, 3) * g             # the corner
+        sgn = np.where(rng.random(3) < 0.5, -1.0, 1.0)
+        d = f32(sgn * rng.uniform(1.0 / 64, 1.0, 3))
+        t0 = g * rng.uniform(1e-3, 4.0)
+        o = f32(c - t0 * d.astype(np.float64))           # rounded: the contact is only nearly exact
+        k = rng.integers(1, 33, 3) * g
+        lo, hi = c.copy(), c.copy()
+        leave = int(rng.integers(3))                      # the axis whose slab ends at the corner
+        for a in range(3):
+            if (d[a] > 0) != (a == leave):                # entering faces sit at the corner, the box beyond
+                hi[a] = c[a] + k[a]
+            else:
+                lo[a] = c[a] - k[a]
+        sib_lo, sib_hi = f32(p), f32(p + 255 * g)
+        lo, hi = f32(lo), f32(hi)
+        if not _exact_slab(o, d, 0.0, float("inf"), lo, hi):
+            continue
+        got = emu.lib.emu_box_hit(o.ctypes.data_as(fp), d.ctypes.data_as(fp), 0.0, float("inf"),
+                                  lo.ctypes.data_as(fp), hi.ctypes.data_as(fp),
+                                  sib_lo.ctypes.data_as(fp), sib_hi.ctypes.data_as(fp))
+        assert got == 1, (o, d, lo, hi, p, g)
+        hits += 1
+    assert hits > 1500
+
+
+def test_slab_test_keeps_contacts_at_rounded_planes(emu):
+    """Corner contacts where the ray leaves through the node's own far plane
+    on an axis whose scale is half the ulp of the node origin p: the builder's
+    255th step rounds p + 127.5 ulp up to the box's face p + 128 ulp, so the
+    decoded plane is the face while the unrounded sum lies half an ulp inside
+    it (a slab arithmetic that skipped the decode's rounding would lose such
+    contacts).  Every box the ray meets in exact arithmetic must be kept (the
+    other two axes enter at grid-exact planes)."""
+    import ctypes as C
+    rng = np.random.default_rng(6)
+    fp = C.POINTER(C.c_float)
+    f32 = lambda v: np.ascontiguousarray(v, np.float32)
+    hits = 0
+    for case in range(8000):
+        ax = int(rng.integers(3))                         # the axis left through the rounded far plane
+        e = int(rng.integers(-4, 16))
+        m = 2 * int(rng.integers(2 ** 21, 2 ** 22))       # even significand: p + 127.5 ulp rounds up
+        sx = 1.0 if rng.random() < 0.5 else -1.0
+        px = sx * m * 2.0 ** (e - 23)
+        ulp = float(np.spacing(np.float32(abs(px))))
+        if float(np.float32(px + 127.5 * ulp)) != px + 128 * ulp:
+            continue
+        g = 2.0 ** int(rng.integers(-12, 4))
+        pg = rng.integers(-2 ** 10, 2 ** 10, 3) * g       # grid-exact node origins on the other axes
+        corner = pg + rng.integers(64, 192, 3) * g
+        node_lo, node_hi = pg.copy(), pg + 255 * g
+        node_lo[ax], node_hi[ax] = px, px + 128 * ulp
+        corner[ax] = px + 128 * ulp
+        d = rng.uniform(1.0 / 64, 1.0, 3) * np.where(rng.random(3) < 0.5, -1.0, 1.0)
+        d[ax] = abs(d[ax])                                # leaving through the high face
+        d = f32(d)
+        lo, hi = corner.copy(), corner.copy()
+        for a in range(3):
+            if a == ax:
+                lo[a] = corner[a] - int(rng.integers(1, 64)) * ulp
+            elif d[a] > 0:
+                hi[a] = corner[a] + int(rng.integers(1, 33)) * g
+            else:
+                lo[a] = corner[a] - int(rng.integers(1, 33)) * g
+        t0 = min(ulp, g) * rng.uniform(0.5, 40.0)
+        o = f32(corner - t0 * d.astype(np.float64))
+        lo, hi = f32(lo), f32(hi)
+        if not _exact_slab(o, d, 0.0, float("inf"), lo, hi):
+            continue
+        sib_lo, sib_hi = f32(node_lo), f32(node_hi)
+        got = emu.lib.emu_box_hit(o.ctypes.data_as(fp), d.ctypes.data_as(fp), 0.0, float("inf"),
+                                  lo.ctypes.data_as(fp), hi.ctypes.data_as(fp),
+                                  sib_lo.ctypes.data_as(fp), sib_hi.ctypes.data_as(fp))
+        assert got == 1, (o, d, lo, hi, node_lo, node_hi)
+        hits += 1
+    assert hits > 1000
+
+
 def test_emu_camera_bit_exact(emu):
     import ctypes as C
     g = load_golden("rng_camera.npz")
