@@ -969,12 +969,15 @@ __global__ void __launch_bounds__(256) k_probe(ProbeArgs a) {
         default: {  // CRG_PROBE_SURFACE_AT
             const crg_probe_surface_in& q = static_cast<const crg_probe_surface_in*>(a.in)[i];
             const uint32_t prim = a.tri_to_prim[q.hit.triangle];
-            const Surf s = surface_at(*a.scene, prim, V3{q.ray.direction[0], q.ray.direction[1], q.ray.direction[2]},
-                                      sf::f64{__builtin_bit_cast(uint64_t, q.hit.u)},
-                                      sf::f64{__builtin_bit_cast(uint64_t, q.hit.v)});
+            // The position as shading computes it (float-float where that
+            // decides its rounding), the low part from the binary64 chain.
+            const V3 dir{q.ray.direction[0], q.ray.direction[1], q.ray.direction[2]};
+            const sf::f64 hu{__builtin_bit_cast(uint64_t, q.hit.u)}, hv{__builtin_bit_cast(uint64_t, q.hit.v)};
+            const Surf s = surface_at(*a.scene, prim, dir, hu, hv, false);
+            const Surf sl = surface_at(*a.scene, prim, dir, hu, hv, true);
             crg_surface r{};
             put3(r.position, s.position);
-            put3(r.position_low, s.position_low);
+            put3(r.position_low, sl.position_low);
             put3(r.geometric_normal, s.ng);
             put3(r.normal, s.normal);
             put3(r.base_color, s.base);

@@ -105,7 +105,7 @@ CR_HDI uint32_t bounce(const DevJob& J, PathState& st, uint32_t prim, ShadowOut&
     const bool in_medium = st.medium >= 0;
     const ExactHit hit = exact_uv_of(sc, q, prim, in_medium);
     const float tf = in_medium ? sf::to_float(hit.t) : 0.0f;
-    const Surf s = surface_at(sc, prim, ray_d, hit.u, hit.v);
+    const Surf s = surface_at(sc, prim, ray_d, hit.u, hit.v, precise);
     if (s.texture_error) {
         set_error(err, kErrTexture, depth, tri_id, s.material, sf::zero(0), s.position, ray_o, ray_d);
         st.L = v3(0);

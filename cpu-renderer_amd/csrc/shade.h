@@ -252,21 +252,29 @@ CR_OUTLINE void apply_maps(const DevScene& sc, const DevMaterial& m, uint32_t pr
     s.texture_error = bad;
 }
 
-CR_OUTLINE Surf surface_at(const DevScene& sc, uint32_t prim, V3 ray_dir, sf::f64 hu, sf::f64 hv) {
+// `want_low`: also the binary64 position's low part (precise origins need it).
+// Without it the float position alone usually follows from a float-float sum,
+// and the binary64 chain runs only where that cannot decide its rounding.
+CR_OUTLINE Surf surface_at(const DevScene& sc, uint32_t prim, V3 ray_dir, sf::f64 hu, sf::f64 hv, bool want_low) {
     const PrimRec p = sc.prims[prim];
     const PrimAux& ax = sc.aux[prim];
     Surf s;
     const sf::f64 w = sf::sub(sf::sub(sf::one(), hu), hv);
-    const D3 P = dadd(dadd(dscale(widen(V3{p.ax, p.ay, p.az}), w), dscale(widen(V3{p.bx, p.by, p.bz}), hu)),
-                      dscale(widen(V3{p.cx, p.cy, p.cz}), hv));
-    s.position = narrow(P);
-    s.position_low = V3{sf::to_float(sf::sub(P.x, sf::from_float(s.position.x))),
-                        sf::to_float(sf::sub(P.y, sf::from_float(s.position.y))),
-                        sf::to_float(sf::sub(P.z, sf::from_float(s.position.z)))};
+    const V3 va{p.ax, p.ay, p.az}, vb{p.bx, p.by, p.bz}, vc{p.cx, p.cy, p.cz};
+    FF fw, fu, fv;
+    const bool have_ff = ff_of(w, fw) && ff_of(hu, fu) && ff_of(hv, fv);
+    s.position_low = V3{0, 0, 0};
+    if (want_low || !have_ff || !interp_point_ff(va, vb, vc, fw, fu, fv, s.position)) {
+        const D3 P = dadd(dadd(dscale(widen(va), w), dscale(widen(vb), hu)), dscale(widen(vc), hv));
+        s.position = narrow(P);
+        s.position_low = V3{sf::to_float(sf::sub(P.x, sf::from_float(s.position.x))),
+                            sf::to_float(sf::sub(P.y, sf::from_float(s.position.y))),
+                            sf::to_float(sf::sub(P.z, sf::from_float(s.position.z)))};
+    }
     s.ng = V3{ax.ng[0], ax.ng[1], ax.ng[2]};
     s.front = dot_sign_exact(ray_dir, s.ng) < 0;
     const V3 n0{ax.n0[0], ax.n0[1], ax.n0[2]}, n1{ax.n1[0], ax.n1[1], ax.n1[2]}, n2{ax.n2[0], ax.n2[1], ax.n2[2]};
-    if (!interp_unit_ff(n0, n1, n2, w, hu, hv, s.normal))
+    if (!have_ff || !interp_unit_ff(n0, n1, n2, fw, fu, fv, s.normal))
         s.normal = dunit_ff(dadd(dadd(dscale(widen(n0), w), dscale(widen(n1), hu)), dscale(widen(n2), hv)), s.ng);
     if (dot(s.normal, s.ng) < 0) s.normal = -s.normal;
     s.material = p.material;

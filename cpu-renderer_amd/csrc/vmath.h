@@ -236,9 +236,7 @@ CR_HDI V3 dunit_ff(const D3& v, V3 fallback) {
 // subnormal range), then unit_ff.  A component whose three
 // inputs are zero is the signed zero of the binary64 sum.  False where the
 // caller's binary64 chain must decide.
-CR_HDI bool interp_unit_ff(V3 n0, V3 n1, V3 n2, sf::f64 w, sf::f64 hu, sf::f64 hv, V3& out) {
-    FF fw, fu, fv;
-    if (!ff_of(w, fw) || !ff_of(hu, fu) || !ff_of(hv, fv)) return false;
+CR_HDI bool interp_unit_ff(V3 n0, V3 n1, V3 n2, FF fw, FF fu, FF fv, V3& out) {
     FF c[3];
     float err[3];
     for (int k = 0; k < 3; ++k) {
@@ -254,6 +252,38 @@ CR_HDI bool interp_unit_ff(V3 n0, V3 n1, V3 n2, sf::f64 w, sf::f64 hu, sf::f64 h
     }
     float o[3];
     if (!unit_ff(c, err, o)) return false;
+    out = V3{o[0], o[1], o[2]};
+    return true;
+}
+CR_HDI bool interp_unit_ff(V3 n0, V3 n1, V3 n2, sf::f64 w, sf::f64 hu, sf::f64 hv, V3& out) {
+    FF fw, fu, fv;
+    if (!ff_of(w, fw) || !ff_of(hu, fu) || !ff_of(hv, fv)) return false;
+    return interp_unit_ff(n0, n1, n2, fw, fu, fv, out);
+}
+
+// float((a * w + b * hu) + e * hv) per component for float vectors a, b, e and
+// binary64 weights, the sum being the reference's binary64 chain (each step
+// rounded): the sum in float-float as in interp_unit_ff (within 2^-44 of the
+// terms' magnitudes, plus 2^-120, of the chain's value), kept only where that
+// distance leaves the float rounding of the chain decided.  A component whose
+// three inputs are zero is the signed zero of the binary64 sum.  False where
+// the caller's binary64 chain must decide.
+CR_HDI bool interp_point_ff(V3 a, V3 b, V3 e, FF fw, FF fu, FF fv, V3& out) {
+    float o[3];
+    for (int k = 0; k < 3; ++k) {
+        const float x = at(a, k), y = at(b, k), z = at(e, k);
+        if (x == 0.0f && y == 0.0f && z == 0.0f) {
+            o[k] = (x * fw.hi + y * fu.hi) + z * fv.hi;
+            continue;
+        }
+        const FF t0 = ff_mul(FF{x, 0.0f}, fw), t1 = ff_mul(FF{y, 0.0f}, fu), t2 = ff_mul(FF{z, 0.0f}, fv);
+        const FF c = ff_add(ff_add(t0, t1), t2);
+        const float err = 0x1p-44f * ((fabsf(t0.hi) + fabsf(t1.hi)) + fabsf(t2.hi)) + 0x1p-120f;
+        const float af = fabsf(c.hi);
+        const float half_ulp = 0.5f * fminf(nextafterf(af, INFINITY) - af, af - nextafterf(af, 0.0f));
+        if (!(af >= 0x1p-100f) || !(af < 0x1p100f) || !(half_ulp - fabsf(c.lo) > 1.01f * err)) return false;
+        o[k] = c.hi;
+    }
     out = V3{o[0], o[1], o[2]};
     return true;
 }

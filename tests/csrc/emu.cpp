@@ -330,6 +330,28 @@ void emu_interp_unit(const float* n, const uint64_t* uv, float* ff, float* chain
     }
 }
 
+// surfaceAt's hit point float((a w + b hu) + c hv) per component: float-float
+// (interp_point_ff, else the chain) and the binary64 chain alone; per case 9 floats a, b, c
+// (triangle vertices) and 2 binary64 weights hu, hv (w = (1 - hu) - hv in binary64).
+// decided[i] = 1 where the float-float path alone gave the result.
+void emu_interp_point(const float* v, const uint64_t* uv, float* ff, float* chain, uint8_t* decided,
+                      uint64_t count) {
+    for (uint64_t i = 0; i < count; ++i) {
+        const V3 a{v[9 * i], v[9 * i + 1], v[9 * i + 2]}, b{v[9 * i + 3], v[9 * i + 4], v[9 * i + 5]},
+            c{v[9 * i + 6], v[9 * i + 7], v[9 * i + 8]};
+        const sf::f64 hu{uv[2 * i]}, hv{uv[2 * i + 1]};
+        const sf::f64 w = sf::sub(sf::sub(sf::one(), hu), hv);
+        const V3 x = narrow(dadd(dadd(dscale(widen(a), w), dscale(widen(b), hu)), dscale(widen(c), hv)));
+        FF fw, fu, fv;
+        V3 f;
+        const bool ok = ff_of(w, fw) && ff_of(hu, fu) && ff_of(hv, fv) && interp_point_ff(a, b, c, fw, fu, fv, f);
+        if (!ok) f = x;
+        decided[i] = ok ? 1 : 0;
+        ff[3 * i] = f.x, ff[3 * i + 1] = f.y, ff[3 * i + 2] = f.z;
+        chain[3 * i] = x.x, chain[3 * i + 1] = x.y, chain[3 * i + 2] = x.z;
+    }
+}
+
 // One coordinate of offsetRay's shifted point: float-float (offset_coord) and the binary64
 // emulation, for coordinate p, scale and normal component n.
 void emu_offset(const float* p, const float* scale, const float* n, float* ff, float* chain, uint64_t count) {

@@ -53,6 +53,7 @@ class Emu:
             "emu_dunit": (None, [C.POINTER(C.c_uint64), fp, fp, u64]),
             "emu_offset": (None, [fp, fp, fp, fp, fp, u64]),
             "emu_interp_unit": (None, [fp, C.POINTER(C.c_uint64), fp, fp, u64]),
+            "emu_interp_point": (None, [fp, C.POINTER(C.c_uint64), fp, fp, C.POINTER(C.c_uint8), u64]),
             "emu_camera_pair": (C.c_int, [C.POINTER(abi.Camera), fp, fp, u64, C.c_int, C.c_int, fp, fp]),
             "emu_mask_check": (C.c_int, [vp, vp, u64, C.POINTER(u64)]),
             "emu_film_png8": (C.c_int, [fp, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_uint8)]),

@@ -625,6 +625,72 @@ def test_interpolated_normal_float_float_equals_the_binary64_chain(emu):
     np.testing.assert_array_equal(chain[ok].view(np.uint32), want.view(np.uint32))
 
 
+def test_interpolated_point_float_float_equals_the_binary64_chain(emu):
+    """surfaceAt's hit point float((a w + b hu) + c hv) (reference src/geometry.cpp:401-412,
+    binary64 sums, then narrowed) in float-float, kept only where that decides the rounding,
+    equals the binary64 chain on 1.2 M cases, and the chain equals numpy's binary64 evaluation:
+    small and large triangles at coordinates from 1e-3 to 1e6, triangles straddling a zero
+    coordinate (cancellation), slivers, axis-aligned triangles with signed zero coordinates,
+    barycentrics at the edges and corners, and points put on a binary32 rounding midpoint (the
+    float-float sum cannot round those; the chain must).  Most of the other cases must take
+    the float-float path."""
+    fp = C.POINTER(C.c_float)
+    g = np.random.Generator(np.random.PCG64(137))
+    m = 200_000
+
+    def tris(scale, size):
+        c = g.normal(size=(m, 1, 3)) * scale
+        return (c + g.normal(size=(m, 3, 3)) * size).astype(np.float32)
+
+    small = tris(10.0, 0.01)
+    large = tris(1e6, 1e3)
+    tiny = tris(1e-3, 1e-5)
+    straddle = (g.normal(size=(m, 3, 3)) * 0.5).astype(np.float32)
+    sliver = tris(100.0, 1.0)
+    sliver[:, 2] = (sliver[:, 0].astype(np.float64) + (sliver[:, 1].astype(np.float64) - sliver[:, 0]) *
+                    g.uniform(0, 1, (m, 1)) + g.normal(size=(m, 3)) * 1e-6).astype(np.float32)
+    flat = tris(5.0, 1.0)
+    ax = g.integers(0, 3, m)
+    flat[np.arange(m), :, ax] = np.where(g.integers(0, 2, m) == 1, 0.0, -0.0)[:, None]
+    # Points on a binary32 rounding midpoint: a + hu (b - a) with a, b on the same binade,
+    # hu the binary64 quotient putting the point on a midpoint between them, so only the
+    # binary64 chain's roundings decide which float the point narrows to.
+    a = np.abs(g.normal(size=(m, 3)) * 10.0 ** g.uniform(-2, 5, (m, 1))).astype(np.float32)
+    ulp = np.spacing(a).astype(np.float64)
+    steps = g.integers(2, 1000, (m, 1)).astype(np.float64)
+    b = (a.astype(np.float64) + steps * ulp).astype(np.float32)
+    mid = a.astype(np.float64) + (np.floor(g.uniform(0, 1, (m, 1)) * (steps - 1)) + 0.5) * ulp
+    axis_mid = g.integers(0, 3, m)
+    hu_mid = (mid[np.arange(m), axis_mid] - a[np.arange(m), axis_mid]) / \
+        (b[np.arange(m), axis_mid].astype(np.float64) - a[np.arange(m), axis_mid])
+    midpoint = np.stack([a, b, (a.astype(np.float64) * 1.5).astype(np.float32)], 1)
+    verts = np.concatenate([small, large, tiny, straddle, sliver, flat, midpoint]).reshape(-1, 9)
+    k = len(verts)
+    hu = g.uniform(0, 1, k)
+    hv = g.uniform(0, 1, k) * (1 - hu)
+    corner = g.integers(0, 6, k)
+    hu = np.where(corner == 0, 0.0, np.where(corner == 1, 1.0, hu))
+    hv = np.where(corner == 0, np.where(g.integers(0, 2, k) == 1, 1.0, 0.0), np.where(corner == 1, 0.0, hv))
+    hv = np.where(corner == 2, 1e-12 * g.uniform(size=k), hv)
+    hu[6 * m:] = hu_mid
+    hv[6 * m:] = 0.0
+    uv = np.ascontiguousarray(np.stack([hu, hv], 1)).view(np.uint64)
+    v32 = np.ascontiguousarray(verts, np.float32)
+    ff = np.empty((k, 3), np.float32)
+    chain = np.empty_like(ff)
+    decided = np.zeros(k, np.uint8)
+    emu.lib.emu_interp_point(v32.ctypes.data_as(fp), uv.ctypes.data_as(C.POINTER(C.c_uint64)),
+                             ff.ctypes.data_as(fp), chain.ctypes.data_as(fp),
+                             decided.ctypes.data_as(C.POINTER(C.c_uint8)), k)
+    bad = np.nonzero(np.any(ff.view(np.uint32) != chain.view(np.uint32), axis=1))[0]
+    assert len(bad) == 0, [(v32[i].tolist(), hu[i], hv[i], ff[i].tolist(), chain[i].tolist()) for i in bad[:3]]
+    assert decided[:6 * m].mean() > 0.99, decided[:6 * m].mean()
+    vd = v32.astype(np.float64).reshape(-1, 3, 3)
+    w = (1.0 - hu) - hv
+    P = (vd[:, 0] * w[:, None] + vd[:, 1] * hu[:, None]) + vd[:, 2] * hv[:, None]
+    np.testing.assert_array_equal(chain.view(np.uint32), P.astype(np.float32).view(np.uint32))
+
+
 def test_camera_float_float_equals_the_binary64_chain(emu):
     """Camera::generate (reference src/geometry.cpp:213-234) in float-float with the binary64
     chain for hard cases equals the chain alone (direction, near and far parameters, accept)
