@@ -195,6 +195,118 @@ CR_OUTLINE ExactHit exact_hit_uv(RayQuery& q, V3 va, V3 vb, V3 vc, bool want_t) 
     return r;
 }
 
+// Barycentrics of a hit the traversal decided, as float pairs with bounds on
+// their distance from the reference's binary64 values (the chain of
+// exact_hit_uv): u = w1/det, v = w2/det and w = (1 - u) - v formed from the
+// pairs u and v, so that w errs by minus their errors plus at most ew.
+struct HitUvFF {
+    FF u, v, w;
+    float eu, ev, ew;
+    // For hit_t_ff: the sheared z offsets' pairs, the edge functions, det and bounds.
+    FF az[3], W[3], det;
+    float eW[3], edet;
+};
+
+// The edge functions in float-float with a running error bound.  Per step the
+// pair's arithmetic errs by at most 2^-46 and the reference's binary64 step by
+// 2^-53 of the step's operand magnitudes; kStep = 2^-45 covers both with room
+// (the vertex offsets are exact pairs; the reference's may round once).  An
+// input error propagates through a product as (|x| + ex) ey + |y| ex and
+// through a quotient as (e_num + |q| e_den) / (|den| - e_den).  False for
+// split-origin queries and where the bound cannot separate det from zero;
+// the binary64 chain decides there.
+CR_HDI bool hit_uv_ff(const RayQuery& q, V3 va, V3 vb, V3 vc, HitUvFF& h) {
+    constexpr float kStep = 0x1p-45f, kFloor = 0x1p-110f;
+    if (q.precise || !q.valid) return false;
+    const int kx = q.kx, ky = q.ky, kz = q.kz;
+    const float dz = at(q.d, kz);
+    const FF sx = ff_div2(FF{at(q.d, kx), 0.0f}, FF{dz, 0.0f}), sy = ff_div2(FF{at(q.d, ky), 0.0f}, FF{dz, 0.0f});
+    const float ox = at(q.o, kx), oy = at(q.o, ky), oz = at(q.o, kz);
+    const V3 vs[3] = {va, vb, vc};
+    FF X[3], Y[3];
+    float eX[3], eY[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const FF ax = ff_two_sum(at(vs[j], kx), -ox), ay = ff_two_sum(at(vs[j], ky), -oy);
+        const FF az = ff_two_sum(at(vs[j], kz), -oz);
+        const FF px = ff_mul(sx, az), py = ff_mul(sy, az);
+        h.az[j] = az;
+        X[j] = ff_add(ax, ff_neg(px));
+        Y[j] = ff_add(ay, ff_neg(py));
+        eX[j] = kStep * (fabsf(ax.hi) + fabsf(px.hi)) + kFloor;
+        eY[j] = kStep * (fabsf(ay.hi) + fabsf(py.hi)) + kFloor;
+    }
+    // w0 = cx*by - cy*bx, w1 = ax*cy - ay*cx, w2 = bx*ay - by*ax
+    const int i0[3] = {2, 0, 1}, i1[3] = {1, 2, 0};
+    FF W[3];
+    float eW[3], mW = 0.0f, sW = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int m = i0[k], n = i1[k];
+        const FF p0 = ff_mul(X[m], Y[n]), p1 = ff_mul(Y[m], X[n]);
+        W[k] = ff_add(p0, ff_neg(p1));
+        eW[k] = ((fabsf(X[m].hi) + eX[m]) * eY[n] + fabsf(Y[n].hi) * eX[m]) +
+                ((fabsf(Y[m].hi) + eY[m]) * eX[n] + fabsf(X[n].hi) * eY[m]);
+        eW[k] = 1.01f * eW[k] + kStep * (fabsf(p0.hi) + fabsf(p1.hi)) + kFloor;
+        sW += eW[k];
+        mW += fabsf(W[k].hi);
+    }
+    const FF det = ff_add(ff_add(W[0], W[1]), W[2]);
+    const float edet = 1.01f * sW + 2.0f * kStep * mW + kFloor;
+    const float adet = fabsf(det.hi);
+    if (!(adet > 4.0f * edet) || !(adet >= 0x1p-100f) || !(adet < 0x1p100f)) return false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        h.W[k] = W[k];
+        h.eW[k] = eW[k];
+    }
+    h.det = det;
+    h.edet = edet;
+    h.u = ff_div2(W[1], det);
+    h.v = ff_div2(W[2], det);
+    const float au = fabsf(h.u.hi), av = fabsf(h.v.hi);
+    h.eu = 1.01f * (eW[1] + 1.01f * au * edet) / (adet - edet) + kStep * au + kFloor;
+    h.ev = 1.01f * (eW[2] + 1.01f * av * edet) / (adet - edet) + kStep * av + kFloor;
+    h.w = ff_add(ff_add(FF{1.0f, 0.0f}, ff_neg(h.u)), ff_neg(h.v));
+    h.ew = kStep * ((1.0f + au) + av) + kFloor;
+    return finite1(h.u.hi) && finite1(h.v.hi) && finite1(h.w.hi) && h.ew < 0x1p-20f;
+}
+
+// The hit's t = (w0 (sz az) + w1 (sz bz) + w2 (sz cz)) / det, sz = 1/d_kz, as
+// a float pair from hit_uv_ff's pairs, with a bound `et` on its distance from
+// the reference's binary64 t (same step model).
+CR_HDI FF hit_t_pair(const RayQuery& q, const HitUvFF& h, float& et) {
+    constexpr float kStep = 0x1p-45f, kFloor = 0x1p-110f;
+    const FF sz = ff_div2(FF{1.0f, 0.0f}, FF{at(q.d, q.kz), 0.0f});
+    FF num{0.0f, 0.0f};
+    float enum_ = 0.0f, mnum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const FF z = ff_mul(sz, h.az[k]);
+        const float ez = 2.0f * kStep * fabsf(z.hi) + kFloor;
+        const FF term = ff_mul(h.W[k], z);
+        enum_ += ((fabsf(h.W[k].hi) + h.eW[k]) * ez + fabsf(z.hi) * h.eW[k]) * 1.01f + kStep * fabsf(term.hi);
+        mnum += fabsf(term.hi);
+        num = k == 0 ? term : ff_add(num, term);
+    }
+    enum_ += 2.0f * kStep * mnum + kFloor;
+    const FF t = ff_div2(num, h.det);
+    const float adet = fabsf(h.det.hi), at_ = fabsf(t.hi);
+    et = 1.01f * (enum_ + 1.01f * at_ * h.edet) / (adet - h.edet) + kStep * at_ + kFloor;
+    return t;
+}
+
+// float(t) where the pair's bound decides the reference's rounding.
+CR_HDI bool hit_t_ff(const RayQuery& q, const HitUvFF& h, float& tf) {
+    float et;
+    const FF t = hit_t_pair(q, h, et);
+    const float at_ = fabsf(t.hi);
+    const float half_ulp = 0.5f * fminf(nextafterf(at_, INFINITY) - at_, at_ - nextafterf(at_, 0.0f));
+    if (!(at_ >= 0x1p-100f) || !(at_ < 0x1p100f) || !(half_ulp - fabsf(t.lo) > 1.01f * et)) return false;
+    tf = t.hi;
+    return true;
+}
+
 enum FastResult { kReject = 0, kAccept = 1, kUnsure = 2 };
 
 // FP32 watertight test with a forward error bound.  On kAccept, [tlo, thi]

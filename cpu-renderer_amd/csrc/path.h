@@ -100,12 +100,26 @@ CR_HDI uint32_t bounce(const DevJob& J, PathState& st, uint32_t prim, ShadowOut&
     RayQuery q;
     setup_query(q, ray_o, ray_d, st.tmin, st.tmax, st.lo, st.prev);
     const uint32_t tri_id = sc.prims[prim].id;
-    // t only feeds the medium's attenuation (and error reports, which
-    // recompute it); u and v always feed the surface reconstruction.
+    // t only feeds the medium's attenuation as float(t) (and error reports,
+    // which recompute it); u and v always feed the surface reconstruction.
+    // Both come from float pairs where their bounds decide the results (no
+    // maps, no split origin), else from the binary64 chain.
     const bool in_medium = st.medium >= 0;
-    const ExactHit hit = exact_uv_of(sc, q, prim, in_medium);
-    const float tf = in_medium ? sf::to_float(hit.t) : 0.0f;
-    const Surf s = surface_at(sc, prim, ray_d, hit.u, hit.v, precise);
+    float tf = 0.0f;
+    Surf s;
+    bool have_surface = false;
+    if (!precise && !sc.materials[sc.prims[prim].material].maps) {
+        const PrimRec& p = sc.prims[prim];
+        HitUvFF h;
+        if (hit_uv_ff(q, V3{p.ax, p.ay, p.az}, V3{p.bx, p.by, p.bz}, V3{p.cx, p.cy, p.cz}, h) &&
+            (!in_medium || hit_t_ff(q, h, tf)))
+            have_surface = surface_at_ff(sc, prim, ray_d, h, s);
+    }
+    if (!have_surface) {
+        const ExactHit hit = exact_uv_of(sc, q, prim, in_medium);
+        if (in_medium) tf = sf::to_float(hit.t);
+        s = surface_at(sc, prim, ray_d, hit.u, hit.v, precise);
+    }
     if (s.texture_error) {
         set_error(err, kErrTexture, depth, tri_id, s.material, sf::zero(0), s.position, ray_o, ray_d);
         st.L = v3(0);
@@ -222,8 +236,7 @@ CR_HDI uint32_t bounce(const DevJob& J, PathState& st, uint32_t prim, ShadowOut&
         if (boundary) {
             sh.emit = false;
             if (!precise) return kReplay;
-            set_error(err, boundary, depth, tri_id, s.material, in_medium ? hit.t : exact_of(sc, q, prim).t,
-                      s.position, ray_o, ray_d);
+            set_error(err, boundary, depth, tri_id, s.material, exact_of(sc, q, prim).t, s.position, ray_o, ray_d);
             st.L = v3(0);
             return kRetire;
         }

@@ -284,6 +284,26 @@ CR_OUTLINE Surf surface_at(const DevScene& sc, uint32_t prim, V3 ray_dir, sf::f6
     return s;
 }
 
+// surface_at from float-pair barycentrics with error bounds (hit_uv_ff), for
+// materials without maps and queries without a split origin: the position and
+// the shading normal where their float rounding is decided; false otherwise
+// (the caller then runs the binary64 chain and surface_at).
+CR_OUTLINE bool surface_at_ff(const DevScene& sc, uint32_t prim, V3 ray_dir, const HitUvFF& h, Surf& s) {
+    const PrimRec p = sc.prims[prim];
+    const PrimAux& ax = sc.aux[prim];
+    const V3 va{p.ax, p.ay, p.az}, vb{p.bx, p.by, p.bz}, vc{p.cx, p.cy, p.cz};
+    if (!interp_point_ff(va, vb, vc, h.w, h.u, h.v, s.position, h.ew, h.eu, h.ev)) return false;
+    const V3 n0{ax.n0[0], ax.n0[1], ax.n0[2]}, n1{ax.n1[0], ax.n1[1], ax.n1[2]}, n2{ax.n2[0], ax.n2[1], ax.n2[2]};
+    if (!interp_unit_ff(n0, n1, n2, h.w, h.u, h.v, s.normal, h.ew, h.eu, h.ev)) return false;
+    s.position_low = V3{0, 0, 0};
+    s.ng = V3{ax.ng[0], ax.ng[1], ax.ng[2]};
+    s.front = dot_sign_exact(ray_dir, s.ng) < 0;
+    if (dot(s.normal, s.ng) < 0) s.normal = -s.normal;
+    s.material = p.material;
+    material_into(sc.materials[p.material], s);
+    return true;
+}
+
 // ---------------------------------------------------------------- sampling angles
 // cos and sin of phi = 2*pi*u for the 24-bit draw k (u = k/2^24) come from a
 // table the host fills with its own cosf/sinf on the reference's float

@@ -157,6 +157,14 @@ CR_HDI FF ff_add(FF a, FF b) {
     const FF s = ff_two_sum(a.hi, b.hi);
     return ff_fast_two_sum(s.hi, s.lo + (a.lo + b.lo));
 }
+CR_HDI FF ff_neg(FF a) { return FF{-a.hi, -a.lo}; }
+// a / b for float pairs, to about 2^-46 relative (the remainder of the first
+// quotient digit is exact in the fused multiply-add).
+CR_HDI FF ff_div2(FF a, FF b) {
+    const float q0 = a.hi / b.hi;
+    const float r = (fmaf(-q0, b.hi, a.hi) + a.lo) - q0 * b.lo;
+    return ff_fast_two_sum(q0, r / b.hi);
+}
 
 CR_HDI V3 dunit(const D3& v, V3 fallback) {
     const sf::f64 sq = ddot(v, v);
@@ -233,10 +241,16 @@ CR_HDI V3 dunit_ff(const D3& v, V3 fallback) {
 // reference evaluating the sum in binary64 ((n0 w + n1 hu) + n2 hv, each step
 // rounded): the sum in float-float, whose distance from the reference's is
 // below 2^-44 of the terms' magnitudes (plus 2^-120 for products near the
-// subnormal range), then unit_ff.  A component whose three
+// subnormal range), then unit_ff.  eu, ev bound how far the pairs fu, fv may
+// lie from the reference's binary64 weights beyond that (0: the pairs hold
+// them to 2^-47), and fw = 1 - fu - fv as a pair errs against the reference's
+// (1 - hu) - hv by the negated errors of fu and fv plus at most ew: so the
+// weights' errors move the sum by at most |n1 - n0| eu + |n2 - n0| ev + |n0| ew
+// per component (an edge-length, not a coordinate, times eu and ev).  A component whose three
 // inputs are zero is the signed zero of the binary64 sum.  False where the
 // caller's binary64 chain must decide.
-CR_HDI bool interp_unit_ff(V3 n0, V3 n1, V3 n2, FF fw, FF fu, FF fv, V3& out) {
+CR_HDI bool interp_unit_ff(V3 n0, V3 n1, V3 n2, FF fw, FF fu, FF fv, V3& out, float ew = 0.0f, float eu = 0.0f,
+                           float ev = 0.0f) {
     FF c[3];
     float err[3];
     for (int k = 0; k < 3; ++k) {
@@ -248,7 +262,8 @@ CR_HDI bool interp_unit_ff(V3 n0, V3 n1, V3 n2, FF fw, FF fu, FF fv, V3& out) {
         }
         const FF t0 = ff_mul(FF{a, 0.0f}, fw), t1 = ff_mul(FF{b, 0.0f}, fu), t2 = ff_mul(FF{e, 0.0f}, fv);
         c[k] = ff_add(ff_add(t0, t1), t2);
-        err[k] = 0x1p-44f * ((fabsf(t0.hi) + fabsf(t1.hi)) + fabsf(t2.hi)) + 0x1p-120f;
+        err[k] = 0x1p-44f * ((fabsf(t0.hi) + fabsf(t1.hi)) + fabsf(t2.hi)) + 0x1p-120f +
+                 1.01f * ((fabsf(b - a) * eu + fabsf(e - a) * ev) + fabsf(a) * ew);
     }
     float o[3];
     if (!unit_ff(c, err, o)) return false;
@@ -265,10 +280,12 @@ CR_HDI bool interp_unit_ff(V3 n0, V3 n1, V3 n2, sf::f64 w, sf::f64 hu, sf::f64 h
 // binary64 weights, the sum being the reference's binary64 chain (each step
 // rounded): the sum in float-float as in interp_unit_ff (within 2^-44 of the
 // terms' magnitudes, plus 2^-120, of the chain's value), kept only where that
-// distance leaves the float rounding of the chain decided.  A component whose
+// distance (plus the weight errors ew, eu, ev, as in interp_unit_ff) leaves the
+// float rounding of the chain decided.  A component whose
 // three inputs are zero is the signed zero of the binary64 sum.  False where
 // the caller's binary64 chain must decide.
-CR_HDI bool interp_point_ff(V3 a, V3 b, V3 e, FF fw, FF fu, FF fv, V3& out) {
+CR_HDI bool interp_point_ff(V3 a, V3 b, V3 e, FF fw, FF fu, FF fv, V3& out, float ew = 0.0f, float eu = 0.0f,
+                            float ev = 0.0f) {
     float o[3];
     for (int k = 0; k < 3; ++k) {
         const float x = at(a, k), y = at(b, k), z = at(e, k);
@@ -278,7 +295,8 @@ CR_HDI bool interp_point_ff(V3 a, V3 b, V3 e, FF fw, FF fu, FF fv, V3& out) {
         }
         const FF t0 = ff_mul(FF{x, 0.0f}, fw), t1 = ff_mul(FF{y, 0.0f}, fu), t2 = ff_mul(FF{z, 0.0f}, fv);
         const FF c = ff_add(ff_add(t0, t1), t2);
-        const float err = 0x1p-44f * ((fabsf(t0.hi) + fabsf(t1.hi)) + fabsf(t2.hi)) + 0x1p-120f;
+        const float err = 0x1p-44f * ((fabsf(t0.hi) + fabsf(t1.hi)) + fabsf(t2.hi)) + 0x1p-120f +
+                          1.01f * ((fabsf(y - x) * eu + fabsf(z - x) * ev) + fabsf(x) * ew);
         const float af = fabsf(c.hi);
         const float half_ulp = 0.5f * fminf(nextafterf(af, INFINITY) - af, af - nextafterf(af, 0.0f));
         if (!(af >= 0x1p-100f) || !(af < 0x1p100f) || !(half_ulp - fabsf(c.lo) > 1.01f * err)) return false;

@@ -330,6 +330,63 @@ void emu_interp_unit(const float* n, const uint64_t* uv, float* ff, float* chain
     }
 }
 
+// Shading's surface reconstruction both ways for each ray's closest hit: from float-pair
+// barycentrics with error bounds (hit_uv_ff + surface_at_ff, and hit_t_ff for float(t))
+// and from the binary64 chain (exact_uv_of + surface_at).  Per ray 7 floats out each
+// (position, shading normal, float(t)); decided bit 0: the pair path gave the surface,
+// bit 1: it gave float(t); rays that miss get decided = 4.
+int emu_hit_surface(void* sp, const crg_ray* rays, uint64_t n, float* ff, float* chain, uint8_t* decided) {
+    const EmuScene& s = *static_cast<EmuScene*>(sp);
+    for (uint64_t i = 0; i < n; ++i) {
+        const crg_ray& r = rays[i];
+        RayQuery q;
+        setup_query(q, V3{r.origin[0], r.origin[1], r.origin[2]}, V3{r.direction[0], r.direction[1], r.direction[2]},
+                    r.tmin, r.tmax, V3{r.origin_low[0], r.origin_low[1], r.origin_low[2]}, r.ignore);
+        TraceCount tc{0, 0, 0};
+        const ClosestResult b = traverse<false, true>(s.dev, q, tc);
+        if (b.overflow || b.tex_error) {
+            lastError = "traversal failed";
+            return 1;
+        }
+        decided[i] = 4;
+        if (b.prim == kNoTriangle) continue;
+        const V3 d{r.direction[0], r.direction[1], r.direction[2]};
+        const PrimRec& p = s.dev.prims[b.prim];
+        HitUvFF h;
+        Surf a{};
+        float tf = 0.0f;
+        const bool uv = hit_uv_ff(q, V3{p.ax, p.ay, p.az}, V3{p.bx, p.by, p.bz}, V3{p.cx, p.cy, p.cz}, h);
+        const bool ok = uv && surface_at_ff(s.dev, b.prim, d, h, a);
+        const bool ok_t = uv && hit_t_ff(q, h, tf);
+        const ExactHit e = exact_uv_of(s.dev, q, b.prim, true);
+        const Surf c = surface_at(s.dev, b.prim, d, e.u, e.v, false);
+        const float ct = sf::to_float(e.t);
+        if (!ok) a = c;
+        if (!ok_t) tf = ct;
+        decided[i] = (ok ? 1 : 0) | (ok_t ? 2 : 0);
+        // The pairs' error bounds themselves: bit 3 when u, v, w or t lie farther
+        // from the chain's binary64 values than claimed.
+        if (uv) {
+            auto dbl = [](sf::f64 x) { double d; std::memcpy(&d, &x.b, 8); return (long double)d; };
+            auto pair = [](FF x) { return (long double)x.hi + (long double)x.lo; };
+            const sf::f64 w = sf::sub(sf::sub(sf::one(), e.u), e.v);
+            const long double du = fabsl(pair(h.u) - dbl(e.u)), dv = fabsl(pair(h.v) - dbl(e.v));
+            const long double dw = fabsl((pair(h.w) - dbl(w)) + ((pair(h.u) - dbl(e.u)) + (pair(h.v) - dbl(e.v))));
+            float et = 0.0f;
+            const FF tp = hit_t_pair(q, h, et);
+            bool bad = du > h.eu || dv > h.ev || dw > h.ew || fabsl(pair(tp) - dbl(e.t)) > et;
+            if (bad) decided[i] |= 8;
+        }
+        const float fo[7] = {a.position.x, a.position.y, a.position.z, a.normal.x, a.normal.y, a.normal.z, tf};
+        const float co[7] = {c.position.x, c.position.y, c.position.z, c.normal.x, c.normal.y, c.normal.z, ct};
+        for (int k = 0; k < 7; ++k) {
+            ff[7 * i + k] = fo[k];
+            chain[7 * i + k] = co[k];
+        }
+    }
+    return 0;
+}
+
 // surfaceAt's hit point float((a w + b hu) + c hv) per component: float-float
 // (interp_point_ff, else the chain) and the binary64 chain alone; per case 9 floats a, b, c
 // (triangle vertices) and 2 binary64 weights hu, hv (w = (1 - hu) - hv in binary64).

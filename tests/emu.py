@@ -54,6 +54,7 @@ class Emu:
             "emu_offset": (None, [fp, fp, fp, fp, fp, u64]),
             "emu_interp_unit": (None, [fp, C.POINTER(C.c_uint64), fp, fp, u64]),
             "emu_interp_point": (None, [fp, C.POINTER(C.c_uint64), fp, fp, C.POINTER(C.c_uint8), u64]),
+            "emu_hit_surface": (C.c_int, [vp, vp, u64, fp, fp, C.POINTER(C.c_uint8)]),
             "emu_camera_pair": (C.c_int, [C.POINTER(abi.Camera), fp, fp, u64, C.c_int, C.c_int, fp, fp]),
             "emu_mask_check": (C.c_int, [vp, vp, u64, C.POINTER(u64)]),
             "emu_film_png8": (C.c_int, [fp, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_uint8)]),
@@ -128,6 +129,21 @@ class EmuScene:
             raise RuntimeError(self.e.error())
         self.fallbacks = fb.value
         return hits
+
+    def hit_surface(self, rays):
+        """(ff, chain, decided): each ray's closest-hit position, shading normal and float(t)
+        from float-pair barycentrics (decided bit 0: surface, bit 1: t) or the binary64
+        chain (4 = miss)."""
+        rays = np.ascontiguousarray(rays, abi.RAY_DTYPE)
+        n = len(rays)
+        ff = np.zeros((n, 7), np.float32)
+        chain = np.zeros((n, 7), np.float32)
+        decided = np.zeros(n, np.uint8)
+        fp = C.POINTER(C.c_float)
+        if self.e.lib.emu_hit_surface(self.h, rays.ctypes.data, n, ff.ctypes.data_as(fp), chain.ctypes.data_as(fp),
+                                      decided.ctypes.data_as(C.POINTER(C.c_uint8))):
+            raise RuntimeError(self.e.error())
+        return ff, chain, decided
 
     def render(self, env, cam, opt):
         film = np.zeros((opt.height, opt.width, 3), np.float32)

@@ -691,6 +691,88 @@ def test_interpolated_point_float_float_equals_the_binary64_chain(emu):
     np.testing.assert_array_equal(chain.view(np.uint32), P.astype(np.float32).view(np.uint32))
 
 
+def _edge_rays(scene_data, count, seed):
+    """Rays at triangle edges and vertices (barycentrics within 1e-7 of 0), from near and
+    far (up to 1e4 scene sizes), some grazing the triangle's plane."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    p = scene_data.positions.reshape(-1, 3, 3).astype(np.float64)
+    idx = g.integers(0, len(p), count)
+    a, b, c = p[idx, 0], p[idx, 1], p[idx, 2]
+    t = g.uniform(0, 1, (count, 1))
+    on_edge = a + (b - a) * t + (c - a) * (g.uniform(-1e-7, 1e-7, (count, 1)))
+    corner = np.where(g.integers(0, 2, (count, 1)) == 1, a, b)
+    target = np.where(g.integers(0, 3, (count, 1)) == 0, corner, on_edge)
+    n = np.cross(b - a, c - a)
+    n /= np.linalg.norm(n, axis=1, keepdims=True) + 1e-300
+    d = g.normal(size=(count, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    graze = g.integers(0, 2, count) == 1
+    d[graze] -= n[graze] * np.sum(d[graze] * n[graze], 1, keepdims=True) * (1 - 1e-4)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    size = np.ptp(p.reshape(-1, 3), 0).max()
+    dist = size * 10.0 ** g.uniform(-3, 4, (count, 1))
+    rays = abi.make_rays((target - d * dist).astype(np.float32), d.astype(np.float32))
+    return rays
+
+
+def _midpoint_t_case(count, seed):
+    """Triangles in planes z = f (floats in [1, 2)) and rays from z = -2^-24 along (dx, dy, 1)
+    with small dx, dy: the exact hit distance f + 2^-24 lies on a binary32 rounding midpoint,
+    so only the binary64 chain's roundings (its shear terms are not exact) decide float(t).
+    Each ray meets its own triangle (grid cells 8 apart, triangles 2 across)."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    side = int(np.ceil(np.sqrt(count)))
+    ij = np.stack(np.meshgrid(np.arange(side), np.arange(side)), -1).reshape(-1, 2)[:count]
+    f = (1.0 + g.integers(0, 2 ** 23, count) * 2.0 ** -23).astype(np.float32)
+    o = np.zeros((count, 3), np.float32)
+    o[:, :2] = (ij * 8.0 + g.uniform(-0.5, 0.5, (count, 2))).astype(np.float32)
+    o[:, 2] = -2.0 ** -24
+    d = np.ones((count, 3), np.float32)
+    d[:, :2] = g.uniform(-0.1, 0.1, (count, 2)).astype(np.float32)
+    cx = o[:, 0] + (f + 2.0 ** -24) * d[:, 0]
+    cy = o[:, 1] + (f + 2.0 ** -24) * d[:, 1]
+    tri = np.zeros((count, 3, 3), np.float32)
+    tri[:, 0, :2] = np.stack([cx - 1.0, cy - 1.0], 1)
+    tri[:, 1, :2] = np.stack([cx + 1.0, cy - 0.7], 1)
+    tri[:, 2, :2] = np.stack([cx - 0.3, cy + 1.0], 1)
+    tri[:, :, 2] = f[:, None]
+    scene = abi.SceneData(tri.reshape(-1, 9), np.zeros(count, np.uint32), [abi.material(double_sided=True)])
+    return scene, abi.make_rays(o, d)
+
+
+def test_surface_from_float_pair_barycentrics_equals_the_binary64_chain(emu):
+    """Shading's hit position, shading normal and float(t) from float-pair barycentrics
+    with running error bounds (hit_uv_ff, surface_at_ff, hit_t_ff) equal those from the
+    reference's binary64 chain (reference src/geometry.cpp:147-182 t and barycentrics,
+    :401-415 surfaceAt) on every hit:
+    random and aimed rays on terrain, sphere, soup and box scenes, rays at triangle edges and
+    vertices from up to 1e4 scene sizes away, rays grazing the triangle planes, and hit
+    distances on binary32 rounding midpoints.  The float-pair path must decide at least 95 %
+    of the random rays' surfaces and 90 % of their float(t) (edge and grazing rays from afar
+    are where its bounds give way to the chain)."""
+    cases = [scenes.config2(grid=96, spheres=12, width=8, height=8).scene,
+             scenes.config3(triangles=20_000, width=8, height=8).scene, scenes.config1(width=8, height=8).scene]
+    hits = 0
+    for k, sd in enumerate(cases):
+        es = emu.scene(sd)
+        for j, rays in enumerate((scenes.random_rays(sd, 20_000, 40 + k), _edge_rays(sd, 20_000, 50 + k))):
+            ff, chain, decided = es.hit_surface(rays)
+            hit = decided != 4
+            assert not np.any(decided & 8), "a barycentric pair outside its error bound"
+            bad = np.nonzero(np.any(ff.view(np.uint32) != chain.view(np.uint32), axis=1))[0]
+            assert len(bad) == 0, [(rays[i], ff[i].tolist(), chain[i].tolist()) for i in bad[:2]]
+            hits += int(hit.sum())
+            if j == 0:
+                for bit, floor in ((1, 0.95), (2, 0.9)):
+                    got = ((decided & bit) != 0) & hit
+                    assert got.sum() > floor * hit.sum(), (k, bit, got.sum() / hit.sum())
+    assert hits > 60_000
+    sd, rays = _midpoint_t_case(20_000, 60)
+    ff, chain, decided = emu.scene(sd).hit_surface(rays)
+    assert np.all(decided != 4)
+    np.testing.assert_array_equal(ff.view(np.uint32), chain.view(np.uint32))
+
+
 def test_camera_float_float_equals_the_binary64_chain(emu):
     """Camera::generate (reference src/geometry.cpp:213-234) in float-float with the binary64
     chain for hard cases equals the chain alone (direction, near and far parameters, accept)
