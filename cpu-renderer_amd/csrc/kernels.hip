@@ -71,6 +71,21 @@ __device__ inline void wave_add_keyed(unsigned long long* arr, uint32_t key, uin
     }
 }
 
+// Per-job counters of one workgroup, summed in LDS over the whole launch and
+// added to the device counters once at its end: one atomic per job and
+// workgroup instead of one per wave and round (same-address device atomics
+// from every XCD serialise).
+struct WorkgroupCounts {
+    unsigned long long* slots;  // kJobSlots entries in LDS
+    __device__ void clear() const {
+        for (uint32_t k = threadIdx.x; k < kJobSlots; k += blockDim.x) slots[k] = 0;
+    }
+    __device__ void flush(unsigned long long* dst) const {
+        for (uint32_t k = threadIdx.x; k < kJobSlots; k += blockDim.x)
+            if (slots[k]) atomicAdd(&dst[k], slots[k]);
+    }
+};
+
 // Errors go to a ring the host drains after every step; a step that raises
 // more than kMaxErrors errors keeps the last kMaxErrors records.
 __device__ inline void report_error(Control* ctl, uint32_t job, uint64_t key, uint32_t code, uint32_t depth,
@@ -329,6 +344,33 @@ __device__ inline uint32_t wave_claim32(uint32_t* counter, bool pred) {
     return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
 }
 
+// Claim of queue entries for the idle lanes of a wave from a queue of n
+// entries split into kClaimParts contiguous parts, each with its own head
+// (heads[p * 32]): one atomic per claim on the wave's current part; a lane
+// whose entry would fall past the part's end gets none, and the wave moves to
+// the next part once its part is used up.  `tried` counts the parts the wave
+// has found used up: all of them (tried == kClaimParts) means the queue is
+// drained for this wave.  Returns the lane's queue index or 0xffffffff.
+__device__ inline uint32_t wave_claim_parted(uint32_t* heads, uint32_t n, uint32_t& part, uint32_t& tried,
+                                             bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    if (!mask || tried >= kClaimParts) return 0xffffffffu;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    const uint32_t want = uint32_t(__popcll(mask));
+    const uint32_t lo = uint32_t(uint64_t(part) * n / kClaimParts);
+    const uint32_t hi = uint32_t(uint64_t(part + 1) * n / kClaimParts);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&heads[part * 32], want);
+    base = __shfl(base, leader);
+    if (base + want >= hi - lo || base >= hi - lo) {  // this part is used up (for every wave)
+        part = part + 1 == kClaimParts ? 0 : part + 1;
+        ++tried;
+    }
+    const uint32_t local = base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+    return pred && local < hi - lo ? lo + local : 0xffffffffu;
+}
+
 // Closest-hit word stored per path: primitive index in bits 0..29 (or one of
 // two codes), shading class in bits 30..31 so that shade can group paths by
 // class without touching the scene.
@@ -366,11 +408,12 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_extend(PathSoA 
     const DevScene* sc = nullptr;
     uint32_t path = 0, job = 0;
     bool busy = false, drained = false;
+    uint32_t part = blockIdx.x % kClaimParts, tried = 0;
     TraceCount tc{0, 0, 0};
     for (;;) {
         if (!drained && __popcll(__ballot(busy)) <= kRefillBelow) {
-            const uint32_t idx = wave_claim32(&ctl->ehead, !busy);
-            if (__ballot(!busy && idx >= n)) drained = true;
+            const uint32_t idx = wave_claim_parted(ctl->eparts, n, part, tried, !busy);
+            if (tried >= kClaimParts) drained = true;
             if (!busy && idx < n) {
                 path = queue[idx];
                 const PathRay r = P.ray[path];
@@ -486,11 +529,11 @@ __device__ inline ShadeResult shade_one(const PathSoA& P, const DevJob& J, uint3
 
 // Wave-level bookkeeping of shade_one's results: counters, the shadow queue
 // and the retire list (the continuing paths' next-queue slot is the caller's).
-__device__ inline void shade_emit(const ShadowSoA& S, JobCounters* cnt, Control* ctl, uint32_t* retire, bool live,
-                                  uint32_t path, uint32_t ji, const ShadeResult& o) {
-    wave_add_keyed(cnt->extend_rays, ji, live ? 1u : 0u);
-    wave_add_keyed(cnt->shadow_rays, ji, live && o.sh.emit ? 1u : 0u);
-    wave_add_keyed(cnt->replays, ji, live && o.replayed ? 1u : 0u);
+__device__ inline void shade_emit(const ShadowSoA& S, const WorkgroupCounts* wc, Control* ctl, uint32_t* retire,
+                                  bool live, uint32_t path, uint32_t ji, const ShadeResult& o) {
+    wave_add_keyed(wc[0].slots, ji, live ? 1u : 0u);
+    wave_add_keyed(wc[1].slots, ji, live && o.sh.emit ? 1u : 0u);
+    wave_add_keyed(wc[2].slots, ji, live && o.replayed ? 1u : 0u);
     const uint32_t spos = wave_append(&ctl->scount, live && o.sh.emit);
     if (live && o.sh.emit) {
         S.path[spos] = path;
@@ -510,6 +553,9 @@ __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, S
                                                                   JobCounters* cnt) {
     __shared__ uint32_t stage[kShadeChunk], sorted[kShadeChunk];
     __shared__ uint32_t counts[kShadeClasses], cursor[kShadeClasses];
+    __shared__ unsigned long long wg_extend[kJobSlots], wg_shadow[kJobSlots], wg_replays[kJobSlots];
+    const WorkgroupCounts wc[3] = {{wg_extend}, {wg_shadow}, {wg_replays}};
+    for (const WorkgroupCounts& c : wc) c.clear();
 #if CR_SHADE_OCTSORT
     __shared__ uint32_t ocount[8], ocursor[8], obase;
 #endif
@@ -558,7 +604,7 @@ __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, S
                 ji = P.ray[path].job;
                 o = shade_one(P, jobs[ji], ji, path, hit_word, ctl);
             }
-            shade_emit(S, cnt, ctl, retire, live, path, ji, o);
+            shade_emit(S, wc, ctl, retire, live, path, ji, o);
 #if CR_SHADE_OCTSORT
             if (live) {
                 const bool cont = o.action == kContinue;
@@ -591,6 +637,9 @@ __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, S
 #endif
         __syncthreads();  // `sorted` is rewritten by the next chunk
     }
+    wc[0].flush(cnt->extend_rays);
+    wc[1].flush(cnt->shadow_rays);
+    wc[2].flush(cnt->replays);
 }
 
 // ------------------------------------------------------------------ shadow
@@ -628,12 +677,13 @@ __global__ void __launch_bounds__(256, CR_TRAVERSE_MIN_BLOCKS) k_shadow(PathSoA 
     const DevScene* sc = nullptr;
     uint32_t i = 0, job = 0;
     bool busy = false, drained = false;
+    uint32_t part = blockIdx.x % kClaimParts, tried = 0;
     TraceCount tc{0, 0, 0};
     for (;;) {
         bool fin = false;
         if (!drained && __popcll(__ballot(busy)) <= kRefillBelow) {
-            const uint32_t idx = wave_claim32(&ctl->shead, !busy);
-            if (__ballot(!busy && idx >= n)) drained = true;
+            const uint32_t idx = wave_claim_parted(ctl->sparts, n, part, tried, !busy);
+            if (tried >= kClaimParts) drained = true;
             if (!busy && idx < n) {
                 i = idx;
                 job = S.job[i];
@@ -708,6 +758,10 @@ __global__ void __launch_bounds__(256) k_retire(PathSoA P, const DevJob* jobs, c
     const uint32_t take = retire_takes(ctl);
     const unsigned long long issued = ctl->issued;
     const uint32_t q0 = ctl->qcount[parity ^ 1], f0 = ctl->fcount;
+    __shared__ unsigned long long wg_retired[kJobSlots];
+    const WorkgroupCounts wc{wg_retired};
+    wc.clear();
+    __syncthreads();
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t rounds = (n + stride - 1) / stride;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -731,7 +785,7 @@ __global__ void __launch_bounds__(256) k_retire(PathSoA P, const DevJob* jobs, c
             const uint64_t at = uint64_t(sample) * npix + lp;
             if (lp < npix && at < J.items) J.samples[at] = make_float4(l.x, l.y, l.z, 0);
         }
-        wave_add_keyed(cnt->retired, ji, live ? 1u : 0u);
+        wave_add_keyed(wc.slots, ji, live ? 1u : 0u);
         if (live) {
             if (i < take) {
                 start_item(P, jobs, path, issued + i, ctl);
@@ -741,6 +795,8 @@ __global__ void __launch_bounds__(256) k_retire(PathSoA P, const DevJob* jobs, c
             }
         }
     }
+    __syncthreads();
+    wc.flush(cnt->retired);
 }
 
 // End of an iteration: apply k_retire's claims, clear the consumed counters
@@ -759,6 +815,7 @@ __global__ void k_reset(int parity, Control* ctl) {
     ctl->xscount = 0;
     ctl->ehead = 0;
     ctl->shead = 0;
+    for (uint32_t p = 0; p < kClaimParts; ++p) ctl->eparts[p * 32] = ctl->sparts[p * 32] = 0;
     ctl->khead = 0;
     const unsigned long long issued = ctl->issued, total = ctl->total;
     const unsigned long long avail = issued < total ? total - issued : 0ull;
@@ -821,6 +878,7 @@ __global__ void __launch_bounds__(256) k_pool_init(uint32_t* freeq, uint32_t poo
         ctl->seq_lo = ctl->seq_hi = 0;
         ctl->err_count = 0;
         ctl->ehead = ctl->shead = ctl->khead = 0;
+        for (uint32_t p = 0; p < kClaimParts; ++p) ctl->eparts[p * 32] = ctl->sparts[p * 32] = 0;
         ctl->pool = pool;
         ctl->issued = ctl->total = ctl->refill_base = 0;
         ctl->fallbacks = 0;

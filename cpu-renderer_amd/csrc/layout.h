@@ -344,6 +344,14 @@ constexpr uint32_t kJobSlots = 1024;
 
 // Wavefront control block.  Queue counters are per iteration; issued/total
 // count global work items (camera samples) over every job ever admitted.
+// The traversal queues are split into this many contiguous parts, claimed
+// through separate counters (workgroups spread over the XCDs round-robin, so
+// each XCD starts on a part of its own).
+#ifndef CR_CLAIM_PARTS
+#define CR_CLAIM_PARTS 8
+#endif
+constexpr uint32_t kClaimParts = CR_CLAIM_PARTS;
+
 struct Control {
     uint32_t qcount[2];
     uint32_t scount, rcount;
@@ -354,7 +362,7 @@ struct Control {
     uint32_t refill_q;         // next-queue index of the first of them
     uint32_t seq_lo, seq_hi;   // live job sequence range
     uint32_t err_count;        // monotonic; records go to errors[n % kMaxErrors]
-    uint32_t ehead, shead;     // next unclaimed entry of the extend / shadow queue
+    uint32_t ehead, shead;     // (unused; the traversal queues are claimed through eparts / sparts)
     uint32_t khead;            // next unclaimed shading chunk (extend-queue index)
     uint32_t pool;             // path slots (capacity of every queue)
     uint32_t masks;            // a live job's scene has alpha-mask primitives: traversal variant with FP32 mask tests
@@ -363,6 +371,10 @@ struct Control {
     unsigned long long refill_base;  // first item of the refill range
     unsigned long long fallbacks;    // exact triangle tests of k_trace_rays
     ErrorRec errors[kMaxErrors];
+    // Claim heads of the extend / shadow queues, one per part, each on a
+    // 128-byte line of its own (see the traversal kernels' claim).
+    uint32_t eparts[kClaimParts * 32];
+    uint32_t sparts[kClaimParts * 32];
 };
 
 // Per-job counters, one array per quantity, indexed by job slot.
