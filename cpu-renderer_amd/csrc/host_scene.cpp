@@ -943,13 +943,17 @@ HostEnv build_env(const crg_env_desc& d) {
         }
         e.threshold[i] = lo;
     }
-    e.guide.resize(65537);
-    for (uint32_t j = 0; j <= 65536; ++j) {
-        const uint32_t k = j << 8;
-        e.guide[j] = uint32_t(std::upper_bound(e.threshold.begin(), e.threshold.end(), k) - e.threshold.begin());
-        if (j == 65536) e.guide[j] = uint32_t(n);
+    // guide[j] = #{i : T_i <= j << kGuideShift} (thresholds are nondecreasing:
+    // one merge pass), the last entry n.  It bounds the search window
+    // [G[k >> s], G[(k >> s) + 1]] for any draw k.
+    e.guide.resize(kGuideEntries);
+    size_t below = 0;
+    for (uint32_t j = 0; j + 1 < kGuideEntries; ++j) {
+        const uint32_t k = j << kGuideShift;
+        while (below < n && e.threshold[below] <= k) ++below;
+        e.guide[j] = uint32_t(below);
     }
-    // The guide bounds the search window [G[k>>8], G[(k>>8)+1]] for any k.
+    e.guide[kGuideEntries - 1] = uint32_t(n);
     e.rowcos.resize(size_t(d.height) + 1);
     for (int y = 0; y <= d.height; ++y) e.rowcos[y] = std::cos(kHostPi * float(y) / float(d.height));
     return e;

@@ -228,11 +228,18 @@ struct DevLights {
     float env_probability;
 };
 
+// Draws per guide bucket of the environment inversion: 2^24 >> kGuideShift
+// buckets, each holding the texel index at its first draw, so that a draw's
+// texel lies between two neighbouring entries (a few texels apart even in the
+// sky's low-weight regions: one or two threshold probes instead of a dozen).
+constexpr uint32_t kGuideShift = 2;
+constexpr uint32_t kGuideEntries = (1u << (24 - kGuideShift)) + 1;
+
 struct DevEnv {
     const float4* texels;       // rgb, 1
     const float* pdf;           // per-texel solid-angle pdf (binary64 on host, rounded)
     const uint32_t* threshold;  // T_i: first 24-bit draw whose CDF pick reaches texel i
-    const uint32_t* guide;      // 65537 entries: texel index at k = j << 8
+    const uint32_t* guide;      // kGuideEntries entries: texel index at k = j << kGuideShift
     const float* rowcos;        // cosf(pi*y/H), y = 0..H
     int32_t width, height;
     float rotation, intensity;

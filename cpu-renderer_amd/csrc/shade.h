@@ -671,7 +671,7 @@ CR_OUTLINE float env_pdf(const DevEnv& e, V3 d) {
 // reference's upper_bound over its binary64 CDF (clamped to the last texel).
 CR_HDI uint32_t env_invert(const DevEnv& e, uint32_t k) {
     const uint32_t n = uint32_t(e.width) * uint32_t(e.height);
-    uint32_t lo = e.guide[k >> 8], hi = e.guide[(k >> 8) + 1];
+    uint32_t lo = e.guide[k >> kGuideShift], hi = e.guide[(k >> kGuideShift) + 1];
     while (lo < hi) {  // first index in [lo, hi) with T > k
         const uint32_t mid = (lo + hi) >> 1;
         if (e.threshold[mid] <= k) lo = mid + 1;
