@@ -545,7 +545,7 @@ def measure_plain(ctx, w, steps: int, warmup: int, cpu_budget) -> dict:
     # records, and a 72-byte shadow record written: 584 B (misses read fewer).
     sh_launches, sh_ms = kt["k_shade"]
     sh_ach = 584 * pr.extend_rays / (sh_ms * 1e-3) / 1e9
-    shade_line = {"bound": "valu (integer binary64 emulation), shown against hbm", "achieved": round(sh_ach, 1),
+    shade_line = {"bound": "latency of per-path scattered reads, shown against hbm", "achieved": round(sh_ach, 1),
                   "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(sh_ach / PEAK_HBM_GBS, 4), "kernel": "k_shade",
                   "avg_launch_ms": round(sh_ms / max(1, sh_launches), 4), "bytes_per_path": 584}
     n_node = cr_.extend_nodes / max(1, cr_.extend_rays)
