@@ -560,18 +560,26 @@ __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, S
     __shared__ uint32_t ocount[8], ocursor[8], obase;
 #endif
     const uint32_t n = ctl->qcount[parity];
-    const uint32_t per = kShadeChunk / blockDim.x;  // entries per thread and chunk
+    // Entries per thread and chunk: kShadeChunk / blockDim.x while the queue
+    // fills the grid that many times over; a shorter queue (the last
+    // iterations of a render, when only the longest paths are left) is cut
+    // into smaller chunks so that it spreads over more workgroups instead of
+    // being shaded by a few threads in turn (same results: the chunk only
+    // orders the work).
+    const uint32_t lanes = gridDim.x * blockDim.x;
+    const uint32_t per = min(kShadeChunk / blockDim.x, max(1u, (n + lanes - 1) / lanes));
+    const uint32_t chunk = per * blockDim.x;
     // Workgroups claim chunks one at a time from a queue counter: only two
     // shading workgroups fit a CU, and with a fixed grid-stride assignment the
     // last round of chunks left most CUs idle (k_shade 4.8 % slower).
     __shared__ uint32_t chunk_base;
     for (;;) {
         if (threadIdx.x < kShadeClasses) counts[threadIdx.x] = cursor[threadIdx.x] = 0;
-        if (threadIdx.x == 0) chunk_base = atomicAdd(&ctl->khead, kShadeChunk);
+        if (threadIdx.x == 0) chunk_base = atomicAdd(&ctl->khead, chunk);
         __syncthreads();
         const uint32_t base = chunk_base;  // the same for the whole workgroup
         if (base >= n) break;
-        const uint32_t m = min(kShadeChunk, n - base);
+        const uint32_t m = min(chunk, n - base);
         for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {  // path ids < 2^25: class fits above
             const uint32_t path = queue[base + k];
             const uint32_t c = P.ray[path].hit >> 30;
