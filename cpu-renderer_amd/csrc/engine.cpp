@@ -16,8 +16,8 @@ constexpr uint32_t kMinPool = 1u << 12, kMaxPool = 1u << 25;
 constexpr int kStallSteps = 96;
 
 uint64_t pool_bytes_for(uint32_t paths) {
-    // ray + state records, 4 float4 + 2 u32 of shadow records, 6 u32 queues.
-    return uint64_t(paths) * (sizeof(PathRay) + sizeof(PathRec) + 4 * 16 + 2 * 4 + 6 * 4 + 2 * sizeof(Pending));
+    // ray + state records, 4 float4 + 2 u32 of shadow records, 7 u32 queues.
+    return uint64_t(paths) * (sizeof(PathRay) + sizeof(PathRec) + 4 * 16 + 2 * 4 + 7 * 4 + 2 * sizeof(Pending));
 }
 
 double seconds_between(Clock::time_point a, Clock::time_point b) {
@@ -296,12 +296,14 @@ void Engine::configure() {
     ray_.alloc(size_t(n) * sizeof(PathRay));
     rec_.alloc(size_t(n) * sizeof(PathRec));
     for (DevBuf* b : {&sh_o_, &sh_d_, &sh_lo_, &sh_c_}) b->alloc(f4);
-    for (DevBuf* b : {&sh_path_, &sh_job_, &q0_, &q1_, &retire_, &xq_, &xsq_, &freeq_}) b->alloc(u32);
+    for (DevBuf* b : {&sh_path_, &sh_job_, &q0_, &q1_, &retire_, &rjob_, &xq_, &xsq_, &freeq_})
+        b->alloc(u32);
     for (DevBuf* b : {&xpend_, &xspend_}) b->alloc(size_t(n) * sizeof(Pending));
     P_ = PathSoA{ray_.as<PathRay>(), rec_.as<PathRec>()};
     S_ = ShadowSoA{sh_path_.as<uint32_t>(), sh_job_.as<uint32_t>(), sh_o_.as<float4>(), sh_d_.as<float4>(),
                    sh_lo_.as<float4>(), sh_c_.as<float4>()};
-    Q_ = Queues{{q0_.as<uint32_t>(), q1_.as<uint32_t>()}, retire_.as<uint32_t>(), xq_.as<uint32_t>(),
+    Q_ = Queues{{q0_.as<uint32_t>(), q1_.as<uint32_t>()}, retire_.as<uint32_t>(), rjob_.as<uint32_t>(),
+                xq_.as<uint32_t>(),
                 xsq_.as<uint32_t>(), freeq_.as<uint32_t>(), xpend_.as<Pending>(), xspend_.as<Pending>()};
     ctl_.alloc(sizeof(Control));
     cnt_.alloc(sizeof(JobCounters));

@@ -156,7 +156,7 @@ private:
     // device state
     DevBuf ray_, rec_;
     DevBuf sh_path_, sh_job_, sh_o_, sh_d_, sh_lo_, sh_c_;
-    DevBuf q0_, q1_, retire_, xq_, xsq_, freeq_, xpend_, xspend_;
+    DevBuf q0_, q1_, retire_, rjob_, xq_, xsq_, freeq_, xpend_, xspend_;
     DevBuf ctl_, cnt_, jobs_;
     PathSoA P_{};
     ShadowSoA S_{};

@@ -70,6 +70,7 @@ constexpr const char* kStageNames[kStageCount] = {"k_extend",       "k_extend_ex
 struct Queues {
     uint32_t* q[2];   // extend queues (current / next iteration)
     uint32_t* retire; // finished paths
+    uint32_t* retire_job;  // job slot of each retire entry (written with it)
     uint32_t* xq;     // extend rays needing the exact pass
     uint32_t* xsq;    // shadow records needing the exact pass
     uint32_t* freeq;  // stack of idle path slots

@@ -530,7 +530,7 @@ __device__ inline ShadeResult shade_one(const PathSoA& P, const DevJob& J, uint3
 // Wave-level bookkeeping of shade_one's results: counters, the shadow queue
 // and the retire list (the continuing paths' next-queue slot is the caller's).
 __device__ inline void shade_emit(const ShadowSoA& S, const WorkgroupCounts* wc, Control* ctl, uint32_t* retire,
-                                  bool live, uint32_t path, uint32_t ji, const ShadeResult& o) {
+                                  uint32_t* retire_job, bool live, uint32_t path, uint32_t ji, const ShadeResult& o) {
     wave_add_keyed(wc[0].slots, ji, live ? 1u : 0u);
     wave_add_keyed(wc[1].slots, ji, live && o.sh.emit ? 1u : 0u);
     wave_add_keyed(wc[2].slots, ji, live && o.replayed ? 1u : 0u);
@@ -544,13 +544,16 @@ __device__ inline void shade_emit(const ShadowSoA& S, const WorkgroupCounts* wc,
         S.c[spos] = make_float4(o.sh.c.x, o.sh.c.y, o.sh.c.z, 0);
     }
     const uint32_t rpos = wave_append(&ctl->rcount, live && o.action == kRetire);
-    if (live && o.action == kRetire) retire[rpos] = path;
+    if (live && o.action == kRetire) {
+        retire[rpos] = path;
+        retire_job[rpos] = ji;
+    }
 }
 
 __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, ShadowSoA S, const DevJob* jobs,
                                                                   const uint32_t* queue, uint32_t* next_queue,
-                                                                  uint32_t* retire, int parity, Control* ctl,
-                                                                  JobCounters* cnt) {
+                                                                  uint32_t* retire, uint32_t* retire_job, int parity,
+                                                                  Control* ctl, JobCounters* cnt) {
     __shared__ uint32_t stage[kShadeChunk], sorted[kShadeChunk];
     __shared__ uint32_t counts[kShadeClasses], cursor[kShadeClasses];
     __shared__ unsigned long long wg_extend[kJobSlots], wg_shadow[kJobSlots], wg_replays[kJobSlots];
@@ -612,7 +615,7 @@ __global__ void __launch_bounds__(256, CR_SHADE_MIN_BLOCKS) k_shade(PathSoA P, S
                 ji = P.ray[path].job;
                 o = shade_one(P, jobs[ji], ji, path, hit_word, ctl);
             }
-            shade_emit(S, wc, ctl, retire, live, path, ji, o);
+            shade_emit(S, wc, ctl, retire, retire_job, live, path, ji, o);
 #if CR_SHADE_OCTSORT
             if (live) {
                 const bool cont = o.action == kContinue;
@@ -760,6 +763,7 @@ __device__ inline uint32_t retire_takes(const Control* ctl) {
 }
 
 __global__ void __launch_bounds__(256) k_retire(PathSoA P, const DevJob* jobs, const uint32_t* retire,
+                                                const uint32_t* retire_job,
                                                 uint32_t* next_queue, uint32_t* freeq, int parity, Control* ctl,
                                                 JobCounters* cnt) {
     const uint32_t n = ctl->rcount;
@@ -778,7 +782,7 @@ __global__ void __launch_bounds__(256) k_retire(PathSoA P, const DevJob* jobs, c
         uint32_t path = 0, ji = 0;
         if (live) {
             path = retire[i];
-            ji = P.ray[path].job;
+            ji = retire_job[i];
             const DevJob& J = jobs[ji];
             const PathRec& c = P.rec[path];
             const V3 l = xyz(c.rad);
@@ -1146,8 +1150,8 @@ hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const
             hipLaunchKernelGGL(k_extend_exact, g, b, 0, st, P, jobs, Q.xq, Q.xpend, ctl, cnt);
             break;
         case kStageShade:
-            hipLaunchKernelGGL(k_shade, g, b, 0, st, P, S, jobs, Q.q[parity], Q.q[parity ^ 1], Q.retire, parity, ctl,
-                               cnt);
+            hipLaunchKernelGGL(k_shade, g, b, 0, st, P, S, jobs, Q.q[parity], Q.q[parity ^ 1], Q.retire, Q.retire_job,
+                               parity, ctl, cnt);
             break;
         case kStageShadow:
             if (shape.width == 8) launch_traversal<8, true>(shape.count, g, b, st, P, S, jobs, Q, parity, ctl, cnt);
@@ -1157,7 +1161,8 @@ hipError_t launch_stage(Stage stage, const PathSoA& P, const ShadowSoA& S, const
             hipLaunchKernelGGL(k_shadow_exact, g, b, 0, st, P, S, jobs, Q.xsq, Q.xspend, ctl, cnt);
             break;
         case kStageRetire:
-            hipLaunchKernelGGL(k_retire, g, b, 0, st, P, jobs, Q.retire, Q.q[parity ^ 1], Q.freeq, parity, ctl, cnt);
+            hipLaunchKernelGGL(k_retire, g, b, 0, st, P, jobs, Q.retire, Q.retire_job, Q.q[parity ^ 1], Q.freeq, parity,
+                               ctl, cnt);
             break;
         case kStageReset:
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, parity, ctl);

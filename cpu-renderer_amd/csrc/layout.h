@@ -297,15 +297,18 @@ struct alignas(64) PathRay {
 };
 static_assert(sizeof(PathRay) == 64, "PathRay is one 64-byte record");
 
+// The first 64 bytes hold what the retire kernel reads (radiance, pixel and
+// sample), so it fetches one 64-byte sector of the record.
 struct alignas(128) PathRec {
     float4 thr;      // throughput, etaScale
     float4 rad;      // radiance, lastPdf
-    float4 lastpos;  // lastPosition
-    float4 absorb;   // active medium absorption
     uint64_t rng;
     uint32_t pixel, sample;
     int32_t medium;
-    uint32_t pad[9];
+    uint32_t pad0[3];
+    float4 lastpos;  // lastPosition
+    float4 absorb;   // active medium absorption
+    uint32_t pad[8];
 };
 static_assert(sizeof(PathRec) == 128, "PathRec is one 128-byte record");
 
