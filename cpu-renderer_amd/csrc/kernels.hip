@@ -522,7 +522,14 @@ __device__ inline ShadeResult shade_one(const PathSoA& P, const DevJob& J, uint3
         begin_sample(J, pixel, sample, true, st);
         o.action = kContinue;
     }
-    store_state(P, path, st);
+    if (o.action == kRetire) {
+        // A finished path's records are read again only for its radiance
+        // (k_shadow adds an unoccluded last contribution, k_retire takes the
+        // sum); the slot's next start rewrites both records whole.
+        P.rec[path].rad = make_float4(st.L.x, st.L.y, st.L.z, st.last_pdf);
+    } else {
+        store_state(P, path, st);
+    }
     o.oct = (st.d.x < 0 ? 1u : 0u) | (st.d.y < 0 ? 2u : 0u) | (st.d.z < 0 ? 4u : 0u);
     return o;
 }
