@@ -124,9 +124,10 @@ CR_OUTLINE bool camera_ray(const DevCamera& c, float px, float py, V3& o, V3& d,
 // exact in binary64) rounded to float.  In float-float: scale * n as an exact
 // pair scaled by 2^-21, its sum with p to about 2^-47 relative, one rounding;
 // within 2^-38 (relative) of a binary32 rounding boundary, or where the pair's
-// low part could leave the normal range, the binary64 emulation decides.  A
-// zero n adds a zero of n's sign, as the binary64 sum does.
-CR_HDI float offset_coord(float p, float scale, float n, sf::f64 distance) {
+// low part could leave the normal range, the binary64 emulation decides (the
+// distance is formed there, off the common path).  A zero n adds a zero of
+// n's sign, as the binary64 sum does.
+CR_HDI float offset_coord(float p, float scale, float n) {
     if (n == 0.0f) return p + n;
     const float ph = scale * n, pl = fmaf(scale, n, -ph);
     const float aph = fabsf(ph);
@@ -138,6 +139,7 @@ CR_HDI float offset_coord(float p, float scale, float n, sf::f64 distance) {
         const float half_ulp = 0.5f * fminf(nextafterf(af, INFINITY) - af, af - nextafterf(af, 0.0f));
         if (af >= 0x1p-100f && half_ulp - d >= 0x1p-38f * af) return f;
     }
+    const sf::f64 distance = sf::mul(sf::from_float(4 * kFltEps), sf::from_float(scale));
     return sf::to_float(sf::add(sf::from_float(p), sf::mul(distance, sf::from_float(n))));
 }
 
@@ -154,10 +156,9 @@ CR_OUTLINE uint32_t offset_ray(V3 p, V3 ng, V3 ung, V3 direction, float max_dist
     float scale = fabsf(p.x);
     if (scale < fabsf(p.y)) scale = fabsf(p.y);
     if (scale < fabsf(p.z)) scale = fabsf(p.z);
-    const sf::f64 distance = sf::mul(sf::from_float(4 * kFltEps), sf::from_float(scale));
     for (int a = 0; a < 3; ++a) {
         const float na = at(n, a);
-        float v = offset_coord(at(p, a), scale, na, distance);
+        float v = offset_coord(at(p, a), scale, na);
         if (na != 0) v = na > 0 ? next_up(v) : next_down(v);
         set(origin, a, v);
     }

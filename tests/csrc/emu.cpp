@@ -414,7 +414,7 @@ void emu_interp_point(const float* v, const uint64_t* uv, float* ff, float* chai
 void emu_offset(const float* p, const float* scale, const float* n, float* ff, float* chain, uint64_t count) {
     for (uint64_t i = 0; i < count; ++i) {
         const sf::f64 distance = sf::mul(sf::from_float(4 * kFltEps), sf::from_float(scale[i]));
-        ff[i] = offset_coord(p[i], scale[i], n[i], distance);
+        ff[i] = offset_coord(p[i], scale[i], n[i]);
         chain[i] = sf::to_float(sf::add(sf::from_float(p[i]), sf::mul(distance, sf::from_float(n[i]))));
     }
 }
