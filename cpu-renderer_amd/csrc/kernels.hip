@@ -26,12 +26,14 @@
 // (register budget = occupancy).  Traversal: 4 (128 VGPRs, 4 waves per SIMD)
 // measured fastest on the 3.1 M and 8.0 M triangle renders, ahead of 5 (96
 // VGPRs, heavy spilling) and 3 (no spills, too few waves to hide node-fetch
-// latency); shading needs registers for the emulated binary64 arithmetic.
+// latency).  Shading: 3 (168 VGPRs, some spilling in the rare binary64
+// chains) measured 0.9 % faster than 2 (256 VGPRs) on the 3.1 M triangle
+// render and the same on the 8.0 M one; 4 spills on the common path (-13 %).
 #ifndef CR_TRAVERSE_MIN_BLOCKS
 #define CR_TRAVERSE_MIN_BLOCKS 4
 #endif
 #ifndef CR_SHADE_MIN_BLOCKS
-#define CR_SHADE_MIN_BLOCKS 2
+#define CR_SHADE_MIN_BLOCKS 3
 #endif
 
 namespace cr {
